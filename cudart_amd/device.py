@@ -1,0 +1,196 @@
+"""Device-side convenience layer over the C ABI for torch tensors.
+
+torch provides device memory and the stream; every computation goes through
+librk.so (include/rk.h). There is no CPU path: without the library or a GPU
+these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import native as N
+
+_U32 = torch.int32  # uint32 payloads are carried in int32 tensors
+
+
+def _dev(a, dtype=None) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.to("cuda") if a.device.type != "cuda" else a
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to("cuda")
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    return int(stream.cuda_stream) or None
+
+
+def u32_view(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint32)
+
+
+def to_u32_tensor(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint32).view(np.int32)).to("cuda")
+
+
+def _check(s, what):
+    N.check(s, what)
+
+
+def calc_mesh_boxes(F: torch.Tensor, V: torch.Tensor, stream=None):
+    n = F.shape[0]
+    boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+    idx = torch.empty(n, dtype=_U32, device="cuda")
+    _check(N.lib().rk_calc_mesh_boxes(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, boxes.data_ptr(),
+                                      idx.data_ptr()), "calc_mesh_boxes")
+    return boxes, idx
+
+
+def morton(boxes: torch.Tensor, root: np.ndarray, mode: int, stream=None) -> torch.Tensor:
+    n = boxes.shape[0]
+    dt = torch.int64 if mode == N.RK_MORTON_63 else torch.int32
+    out = torch.empty(n, dtype=dt, device="cuda")
+    rb = N.aabox(root)
+    _check(N.lib().rk_calc_morton_codes(_stream_ptr(stream), boxes.data_ptr(), n, C.byref(rb), mode, out.data_ptr()),
+           "morton")
+    return out
+
+
+def workspace(nbytes: int) -> torch.Tensor:
+    return torch.empty(max(1, nbytes), dtype=torch.uint8, device="cuda")
+
+
+def sort_general(keys: torch.Tensor, vals: torch.Tensor, bits: int = 32, init: bool = False, stream=None):
+    n = keys.shape[0]
+    ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
+    _check(N.lib().rk_radix_sort_general(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits, int(init),
+                                         ws.data_ptr(), ws.numel()), "radix_sort_general")
+    return keys, vals
+
+
+def sort_values(values: torch.Tensor, bits: int = 32, stream=None):
+    n = values.shape[0]
+    ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
+    _check(N.lib().rk_radix_sort(_stream_ptr(stream), values.data_ptr(), n, bits, ws.data_ptr(), ws.numel()),
+           "radix_sort")
+    return values
+
+
+def sort_pairs_u64(keys: torch.Tensor, vals: torch.Tensor, bits: int = 64, stream=None):
+    n = keys.shape[0]
+    ws = workspace(N.lib().rk_sort_workspace_size(n, 8))
+    _check(N.lib().rk_radix_sort_pairs_u64(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits,
+                                           ws.data_ptr(), ws.numel()), "radix_sort_pairs_u64")
+    return keys, vals
+
+
+@dataclass
+class RefTree:
+    faces: torch.Tensor      # (n,4) int32, reordered when complete
+    boxes: torch.Tensor      # (n,2,4) float32
+    indices: torch.Tensor    # (n,) int32
+    nodes: torch.Tensor      # (2n, 8) int32 (32-B bvnodes), first node_count valid
+    info: torch.Tensor       # (8,) int32
+    node_count: int = -1
+
+    def host_info(self) -> np.ndarray:
+        return u32_view(self.info)
+
+
+class LBVHBuilder:
+    """Reusable buffers for rk_build_lbvh on meshes of up to n faces."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.ws = workspace(N.lib().rk_build_lbvh_workspace_size(n))
+        self.boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+        self.indices = torch.empty(n, dtype=_U32, device="cuda")
+        self.nodes = torch.empty((2 * n, 8), dtype=_U32, device="cuda")
+        self.info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
+
+    def build(self, F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, tree_depth: int = 12,
+              complete: bool = True, stream=None) -> RefTree:
+        n = F.shape[0]
+        assert n <= self.n
+        hb = N.aabox(hitbox)
+        _check(N.lib().rk_build_lbvh(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, C.byref(hb), int(complete),
+                                     tree_depth, self.boxes.data_ptr(), self.indices.data_ptr(),
+                                     self.nodes.data_ptr(), 2 * self.n, self.info.data_ptr(), self.ws.data_ptr(),
+                                     self.ws.numel()), "build_lbvh")
+        return RefTree(F, self.boxes[:n], self.indices[:n], self.nodes, self.info)
+
+
+def build_lbvh(F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, tree_depth: int = 12, complete: bool = True,
+               stream=None) -> RefTree:
+    t = LBVHBuilder(F.shape[0]).build(F, V, hitbox, tree_depth, complete, stream)
+    torch.cuda.synchronize()
+    t.node_count = int(t.host_info()[N.RK_INFO_NODE_COUNT])
+    return t
+
+
+def new_outputs(n: int, tmax: float | None = None):
+    fid = torch.zeros(n, dtype=_U32, device="cuda")
+    uv = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+    dep = torch.full((n,), float(np.finfo(np.float32).max) if tmax is None else tmax, dtype=torch.float32,
+                     device="cuda")
+    return fid, uv, dep
+
+
+def trace_ref(tree: RefTree, V: torch.Tensor, O: torch.Tensor, D: torch.Tensor, width: int, height: int,
+              coherency: bool = True, out=None, stream=None):
+    n = width * height
+    fid, uv, dep = out if out is not None else new_outputs(n)
+    node_count = tree.node_count if tree.node_count > 0 else tree.nodes.shape[0]
+    _check(N.lib().rk_raytrace_mesh_tree(_stream_ptr(stream), tree.nodes.data_ptr(), node_count, tree.faces.data_ptr(),
+                                         tree.faces.shape[0], V.data_ptr(), O.data_ptr(), D.data_ptr(), width, height, fid.data_ptr(),
+                                         uv.data_ptr(), dep.data_ptr(), int(coherency)), "raytrace_mesh_tree")
+    return fid, uv, dep
+
+
+@dataclass
+class FastTree:
+    nodes: torch.Tensor
+    tris: torch.Tensor
+    indices: torch.Tensor
+    boxes: torch.Tensor
+    info: torch.Tensor
+    desc: N.FastBVH
+
+
+class FastBuilder:
+    """Reusable buffers for rk_fast_build on meshes of up to n faces."""
+
+    def __init__(self, n: int, morton_mode: int = N.RK_MORTON_30):
+        self.n = n
+        self.mode = morton_mode
+        self.ws = workspace(N.lib().rk_fast_build_workspace_size(n, morton_mode))
+        self.nodes = torch.empty((max(1, n - 1), 16), dtype=_U32, device="cuda")
+        self.tris = torch.empty((n, 12), dtype=torch.float32, device="cuda")
+        self.indices = torch.empty(n, dtype=_U32, device="cuda")
+        self.boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+        self.info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
+
+    def build(self, F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, leaf_max: int = 4,
+              stream=None) -> FastTree:
+        n = F.shape[0]
+        assert n <= self.n
+        desc = N.FastBVH(self.nodes.data_ptr(), self.tris.data_ptr(), self.indices.data_ptr(), self.boxes.data_ptr(),
+                         n, 0)
+        hb = N.aabox(hitbox)
+        _check(N.lib().rk_fast_build(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, C.byref(hb), self.mode,
+                                     leaf_max, C.byref(desc), self.info.data_ptr(), self.ws.data_ptr(),
+                                     self.ws.numel()), "fast_build")
+        return FastTree(self.nodes, self.tris, self.indices[:n], self.boxes[:n], self.info, desc)
+
+
+def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None):
+    n = O.shape[0]
+    fid, uv, dep = out if out is not None else new_outputs(n)
+    _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                 fid.data_ptr(), uv.data_ptr(), dep.data_ptr()), "fast_trace")
+    return fid, uv, dep

@@ -1,0 +1,202 @@
+"""ctypes binding of the C ABI in include/rk.h and include/rk_scenes.h.
+
+The library is loaded from cudart_amd/lib/librk.so (built in-tree by
+cudart_amd/build.py). There is no fallback: if the library is missing the
+import raises, so nothing silently runs on the CPU instead of the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+LIB_PATH = LIB_DIR / "librk.so"
+
+RK_SUCCESS = 0
+RK_MORTON_REF32, RK_MORTON_30, RK_MORTON_63 = 0, 1, 2
+RK_INFO_NODE_COUNT, RK_INFO_REAL_DEPTH, RK_INFO_FLAGS, RK_INFO_SPLITS, RK_INFO_LEAVES = 0, 1, 2, 3, 4
+RK_INFO_WORDS = 8
+RK_FLAG_WALK_PAST_END, RK_FLAG_WALK_PAST_DEPTH, RK_FLAG_MERGE2X_OOB, RK_FLAG_NODES_GT_N = 1, 2, 4, 8
+RK_FLAG_SAH_NO_CUT, RK_FLAG_SPIN_LIMIT = 16, 64
+
+
+class RkError(RuntimeError):
+    pass
+
+
+class Float4(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Float2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class AABox(C.Structure):
+    _fields_ = [("min", Float4), ("max", Float4)]
+
+
+class FastBVH(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("indices", C.c_void_p), ("boxes", C.c_void_p),
+                ("face_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I = C.c_int
+_SZ = C.c_size_t
+
+_SIGS = {
+    "rk_status_string": (C.c_char_p, [_I]),
+    "rk_device_count": (_I, [C.POINTER(_I)]),
+    "rk_malloc": (_I, [C.POINTER(_P), _SZ]),
+    "rk_free": (_I, [_P]),
+    "rk_malloc_host": (_I, [C.POINTER(_P), _SZ]),
+    "rk_free_host": (_I, [_P]),
+    "rk_memcpy": (_I, [_P, _P, _SZ, _I, _P]),
+    "rk_stream_create": (_I, [C.POINTER(_P)]),
+    "rk_stream_destroy": (_I, [_P]),
+    "rk_stream_synchronize": (_I, [_P]),
+    "rk_fill_u32": (_I, [_P, _P, _SZ, _U32]),
+    "rk_fill_f32": (_I, [_P, _P, _SZ, C.c_float]),
+    "rk_fill_f2": (_I, [_P, _P, _SZ, Float2]),
+    "rk_fill_f4": (_I, [_P, _P, _SZ, Float4]),
+    "rk_calc_primary_rays": (_I, [_P, _I, _I, Float4, Float4, Float4, Float4, _P, _P]),
+    "rk_calc_mesh_boxes": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "rk_reorder_workspace_size": (_SZ, [_U32]),
+    "rk_reorder_mesh_faces": (_I, [_P, _P, _P, _U32, _P, _SZ]),
+    "rk_sort_workspace_size": (_SZ, [_U32, _U32]),
+    "rk_radix_sort_general": (_I, [_P, _P, _P, _U32, _U32, _I, _P, _SZ]),
+    "rk_radix_sort": (_I, [_P, _P, _U32, _U32, _P, _SZ]),
+    "rk_radix_sort_pairs_u64": (_I, [_P, _P, _P, _U32, _U32, _P, _SZ]),
+    "rk_calc_morton_codes": (_I, [_P, _P, _U32, C.POINTER(AABox), _I, _P]),
+    "rk_build_lbvh_workspace_size": (_SZ, [_U32]),
+    "rk_build_lbvh": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, _P, _P, _P, _U32, _P, _P, _SZ]),
+    "rk_raytrace_mesh_tree": (_I, [_P, _P, _U32, _P, _U32, _P, _P, _P, _I, _I, _P, _P, _P, _I]),
+    "rk_fast_build_workspace_size": (_SZ, [_U32, _I]),
+    "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
+    "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P]),
+    # scenes
+    "rk_scene_icosphere": (_I, [_U32, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "rk_scene_heightfield": (_I, [_U32, C.c_uint64, C.c_float, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "rk_scene_soup": (_I, [_U32, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float, _P, _P,
+                           C.POINTER(_U32), C.POINTER(_U32)]),
+    "rk_scene_hitbox": (_I, [_P, _U32, C.POINTER(AABox)]),
+    "rk_camera_lookat": (_I, [Float4, Float4, C.c_float, C.c_float, _I, _I, _P]),
+    "rk_camera_lookat_up": (_I, [Float4, Float4, Float4, C.c_float, C.c_float, _I, _I, _P]),
+    "rk_primary_rays_host": (_I, [_P, _I, _I, _P, _P]),
+    "rk_rays_random": (_I, [_U32, C.c_uint64, C.c_float, C.c_float, _P, _P]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load librk.so once and attach the signatures. Raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RkError(f"{LIB_PATH} is not built; run `python -m cudart_amd.build` (or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def check(status: int, what: str = "") -> None:
+    if status != RK_SUCCESS:
+        msg = lib().rk_status_string(status).decode()
+        raise RkError(f"{what}: {msg} ({status})")
+
+
+def ptr(a) -> int:
+    """Device or host address of a torch tensor or numpy array."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def f4(v) -> Float4:
+    v = list(v) + [0.0] * (4 - len(v))
+    return Float4(*[float(x) for x in v[:4]])
+
+
+def aabox(b: np.ndarray) -> AABox:
+    """np.float32 array (2,4) -> AABox."""
+    b = np.asarray(b, dtype=np.float32).reshape(2, 4)
+    return AABox(f4(b[0]), f4(b[1]))
+
+
+# ---- host scene generators (numpy) ----------------------------------------
+
+def icosphere(levels: int):
+    L = lib()
+    nv, nf = _U32(), _U32()
+    check(L.rk_scene_icosphere(levels, None, None, C.byref(nv), C.byref(nf)), "icosphere")
+    V = np.zeros((nv.value, 4), np.float32)
+    F = np.zeros((nf.value, 4), np.uint32)
+    check(L.rk_scene_icosphere(levels, V.ctypes.data, F.ctypes.data, C.byref(nv), C.byref(nf)), "icosphere")
+    return V, F
+
+
+def heightfield(grid: int, seed: int = 0xC2, amplitude: float = 0.2):
+    L = lib()
+    nv, nf = _U32(), _U32()
+    check(L.rk_scene_heightfield(grid, seed, amplitude, None, None, C.byref(nv), C.byref(nf)), "heightfield")
+    V = np.zeros((nv.value, 4), np.float32)
+    F = np.zeros((nf.value, 4), np.uint32)
+    check(L.rk_scene_heightfield(grid, seed, amplitude, V.ctypes.data, F.ctypes.data, C.byref(nv), C.byref(nf)),
+          "heightfield")
+    return V, F
+
+
+def soup(n: int, seed: int = 0xC3, mu: float = float(np.log(0.002)), sigma: float = 0.5, lo: float = 0.0,
+         hi: float = 1.0):
+    L = lib()
+    V = np.zeros((3 * n, 4), np.float32)
+    F = np.zeros((n, 4), np.uint32)
+    nv, nf = _U32(), _U32()
+    check(L.rk_scene_soup(n, seed, mu, sigma, lo, hi, V.ctypes.data, F.ctypes.data, C.byref(nv), C.byref(nf)),
+          "soup")
+    return V, F
+
+
+def hitbox(V: np.ndarray) -> np.ndarray:
+    b = AABox()
+    check(lib().rk_scene_hitbox(V.ctypes.data, V.shape[0], C.byref(b)), "hitbox")
+    return np.array([[b.min.x, b.min.y, b.min.z, b.min.w], [b.max.x, b.max.y, b.max.z, b.max.w]], np.float32)
+
+
+def camera_lookat(eye, target, fov: float, width: int, height: int, up=None) -> np.ndarray:
+    out = np.zeros((4, 4), np.float32)
+    ratio = float(width) / float(height)
+    if up is None:
+        check(lib().rk_camera_lookat(f4(eye), f4(target), fov, ratio, width, height, out.ctypes.data), "camera")
+    else:
+        check(lib().rk_camera_lookat_up(f4(eye), f4(target), f4(up), fov, ratio, width, height, out.ctypes.data),
+              "camera")
+    return out
+
+
+def primary_rays_host(cam: np.ndarray, width: int, height: int):
+    O = np.zeros((width * height, 4), np.float32)
+    D = np.zeros((width * height, 4), np.float32)
+    cam = np.ascontiguousarray(cam, np.float32)
+    check(lib().rk_primary_rays_host(cam.ctypes.data, width, height, O.ctypes.data, D.ctypes.data), "rays")
+    return O, D
+
+
+def random_rays(n: int, seed: int = 0xC3, lo: float = 0.0, hi: float = 1.0):
+    O = np.zeros((n, 4), np.float32)
+    D = np.zeros((n, 4), np.float32)
+    check(lib().rk_rays_random(n, seed, lo, hi, O.ctypes.data, D.ctypes.data), "random rays")
+    return O, D

@@ -1,0 +1,184 @@
+/*
+ * rk.h — C ABI of the MI355X (gfx950) mesh-tree build and closest-hit trace
+ * library. This is the drop-in boundary: the C++ host classes in
+ * include/renderkit/ (CUDAMeshTree, CUDAMesh, CUDABuffer, CUDARaytracer, ...)
+ * call only these functions. The reference's own boundary is a set of C++
+ * free functions declared as prototypes by their callers
+ * (CUDAMeshTree.cpp:26-38, CUDARaytracer.h:24-36, CUDA_buildMeshTreeLBVH.cu:23-29);
+ * each declaration below names the one it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Pointers marked "device" address GPU
+ *    memory (from rk_malloc or any HIP/torch allocation on the current device).
+ *  - Every call is stream-ordered on 'stream' (NULL = the default stream) and
+ *    returns without waiting for the GPU, except where noted.
+ *  - Temporaries come from a caller-provided workspace ('ws', 'ws_bytes'), sized
+ *    by the matching *_workspace_size function. The library keeps no global
+ *    device state, so calls on different streams with different workspaces are
+ *    independent and may run concurrently.
+ *  - Errors are returned as rk_status; nothing exits the process.
+ */
+#ifndef RK_H
+#define RK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rk_status;
+#define RK_SUCCESS 0
+#define RK_ERROR_INVALID_VALUE 1
+#define RK_ERROR_OUT_OF_MEMORY 2
+#define RK_ERROR_LAUNCH 3
+#define RK_ERROR_WORKSPACE 4
+#define RK_ERROR_DEVICE 5
+#define RK_ERROR_UNSUPPORTED 6
+
+/* POD types with the reference's layouts (16-byte aligned where the reference
+ * uses __builtin_align__(16)). */
+typedef struct rk_float2 { float x, y; } rk_float2;
+typedef struct rk_float4 { float x, y, z, w; } rk_float4;
+typedef struct rk_uint4 { uint32_t x, y, z, w; } rk_uint4;
+/* aabox, CUDABox.h:17-148: float4 min, float4 max (32 B). */
+typedef struct rk_aabox { rk_float4 min, max; } rk_aabox;
+/* bvnode, CUDANode.h:20-48 (32 B). word0 bit 0 = type (0 node, 1 leaf);
+ * node: bits 1-2 = split axis, word1 = child_id (children child_id, child_id+1);
+ * leaf: bits 1-31 = obj_count, word1 = obj_id. Then float3 box_min, box_max. */
+typedef struct rk_bvnode { uint32_t word0, word1; float box_min[3], box_max[3]; } rk_bvnode;
+
+/* An opaque stream handle (a hipStream_t). NULL is the default stream. */
+typedef struct rk_stream_st* rk_stream;
+
+/* Build diagnostics written to the device array 'info' by the builders. */
+#define RK_INFO_NODE_COUNT 0 /* number of nodes M */
+#define RK_INFO_REAL_DEPTH 1 /* max leaf level + 1 (LBVH.cu:452) */
+#define RK_INFO_FLAGS 2      /* RK_FLAG_* bits */
+#define RK_INFO_SPLITS 3     /* number of Morton splits S (LBVH.cu:569) */
+#define RK_INFO_LEAVES 4     /* number of leaves */
+#define RK_INFO_WORDS 8
+#define RK_FLAG_WALK_PAST_END 1   /* split walk ran past the last split (reference reads past its array) */
+#define RK_FLAG_WALK_PAST_DEPTH 2 /* split walk reached levels > tree_depth (reference order there is nondeterministic) */
+#define RK_FLAG_MERGE2X_OOB 4     /* reference bitonic Merge2x would write out of bounds (BSG.cu:254, BS.cu:196) */
+#define RK_FLAG_NODES_GT_N 8      /* M > n: the reference's N-node buffers would overflow (LBVH.cu:466-470) */
+#define RK_FLAG_SAH_NO_CUT 16     /* SAH node with >= 224 objects and no finite-cost cut (reference loops forever) */
+#define RK_FLAG_SPIN_LIMIT 64     /* a look-back wait hit its iteration limit; results invalid */
+
+/* ---- runtime ------------------------------------------------------------- */
+const char* rk_status_string(rk_status s);
+/* Returns the number of visible GPUs in *count (0 if none). */
+rk_status rk_device_count(int* count);
+/* CUDAArray.h:144-148 cudaMalloc / :150-154 cudaFree. */
+rk_status rk_malloc(void** ptr, size_t bytes);
+rk_status rk_free(void* ptr);
+/* Pinned host memory (cudaMallocHost in the reference's quickLoad/quickSave). */
+rk_status rk_malloc_host(void** ptr, size_t bytes);
+rk_status rk_free_host(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous when
+ * stream is NULL (CUDAArray.h:169-208 copyData*), else stream-ordered. */
+rk_status rk_memcpy(void* dst, const void* src, size_t bytes, int kind, rk_stream stream);
+rk_status rk_stream_create(rk_stream* stream);
+rk_status rk_stream_destroy(rk_stream stream);
+rk_status rk_stream_synchronize(rk_stream stream);
+
+/* ---- surfaces (CUDA_fillSurface.cu:17-61) ---------------------------------- */
+/* template<T> CUDA_fillSurface(CUDASurface<T>*, T), CUDABuffer.cpp:27-32. */
+rk_status rk_fill_u32(rk_stream stream, uint32_t* dst, size_t count, uint32_t value);
+rk_status rk_fill_f32(rk_stream stream, float* dst, size_t count, float value);
+rk_status rk_fill_f2(rk_stream stream, rk_float2* dst, size_t count, rk_float2 value);
+rk_status rk_fill_f4(rk_stream stream, rk_float4* dst, size_t count, rk_float4 value);
+
+/* ---- primary rays (CUDA_calcPrimaryRays.cu:20-32, kernel part) ------------- */
+/* origins[y*w+x] = ray_orig; directions[y*w+x] = (ray_dir - x*ray_x) - y*ray_y. */
+rk_status rk_calc_primary_rays(rk_stream stream, int width, int height, rk_float4 ray_orig, rk_float4 ray_dir,
+                               rk_float4 ray_x, rk_float4 ray_y, rk_float4* origins, rk_float4* directions);
+
+/* ---- build pieces ----------------------------------------------------------- */
+/* void CUDA_calcMeshBoxes(aabox*, uint*, uint4*, float4*, uint), CUDA_calcMeshBoxes.cu:65. */
+rk_status rk_calc_mesh_boxes(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t count,
+                             rk_aabox* boxes, uint32_t* indices);
+/* void CUDA_reorderMeshFaces(uint4*, uint*, uint), CUDA_reorderMeshFaces.cu:44. */
+size_t rk_reorder_workspace_size(uint32_t count);
+rk_status rk_reorder_mesh_faces(rk_stream stream, rk_uint4* faces, const uint32_t* indices, uint32_t count, void* ws,
+                                size_t ws_bytes);
+
+/* ---- sorts ------------------------------------------------------------------ */
+/* Workspace for any sort below of 'length' elements with keys of key_bytes (4 or 8). */
+size_t rk_sort_workspace_size(uint32_t length, uint32_t key_bytes);
+/* void CUDA_radixSortGeneral(uint* keys, uint* indices, uint length, uint bit_depth, bool init_indices),
+ * CUDA_buildMeshTreeLBVH.cu:29 / CUDA_radixSortGeneral.cu:397. Same tie order by size class:
+ * length <= 512 stable; <= byte_depth*8192 bitonic order within 128-chunks then stable by key;
+ * else stable by the low 8*byte_depth bits. */
+rk_status rk_radix_sort_general(rk_stream stream, uint32_t* keys, uint32_t* indices, uint32_t length,
+                                uint32_t bit_depth, int init_indices, void* ws, size_t ws_bytes);
+/* void CUDA_radixSort(uint* values, uint length, uint bit_depth), CUDA_buildMeshTreeLBVH.cu:27 / CUDA_radixSort.cu:357. */
+rk_status rk_radix_sort(rk_stream stream, uint32_t* values, uint32_t length, uint32_t bit_depth, void* ws,
+                        size_t ws_bytes);
+/* Stable (u64 key, u32 value) sort on the low 'bit_depth' bits (63-bit Morton keys; no reference counterpart). */
+rk_status rk_radix_sort_pairs_u64(rk_stream stream, uint64_t* keys, uint32_t* values, uint32_t length,
+                                  uint32_t bit_depth, void* ws, size_t ws_bytes);
+
+/* ---- Morton codes (kernel_calcMortonCodes, CUDA_buildMeshTreeLBVH.cu:115-173) ---- */
+#define RK_MORTON_REF32 0 /* the reference 11/11/10-bit code */
+#define RK_MORTON_30 1    /* 10/10/10 bits */
+#define RK_MORTON_63 2    /* 21/21/21 bits (written as uint64) */
+rk_status rk_calc_morton_codes(rk_stream stream, const rk_aabox* boxes, uint32_t count, const rk_aabox* root_box,
+                               int mode, void* codes);
+
+/* ---- reference LBVH build --------------------------------------------------- */
+/* void CUDA_buildMeshTreeLBVH(CUDAMesh*, CUDAMeshTree*, bool tree_complete, uint tree_depth),
+ * CUDAMeshTree.cpp:26-32 / CUDA_buildMeshTreeLBVH.cu:457. Inputs are the mesh arrays and its
+ * hitbox (host value, the Morton root box). Writes boxes[n], indices[n], nodes[M] (M <= 2n-1,
+ * nodes_capacity >= 2n-1), info[RK_INFO_WORDS] (device) and, when tree_complete, reorders
+ * faces in place. tree_depth <= 31. */
+size_t rk_build_lbvh_workspace_size(uint32_t face_count);
+rk_status rk_build_lbvh(rk_stream stream, rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                        const rk_aabox* hitbox, int tree_complete, uint32_t tree_depth, rk_aabox* boxes,
+                        uint32_t* indices, rk_bvnode* nodes, uint32_t nodes_capacity, uint32_t* info, void* ws,
+                        size_t ws_bytes);
+
+/* ---- reference trace -------------------------------------------------------- */
+/* void CUDA_raytraceMeshTree(CUDABuffer*, CUDAMesh*, CUDAMeshTree*, bool coherency),
+ * CUDARaytracer.h:30-36 / CUDA_raytraceMeshTree.cu:434. faces_id/coords/depths are the
+ * buffer's output surfaces; depths is read as tmax. On a miss the three outputs are left
+ * unchanged. coherency != 0 selects the 16x12 packet traversal (the reference default). */
+rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32_t node_count,
+                                const rk_uint4* faces, uint32_t face_count, const rk_float4* vertices,
+                                const rk_float4* origins,
+                                const rk_float4* directions, int width, int height, uint32_t* faces_id,
+                                rk_float2* coords, float* depths, int coherency);
+
+/* ---- wide-BVH fast path (no reference counterpart) --------------------------- */
+/* A second tree for throughput: Morton LBVH (Karras topology, atomic bottom-up refit,
+ * leaf collapse), stored as 64-B two-child nodes and leaf-ordered triangle records
+ * (v0, v1-v0, v2-v0; 48 B). The trace uses the reference triangle arithmetic, so a
+ * hit's t/u/v are bit-identical to the reference's for the same triangle; ties on t
+ * go to the smallest original face id.
+ *   nodes: capacity max(1, n-1) x 64 B; tris: n x 48 B; indices: n (sorted
+ *   position -> original face id); boxes: n (API compatibility, original order).
+ * morton_mode is RK_MORTON_30 or RK_MORTON_63; leaf_max in [1, 64]. */
+typedef struct rk_fast_bvh {
+  void* nodes;
+  void* tris;
+  uint32_t* indices;
+  rk_aabox* boxes;
+  uint32_t face_count;
+  uint32_t reserved;
+} rk_fast_bvh;
+size_t rk_fast_build_workspace_size(uint32_t face_count, int morton_mode);
+rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                        const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
+                        uint32_t* info, void* ws, size_t ws_bytes);
+/* Closest hit for n_rays rays. faces_id receives the sorted position (index into
+ * bvh->tris; original face id = bvh->indices[position]); depths is read as tmax. */
+rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                        const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
+                        float* depths);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RK_H */

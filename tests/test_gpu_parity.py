@@ -1,0 +1,273 @@
+"""Parity of the gfx950 path (through the C ABI) with the oracle, bit-exact:
+boxes, Morton codes, sort permutations with the reference tie order, the
+reference LBVH (topology, boxes, indices, reordered faces, real_depth), both
+reference traversals, and the fast tree's closest hits."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32MAX = np.finfo(np.float32).max
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    from cudart_amd import device
+    torch.cuda.init()
+    return device
+
+
+def _t(a):
+    import torch
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a).to("cuda")
+
+
+def _h(t, dtype=None):
+    import torch
+    torch.cuda.synchronize()
+    a = t.cpu().numpy()
+    if dtype is not None:
+        a = a.view(dtype)
+    return a
+
+
+def _scene(rk, name):
+    if name == "ico3":
+        V, F = rk.icosphere(3)
+    elif name == "ico5":
+        V, F = rk.icosphere(5)
+    elif name == "hf128":
+        V, F = rk.heightfield(128)
+    elif name == "hf708":
+        V, F = rk.heightfield(708)
+    elif name == "soup50k":
+        V, F = rk.soup(50000, seed=11)
+    elif name == "tiny":
+        V, F = rk.soup(37, seed=2)
+    else:
+        raise KeyError(name)
+    return V, F
+
+
+@pytest.mark.parametrize("name", ["ico3", "hf128", "soup50k"])
+def test_boxes_and_morton(dev, rk, ora, name):
+    V, F = _scene(rk, name)
+    hb = rk.hitbox(V)
+    boxes, idx = dev.calc_mesh_boxes(_t(F), _t(V))
+    ob, oi = ora.calc_boxes(F, V)
+    np.testing.assert_array_equal(_h(boxes), ob)
+    np.testing.assert_array_equal(_h(idx, np.uint32), oi)
+    m32 = dev.morton(boxes, hb, rk.RK_MORTON_REF32)
+    np.testing.assert_array_equal(_h(m32, np.uint32), ora.morton32(ob, hb))
+    m30 = dev.morton(boxes, hb, rk.RK_MORTON_30)
+    np.testing.assert_array_equal(_h(m30, np.uint32).astype(np.uint64), ora.morton_wide(ob, hb, 10))
+    m63 = dev.morton(boxes, hb, rk.RK_MORTON_63)
+    np.testing.assert_array_equal(_h(m63, np.uint64), ora.morton_wide(ob, hb, 21))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 511, 512, 513, 700, 4096, 20480, 32768, 32769, 100000, 1 << 20])
+@pytest.mark.parametrize("dist", ["dups", "wide"])
+def test_sort_general_reference_tie_order(dev, ora, n, dist):
+    rng = np.random.default_rng(n * 7 + len(dist))
+    if dist == "dups":
+        keys = rng.integers(0, 5, n, dtype=np.uint32)
+    else:
+        keys = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+        keys[::17] = keys[0]
+    ok, ov, _ = ora.sort_pairs_ref(keys, np.zeros(n, np.uint32), 32, init=True)
+    k, v = _t(keys), _t(np.zeros(n, np.uint32))
+    dev.sort_general(k, v, 32, init=True)
+    np.testing.assert_array_equal(_h(k, np.uint32), ok)
+    np.testing.assert_array_equal(_h(v, np.uint32), ov)
+
+
+@pytest.mark.parametrize("n,bits", [(9000, 8), (70000, 8), (50000, 21), (300000, 16)])
+def test_sort_general_low_byte_classes(dev, ora, n, bits):
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 1 << bits, n, dtype=np.uint32)
+    vals = rng.integers(0, 1 << 31, n, dtype=np.uint32)
+    ok, ov, _ = ora.sort_pairs_ref(keys, vals, bits, init=False)
+    k, v = _t(keys), _t(vals)
+    dev.sort_general(k, v, bits, init=False)
+    np.testing.assert_array_equal(_h(k, np.uint32), ok)
+    np.testing.assert_array_equal(_h(v, np.uint32), ov)
+
+
+@pytest.mark.parametrize("n", [5, 1000, 40000, 500000])
+def test_sort_values(dev, n):
+    rng = np.random.default_rng(n)
+    vals = rng.permutation(n * 3)[:n].astype(np.uint32)
+    t = _t(vals)
+    dev.sort_values(t)
+    np.testing.assert_array_equal(_h(t, np.uint32), np.sort(vals))
+
+
+@pytest.mark.parametrize("n", [2, 777, 65537, 1 << 20])
+def test_sort_pairs_u64_stable(dev, ora, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+    keys[::5] = keys[1]
+    ok, ov = ora.sort_pairs_u64(keys, np.arange(n, dtype=np.uint32))
+    k, v = _t(keys), _t(np.arange(n, dtype=np.uint32))
+    dev.sort_pairs_u64(k, v)
+    np.testing.assert_array_equal(_h(k, np.uint64), ok)
+    np.testing.assert_array_equal(_h(v, np.uint32), ov)
+
+
+def _compare_tree(dev, rk, ora, V, F, depth, complete):
+    hb = rk.hitbox(V)
+    ref = ora.build_lbvh_ref(F, V, hb, depth, complete)
+    Ft = _t(F)
+    tree = dev.build_lbvh(Ft, _t(V), hb, depth, complete)
+    info = tree.host_info()
+    M = int(info[rk.RK_INFO_NODE_COUNT])
+    assert M == ref.node_count
+    assert int(info[rk.RK_INFO_REAL_DEPTH]) == ref.real_depth
+    assert int(info[rk.RK_INFO_FLAGS]) & 0xFF == ref.flags
+    np.testing.assert_array_equal(_h(tree.indices, np.uint32), ref.indices)
+    np.testing.assert_array_equal(_h(tree.boxes), ref.boxes)
+    np.testing.assert_array_equal(_h(Ft, np.uint32), ref.faces)
+    nodes = _h(tree.nodes, np.uint32)[:M].copy().view(ora.NODE_DTYPE).reshape(M)
+    np.testing.assert_array_equal(nodes["w0"], ref.nodes["w0"])
+    np.testing.assert_array_equal(nodes["w1"], ref.nodes["w1"])
+    # boxes compare as floats: -0 == +0 (min/max of signed zeros may differ)
+    assert np.array_equal(nodes["bmin"], ref.nodes["bmin"])
+    assert np.array_equal(nodes["bmax"], ref.nodes["bmax"])
+    return tree, ref
+
+
+@pytest.mark.parametrize("name", ["tiny", "ico3", "ico5", "hf128", "soup50k"])
+@pytest.mark.parametrize("depth", [0, 4, 12, 31])
+def test_reference_lbvh_bit_exact(dev, rk, ora, name, depth):
+    V, F = _scene(rk, name)
+    _compare_tree(dev, rk, ora, V, F, depth, True)
+
+
+@pytest.mark.parametrize("name", ["ico5", "soup50k"])
+def test_reference_lbvh_incomplete(dev, rk, ora, name):
+    V, F = _scene(rk, name)
+    _compare_tree(dev, rk, ora, V, F, 12, False)
+
+
+def test_reference_lbvh_heightfield_full_size(dev, rk, ora):
+    V, F = _scene(rk, "hf708")
+    assert F.shape[0] == 999698
+    _compare_tree(dev, rk, ora, V, F, 12, True)
+
+
+def _camera_rays(rk, name, w, h):
+    if name.startswith("ico"):
+        cam = rk.camera_lookat([0, 0, 3], [0, 0, 0], 60 * 0.01745329252, w, h)
+    else:
+        el, az, dist = np.radians(30.0), np.radians(45.0), 1.5
+        tgt = np.array([0.5, 0.5, 0.1])
+        eye = tgt + dist * np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+        cam = rk.camera_lookat(eye, tgt, 60 * 0.01745329252, w, h, up=[0, 0, 1])
+    return rk.primary_rays_host(cam, w, h)
+
+
+@pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29)])
+@pytest.mark.parametrize("coherency", [False, True])
+def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
+    V, F = _scene(rk, name)
+    tree, ref = _compare_tree(dev, rk, ora, V, F, 12, True)
+    O, D = _camera_rays(rk, name, w, h)
+    n = w * h
+    rng = np.random.default_rng(3)
+    fid0 = rng.integers(0, 1000, n, dtype=np.uint32)
+    uv0 = rng.random((n, 2), dtype=np.float32)
+    dep0 = np.full(n, F32MAX, np.float32)
+    dep0[::7] = 2.5  # some rays start with a finite tmax
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, coherency, out=out)
+    if coherency:
+        of, ouv, od, ovf = ora.trace_ref_packet(ref.nodes, ref.faces, V, O, D, w, h, fid0, uv0, dep0)
+    else:
+        of, ouv, od, ovf = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0)
+    assert ovf == 0
+    np.testing.assert_array_equal(_h(out[0], np.uint32), of)
+    np.testing.assert_array_equal(_h(out[1]), ouv)
+    np.testing.assert_array_equal(_h(out[2]), od)
+    assert (od < dep0).sum() > n // 10
+
+
+def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None):
+    hb = rk.hitbox(V)
+    b = dev.FastBuilder(F.shape[0], mode)
+    tree = b.build(_t(F), _t(V), hb, leaf_max)
+    n = O.shape[0]
+    fid, uv, dep = dev.new_outputs(n)
+    if tmax is not None:
+        dep.fill_(tmax)
+    dev.fast_trace(tree, _t(O), _t(D), out=(fid, uv, dep))
+    bf, buv, bdep = ora.trace_bruteforce(F, V, O, D, depths=None if tmax is None else np.full(n, tmax, np.float32))
+    g_dep = _h(dep)
+    hit = bdep < (F32MAX if tmax is None else tmax)
+    np.testing.assert_array_equal(g_dep, bdep)
+    g_pos = _h(fid, np.uint32)
+    order = _h(tree.indices, np.uint32)
+    np.testing.assert_array_equal(order[g_pos[hit]], bf[hit])
+    np.testing.assert_array_equal(_h(uv)[hit], buv[hit])
+    assert int(_h(tree.info, np.uint32)[rk.RK_INFO_FLAGS]) == 0
+    return hit.mean()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("leaf_max", [1, 4, 8])
+def test_fast_tree_closest_hit_icosphere(dev, rk, ora, mode, leaf_max):
+    V, F = _scene(rk, "ico3")
+    O, D = _camera_rays(rk, "ico3", 64, 64)
+    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max) > 0.2
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fast_tree_closest_hit_soup(dev, rk, ora, mode):
+    V, F = rk.soup(3000, seed=9, mu=float(np.log(0.05)))
+    O, D = rk.random_rays(4000, seed=10)
+    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, 4) > 0.05
+
+
+def test_fast_tree_heightfield_and_tmax(dev, rk, ora):
+    V, F = _scene(rk, "hf128")
+    O, D = _camera_rays(rk, "hf128", 80, 60)
+    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4) > 0.2
+    _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4, tmax=1.2)
+
+
+def test_fast_tree_single_face(dev, rk, ora):
+    V = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+    F = np.array([[0, 1, 2, 0]], np.uint32)
+    O = np.array([[0.25, 0.25, 1, 0], [2, 2, 1, 0]], np.float32)
+    D = np.array([[0, 0, -1, 0], [0, 0, -1, 0]], np.float32)
+    _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4)
+
+
+def test_fast_vs_reference_trace_c1(dev, rk, ora):
+    """The fast tree and the reference tree give the same closest hits on the
+    256x256 icosphere view, in original face ids, except where two faces are
+    hit at exactly the same t (the reference keeps the first one it visits)."""
+    V, F = _scene(rk, "ico5")
+    O, D = _camera_rays(rk, "ico5", 256, 256)
+    hb = rk.hitbox(V)
+    ref = ora.build_lbvh_ref(F, V, hb, 12, True)
+    rf, ruv, rd, _ = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D)
+    b = dev.FastBuilder(F.shape[0], 1)
+    tree = b.build(_t(F), _t(V), hb, 4)
+    fid, uv, dep = dev.fast_trace(tree, _t(O), _t(D))
+    g_dep = _h(dep)
+    np.testing.assert_array_equal(g_dep, rd)
+    hit = rd < F32MAX
+    g_orig = _h(tree.indices, np.uint32)[_h(fid, np.uint32)[hit]]
+    r_orig = ref.indices[rf[hit]]
+    diff = g_orig != r_orig
+    assert diff.mean() < 0.01
+    # every difference is an exact tie
+    gt = ora.hit_t_f64(F, V, O[hit][diff], D[hit][diff], g_orig[diff])
+    rt = ora.hit_t_f64(F, V, O[hit][diff], D[hit][diff], r_orig[diff])
+    assert np.allclose(gt, rt, rtol=1e-6)

@@ -1,0 +1,424 @@
+"""Pins of the oracle (CPU, no GPU): closed forms, known answers, hand-drawn
+trees, invariants and independent formulations, chosen so that a dropped term,
+a swapped axis or an off-by-one in the oracle fails one of them."""
+import itertools
+
+import numpy as np
+import pytest
+
+F32MAX = np.finfo(np.float32).max
+
+
+def box_of_center(c):
+    c = np.asarray(c, np.float32)
+    b = np.zeros((1, 2, 4), np.float32)
+    b[0, 0, :3] = c
+    b[0, 1, :3] = c
+    return b
+
+
+UNIT = np.array([[0, 0, 0, 0], [1, 1, 1, 0]], np.float32)
+
+
+# ---- Morton -----------------------------------------------------------------
+
+@pytest.mark.parametrize("center,code", [
+    ((0, 0, 0), 0x00000000),
+    ((1, 1, 1), 0xFFFFFFFF),
+    ((1, 0, 0), 0x92492492),   # x bits at 3k+1
+    ((0, 1, 0), 0x49249249),   # y bits at 3k
+    ((0, 0, 1), 0x24924924),   # z bits at 3k+2 (10 bits)
+])
+def test_morton32_known_answers(ora, center, code):
+    assert ora.morton32(box_of_center(center), UNIT)[0] == code
+
+
+def _bitloop32(q):
+    qx, qy, qz = q
+    code = 0
+    for k in range(11):
+        code |= ((qy >> k) & 1) << (3 * k)
+        code |= ((qx >> k) & 1) << (3 * k + 1)
+        if k < 10:
+            code |= ((qz >> k) & 1) << (3 * k + 2)
+    return code & 0xFFFFFFFF
+
+
+def test_morton32_matches_bit_loop(ora):
+    rng = np.random.default_rng(1)
+    root = np.array([[-1.5, -0.5, 2.0, 0], [2.5, 3.0, 2.75, 0]], np.float32)
+    n = 4000
+    boxes = np.zeros((n, 2, 4), np.float32)
+    lo = root[0, :3] + rng.random((n, 3), dtype=np.float32) * (root[1, :3] - root[0, :3])
+    ext = rng.random((n, 3), dtype=np.float32) * 0.01
+    boxes[:, 0, :3] = lo
+    boxes[:, 1, :3] = np.minimum(lo + ext, root[1, :3])
+    codes = ora.morton32(boxes, root)
+    length = root[1] - root[0]
+    s = np.array([np.float32(2047) / length[0], np.float32(2047) / length[1], np.float32(1023) / length[2]],
+                 np.float32)
+    c = (boxes[:, 0] + boxes[:, 1]) * np.float32(0.5)
+    q = ((c[:, :3] - root[0, :3]) * s).astype(np.float32)
+    qi = np.floor(np.maximum(q, 0)).astype(np.uint64)
+    for i in range(0, n, 37):
+        assert codes[i] == _bitloop32(tuple(int(x) for x in qi[i])), i
+
+
+@pytest.mark.parametrize("bits", [10, 21])
+def test_morton_wide_matches_bit_loop(ora, bits):
+    rng = np.random.default_rng(bits)
+    n = 2000
+    boxes = np.zeros((n, 2, 4), np.float32)
+    c = rng.random((n, 3), dtype=np.float32)
+    c[0] = 1.0  # the top cell clamps to 2^b - 1
+    c[1] = 0.0
+    boxes[:, 0, :3] = c
+    boxes[:, 1, :3] = c
+    codes = ora.morton_wide(boxes, UNIT, bits)
+    cells = np.float32(1 << bits)
+    q = np.minimum(np.floor(c * (cells / np.float32(1.0))).astype(np.int64), (1 << bits) - 1)
+    for i in range(0, n, 13):
+        ref = 0
+        for k in range(bits):
+            ref |= ((int(q[i, 0]) >> k) & 1) << (3 * k + 2)
+            ref |= ((int(q[i, 1]) >> k) & 1) << (3 * k + 1)
+            ref |= ((int(q[i, 2]) >> k) & 1) << (3 * k)
+        assert int(codes[i]) == ref, i
+    assert int(codes[0]) == (1 << (3 * bits)) - 1
+
+
+# ---- boxes ------------------------------------------------------------------
+
+def test_boxes_closed_form(ora):
+    V = np.array([[0, 1, 2, 5], [3, -1, 2, -7], [1, 1, -4, 0.5], [9, 9, 9, 9]], np.float32)
+    F = np.array([[0, 1, 2, 0], [3, 3, 3, 7], [2, 0, 1, 1]], np.uint32)
+    boxes, idx = ora.calc_boxes(F, V)
+    assert list(idx) == [0, 1, 2]
+    np.testing.assert_array_equal(boxes[0, 0], [0, -1, -4, -7])   # w is merged too
+    np.testing.assert_array_equal(boxes[0, 1], [3, 1, 2, 5])
+    np.testing.assert_array_equal(boxes[1, 0], V[3])
+    np.testing.assert_array_equal(boxes[1, 1], V[3])
+    np.testing.assert_array_equal(boxes[2], boxes[0])
+
+
+# ---- sorts ------------------------------------------------------------------
+
+def test_sort_small_all_permutations_stable(ora):
+    for keys in ([3, 1, 2, 1, 3, 0, 1], [5, 5, 5, 5], [2, 1, 2, 1, 2, 1, 2, 1]):
+        for perm in set(itertools.permutations(range(len(keys)))):
+            k = np.array([keys[p] for p in perm], np.uint32)
+            sk, sv, _ = ora.sort_pairs_ref(k, np.arange(len(k), dtype=np.uint32), 32, init=True)
+            expect = sorted(range(len(k)), key=lambda i: (k[i], i))
+            assert list(sv) == expect
+
+
+def _batcher_bitonic(k, v):
+    """Bitonic sorting network on 128 elements (Batcher 1968): in stage (i, j),
+    element n is compare-exchanged with n^j, ascending when n & i == 0."""
+    k, v = k.copy(), v.copy()
+    n = np.arange(128)
+    i = 2
+    while i <= 128:
+        j = i // 2
+        while j >= 1:
+            partner = n ^ j
+            lo = n[partner > n]
+            hi = lo ^ j
+            up = (lo & i) == 0
+            a, b = k[lo], k[hi]
+            sw = np.where(up, a > b, a < b)
+            kl, kh = k[lo].copy(), k[hi].copy()
+            vl, vh = v[lo].copy(), v[hi].copy()
+            k[lo] = np.where(sw, kh, kl)
+            k[hi] = np.where(sw, kl, kh)
+            v[lo] = np.where(sw, vh, vl)
+            v[hi] = np.where(sw, vl, vh)
+            j //= 2
+        i *= 2
+    return k, v
+
+
+@pytest.mark.parametrize("n", [513, 640, 1000, 4096, 20480, 32768])
+def test_sort_bitonic_class_tie_order(ora, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 7, n, dtype=np.uint32)  # heavy duplicates
+    sk, sv, _ = ora.sort_pairs_ref(keys, np.zeros(n, np.uint32), 32, init=True)
+    # independent formulation: network per full 128-chunk, stable for the tail
+    k2, v2 = keys.copy(), np.arange(n, dtype=np.uint32)
+    for o in range(0, n, 128):
+        m = min(128, n - o)
+        if m == 128:
+            k2[o:o + 128], v2[o:o + 128] = _batcher_bitonic(k2[o:o + 128], v2[o:o + 128])
+        else:
+            order = np.argsort(k2[o:o + m], kind="stable")
+            k2[o:o + m], v2[o:o + m] = k2[o:o + m][order], v2[o:o + m][order]
+    order = np.argsort(k2, kind="stable")
+    np.testing.assert_array_equal(sk, k2[order])
+    np.testing.assert_array_equal(sv, v2[order])
+    assert np.all(np.diff(sk.astype(np.int64)) >= 0)
+    assert sorted(sv.tolist()) == list(range(n))
+
+
+@pytest.mark.parametrize("n,bits", [(32769, 32), (40000, 21), (9000, 8), (70000, 8)])
+def test_sort_radix_class_stable_on_low_bytes(ora, n, bits):
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 1 << min(bits, 31), n, dtype=np.uint32)
+    keys[::3] = keys[0]
+    sk, sv, _ = ora.sort_pairs_ref(keys, np.zeros(n, np.uint32), bits, init=True)
+    byte_depth = 1 + (bits - 1) // 8
+    if n <= byte_depth * 8192:
+        pytest.skip("bitonic class")
+    mask = np.uint32((1 << (8 * byte_depth)) - 1) if byte_depth < 4 else np.uint32(0xFFFFFFFF)
+    order = np.argsort(keys & mask, kind="stable")
+    np.testing.assert_array_equal(sv, order.astype(np.uint32))
+
+
+def test_merge2x_oob_trigger(ora):
+    # 2 bins with a partial second bin: the reference writes past n (BSG.cu:254)
+    _, _, oob = ora.sort_pairs_ref(np.arange(700, dtype=np.uint32)[::-1].copy(), np.zeros(700, np.uint32))
+    assert oob  # 700 -> 6 bins, 700 % 128 != 0, last pair (4,5) partial
+    _, _, oob = ora.sort_pairs_ref(np.arange(640, dtype=np.uint32), np.zeros(640, np.uint32))
+    assert not oob  # 640 = 5 full bins
+
+
+# ---- splits / topology ------------------------------------------------------
+
+def test_find_splits_levels(ora):
+    codes = np.array([0, 0, 0x40000000, 0x80000000, 0x80000001, 0xA0000000], np.uint32)
+    A, cnt = ora.find_splits(codes)
+    # splits at 2 (level 1), 3 (level 0), 4 (level 31), 5 (level 2)
+    assert cnt[0] == 1 and cnt[1] == 1 and cnt[2] == 1 and cnt[31] == 1 and cnt.sum() == 4
+    assert list(A) == [3, 2, 5, 4]
+
+
+def _mesh_from_codes(codes):
+    """A mesh whose i-th face (one point triangle) has Morton code codes[i] in the
+    unit root box, listed in code order so the sort is the identity."""
+    codes = np.asarray(codes, np.uint32)
+    n = codes.shape[0]
+    q = np.zeros((n, 3), np.int64)
+    for k in range(11):
+        q[:, 1] |= ((codes >> (3 * k)) & 1).astype(np.int64) << k
+        q[:, 0] |= ((codes >> (3 * k + 1)) & 1).astype(np.int64) << k
+        if k < 10:
+            q[:, 2] |= ((codes >> (3 * k + 2)) & 1).astype(np.int64) << k
+    # cell centres: (q + 0.5) / cells, exact in float32
+    c = np.stack([(q[:, 0] + 0.5) / 2047.0, (q[:, 1] + 0.5) / 2047.0, (q[:, 2] + 0.5) / 1023.0], 1)
+    c = c.astype(np.float32)
+    V = np.zeros((3 * n, 4), np.float32)
+    for m in range(3):
+        V[m::3, :3] = c
+    V[1::3, 2] += np.float32(1e-6)  # non-zero z extent for each leaf
+    F = np.stack([np.arange(0, 3 * n, 3), np.arange(1, 3 * n, 3), np.arange(2, 3 * n, 3), np.zeros(n)], 1)
+    return F.astype(np.uint32), V
+
+
+def _semantic(nodes):
+    out = []
+    for nd in nodes:
+        if nd["w0"] & 1:
+            out.append(("L", int(nd["w1"]), int(nd["w0"] >> 1)))
+        else:
+            out.append(("N", int((nd["w0"] >> 1) & 3), int(nd["w1"])))
+    return out
+
+
+def test_topology_walk_into_next_level(ora):
+    # Splits: 16 (level 1), 32 (level 0), 48 (level 2). The right level-1 range
+    # [32,64) has no level-1 split; its split pointer walks on into the level-2
+    # list and splits at 48 (CUDA_buildMeshTreeLBVH.cu:381-392).
+    codes = [0] * 16 + [0x40000000] * 16 + [0x80000000] * 16 + [0xA0000000] * 16
+    F, V = _mesh_from_codes(codes)
+    r = ora.build_lbvh_ref(F, V, UNIT, tree_depth=12, complete=True)
+    assert list(r.codes) == codes
+    assert _semantic(r.nodes) == [
+        ("N", 0, 1),             # 0: root [0,64) split 32, axis 0
+        ("N", 1, 3),             # 1: [0,32) L1 split 16, axis 1
+        ("N", 1, 5),             # 2: [32,64) L1 split 48 (a level-2 split), axis 1
+        ("L", 0, 16), ("L", 16, 16),
+        ("L", 32, 16), ("L", 48, 16),
+    ]
+    assert r.flags & ora.FLAG_WALK_PAST_END  # [48,64) at L2 walks past the last split
+    assert r.real_depth == 3
+
+
+def test_topology_leaf_rules(ora):
+    # all codes equal: no split at all; 300 > 224 so the root is median-split
+    F, V = _mesh_from_codes([7] * 300)
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, True)
+    assert _semantic(r.nodes) == [("N", 0, 1), ("L", 0, 150), ("L", 150, 150)]
+    # incomplete tree keeps the 300-face leaf
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, False)
+    assert _semantic(r.nodes) == [("L", 0, 300)]
+    # exactly 224 faces: a leaf (createLeaf splits only > 224)
+    F, V = _mesh_from_codes([7] * 224)
+    assert _semantic(ora.build_lbvh_ref(F, V, UNIT, 12, True).nodes) == [("L", 0, 224)]
+    # 448 faces: 224/224 from createLeaf, then each 224 splits again (median rule '< 224')
+    F, V = _mesh_from_codes([7] * 448)
+    assert _semantic(ora.build_lbvh_ref(F, V, UNIT, 12, True).nodes) == [
+        ("N", 0, 1), ("N", 1, 3), ("N", 1, 5), ("L", 0, 112), ("L", 112, 112), ("L", 224, 112), ("L", 336, 112)]
+    # size < 16 is a leaf even with a split inside
+    F, V = _mesh_from_codes([0] * 7 + [0x80000000] * 8)
+    assert _semantic(ora.build_lbvh_ref(F, V, UNIT, 12, True).nodes) == [("L", 0, 15)]
+    # 16 faces with a level-0 split: splits
+    F, V = _mesh_from_codes([0] * 8 + [0x80000000] * 8)
+    assert _semantic(ora.build_lbvh_ref(F, V, UNIT, 12, True).nodes) == [("N", 0, 1), ("L", 0, 8), ("L", 8, 8)]
+    # tree_depth 0: the root is a leaf
+    F, V = _mesh_from_codes([0] * 8 + [0x80000000] * 8)
+    assert _semantic(ora.build_lbvh_ref(F, V, UNIT, 0, True).nodes) == [("L", 0, 16)]
+
+
+def _check_tree_invariants(r, n):
+    nodes = r.nodes
+    M = nodes.shape[0]
+    assert M % 2 == 1
+    leaves = []
+    rank = 0
+    seen_children = set()
+
+    def walk(i, lo, hi, depth):
+        nonlocal rank
+        nd = nodes[i]
+        if nd["w0"] & 1:
+            a, c = int(nd["w1"]), int(nd["w0"] >> 1)
+            leaves.append((a, a + c))
+            return depth
+        c = int(nd["w1"])
+        assert c == 2 * rank + 1, "children of the internal node of preorder rank r are 2r+1, 2r+2"
+        rank += 1
+        seen_children.update([c, c + 1])
+        return max(walk(c, lo, hi, depth + 1), walk(c + 1, lo, hi, depth + 1))
+
+    depth = walk(0, 0, n, 0)
+    assert sorted(seen_children) == list(range(1, M))
+    leaves.sort()
+    assert leaves[0][0] == 0 and leaves[-1][1] == n
+    for (a0, b0), (a1, b1) in zip(leaves, leaves[1:]):
+        assert b0 == a1 and a0 < b0
+    assert r.real_depth == depth + 1
+
+
+@pytest.mark.parametrize("scene", ["ico3", "hf64", "soup"])
+@pytest.mark.parametrize("depth", [4, 12, 31])
+def test_topology_invariants_and_refit(ora, rk, scene, depth):
+    if scene == "ico3":
+        V, F = rk.icosphere(3)
+    elif scene == "hf64":
+        V, F = rk.heightfield(64)
+    else:
+        V, F = rk.soup(3000, seed=7)
+    hb = rk.hitbox(V)
+    r = ora.build_lbvh_ref(F, V, hb, depth, True)
+    _check_tree_invariants(r, F.shape[0])
+    assert np.all(np.diff(r.codes.astype(np.int64)) >= 0)
+    assert sorted(r.indices.tolist()) == list(range(F.shape[0]))
+    # the pass emulation of the refit equals the recursive rule
+    rec = ora.refit_recursive(r.nodes)
+    for f in ("bmin", "bmax"):
+        np.testing.assert_array_equal(rec[f], r.nodes[f])
+
+
+def test_refit_flat_leaf_empties_ancestors(ora):
+    # two leaves; the right leaf is flat in z, so the root keeps the empty box
+    codes = [0] * 16 + [0x80000000] * 16
+    F, V = _mesh_from_codes(codes)
+    V[3 * 16 + 1::3, 2] = V[3 * 16::3, 2][: 16]  # right leaf: remove the z offset
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, True)
+    assert r.nodes[2]["bmax"][2] - r.nodes[2]["bmin"][2] == 0.0
+    np.testing.assert_array_equal(r.nodes[0]["bmin"], [F32MAX] * 3)
+    np.testing.assert_array_equal(r.nodes[0]["bmax"], [-F32MAX] * 3)
+    assert r.nodes[1]["bmax"][2] > r.nodes[1]["bmin"][2]
+
+
+# ---- intersection -----------------------------------------------------------
+
+def test_moller_trumbore_known_answer(ora):
+    hit, tuv = ora.tri_ray([0.25, 0.25, 1, 0], [0, 0, -1, 0], [0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0])
+    assert hit and tuv[0] == 1.0 and tuv[1] == 0.25 and tuv[2] == 0.25
+
+
+@pytest.mark.parametrize("b,expect", [(0.999e-3, False), (1.001e-3, True), (-0.999e-3, False), (-1.001e-3, True)])
+def test_moller_trumbore_det_threshold(ora, b, expect):
+    # det = e1.x * e2.y for d = (0,0,-1): the reference rejects |det| < 1e-6 (absolute)
+    o = [0.0001, 0.0001 * np.sign(b), 1, 0]
+    hit, _ = ora.tri_ray(o, [0, 0, -1, 0], [0, 0, 0, 0], [1e-3, 0, 0, 0], [0, b, 0, 0])
+    assert hit == expect
+
+
+def test_box_zero_direction_component_always_enters(ora):
+    # 1/0 = inf, normalize gives NaN there and the slab test passes (RT.cu:128)
+    hit, _ = ora.box_ray([5, 5, 5, 0], [6, 6, 6, 0], [0, 0, 0, 0], [1, 0, 0, 0])
+    assert hit
+    hit, _ = ora.box_ray([5, 5, 5, 0], [6, 6, 6, 0], [0, 0, 0, 0], [1, 0.01, 0.02, 0])
+    assert not hit
+    # an inverted (empty) box is entered by every ray
+    hit, _ = ora.box_ray([F32MAX] * 3 + [0], [-F32MAX] * 3 + [0], [0, 0, 0, 0], [1, 0.3, 0.2, 0])
+    assert hit
+
+
+def test_trace_ref_matches_bruteforce_on_icosphere(ora, rk):
+    V, F = rk.icosphere(3)
+    hb = rk.hitbox(V)
+    r = ora.build_lbvh_ref(F, V, hb, 12, True)
+    cam = rk.camera_lookat([0, 0, 3], [0, 0, 0], 60 * 0.01745329252, 48, 48)
+    O, D = rk.primary_rays_host(cam, 48, 48)
+    fid, uv, dep, ovf = ora.trace_ref_ray(r.nodes, r.faces, V, O, D)
+    assert ovf == 0
+    bf, buv, bdep = ora.trace_bruteforce(F, V, O, D)
+    hit = dep < F32MAX
+    assert hit.sum() > 0.2 * hit.size  # the unit sphere covers ~29% of a 60 degree view from z = 3
+    np.testing.assert_array_equal(dep, bdep)
+    same = r.indices[fid[hit]] == bf[hit]
+    # ties on t (a ray through a shared edge or vertex) may pick either face
+    assert same.mean() > 0.99
+    np.testing.assert_array_equal(uv[hit][same], buv[hit][same])
+    pf, puv, pdep, povf = ora.trace_ref_packet(r.nodes, r.faces, V, O, D, 48, 48)
+    assert povf == 0
+    np.testing.assert_array_equal(pdep, dep)
+    t64 = ora.hit_t_f64(r.faces, V, O, D, fid)
+    err = np.abs(dep[hit].astype(np.float64) - t64[hit]) / t64[hit]
+    assert err.max() < 1e-5
+
+
+def test_trace_miss_leaves_outputs_unchanged(ora, rk):
+    V, F = rk.icosphere(1)
+    r = ora.build_lbvh_ref(F, V, rk.hitbox(V), 12, True)
+    O = np.array([[0, 0, 5, 0], [10, 10, 10, 0]], np.float32)
+    D = np.array([[0, 0, -1, 0], [1, 0, 0, 0]], np.float32)
+    fid, uv, dep, _ = ora.trace_ref_ray(r.nodes, r.faces, V, O, D, face_id=np.array([77, 88], np.uint32),
+                                        coords=np.full((2, 2), 0.5, np.float32),
+                                        depths=np.array([F32MAX, 9.0], np.float32))
+    assert dep[0] < 5 and fid[0] != 77
+    assert fid[1] == 88 and dep[1] == 9.0 and list(uv[1]) == [0.5, 0.5]
+
+
+def test_trace_depth_is_tmax(ora, rk):
+    V, F = rk.icosphere(2)
+    r = ora.build_lbvh_ref(F, V, rk.hitbox(V), 12, True)
+    O = np.array([[0, 0, 3, 0]], np.float32)
+    D = np.array([[0.01, 0.02, -1, 0]], np.float32)
+    _, _, d0, _ = ora.trace_ref_ray(r.nodes, r.faces, V, O, D)
+    _, _, d1, _ = ora.trace_ref_ray(r.nodes, r.faces, V, O, D, depths=np.array([d0[0]], np.float32))
+    assert d1[0] == d0[0]  # strict t < depth: an equal hit is not taken, depth unchanged
+
+
+def test_sah_two_clusters(ora):
+    # two well-separated clusters of 20 unit boxes: the root cut separates them
+    n = 40
+    boxes = np.zeros((n, 2, 4), np.float32)
+    xs = np.concatenate([np.linspace(0, 1, 20), np.linspace(100, 101, 20)]).astype(np.float32)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(n)
+    boxes[:, 0, 0] = xs[perm]
+    boxes[:, 1, 0] = xs[perm] + 0.01
+    boxes[:, 1, 1:3] = 0.01
+    root = np.array([[0, 0, 0, 0], [101.01, 0.01, 0.01, 0]], np.float32)
+    nodes, idx, fl = ora.build_sah(boxes, np.arange(n, dtype=np.uint32), root)
+    assert fl == 0
+    assert not (nodes[0]["w0"] & 1) and ((nodes[0]["w0"] >> 1) & 3) == 0
+    c = nodes[0]["w1"]
+    left, right = nodes[c], nodes[c + 1]
+    assert left["w0"] & 1 and right["w0"] & 1
+    assert (left["w0"] >> 1) == 20 and (right["w0"] >> 1) == 20
+    assert np.all(boxes[idx[:20], 0, 0] < 50) and np.all(boxes[idx[20:], 0, 0] > 50)
+    assert left["bmax"][0] == np.float32(1.01) and right["bmin"][0] == np.float32(100)
