@@ -1,0 +1,242 @@
+"""Benchmark of the mesh-tree build + closest-hit trace path on one MI355X.
+
+Workload (default): the 708x708 fBm heightfield (999,698 FP32 triangles),
+1920x1080 primary rays from a camera at 30 degrees elevation, 45 degrees
+azimuth, distance 1.5 from (0.5, 0.5, 0.1), 60 degree fov.
+
+A step is one frame of the trace with the tree resident in HBM: the closest
+hit of all 2,073,600 rays (rk_fast_trace). `value` is Mrays/s over the timed
+steps. The LBVH build rate (rk_fast_build, 30-bit Morton, leaf <= 4 faces) is
+timed the same way over as many builds and reported as lbvh_mtri_s; the
+reference tree (rk_build_lbvh, tree_depth 12) and its packet traversal are
+timed once beside it. cpu_baseline is the oracle (the reference algorithm on
+the host: its LBVH build and per-ray traversal) on a bounded sample.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--profile-only]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s closest-hit + LBVH Mtri/s, batched vs plain on 1 GPU; p99 latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+OPS_PER_BOX_TEST = 24   # algorithmic FP32 ops of one slab test
+OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def heightfield_scene(native, width=1920, height=1080, grid=708):
+    V, F = native.heightfield(grid, seed=0xC2, amplitude=0.2)
+    hb = native.hitbox(V)
+    el, az, dist = np.radians(30.0), np.radians(45.0), 1.5
+    tgt = np.array([0.5, 0.5, 0.1])
+    eye = tgt + dist * np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    cam = native.camera_lookat(eye, tgt, 60 * 0.01745329252, width, height, up=[0, 0, 1])
+    O, D = native.primary_rays_host(cam, width, height)
+    return V, F, hb, O, D
+
+
+class Timer:
+    """HIP events on torch's current stream, which is the stream every rk_* call
+    here is issued on (the default stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def time(self, fn, reps):
+        t = self.torch
+        s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        t.cuda.synchronize()
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        t.cuda.synchronize()
+        return s.elapsed_time(e) / reps  # ms
+
+
+def cpu_baseline(native, V, F, hb, O, D, budget_s=20.0):
+    """The reference algorithm on the host (the oracle): LBVH build (depth 12)
+    and per-ray traversal on a sample of the frame's rays."""
+    import oracle
+    threads = oracle.num_threads()
+    t0 = time.perf_counter()
+    ref = oracle.build_lbvh_ref(F, V, hb, 12, True)
+    build_s = time.perf_counter() - t0
+    # calibrate the sample so the trace takes a few seconds up to budget_s
+    n = O.shape[0]
+    stride = 256
+    t0 = time.perf_counter()
+    oracle.trace_ref_ray(ref.nodes, ref.faces, V, np.ascontiguousarray(O[::stride]), np.ascontiguousarray(D[::stride]))
+    dt = time.perf_counter() - t0
+    rays = (n + stride - 1) // stride
+    if dt < budget_s / 4:
+        stride2 = max(1, int(stride * dt / (budget_s / 2)))
+        O2, D2 = np.ascontiguousarray(O[::stride2]), np.ascontiguousarray(D[::stride2])
+        t0 = time.perf_counter()
+        oracle.trace_ref_ray(ref.nodes, ref.faces, V, O2, D2)
+        dt = time.perf_counter() - t0
+        rays, stride = O2.shape[0], stride2
+    return {
+        "value": round(rays / dt / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"every {stride}th ray of the 1920x1080 frame ({rays} rays), reference LBVH depth 12, "
+                  f"per-ray DFS (CUDA_raytraceMeshTree.cu:111-219); reference build of all "
+                  f"{F.shape[0]} faces took {build_s:.3f} s ({F.shape[0] / build_s / 1e6:.3f} Mtri/s)",
+        "lbvh_mtri_s": round(F.shape[0] / build_s / 1e6, 4),
+        "host_cores_visible": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref", action="store_true")
+    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--profile-only", action="store_true", help="a few builds and traces, no JSON (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own "
+              "share of rays or meshes with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cudart_amd import device as dev
+    from cudart_amd import native
+
+    torch.cuda.set_device(0)
+    timer = Timer(torch)
+    W, H = 1920, 1080
+    log("generating scene")
+    V, F, hb, O, D = heightfield_scene(native, W, H)
+    n, nr = F.shape[0], O.shape[0]
+    Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
+    Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
+    f32max = float(np.finfo(np.float32).max)
+
+    # ---- fast tree: build rate
+    builder = dev.FastBuilder(n, native.RK_MORTON_30)
+    build = lambda: builder.build(Ft, Vt, hb, args.leaf)
+    tree = build()
+    torch.cuda.synchronize()
+    if args.profile_only:
+        for _ in range(3):
+            build()
+        fid, uv, dep = dev.new_outputs(nr)
+        for _ in range(3):
+            dep.fill_(f32max)
+            dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep))
+        torch.cuda.synchronize()
+        log("profile run done")
+        return
+    log("timing builds")
+    for _ in range(args.warmup):
+        build()
+    build_ms = timer.time(build, args.steps)
+    flags = int(dev.u32_view(tree.info)[native.RK_INFO_FLAGS])
+
+    # ---- fast tree: trace frames (tmax reset with a fill kernel each frame)
+    fid, uv, dep = dev.new_outputs(nr)
+
+    def frame():
+        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep))
+
+    log("timing trace")
+    for _ in range(args.warmup):
+        frame()
+    frame_ms = timer.time(frame, args.steps)
+    fill_ms = timer.time(lambda: native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max)),
+                         args.steps)
+    trace_ms = max(frame_ms - fill_ms, 1e-6)
+    hits = int((dep < f32max).sum().item())
+    # algorithmic work of the trace kernel, counted on the same rays and tree
+    dep.fill_(f32max)
+    _, _, _, cnt = dev.fast_trace_counted(tree, Ot, Dt, out=(fid, uv, dep))
+    torch.cuda.synchronize()
+    n_box, n_tri = (int(x) for x in cnt.cpu().numpy())
+    flops = OPS_PER_BOX_TEST * n_box + OPS_PER_TRI_TEST * n_tri
+    achieved_tflops = flops / (trace_ms * 1e-3) / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": round(nr / (frame_ms * 1e-3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(frame_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "heightfield 708x708 fBm (999,698 FP32 triangles), 1920x1080 primary rays, closest hit",
+            "faces": n, "rays": nr, "tree": f"LBVH 30-bit Morton (Karras), leaf <= {args.leaf} faces",
+            "step": "one frame: reset tmax + closest-hit trace of every ray, tree resident in HBM",
+        },
+        "lbvh_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
+        "lbvh_build_ms": round(build_ms, 4),
+        "trace_kernel_ms": round(trace_ms, 4),
+        "hit_fraction": round(hits / nr, 4),
+        "box_tests_per_ray": round(n_box / nr, 2),
+        "tri_tests_per_ray": round(n_tri / nr, 2),
+        "build_flags": flags,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_fast_trace",
+            "achieved": round(achieved_tflops, 4),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
+            "traffic": None,
+            "note": "algorithmic FP32 ops = 24 per slab test + 53 per triangle test, counted on this frame",
+        },
+    }
+
+    # ---- the reference tree and traversal on the same frame (timed once)
+    if not args.no_ref:
+        log("reference build + packet trace")
+        rb = dev.LBVHBuilder(n)
+        rbuild = lambda: rb.build(Ft.clone(), Vt, hb, 12, True)
+        rtree = rbuild()
+        torch.cuda.synchronize()
+        Fr = rtree.faces
+        ref_build_ms = timer.time(lambda: rb.build(Fr, Vt, hb, 12, True), 3)
+        rtree = rb.build(Ft.clone(), Vt, hb, 12, True)
+        torch.cuda.synchronize()
+        rtree.node_count = int(rtree.host_info()[native.RK_INFO_NODE_COUNT])
+        rf, ruv, rdep = dev.new_outputs(nr)
+        ref_ms = timer.time(lambda: dev.trace_ref(rtree, Vt, Ot, Dt, W, H, True, out=(rf, ruv, rdep)), 1)
+        result["reference_lbvh_mtri_s"] = round(n / (ref_build_ms * 1e-3) / 1e6, 2)
+        result["reference_packet_mrays_s"] = round(nr / (ref_ms * 1e-3) / 1e6, 2)
+        result["reference_node_count"] = rtree.node_count
+
+    if not args.no_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(native, V, F, hb, O, D)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

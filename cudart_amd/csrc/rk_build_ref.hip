@@ -34,26 +34,6 @@ __device__ __forceinline__ Box face_box(const uint4* __restrict__ faces, const f
   return b;
 }
 
-// kernel_calcMortonCodes, CUDA_buildMeshTreeLBVH.cu:115-173.
-__device__ __forceinline__ uint32_t morton_ref32(const Box& b, float4 rmin, float4 rmax) {
-  float4 len = rk::sub4(rmax, rmin);
-  float sx = 2047.0f / len.x, sy = 2047.0f / len.y, sz = 1023.0f / len.z;
-  float4 c = rk::muls4(rk::add4(b.mn, b.mx), 0.5f);
-  uint32_t cx = rk::f2u_sat((c.x - rmin.x) * sx);
-  uint32_t cy = rk::f2u_sat((c.y - rmin.y) * sy);
-  uint32_t cz = rk::f2u_sat((c.z - rmin.z) * sz);
-  // Spread: y bit k -> 3k, x bit k -> 3k+1 (k <= 10), z bit k -> 3k+2 (k <= 9).
-  auto spread = [](uint32_t v) {  // bit k -> bit 3k for k < 11
-    v &= 0x7FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-  };
-  return spread(cy) | (spread(cx) << 1) | (spread(cz & 0x3FFu) << 2);
-}
-
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {  // bit k -> bit 3k, k < 10
   v &= 0x3FFu;
   v = (v | (v << 16)) & 0x030000FFu;
@@ -72,6 +52,19 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v) {  // bit k -> bit 3k, 
   return v;
 }
 
+// kernel_calcMortonCodes, CUDA_buildMeshTreeLBVH.cu:115-173.
+__device__ __forceinline__ uint32_t morton_ref32(const Box& b, float4 rmin, float4 rmax) {
+  float4 len = rk::sub4(rmax, rmin);
+  float sx = 2047.0f / len.x, sy = 2047.0f / len.y, sz = 1023.0f / len.z;
+  float4 c = rk::muls4(rk::add4(b.mn, b.mx), 0.5f);
+  uint32_t cx = rk::f2u_sat((c.x - rmin.x) * sx);
+  uint32_t cy = rk::f2u_sat((c.y - rmin.y) * sy);
+  uint32_t cz = rk::f2u_sat((c.z - rmin.z) * sz);
+  // y bit k -> 3k, x bit k -> 3k+1 (k <= 10), z bit k -> 3k+2 (k <= 9); bit 10
+  // of y and x land on bits 30 and 31.
+  return spread10(cy) | (spread10(cx) << 1) | (spread10(cz) << 2) | (((cy >> 10) & 1u) << 30) |
+         (((cx >> 10) & 1u) << 31);
+}
 // 30- and 63-bit codes (no reference counterpart): q = min((uint)((c-min)*(2^b/len)), 2^b-1),
 // x in the highest bit of each triple.
 __device__ __forceinline__ uint32_t morton30(const Box& b, float4 rmin, float4 rmax) {

@@ -250,11 +250,14 @@ __device__ __forceinline__ float safe_inv(float d) {
   return 1.0f / a;
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                     uint32_t n_tris, const float4* __restrict__ origins,
                                                     const float4* __restrict__ dirs, uint32_t n_rays,
                                                     uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                    float* __restrict__ depths) {
+                                                    float* __restrict__ depths,
+                                                    unsigned long long* __restrict__ counters) {
+  uint32_t n_box = 0, n_tri = 0;
   const uint32_t n_nodes = n_tris > 1 ? n_tris - 1 : 1;
   const uint32_t max_visits = 4 * n_tris + 64;  // a tree visit touches each node once
   const uint32_t l = blockIdx.x * 256 + threadIdx.x;
@@ -285,6 +288,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
       const float tmin1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
       const float tmax1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), depth));
       const bool h0 = tmin0 <= tmax0, h1 = tmin1 <= tmax1;
+      if (kCount) n_box += 2;
       if (h0 && h1) {
         const bool first0 = tmin0 <= tmin1;
         const uint32_t nearr = first0 ? nd.d.x : nd.d.y, farr = first0 ? nd.d.y : nd.d.x;
@@ -304,6 +308,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
         v0.w = 0.0f;
         const float4 e1 = tris[3 * (first + k) + 1], e2 = tris[3 * (first + k) + 2];
         float t, u, v;
+        if (kCount) ++n_tri;
         if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
           const uint32_t oid = __float_as_uint(r0.w);
           if (t < depth || (best != 0xFFFFFFFFu && t == depth && oid < best_oid)) {
@@ -319,6 +324,10 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
       if (stack_t[sp] <= depth) { cur = stack_ref[sp]; found = true; break; }
     }
     if (!found) break;
+  }
+  if (kCount) {
+    atomicAdd(counters, (unsigned long long)n_box);
+    atomicAdd(counters + 1, (unsigned long long)n_tri);
   }
   if (best != 0xFFFFFFFFu) {
     faces_id[l] = best;
@@ -418,10 +427,25 @@ extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, con
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_fast_trace, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
-                     faces_id, reinterpret_cast<float2*>(coords), depths);
+                     faces_id, reinterpret_cast<float2*>(coords), depths, (unsigned long long*)nullptr);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                                           const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id,
+                                           rk_float2* coords, float* depths, uint64_t* counters) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !counters) return RK_ERROR_INVALID_VALUE;
+  if (n_rays == 0) return RK_SUCCESS;
+  if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(k_fast_trace<true>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
+                     static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
+                     reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
+                     faces_id, reinterpret_cast<float2*>(coords), depths,
+                     reinterpret_cast<unsigned long long*>(counters));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

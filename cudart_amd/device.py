@@ -194,3 +194,14 @@ def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, strea
     _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
                                  fid.data_ptr(), uv.data_ptr(), dep.data_ptr()), "fast_trace")
     return fid, uv, dep
+
+
+def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None):
+    """fast_trace plus the total child-box and triangle tests performed."""
+    n = O.shape[0]
+    fid, uv, dep = out if out is not None else new_outputs(n)
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    _check(N.lib().rk_fast_trace_counted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                         fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), cnt.data_ptr()),
+           "fast_trace_counted")
+    return fid, uv, dep, cnt

