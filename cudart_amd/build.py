@@ -83,10 +83,21 @@ def build_oracle() -> Path:
     return ORACLE / "_build" / "liboracle.so"
 
 
+def build_cpp_tests() -> None:
+    """The C++ programs the GPU tests run (drop-in usage of include/renderkit/)."""
+    src_dir = ROOT / "tests" / "cpp"
+    out_dir = src_dir / "_build"
+    out_dir.mkdir(exist_ok=True)
+    for src in sorted(src_dir.glob("*.cpp")):
+        exe = out_dir / src.stem
+        _run(["g++", *HOST_FLAGS, str(src), f"-L{LIB}", "-lrenderkit", "-lrk", f"-Wl,-rpath,{LIB}", "-o", str(exe)])
+
+
 def build_all(jobs: int = 8) -> None:
     build_rk(jobs)
     build_renderkit()
     build_oracle()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

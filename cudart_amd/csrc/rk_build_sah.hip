@@ -1,0 +1,610 @@
+// Binned SAH build (CUDA_buildMeshTreeSAH, CUDA_buildMeshTreeSAH.cu:224-789)
+// and the Hybrid refinement of LBVH leaves (kernel_initStack, :647-666), for
+// gfx950. The reference processes one node per 32-thread block with thread 0
+// doing the binning, sweep and partition serially, and a host round trip per
+// level through a 4096-entry global stack. Here each level's nodes ("items")
+// are cut into 1024-face chunks, one workgroup per chunk:
+//   k_sah_prep       chunk offsets per item (one workgroup scan), accumulator reset
+//   k_sah_bounds     centroid bounds per item (ordered-int atomics)
+//   k_sah_bins       32 bins per item: count + primitive-box bounds (LDS, then atomics)
+//   k_sah_decide     one lane per item: the reference sweep (:424-474), leaf
+//                    test (:479-496), children boxes (:501-530), node record,
+//                    next-level items; child ids in breadth-first order
+//   k_sah_count      per-chunk left counts
+//   k_sah_offsets    per-item exclusive scan over chunks
+//   k_sah_scatter    stable partition (bin <= m_min goes left, :572-594)
+//   k_sah_copyback   partitioned ranges back into indices
+// Min/max are exact in any order, so the atomics reproduce the reference's
+// serial merges; the sweep itself runs in the reference's order on one lane.
+// Node numbering: items are processed level by level in queue order, and the
+// k-th internal item of a level takes children (count + 2k, count + 2k + 1),
+// the breadth-first equivalent of the reference's atomicAdd(node_count, 2).
+#include "rk_device.h"
+
+namespace {
+
+using rk::Box;
+using rk::Node;
+
+constexpr uint32_t kChunk = 1024;
+constexpr int kBins = 32;
+constexpr uint32_t kLeafMin = 16;    // MIN_LEAF_SIZE, :55
+constexpr uint32_t kLeafMax = 224;   // MAX_LEAF_SIZE, :56
+
+struct Item {
+  uint32_t node, lo, hi, pad;
+  float bmin[3], bmax[3];
+  float pad2[2];
+};
+static_assert(sizeof(Item) == 48, "item is 48 B");
+
+// Per-item accumulators: centroid bounds (6 ordered ints) and 32 bins of
+// (count, 3 min, 3 max ordered ints).
+constexpr int kAccWords = 6 + kBins * 7;
+
+// Per-item decision written by k_sah_decide.
+struct Decision {
+  int axis;
+  float scale, bmin0;
+  int m_min;        // -1: leaf
+  uint32_t lcount;
+  uint32_t pad[3];
+};
+
+__device__ __forceinline__ int f2o(float f) {  // order-preserving float -> int
+  int i = __float_as_int(f);
+  return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+__device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : (i ^ 0x7FFFFFFF)); }
+
+__device__ __forceinline__ uint32_t find_item(const uint32_t* __restrict__ chunk_base, uint32_t m, uint32_t c) {
+  uint32_t lo = 0, hi = m;  // last item with chunk_base <= c
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (chunk_base[mid] <= c) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float box_area(float mnx, float mny, float mnz, float mxx, float mxy, float mxz) {
+  float vx = mxx - mnx, vy = mxy - mny, vz = mxz - mnz;  // CUDABox.h:35-44
+  return 2.f * (vx * vy + vx * vz + vy * vz);
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total, int nthreads) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == 63) s_tmp[wave] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0, tot = 0;
+  for (int w = 0; w < nthreads / 64; ++w) {
+    uint32_t c = s_tmp[w];
+    wprefix += ((uint32_t)w < wave) ? c : 0u;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return wprefix + incl - v;
+}
+
+// ctl words: [0] items this level (m), [1] chunks this level (T), [2] items next level,
+// [3] node count, [4] flags
+__global__ __launch_bounds__(1024) void k_sah_prep(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                   uint32_t* __restrict__ chunk_base, int* __restrict__ acc) {
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_carry;
+  const uint32_t m = ctl[0];
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < m; c0 += 1024) {
+    uint32_t i = c0 + threadIdx.x;
+    uint32_t nch = 0;
+    if (i < m) {
+      uint32_t size = items[i].hi - items[i].lo;
+      nch = size > kLeafMin ? (size + kChunk - 1) / kChunk : 0u;
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(nch, s_tmp, &total, 1024);
+    if (i < m) {
+      chunk_base[i] = s_carry + ex;
+      if (nch) {  // centroid-bound accumulators; the bins are reset by k_sah_bounds
+        int* a = acc + (size_t)i * kAccWords;
+        a[0] = a[1] = a[2] = f2o(RK_HUGE);
+        a[3] = a[4] = a[5] = f2o(-RK_HUGE);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ctl[1] = s_carry;
+}
+
+struct ChunkRange { uint32_t item, lo, hi; bool ok; };
+
+__device__ __forceinline__ ChunkRange chunk_range(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                  const uint32_t* __restrict__ chunk_base, uint32_t c) {
+  ChunkRange r;
+  r.ok = c < ctl[1];
+  if (!r.ok) return r;
+  const uint32_t m = ctl[0];
+  r.item = find_item(chunk_base, m, c);
+  const Item& it = items[r.item];
+  r.lo = it.lo + (c - chunk_base[r.item]) * kChunk;
+  r.hi = min(it.hi, r.lo + kChunk);
+  return r;
+}
+
+__device__ __forceinline__ float center_axis(const Box& b, int a) {  // 0.5f * (min + max), :292
+  return a == 0 ? (b.mn.x + b.mx.x) * 0.5f : (a == 1 ? (b.mn.y + b.mx.y) * 0.5f : (b.mn.z + b.mx.z) * 0.5f);
+}
+
+__global__ __launch_bounds__(256) void k_sah_bounds(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                    const uint32_t* __restrict__ chunk_base,
+                                                    const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
+                                                    int* __restrict__ acc) {
+  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  if (!r.ok) return;
+  if (blockIdx.x == chunk_base[r.item]) {  // the item's first chunk resets its bins for k_sah_bins
+    int* bins = acc + (size_t)r.item * kAccWords + 6;
+    for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+      int k = w % 7;
+      bins[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
+    }
+  }
+  float mn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, mx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) {
+    Box b = rk::load_box(boxes + indices[l]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float c = center_axis(b, a);
+      mn[a] = rk::fmn(mn[a], c);
+      mx[a] = rk::fmx(mx[a], c);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = rk::fmn(mn[a], __shfl_xor(mn[a], off, 64));
+      mx[a] = rk::fmx(mx[a], __shfl_xor(mx[a], off, 64));
+    }
+  if ((threadIdx.x & 63) == 0) {
+    int* a = acc + (size_t)r.item * kAccWords;
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(a + k, f2o(mn[k]));
+      atomicMax(a + 3 + k, f2o(mx[k]));
+    }
+  }
+}
+
+// Axis and bin scale of an item from its centroid bounds (:302-342).
+__device__ __forceinline__ void item_axis(const int* __restrict__ a, int& axis, float& scale, float& bmin0) {
+  float ex = o2f(a[3]) - o2f(a[0]), ey = o2f(a[4]) - o2f(a[1]), ez = o2f(a[5]) - o2f(a[2]);
+  if (ex > ey) axis = (ex > ez) ? 0 : 2;
+  else axis = (ey > ez) ? 1 : 2;
+  float e = axis == 0 ? ex : (axis == 1 ? ey : ez);
+  scale = (float)(kBins - 1) / e;
+  bmin0 = o2f(a[axis]);
+}
+
+__device__ __forceinline__ int bin_of(const Box& b, int axis, float scale, float bmin0) {  // :375-385
+  int m = rk::f2i_sat((center_axis(b, axis) - bmin0) * scale);
+  if (m < 0) m = 0;
+  else if (m >= kBins - 1) m = kBins - 1;
+  return m;
+}
+
+__global__ __launch_bounds__(256) void k_sah_bins(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                  const uint32_t* __restrict__ chunk_base,
+                                                  const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
+                                                  int* __restrict__ acc) {
+  __shared__ int s_bin[kBins * 7];
+  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  if (!r.ok) return;
+  for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+    int k = w % 7;
+    s_bin[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
+  }
+  __syncthreads();
+  int* a = acc + (size_t)r.item * kAccWords;
+  int axis;
+  float scale, bmin0;
+  item_axis(a, axis, scale, bmin0);
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) {
+    Box b = rk::load_box(boxes + indices[l]);
+    int m = bin_of(b, axis, scale, bmin0);
+    int* s = s_bin + m * 7;
+    atomicAdd(s, 1);
+    atomicMin(s + 1, f2o(b.mn.x)); atomicMin(s + 2, f2o(b.mn.y)); atomicMin(s + 3, f2o(b.mn.z));
+    atomicMax(s + 4, f2o(b.mx.x)); atomicMax(s + 5, f2o(b.mx.y)); atomicMax(s + 6, f2o(b.mx.z));
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+    int k = w % 7, v = s_bin[w];
+    int* g = a + 6 + w;
+    if (k == 0) { if (v) atomicAdd(g, v); }
+    else if (k < 4) { if (v != f2o(RK_HUGE)) atomicMin(g, v); }
+    else { if (v != f2o(-RK_HUGE)) atomicMax(g, v); }
+  }
+}
+
+__device__ __forceinline__ Node make_node(uint32_t axis, uint32_t child, const float* bmin, const float* bmax) {
+  Node n;
+  n.w0 = (axis & 3u) << 1;
+  n.w1 = child;
+  for (int a = 0; a < 3; ++a) { n.bmin[a] = bmin[a]; n.bmax[a] = bmax[a]; }
+  return n;
+}
+__device__ __forceinline__ Node make_leaf(uint32_t first, uint32_t count, const float* bmin, const float* bmax) {
+  Node n;
+  n.w0 = 1u | (count << 1);
+  n.w1 = first;
+  for (int a = 0; a < 3; ++a) { n.bmin[a] = bmin[a]; n.bmax[a] = bmax[a]; }
+  return n;
+}
+
+// One lane per item: the reference's per-node decision, in the reference's order.
+__global__ __launch_bounds__(1024) void k_sah_decide(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                     const int* __restrict__ acc, Decision* __restrict__ dec,
+                                                     Item* __restrict__ next, Node* __restrict__ nodes,
+                                                     uint32_t nodes_cap) {
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_carry, s_flags;
+  const uint32_t m = ctl[0];
+  const uint32_t count0 = ctl[3];
+  if (threadIdx.x == 0) { s_carry = 0; s_flags = 0; }
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < m; c0 += 1024) {
+    const uint32_t i = c0 + threadIdx.x;
+    uint32_t internal = 0;
+    Item it;
+    Decision d;
+    float L[6], R[6];
+    if (i < m) {
+      it = items[i];
+      const uint32_t size = it.hi - it.lo;
+      d.axis = 0; d.scale = 0.f; d.bmin0 = 0.f; d.m_min = -1; d.lcount = 0;
+      if (size > kLeafMin) {
+        const int* a = acc + (size_t)i * kAccWords;
+        item_axis(a, d.axis, d.scale, d.bmin0);
+        const int* bins = a + 6;
+        float pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
+        // left sweep :427-441
+        float larea[kBins];
+        int lcnt[kBins];
+        float lmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, lmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+        int lc = 0;
+        for (int k = 0; k < kBins - 1; ++k) {
+          const int* b = bins + k * 7;
+          lc += b[0];
+          for (int q = 0; q < 3; ++q) { lmn[q] = rk::fmn(lmn[q], o2f(b[1 + q])); lmx[q] = rk::fmx(lmx[q], o2f(b[4 + q])); }
+          larea[k] = 0.f + box_area(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2]);
+          lcnt[k] = 0 + lc;
+        }
+        // right sweep and costs :447-473 (box_cost 1.0, obj_cost 0.125, strict <)
+        float T_min = RK_HUGE;
+        int m_min = -1;
+        float rmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, rmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+        int rc = 0;
+        for (int k = kBins - 1; k > 0; --k) {
+          const int* b = bins + k * 7;
+          rc += b[0];
+          for (int q = 0; q < 3; ++q) { rmn[q] = rk::fmn(rmn[q], o2f(b[1 + q])); rmx[q] = rk::fmx(rmx[q], o2f(b[4 + q])); }
+          float rarea = 0.f + box_area(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2]);
+          int rcount = 0 + rc;
+          float T = 1.0f * 2.f + 0.125f * ((larea[k - 1] / pa) * (float)lcnt[k - 1] + (rarea / pa) * (float)rcount);
+          if (T < T_min) { T_min = T; m_min = k - 1; }
+        }
+        bool leaf = size < kLeafMax && T_min > (float)(size + 1) * 0.125f;
+        if (!leaf && m_min < 0) { leaf = true; atomicOr(&s_flags, (uint32_t)RK_FLAG_SAH_NO_CUT); }
+        if (!leaf) {
+          internal = 1;
+          d.m_min = m_min;
+          for (int q = 0; q < 3; ++q) { L[q] = RK_HUGE; L[3 + q] = -RK_HUGE; R[q] = RK_HUGE; R[3 + q] = -RK_HUGE; }
+          int lcount = 0;
+          for (int k = 0; k <= m_min; ++k) {
+            const int* b = bins + k * 7;
+            lcount += b[0];
+            for (int q = 0; q < 3; ++q) { L[q] = rk::fmn(L[q], o2f(b[1 + q])); L[3 + q] = rk::fmx(L[3 + q], o2f(b[4 + q])); }
+          }
+          for (int k = m_min + 1; k < kBins; ++k) {
+            const int* b = bins + k * 7;
+            for (int q = 0; q < 3; ++q) { R[q] = rk::fmn(R[q], o2f(b[1 + q])); R[3 + q] = rk::fmx(R[3 + q], o2f(b[4 + q])); }
+          }
+          d.lcount = (uint32_t)lcount;
+        }
+      }
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(internal, s_tmp, &total, 1024);
+    if (i < m) {
+      dec[i] = d;
+      if (internal) {
+        const uint32_t k = s_carry + ex;
+        const uint32_t c = count0 + 2 * k;
+        if (it.node < nodes_cap) nodes[it.node] = make_node((uint32_t)d.axis, c, it.bmin, it.bmax);
+        Item li, ri;
+        li.node = c; li.lo = it.lo; li.hi = it.lo + d.lcount; li.pad = 0;
+        ri.node = c + 1; ri.lo = it.lo + d.lcount; ri.hi = it.hi; ri.pad = 0;
+        for (int q = 0; q < 3; ++q) {
+          li.bmin[q] = L[q]; li.bmax[q] = L[3 + q];
+          ri.bmin[q] = R[q]; ri.bmax[q] = R[3 + q];
+        }
+        li.pad2[0] = li.pad2[1] = ri.pad2[0] = ri.pad2[1] = 0.f;
+        next[2 * k] = li;
+        next[2 * k + 1] = ri;
+      } else if (it.node < nodes_cap) {
+        nodes[it.node] = make_leaf(it.lo, it.hi - it.lo, it.bmin, it.bmax);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ctl[2] = 2 * s_carry;
+    ctl[3] = count0 + 2 * s_carry;
+    ctl[4] |= s_flags;
+    if (count0 + 2 * s_carry > nodes_cap) ctl[4] |= 0x20u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sah_count(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                   const uint32_t* __restrict__ chunk_base,
+                                                   const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
+                                                   const Decision* __restrict__ dec, uint32_t* __restrict__ chunk_left) {
+  __shared__ uint32_t s_cnt;
+  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  if (!r.ok) return;
+  const Decision d = dec[r.item];
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  if (d.m_min >= 0) {
+    uint32_t c = 0;
+    for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256)
+      c += bin_of(rk::load_box(boxes + indices[l]), d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
+    atomicAdd(&s_cnt, c);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) chunk_left[blockIdx.x] = s_cnt;
+}
+
+// Exclusive scan of chunk_left within each item (an item's chunks are contiguous).
+__global__ __launch_bounds__(1024) void k_sah_offsets(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                      const uint32_t* __restrict__ chunk_base,
+                                                      const uint32_t* __restrict__ chunk_left,
+                                                      uint32_t* __restrict__ chunk_loff) {
+  const uint32_t m = ctl[0];
+  for (uint32_t i = threadIdx.x; i < m; i += 1024) {
+    const uint32_t size = items[i].hi - items[i].lo;
+    if (size <= kLeafMin) continue;
+    const uint32_t b = chunk_base[i], nch = (size + kChunk - 1) / kChunk;
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < nch; ++c) {
+      chunk_loff[b + c] = run;
+      run += chunk_left[b + c];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sah_scatter(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                     const uint32_t* __restrict__ chunk_base,
+                                                     const uint32_t* __restrict__ indices,
+                                                     const Box* __restrict__ boxes, const Decision* __restrict__ dec,
+                                                     const uint32_t* __restrict__ chunk_loff,
+                                                     uint32_t* __restrict__ tmp) {
+  __shared__ uint32_t s_tmp[8];
+  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  if (!r.ok) return;
+  const Decision d = dec[r.item];
+  if (d.m_min < 0) return;
+  const Item it = items[r.item];
+  const uint32_t loff = chunk_loff[blockIdx.x];
+  const uint32_t chunk_start = r.lo - it.lo;  // elements of this item before the chunk
+  uint32_t left_before = 0;                    // left elements of this chunk before the current step
+  for (uint32_t s = r.lo; s < r.hi; s += 256) {
+    const uint32_t l = s + threadIdx.x;
+    uint32_t id = 0, is_left = 0;
+    const bool in = l < r.hi;
+    if (in) {
+      id = indices[l];
+      is_left = bin_of(rk::load_box(boxes + id), d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(is_left, s_tmp, &total, 256);
+    if (in) {
+      const uint32_t pos_in_item = chunk_start + (l - r.lo);
+      const uint32_t lrank = loff + left_before + ex;            // left elements before this one
+      const uint32_t out = is_left ? (it.lo + lrank) : (it.lo + d.lcount + (pos_in_item - lrank));
+      tmp[out] = id;
+    }
+    left_before += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sah_copyback(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                      const uint32_t* __restrict__ chunk_base,
+                                                      const Decision* __restrict__ dec,
+                                                      const uint32_t* __restrict__ tmp, uint32_t* __restrict__ indices) {
+  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  if (!r.ok) return;
+  if (dec[r.item].m_min < 0) return;
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) indices[l] = tmp[l];
+}
+
+// Hybrid entry (kernel_initStack, :647-666): every leaf of the existing tree
+// becomes an item, in ascending node order, with the leaf's box.
+__global__ __launch_bounds__(1024) void k_sah_init_from_leaves(const Node* __restrict__ nodes, uint32_t existing,
+                                                               Item* __restrict__ items, uint32_t* __restrict__ ctl) {
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < existing; c0 += 1024) {
+    const uint32_t k = c0 + threadIdx.x;
+    uint32_t leaf = 0;
+    Node nd;
+    if (k < existing) {
+      nd = nodes[k];
+      leaf = rk::node_is_leaf(nd.w0);
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(leaf, s_tmp, &total, 1024);
+    if (leaf) {
+      Item it;
+      it.node = k; it.lo = nd.w1; it.hi = nd.w1 + rk::leaf_count(nd.w0); it.pad = 0;
+      for (int q = 0; q < 3; ++q) { it.bmin[q] = nd.bmin[q]; it.bmax[q] = nd.bmax[q]; }
+      it.pad2[0] = it.pad2[1] = 0.f;
+      items[s_carry + ex] = it;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { ctl[0] = s_carry; ctl[3] = existing; ctl[4] = 0; }
+}
+
+__global__ void k_sah_init_root(Item* __restrict__ items, uint32_t* __restrict__ ctl, uint32_t n, float4 rmin,
+                                float4 rmax) {
+  Item it;
+  it.node = 0; it.lo = 0; it.hi = n; it.pad = 0;
+  it.bmin[0] = rmin.x; it.bmin[1] = rmin.y; it.bmin[2] = rmin.z;
+  it.bmax[0] = rmax.x; it.bmax[1] = rmax.y; it.bmax[2] = rmax.z;
+  it.pad2[0] = it.pad2[1] = 0.f;
+  items[0] = it;
+  ctl[0] = 1; ctl[3] = 1; ctl[4] = 0;
+}
+
+__global__ void k_sah_next_level(uint32_t* __restrict__ ctl) { ctl[0] = ctl[2]; }
+
+__global__ void k_sah_finish(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ info) {
+  info[RK_INFO_NODE_COUNT] = ctl[3];
+  info[RK_INFO_FLAGS] |= ctl[4];
+}
+
+__global__ __launch_bounds__(256) void k_boxes_iota(const uint4* __restrict__ faces, const float4* __restrict__ verts,
+                                                    uint32_t n, Box* __restrict__ boxes, uint32_t* __restrict__ idx) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  uint4 f = faces[k];
+  float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+  float4* o = reinterpret_cast<float4*>(boxes + k);
+  o[0] = rk::min4(rk::min4(v0, v1), v2);
+  o[1] = rk::max4(rk::max4(v0, v1), v2);
+  idx[k] = k;
+}
+
+__global__ __launch_bounds__(256) void k_gather_faces2(const uint4* __restrict__ src, const uint32_t* __restrict__ idx,
+                                                       uint4* __restrict__ dst, uint32_t n) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ __launch_bounds__(256) void k_copy_faces2(const uint4* __restrict__ src, uint4* __restrict__ dst, uint32_t n) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t existing_nodes) {
+  size_t n = face_count ? face_count : 1;
+  size_t m = n + existing_nodes + 1;   // items per level
+  size_t T = n / kChunk + m + 1;       // chunks per level (upper bound)
+  size_t b = 0;
+  b += rk_align_up(m * sizeof(Item)) * 2;
+  b += rk_align_up(m * 4);                         // chunk_base
+  b += rk_align_up(m * kAccWords * 4);             // accumulators
+  b += rk_align_up(m * sizeof(Decision));
+  b += rk_align_up(T * 4) * 2;                     // chunk_left, chunk_loff
+  b += rk_align_up(n * 4);                         // partition temp
+  b += rk_align_up(n * 16);                        // face reorder temp
+  b += rk_align_up(64);
+  return b + 4096;
+}
+
+// void CUDA_buildMeshTreeSAH(CUDAMesh*, CUDAMeshTree*), CUDAMeshTree.cpp:34-38 /
+// CUDA_buildMeshTreeSAH.cu:672-789. existing_nodes == 0: boxes and indices are
+// (re)computed and the root is the mesh hitbox; existing_nodes > 0 (Hybrid):
+// boxes/indices come from the LBVH pass and every leaf among nodes[0,
+// existing_nodes) is refined. Faces are reordered by the final indices.
+// Synchronises 'stream' once per tree level (the level's item count sizes the
+// next launch), so it returns after the build has finished.
+extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                                  const rk_aabox* hitbox, rk_aabox* boxes, uint32_t* indices, rk_bvnode* nodes,
+                                  uint32_t existing_nodes, uint32_t nodes_capacity, uint32_t* info, void* ws,
+                                  size_t ws_bytes) {
+  const uint32_t n = face_count;
+  if (n == 0 || !faces || !vertices || !hitbox || !boxes || !indices || !nodes || !info) return RK_ERROR_INVALID_VALUE;
+  if (n >= (1u << 30)) return RK_ERROR_UNSUPPORTED;
+  if ((size_t)nodes_capacity < (size_t)existing_nodes + 2 * (size_t)n - 1) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  const size_t m_cap = (size_t)n + existing_nodes + 1;
+  const size_t T_cap = n / kChunk + m_cap + 1;
+  RkArena ar(ws, ws_bytes);
+  Item* items_a = ar.take<Item>(m_cap);
+  Item* items_b = ar.take<Item>(m_cap);
+  uint32_t* chunk_base = ar.take<uint32_t>(m_cap);
+  int* acc = ar.take<int>(m_cap * kAccWords);
+  Decision* dec = ar.take<Decision>(m_cap);
+  uint32_t* chunk_left = ar.take<uint32_t>(T_cap);
+  uint32_t* chunk_loff = ar.take<uint32_t>(T_cap);
+  uint32_t* tmp = ar.take<uint32_t>(n);
+  uint4* ftmp = ar.take<uint4>(n);
+  uint32_t* ctl = ar.take<uint32_t>(16);
+  if (!items_a || !items_b || !chunk_base || !acc || !dec || !chunk_left || !chunk_loff || !tmp || !ftmp || !ctl)
+    return RK_ERROR_WORKSPACE;
+  rk_status s;
+  if ((s = rk_fill_u32(stream, ctl, 16, 0u)) != RK_SUCCESS) return s;
+  if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
+  if (existing_nodes == 0) {
+    hipLaunchKernelGGL(k_boxes_iota, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<const uint4*>(faces),
+                       reinterpret_cast<const float4*>(vertices), n, reinterpret_cast<Box*>(boxes), indices);
+    RK_CHECK_LAUNCH();
+    float4 rmin = make_float4(hitbox->min.x, hitbox->min.y, hitbox->min.z, hitbox->min.w);
+    float4 rmax = make_float4(hitbox->max.x, hitbox->max.y, hitbox->max.z, hitbox->max.w);
+    hipLaunchKernelGGL(k_sah_init_root, dim3(1), dim3(1), 0, st, items_a, ctl, n, rmin, rmax);
+  } else {
+    hipLaunchKernelGGL(k_sah_init_from_leaves, dim3(1), dim3(1024), 0, st, reinterpret_cast<const Node*>(nodes),
+                       existing_nodes, items_a, ctl);
+  }
+  RK_CHECK_LAUNCH();
+  uint32_t m = 0;
+  RK_HIP(hipMemcpyAsync(&m, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  Item* cur = items_a;
+  Item* nxt = items_b;
+  const Box* bx = reinterpret_cast<const Box*>(boxes);
+  Node* nd = reinterpret_cast<Node*>(nodes);
+  for (uint32_t level = 0; m > 0; ++level) {
+    if (level > 4096) return RK_ERROR_LAUNCH;
+    const unsigned grid = (unsigned)(n / kChunk + m + 1);
+    hipLaunchKernelGGL(k_sah_prep, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, acc);
+    hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, acc);
+    hipLaunchKernelGGL(k_sah_bins, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, acc);
+    hipLaunchKernelGGL(k_sah_decide, dim3(1), dim3(1024), 0, st, cur, ctl, acc, dec, nxt, nd, nodes_capacity);
+    hipLaunchKernelGGL(k_sah_count, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_left);
+    hipLaunchKernelGGL(k_sah_offsets, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, chunk_left, chunk_loff);
+    hipLaunchKernelGGL(k_sah_scatter, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_loff,
+                       tmp);
+    hipLaunchKernelGGL(k_sah_copyback, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, dec, tmp, indices);
+    hipLaunchKernelGGL(k_sah_next_level, dim3(1), dim3(1), 0, st, ctl);
+    RK_CHECK_LAUNCH();
+    RK_HIP(hipMemcpyAsync(&m, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RK_HIP(hipStreamSynchronize(st));
+    std::swap(cur, nxt);
+  }
+  hipLaunchKernelGGL(k_sah_finish, dim3(1), dim3(1), 0, st, ctl, info);
+  RK_CHECK_LAUNCH();
+  // CUDA_reorderMeshFaces, :778-784
+  uint4* f = reinterpret_cast<uint4*>(faces);
+  hipLaunchKernelGGL(k_gather_faces2, dim3((n + 255) / 256), dim3(256), 0, st, f, indices, ftmp, n);
+  hipLaunchKernelGGL(k_copy_faces2, dim3((n + 255) / 256), dim3(256), 0, st, ftmp, f, n);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}

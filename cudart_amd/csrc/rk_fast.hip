@@ -187,19 +187,33 @@ __global__ __launch_bounds__(256) void k_fast_refit(const Box* __restrict__ boxe
   uint32_t self = kLeafBit | i;
   // A Karras tree over n keys with index tie-breaks is at most 64 + 32 deep;
   // the cap only bounds the loop should the parent links ever be corrupt.
+  // Sibling exchange without release/acquire: on a multi-XCD device an
+  // agent-scope release writes back the whole local L2 and an acquire
+  // invalidates it. Instead the exchanged words go through relaxed
+  // agent-scope atomic stores and loads (coherent across XCDs), and the
+  // counter increment waits until those stores are acknowledged.
   for (uint32_t guard = 0; guard < 4096u && node < n - 1; ++guard) {
     const uint32_t side = (child[2 * node] == self) ? 0u : 1u;
     write_slot(nodes, node, side, lox, loy, loz, hix, hiy, hiz, ref, count);
-    // Stash the unpadded box of this child for the sibling that finishes the node.
-    float* mine = nbox + (size_t)(2 * node + side) * 6;
-    mine[0] = lox; mine[1] = loy; mine[2] = loz; mine[3] = hix; mine[4] = hiy; mine[5] = hiz;
-    ncount[2 * node + side] = count;
-    const uint32_t old = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t* mine = reinterpret_cast<uint32_t*>(nbox) + (size_t)(2 * node + side) * 6;
+    rk::atomic_store_relaxed(mine + 0, __float_as_uint(lox));
+    rk::atomic_store_relaxed(mine + 1, __float_as_uint(loy));
+    rk::atomic_store_relaxed(mine + 2, __float_as_uint(loz));
+    rk::atomic_store_relaxed(mine + 3, __float_as_uint(hix));
+    rk::atomic_store_relaxed(mine + 4, __float_as_uint(hiy));
+    rk::atomic_store_relaxed(mine + 5, __float_as_uint(hiz));
+    rk::atomic_store_relaxed(ncount + 2 * node + side, count);
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const uint32_t old = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;  // the sibling finishes this node
-    const float* other = nbox + (size_t)(2 * node + (side ^ 1u)) * 6;
-    lox = rk::fmn(lox, other[0]); loy = rk::fmn(loy, other[1]); loz = rk::fmn(loz, other[2]);
-    hix = rk::fmx(hix, other[3]); hiy = rk::fmx(hiy, other[4]); hiz = rk::fmx(hiz, other[5]);
-    count += ncount[2 * node + (side ^ 1u)];
+    const uint32_t* other = reinterpret_cast<const uint32_t*>(nbox) + (size_t)(2 * node + (side ^ 1u)) * 6;
+    lox = rk::fmn(lox, __uint_as_float(rk::atomic_load_relaxed(other + 0)));
+    loy = rk::fmn(loy, __uint_as_float(rk::atomic_load_relaxed(other + 1)));
+    loz = rk::fmn(loz, __uint_as_float(rk::atomic_load_relaxed(other + 2)));
+    hix = rk::fmx(hix, __uint_as_float(rk::atomic_load_relaxed(other + 3)));
+    hiy = rk::fmx(hiy, __uint_as_float(rk::atomic_load_relaxed(other + 4)));
+    hiz = rk::fmx(hiz, __uint_as_float(rk::atomic_load_relaxed(other + 5)));
+    count += rk::atomic_load_relaxed(ncount + 2 * node + (side ^ 1u));
     if (node == 0) return;
     ref = (count <= leaf_max) ? (kLeafBit | (range_first[node] << 4) | (count - 1)) : node;
     self = node;
@@ -224,16 +238,16 @@ __global__ __launch_bounds__(256) void k_fast_tris(const uint4* __restrict__ fac
 
 // ---- trace -------------------------------------------------------------------
 
-constexpr float kEps = 0.000001f;  // CUDA_raytraceMeshTree.cu:36
 constexpr int kStack = 64;
 
 // intersectTriRay, CUDA_raytraceMeshTree.cu:57-89, with vert0.w = edge1.w =
-// edge2.w = 0 (:174-176) and the reference's four-term dot products.
-__device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, float4 e1, float4 e2, float& t, float& u,
-                                        float& v) {
+// edge2.w = 0 (:174-176) and the reference's four-term dot products. eps is
+// the reference's EPSILON (1e-6, :36) unless the caller chose another.
+__device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, float4 e1, float4 e2, float eps, float& t,
+                                        float& u, float& v) {
   float4 pvec = rk::cross4(dir, e2);
   float det = rk::dot4(e1, pvec);
-  if (det > -kEps && det < kEps) return false;
+  if (det > -eps && det < eps) return false;
   float inv_det = 1.0f / det;
   float4 tvec = rk::sub4(orig, v0);
   u = rk::dot4(tvec, pvec) * inv_det;
@@ -255,7 +269,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
                                                     uint32_t n_tris, const float4* __restrict__ origins,
                                                     const float4* __restrict__ dirs, uint32_t n_rays,
                                                     uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                    float* __restrict__ depths,
+                                                    float* __restrict__ depths, float eps,
                                                     unsigned long long* __restrict__ counters) {
   uint32_t n_box = 0, n_tri = 0;
   const uint32_t n_nodes = n_tris > 1 ? n_tris - 1 : 1;
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
         const float4 e1 = tris[3 * (first + k) + 1], e2 = tris[3 * (first + k) + 2];
         float t, u, v;
         if (kCount) ++n_tri;
-        if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
+        if (tri_ray(orig, dir, v0, e1, e2, eps, t, u, v)) {
           const uint32_t oid = __float_as_uint(r0.w);
           if (t < depth || (best != 0xFFFFFFFFu && t == depth && oid < best_oid)) {
             depth = t; best = first + k; best_oid = oid; bu = u; bv = v;
@@ -421,30 +435,36 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
   return RK_SUCCESS;
 }
 
+static inline float det_eps_of(const rk_trace_options* o) {
+  return o ? o->det_epsilon : RK_DET_EPSILON_REFERENCE;
+}
+
 extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                    const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
-                                   float* depths) {
+                                   float* depths, const rk_trace_options* options) {
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
-                     faces_id, reinterpret_cast<float2*>(coords), depths, (unsigned long long*)nullptr);
+                     faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
+                     (unsigned long long*)nullptr);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
 
 extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                            const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id,
-                                           rk_float2* coords, float* depths, uint64_t* counters) {
+                                           rk_float2* coords, float* depths, const rk_trace_options* options,
+                                           uint64_t* counters) {
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !counters) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   hipLaunchKernelGGL(k_fast_trace<true>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
-                     faces_id, reinterpret_cast<float2*>(coords), depths,
+                     faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
                      reinterpret_cast<unsigned long long*>(counters));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;

@@ -188,20 +188,51 @@ class FastBuilder:
         return FastTree(self.nodes, self.tris, self.indices[:n], self.boxes[:n], self.info, desc)
 
 
-def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None):
+def _opt(det_epsilon):
+    o = N.trace_options(det_epsilon)
+    return None if o is None else C.byref(o)
+
+
+def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
+    """det_epsilon None = the reference threshold (1e-6); 0.0 = only parallel rays rejected."""
     n = O.shape[0]
     fid, uv, dep = out if out is not None else new_outputs(n)
     _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                 fid.data_ptr(), uv.data_ptr(), dep.data_ptr()), "fast_trace")
+                                 fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)), "fast_trace")
     return fid, uv, dep
 
 
-def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None):
+def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
     """fast_trace plus the total child-box and triangle tests performed."""
     n = O.shape[0]
     fid, uv, dep = out if out is not None else new_outputs(n)
     cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
     _check(N.lib().rk_fast_trace_counted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                         fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), cnt.data_ptr()),
-           "fast_trace_counted")
+                                         fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                         cnt.data_ptr()), "fast_trace_counted")
     return fid, uv, dep, cnt
+
+
+def build_sah(F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, lbvh: RefTree | None = None, stream=None):
+    """rk_build_sah: a pure SAH tree (lbvh None) or the Hybrid refinement of an
+    LBVH built with complete=False (its boxes, indices and nodes are reused)."""
+    n = F.shape[0]
+    existing = 0 if lbvh is None else int(lbvh.node_count)
+    cap = existing + 2 * n + 1
+    nodes = torch.zeros((cap, 8), dtype=_U32, device="cuda")
+    if lbvh is None:
+        boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+        indices = torch.empty(n, dtype=_U32, device="cuda")
+    else:
+        boxes, indices = lbvh.boxes, lbvh.indices
+        nodes[:existing] = lbvh.nodes[:existing]
+    info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
+    ws = workspace(N.lib().rk_build_sah_workspace_size(n, existing))
+    hb = N.aabox(hitbox)
+    _check(N.lib().rk_build_sah(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, C.byref(hb), boxes.data_ptr(),
+                                indices.data_ptr(), nodes.data_ptr(), existing, cap, info.data_ptr(), ws.data_ptr(),
+                                ws.numel()), "build_sah")
+    torch.cuda.synchronize()
+    t = RefTree(F, boxes, indices, nodes, info)
+    t.node_count = int(t.host_info()[N.RK_INFO_NODE_COUNT])
+    return t

@@ -38,6 +38,20 @@ class AABox(C.Structure):
     _fields_ = [("min", Float4), ("max", Float4)]
 
 
+class TraceOptions(C.Structure):
+    _fields_ = [("det_epsilon", C.c_float), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+DET_EPSILON_REFERENCE = 0.000001
+
+
+def trace_options(det_epsilon: float | None) -> TraceOptions | None:
+    """None = the reference threshold (1e-6); 0.0 = reject only parallel rays."""
+    if det_epsilon is None:
+        return None
+    return TraceOptions(float(det_epsilon), 0, (C.c_uint32 * 2)(0, 0))
+
+
 class FastBVH(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("indices", C.c_void_p), ("boxes", C.c_void_p),
                 ("face_count", C.c_uint32), ("reserved", C.c_uint32)]
@@ -76,11 +90,13 @@ _SIGS = {
     "rk_calc_morton_codes": (_I, [_P, _P, _U32, C.POINTER(AABox), _I, _P]),
     "rk_build_lbvh_workspace_size": (_SZ, [_U32]),
     "rk_build_lbvh": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, _P, _P, _P, _U32, _P, _P, _SZ]),
+    "rk_build_sah_workspace_size": (_SZ, [_U32, _U32]),
+    "rk_build_sah": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _P, _P, _P, _U32, _U32, _P, _P, _SZ]),
     "rk_raytrace_mesh_tree": (_I, [_P, _P, _U32, _P, _U32, _P, _P, _P, _I, _I, _P, _P, _P, _I]),
     "rk_fast_build_workspace_size": (_SZ, [_U32, _I]),
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
-    "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P]),
-    "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
+    "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
+    "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P]),
     # scenes
     "rk_scene_icosphere": (_I, [_U32, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rk_scene_heightfield": (_I, [_U32, C.c_uint64, C.c_float, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),

@@ -13,6 +13,7 @@
 #include <string>
 
 #include "../rk.h"
+#include "../rk_scenes.h"
 
 typedef unsigned long long qword;
 typedef unsigned int dword;

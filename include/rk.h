@@ -139,6 +139,20 @@ rk_status rk_build_lbvh(rk_stream stream, rk_uint4* faces, const rk_float4* vert
                         uint32_t* indices, rk_bvnode* nodes, uint32_t nodes_capacity, uint32_t* info, void* ws,
                         size_t ws_bytes);
 
+/* ---- reference binned SAH build ------------------------------------------------ */
+/* void CUDA_buildMeshTreeSAH(CUDAMesh*, CUDAMeshTree*), CUDAMeshTree.cpp:34-38 /
+ * CUDA_buildMeshTreeSAH.cu:672. 32-bin SAH with the reference's costs, leaf rules and
+ * stable partition. existing_nodes == 0: boxes/indices are computed and the root box is
+ * the hitbox; existing_nodes > 0 (Hybrid, CUDAMeshTree.cpp:98-103): the first
+ * existing_nodes entries of 'nodes' are an LBVH built with tree_complete = 0 (boxes and
+ * indices from that build) and each of its leaves is refined. Node ids are breadth-first
+ * (the reference's are in atomicAdd order). nodes_capacity >= existing_nodes + 2n - 1.
+ * Faces are reordered in place. Synchronises 'stream' once per tree level. */
+size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t existing_nodes);
+rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                       const rk_aabox* hitbox, rk_aabox* boxes, uint32_t* indices, rk_bvnode* nodes,
+                       uint32_t existing_nodes, uint32_t nodes_capacity, uint32_t* info, void* ws, size_t ws_bytes);
+
 /* ---- reference trace -------------------------------------------------------- */
 /* void CUDA_raytraceMeshTree(CUDABuffer*, CUDAMesh*, CUDAMeshTree*, bool coherency),
  * CUDARaytracer.h:30-36 / CUDA_raytraceMeshTree.cu:434. faces_id/coords/depths are the
@@ -171,16 +185,26 @@ size_t rk_fast_build_workspace_size(uint32_t face_count, int morton_mode);
 rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
                         const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
                         uint32_t* info, void* ws, size_t ws_bytes);
+/* Trace options. det_epsilon is the triangle test's parallel-ray threshold:
+ * the reference rejects |det| < 1e-6 (CUDA_raytraceMeshTree.cu:36,65), an
+ * absolute bound that rejects every hit once edges are ~1e-3 long; 0 rejects
+ * only exactly parallel rays. NULL options = the reference threshold. */
+#define RK_DET_EPSILON_REFERENCE 0.000001f
+typedef struct rk_trace_options {
+  float det_epsilon;
+  uint32_t flags;
+  uint32_t reserved[2];
+} rk_trace_options;
 /* Closest hit for n_rays rays. faces_id receives the sorted position (index into
  * bvh->tris; original face id = bvh->indices[position]); depths is read as tmax. */
 rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                         const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
-                        float* depths);
+                        float* depths, const rk_trace_options* options);
 /* rk_fast_trace that also adds the number of child-box tests and triangle tests
  * it performs to counters[0] and counters[1] (device, uint64). Same results. */
 rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                 const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
-                                float* depths, uint64_t* counters);
+                                float* depths, const rk_trace_options* options, uint64_t* counters);
 
 #ifdef __cplusplus
 }

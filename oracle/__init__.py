@@ -35,7 +35,8 @@ _SIGS = {
     "ora_box_ray": (_I, [_P, _P, _P, _P, _P]),
     "ora_trace_ref_ray": (_U, [_P, _P, _P, _P, _P, _U, _P, _P, _P, _P]),
     "ora_trace_ref_packet": (_U, [_P, _P, _P, _P, _P, _U, _U, _P, _P, _P]),
-    "ora_trace_bruteforce": (None, [_P, _P, _U, _P, _P, _U, _P, _P, _P]),
+    "ora_trace_bruteforce": (None, [_P, _P, _U, _P, _P, _U, _P, _P, _P, C.c_float]),
+    "ora_trace_closest": (None, [_P, _P, _P, _P, _P, _P, _U, _P, _P, _P, C.c_float]),
     "ora_hit_t_f64": (None, [_P, _P, _P, _P, _P, _U, _P]),
     "ora_build_sah": (_I, [_P, _P, _U, _P, _P, _U, _U, _P, _P]),
     "ora_reorder_faces": (None, [_P, _P, _U]),
@@ -182,10 +183,25 @@ def trace_ref_packet(nodes, F, V, O, D, width, height, face_id=None, coords=None
     return fid, uv, dep, ovf
 
 
-def trace_bruteforce(F, V, O, D, face_id=None, coords=None, depths=None):
+DET_EPSILON_REFERENCE = 0.000001
+
+
+def trace_bruteforce(F, V, O, D, face_id=None, coords=None, depths=None, det_epsilon=DET_EPSILON_REFERENCE):
+    """Closest hit over every face, ties to the smallest face index."""
     n = O.shape[0]
     fid, uv, dep = _outputs(n, face_id, coords, depths)
-    lib().ora_trace_bruteforce(_p(F), _p(V), F.shape[0], _p(O), _p(D), n, _p(fid), _p(uv), _p(dep))
+    lib().ora_trace_bruteforce(_p(F), _p(V), F.shape[0], _p(O), _p(D), n, _p(fid), _p(uv), _p(dep), det_epsilon)
+    return fid, uv, dep
+
+
+def trace_closest(tree: "LBVH", V, O, D, face_id=None, coords=None, depths=None,
+                  det_epsilon=DET_EPSILON_REFERENCE):
+    """The trace_bruteforce result found through a reference-layout tree
+    (faces reordered, indices mapping back); face ids are original ids."""
+    n = O.shape[0]
+    fid, uv, dep = _outputs(n, face_id, coords, depths)
+    lib().ora_trace_closest(tree.nodes.ctypes.data, _p(tree.faces), _p(tree.indices), _p(V), _p(O), _p(D), n,
+                            _p(fid), _p(uv), _p(dep), det_epsilon)
     return fid, uv, dep
 
 

@@ -528,10 +528,11 @@ namespace {
 const float EPS = 0.000001f;  // RT.cu:36
 
 // intersectTriRay RT.cu:57-89 (vert0/edge1/edge2 have w = 0, RT.cu:174-176).
-inline bool tri_ray(F4 orig, F4 dir, F4 v0, F4 e1, F4 e2, float& t, float& u, float& v) {
+// eps is the reference's EPSILON unless a caller asks for another threshold.
+inline bool tri_ray(F4 orig, F4 dir, F4 v0, F4 e1, F4 e2, float& t, float& u, float& v, float eps = EPS) {
   F4 pvec = cross4(dir, e2);
   float det = dot4(e1, pvec);
-  if (det > -EPS && det < EPS) return false;
+  if (det > -eps && det < eps) return false;
   float inv_det = 1.0f / det;
   F4 tvec = sub4(orig, v0);
   u = dot4(tvec, pvec) * inv_det;
@@ -726,7 +727,7 @@ ORA_API uint32_t ora_trace_ref_packet(const Node* nodes, const U4* faces, const 
 // face ids are indices into 'faces' as given.
 ORA_API void ora_trace_bruteforce(const U4* faces, const F4* verts, uint32_t n_faces, const F4* origs,
                                   const F4* dirs, uint32_t n_rays, uint32_t* face_id, float* coords,
-                                  float* depths) {
+                                  float* depths, float eps) {
 #pragma omp parallel for schedule(dynamic, 16)
   for (long l = 0; l < (long)n_rays; ++l) {
     F4 orig = origs[l], dir = dirs[l];
@@ -738,9 +739,68 @@ ORA_API void ora_trace_bruteforce(const U4* faces, const F4* verts, uint32_t n_f
       F4 v0, e1, e2;
       tri_setup(faces, verts, f, v0, e1, e2);
       float t, u, v;
-      if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
+      if (tri_ray(orig, dir, v0, e1, e2, t, u, v, eps)) {
         if (t < depth || (any && t == depth && f < fid)) { fid = f; cu = u; cv = v; depth = t; any = true; }
       }
+    }
+    face_id[l] = fid; coords[2 * l] = cu; coords[2 * l + 1] = cv; depths[l] = depth;
+  }
+}
+
+// The same closest hit as ora_trace_bruteforce, found through a tree in the
+// reference layout (any of the builds above) with a conservative box test in
+// double precision (boxes widened by 1e-6 relative), so it scales to large
+// meshes. 'faces' are the tree's (reordered) faces and indices[] maps a
+// position to the original face id used for ties and returned in face_id.
+ORA_API void ora_trace_closest(const Node* nodes, const U4* faces, const uint32_t* indices, const F4* verts,
+                               const F4* origs, const F4* dirs, uint32_t n_rays, uint32_t* face_id, float* coords,
+                               float* depths, float eps) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long l = 0; l < (long)n_rays; ++l) {
+    F4 orig = origs[l], dir = dirs[l];
+    float depth = depths[l];
+    uint32_t fid = face_id[l];
+    float cu = coords[2 * l], cv = coords[2 * l + 1];
+    bool any = false;
+    const double o[3] = {orig.x, orig.y, orig.z};
+    double inv[3];
+    const float dd[3] = {dir.x, dir.y, dir.z};
+    for (int a = 0; a < 3; ++a) inv[a] = dd[a] != 0.0f ? 1.0 / (double)dd[a] : 1e300;
+    std::vector<uint32_t> stack;
+    stack.reserve(128);
+    stack.push_back(0);
+    while (!stack.empty()) {
+      const Node& nd = nodes[stack.back()];
+      stack.pop_back();
+      if (is_leaf(nd)) {
+        for (uint32_t k = 0; k < leaf_count(nd); ++k) {
+          uint32_t pos = nd.w1 + k;
+          F4 v0, e1, e2;
+          tri_setup(faces, verts, pos, v0, e1, e2);
+          float t, u, v;
+          if (tri_ray(orig, dir, v0, e1, e2, t, u, v, eps)) {
+            uint32_t id = indices[pos];
+            if (t < depth || (any && t == depth && id < fid)) { fid = id; cu = u; cv = v; depth = t; any = true; }
+          }
+        }
+        continue;
+      }
+      double t0 = 0.0, t1 = (double)depth * (1.0 + 1e-6) + 1e-30;
+      bool empty = nd.bmin[0] > nd.bmax[0] || nd.bmin[1] > nd.bmax[1] || nd.bmin[2] > nd.bmax[2];
+      bool hit = true;
+      if (!empty) {
+        for (int a = 0; a < 3 && hit; ++a) {
+          double lo = nd.bmin[a], hi = nd.bmax[a], pad = 1e-6 * (std::fabs(lo) + std::fabs(hi) + (hi - lo)) + 1e-30;
+          lo -= pad; hi += pad;
+          if (dd[a] == 0.0f) { hit = o[a] >= lo && o[a] <= hi; continue; }
+          double ta = (lo - o[a]) * inv[a], tb = (hi - o[a]) * inv[a];
+          if (ta > tb) std::swap(ta, tb);
+          t0 = std::max(t0, ta);
+          t1 = std::min(t1, tb);
+          hit = t0 <= t1;
+        }
+      }
+      if (hit) { stack.push_back(nd.w1); stack.push_back(nd.w1 + 1); }
     }
     face_id[l] = fid; coords[2 * l] = cu; coords[2 * l + 1] = cv; depths[l] = depth;
   }

@@ -197,7 +197,38 @@ def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
     assert (od < dep0).sum() > n // 10
 
 
-def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None):
+def _check_fast_boxes(tree, F, V, leaf_max):
+    """Every child slot of every reachable node holds the padded union of the
+    boxes of the faces below it, the right face count, and covers them."""
+    nodes = _h(tree.nodes, np.uint32)
+    order = _h(tree.indices, np.uint32)
+    fb = np.stack([V[F[:, 0], :3], V[F[:, 1], :3], V[F[:, 2], :3]], 1)
+    lo_f, hi_f = fb.min(1)[order], fb.max(1)[order]
+    nf = nodes.view(np.float32)
+    n = F.shape[0]
+    stack, seen = [0], 0
+    while stack:
+        i = stack.pop()
+        seen += 1
+        for side in (0, 1):
+            ref = int(nodes[i, 12 + side])
+            cnt = int(nodes[i, 14 + side])
+            box = nf[i, 4 * side:4 * side + 4]
+            lo = np.array([box[0], box[2], nf[i, 8 + 2 * side]])
+            hi = np.array([box[1], box[3], nf[i, 9 + 2 * side]])
+            if ref & 0x80000000:
+                first, c = (ref & 0x7FFFFFFF) >> 4, (ref & 15) + 1
+                assert c == cnt and c <= leaf_max and first + c <= n
+                rng = slice(first, first + c)
+            else:
+                stack.append(ref)
+                rng = None
+            if rng is not None:
+                assert np.all(lo <= lo_f[rng].min(0)) and np.all(hi >= hi_f[rng].max(0))
+    assert seen >= max(1, (n - 1) // leaf_max // 2)
+
+
+def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None, det_epsilon=1e-6):
     hb = rk.hitbox(V)
     b = dev.FastBuilder(F.shape[0], mode)
     tree = b.build(_t(F), _t(V), hb, leaf_max)
@@ -205,8 +236,9 @@ def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None):
     fid, uv, dep = dev.new_outputs(n)
     if tmax is not None:
         dep.fill_(tmax)
-    dev.fast_trace(tree, _t(O), _t(D), out=(fid, uv, dep))
-    bf, buv, bdep = ora.trace_bruteforce(F, V, O, D, depths=None if tmax is None else np.full(n, tmax, np.float32))
+    dev.fast_trace(tree, _t(O), _t(D), out=(fid, uv, dep), det_epsilon=det_epsilon)
+    bf, buv, bdep = ora.trace_bruteforce(F, V, O, D, depths=None if tmax is None else np.full(n, tmax, np.float32),
+                                         det_epsilon=det_epsilon)
     g_dep = _h(dep)
     hit = bdep < (F32MAX if tmax is None else tmax)
     np.testing.assert_array_equal(g_dep, bdep)
@@ -215,7 +247,37 @@ def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None):
     np.testing.assert_array_equal(order[g_pos[hit]], bf[hit])
     np.testing.assert_array_equal(_h(uv)[hit], buv[hit])
     assert int(_h(tree.info, np.uint32)[rk.RK_INFO_FLAGS]) == 0
+    if F.shape[0] > 1:
+        _check_fast_boxes(tree, F, V, leaf_max)
     return hit.mean()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fast_tree_det_epsilon_zero(dev, rk, ora, mode):
+    V, F = _scene(rk, "hf128")
+    O, D = _camera_rays(rk, "hf128", 80, 60)
+    assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, 4, det_epsilon=0.0) > 0.05
+
+
+def test_fast_tree_full_size_heightfield(dev, rk, ora):
+    """1M faces, 1920x1080 camera: every 37th ray of the frame against the
+    oracle's closest hit through its own depth-31 tree, det_epsilon 0."""
+    V, F = _scene(rk, "hf708")
+    hb = rk.hitbox(V)
+    O, D = _camera_rays(rk, "hf708", 1920, 1080)
+    tree = dev.FastBuilder(F.shape[0], 1).build(_t(F), _t(V), hb, 4)
+    fid, uv, dep = dev.fast_trace(tree, _t(O), _t(D), det_epsilon=0.0)
+    sel = np.arange(0, O.shape[0], 37)
+    Os, Ds = np.ascontiguousarray(O[sel]), np.ascontiguousarray(D[sel])
+    otree = ora.build_lbvh_ref(F, V, hb, 31, True)
+    of, ouv, od = ora.trace_closest(otree, V, Os, Ds, det_epsilon=0.0)
+    g_dep = _h(dep)[sel]
+    np.testing.assert_array_equal(g_dep, od)
+    hit = od < F32MAX
+    assert hit.sum() > 1000
+    g_orig = _h(tree.indices, np.uint32)[_h(fid, np.uint32)[sel][hit]]
+    np.testing.assert_array_equal(g_orig, of[hit])
+    np.testing.assert_array_equal(_h(uv)[sel][hit], ouv[hit])
 
 
 @pytest.mark.parametrize("mode", [1, 2])
@@ -273,3 +335,61 @@ def test_fast_vs_reference_trace_c1(dev, rk, ora):
     gt = ora.hit_t_f64(F, V, O[hit][diff], D[hit][diff], g_orig[diff])
     rt = ora.hit_t_f64(F, V, O[hit][diff], D[hit][diff], r_orig[diff])
     assert np.allclose(gt, rt, rtol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["tiny", "ico3", "ico5", "hf128", "soup50k"])
+def test_sah_bit_exact(dev, rk, ora, name):
+    V, F = _scene(rk, name)
+    hb = rk.hitbox(V)
+    boxes, idx = ora.calc_boxes(F, V)
+    onodes, oidx, oflags = ora.build_sah(boxes, idx, hb)
+    Ft = _t(F)
+    tree = dev.build_sah(Ft, _t(V), hb)
+    M = tree.node_count
+    assert M == onodes.shape[0]
+    assert int(tree.host_info()[rk.RK_INFO_FLAGS]) == oflags
+    np.testing.assert_array_equal(_h(tree.indices, np.uint32), oidx)
+    np.testing.assert_array_equal(_h(Ft, np.uint32), ora.reorder_faces(F, oidx))
+    nodes = _h(tree.nodes, np.uint32)[:M].copy().view(ora.NODE_DTYPE).reshape(M)
+    np.testing.assert_array_equal(nodes["w0"], onodes["w0"])
+    np.testing.assert_array_equal(nodes["w1"], onodes["w1"])
+    assert np.array_equal(nodes["bmin"], onodes["bmin"]) and np.array_equal(nodes["bmax"], onodes["bmax"])
+
+
+@pytest.mark.parametrize("name,depth", [("ico5", 12), ("hf128", 6), ("soup50k", 12)])
+def test_hybrid_bit_exact(dev, rk, ora, name, depth):
+    V, F = _scene(rk, name)
+    hb = rk.hitbox(V)
+    ref = ora.build_lbvh_ref(F, V, hb, depth, False)
+    onodes, oidx, oflags = ora.build_sah(ref.boxes, ref.indices, hb, ref.nodes, ref.node_count)
+    Ft = _t(F)
+    lb = dev.build_lbvh(Ft, _t(V), hb, depth, False)
+    assert lb.node_count == ref.node_count
+    tree = dev.build_sah(Ft, _t(V), hb, lbvh=lb)
+    M = tree.node_count
+    assert M == onodes.shape[0]
+    np.testing.assert_array_equal(_h(tree.indices, np.uint32), oidx)
+    np.testing.assert_array_equal(_h(Ft, np.uint32), ora.reorder_faces(F, oidx))
+    nodes = _h(tree.nodes, np.uint32)[:M].copy().view(ora.NODE_DTYPE).reshape(M)
+    np.testing.assert_array_equal(nodes["w0"], onodes["w0"])
+    np.testing.assert_array_equal(nodes["w1"], onodes["w1"])
+    assert np.array_equal(nodes["bmin"], onodes["bmin"]) and np.array_equal(nodes["bmax"], onodes["bmax"])
+
+
+@pytest.mark.parametrize("coherency", [False, True])
+def test_sah_tree_trace_bit_exact(dev, rk, ora, coherency):
+    V, F = _scene(rk, "ico5")
+    hb = rk.hitbox(V)
+    boxes, idx = ora.calc_boxes(F, V)
+    onodes, oidx, _ = ora.build_sah(boxes, idx, hb)
+    OF = ora.reorder_faces(F, oidx)
+    tree = dev.build_sah(_t(F), _t(V), hb)
+    O, D = _camera_rays(rk, "ico5", 128, 128)
+    fid, uv, dep = dev.trace_ref(tree, _t(V), _t(O), _t(D), 128, 128, coherency)
+    if coherency:
+        of, ouv, od, _ = ora.trace_ref_packet(onodes, OF, V, O, D, 128, 128)
+    else:
+        of, ouv, od, _ = ora.trace_ref_ray(onodes, OF, V, O, D)
+    np.testing.assert_array_equal(_h(fid, np.uint32), of)
+    np.testing.assert_array_equal(_h(uv), ouv)
+    np.testing.assert_array_equal(_h(dep), od)

@@ -380,6 +380,28 @@ def test_trace_ref_matches_bruteforce_on_icosphere(ora, rk):
     assert err.max() < 1e-5
 
 
+@pytest.mark.parametrize("eps", [1e-6, 0.0])
+def test_trace_closest_equals_bruteforce(ora, rk, eps):
+    # the tree-based closest-hit oracle agrees with brute force (all faces) on
+    # t, u, v and the tie rule, with the reference threshold and with eps = 0
+    V, F = rk.heightfield(40, seed=5)
+    hb = rk.hitbox(V)
+    el, az = np.radians(20.0), np.radians(30.0)
+    tgt = np.array([0.5, 0.5, 0.1])
+    eye = tgt + 1.5 * np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    cam = rk.camera_lookat(eye, tgt, 60 * 0.01745329252, 64, 48, up=[0, 0, 1])
+    O, D = rk.primary_rays_host(cam, 64, 48)
+    bf, buv, bd = ora.trace_bruteforce(F, V, O, D, det_epsilon=eps)
+    for depth in (6, 31):
+        tree = ora.build_lbvh_ref(F, V, hb, depth, True)
+        cf, cuv, cd = ora.trace_closest(tree, V, O, D, det_epsilon=eps)
+        np.testing.assert_array_equal(cd, bd)
+        hit = bd < F32MAX
+        np.testing.assert_array_equal(cf[hit], bf[hit])
+        np.testing.assert_array_equal(cuv[hit], buv[hit])
+    assert hit.mean() > 0.05  # the terrain fills about a tenth of this view
+
+
 def test_trace_miss_leaves_outputs_unchanged(ora, rk):
     V, F = rk.icosphere(1)
     r = ora.build_lbvh_ref(F, V, rk.hitbox(V), 12, True)
