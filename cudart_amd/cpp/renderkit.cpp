@@ -243,7 +243,7 @@ void CUDAMeshTree::buildMeshTreeFast(uint leaf_max, bool morton63) {
   delete nodes;
   nodes = nullptr;
   int mode = morton63 ? RK_MORTON_63 : RK_MORTON_30;
-  fast_nodes = new CUDAArray<float4>((int)(4 * (n > 1 ? n - 1 : 1)));
+  fast_nodes = new CUDAArray<float4>((int)(4 * n));
   fast_tris = new CUDAArray<float4>((int)(3 * n));
   Workspace ws(rk_fast_build_workspace_size(n, mode));
   CUDAArray<uint> info(RK_INFO_WORDS);

@@ -7,19 +7,17 @@
 // Build:
 //   k_fast_prep     per-face box (API array) and 30/63-bit Morton key
 //   radix sort      (key, face) pairs
-//   k_karras        internal node i of n-1 (Karras 2012): range, split,
-//                   children, parent links
-//   k_fast_refit    one thread per leaf walks to the root; the second thread
-//                   to reach a node finishes it (atomic counter), writing each
-//                   child's padded box, count and reference into the parent's
-//                   64-B record; subtrees of <= leaf_max faces become leaves
-//   k_fast_tris     leaf-ordered triangle records (v0, v1-v0, v2-v0), 48 B
+//   k_fast_hierarchy  topology, node boxes and leaf-ordered triangle records
+//                     (v0, v1-v0, v2-v0; 48 B) in one bottom-up pass
+//                     (agglomerative LBVH, Apetrei 2014); subtrees of
+//                     <= leaf_max faces become leaves
 //
 // Node record (64 B): float4 (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y),
 //                     float4 (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y),
 //                     float4 (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z),
 //                     uint4  (ref0, ref1, count0, count1)
-// ref: internal node index, or kLeafBit | first << 4 | (count - 1).
+// ref: node index (root 0, the split between sorted positions p and p+1 is
+// p + 1), or kLeafBit | first << 4 | (count - 1). n records for n faces.
 #include "rk_device.h"
 
 template <typename K>
@@ -86,50 +84,16 @@ __global__ __launch_bounds__(256) void k_fast_prep(const uint4* __restrict__ fac
   vals[k] = k;
 }
 
+// Split key between sorted positions i and i+1 (Apetrei 2014, "Fast and
+// simple agglomerative LBVH construction"): the XOR of the two Morton keys,
+// with the top bit set so that any difference in the keys outranks the index
+// tie-break used for equal keys (the XOR of the two positions). Larger means
+// the two positions separate higher in the tree. Two splits that bound the
+// same node never compare equal.
 template <typename K>
-__device__ __forceinline__ int kdelta(const K* __restrict__ keys, uint32_t n, int i, int j) {
-  if (j < 0 || j >= (int)n) return -1;
-  K a = keys[i], b = keys[j];
-  const int kb = (int)sizeof(K) * 8;
-  if (a == b) return kb + __clz((int)((uint32_t)i ^ (uint32_t)j));
-  if (sizeof(K) == 4) return __clz((int)(uint32_t)(a ^ b));
-  return __clzll((long long)(uint64_t)(a ^ b));
-}
-
-// Internal node i (Karras 2012, "Maximizing parallelism in the construction of
-// BVHs, octrees and k-d trees"). Children are leaf (sorted position) or
-// internal index; parents are recorded for the bottom-up pass.
-template <typename K>
-__global__ __launch_bounds__(256) void k_karras(const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ child,
-                                                uint32_t* __restrict__ range_first, uint32_t* __restrict__ parent_node,
-                                                uint32_t* __restrict__ parent_leaf) {
-  const int i = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (i >= (int)n - 1) return;
-  const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = kdelta(keys, n, i, i - d);
-  int lmax = 2;
-  while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-  const int j = i + l * d;
-  const int dnode = kdelta(keys, n, i, j);
-  int s = 0;
-  int t = l;
-  do {
-    t = (t + 1) >> 1;
-    if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-  } while (t > 1);
-  const int gamma = i + s * d + (d < 0 ? -1 : 0);
-  const int lo = i < j ? i : j, hi = i < j ? j : i;
-  uint32_t left, right;
-  if (lo == gamma) { left = kLeafBit | (uint32_t)gamma; parent_leaf[gamma] = (uint32_t)i; }
-  else { left = (uint32_t)gamma; parent_node[gamma] = (uint32_t)i; }
-  if (hi == gamma + 1) { right = kLeafBit | (uint32_t)(gamma + 1); parent_leaf[gamma + 1] = (uint32_t)i; }
-  else { right = (uint32_t)(gamma + 1); parent_node[gamma + 1] = (uint32_t)i; }
-  child[2 * i] = left;
-  child[2 * i + 1] = right;
-  range_first[i] = (uint32_t)lo;
+__device__ __forceinline__ uint64_t split_key(const K* __restrict__ keys, uint32_t i) {
+  const uint64_t x = (uint64_t)(keys[i] ^ keys[i + 1]);
+  return x ? (x | 0x8000000000000000ull) : (uint64_t)(i ^ (i + 1));
 }
 
 // Outward padding: a relative 2^-18 of the coordinate magnitude plus the same
@@ -143,97 +107,243 @@ __device__ __forceinline__ void pad_box(float& lo, float& hi) {
   hi = hi + p;
 }
 
-__device__ __forceinline__ void write_slot(FNode* __restrict__ nodes, uint32_t parent, uint32_t side, float lox,
-                                           float loy, float loz, float hix, float hiy, float hiz, uint32_t ref,
-                                           uint32_t count) {
-  pad_box(lox, hix);
-  pad_box(loy, hiy);
-  pad_box(loz, hiz);
-  FNode* p = nodes + parent;
-  float* ab = side ? reinterpret_cast<float*>(&p->b) : reinterpret_cast<float*>(&p->a);
-  ab[0] = lox; ab[1] = hix; ab[2] = loy; ab[3] = hiy;
-  float* cz = reinterpret_cast<float*>(&p->c) + 2 * side;
-  cz[0] = loz; cz[1] = hiz;
-  uint32_t* dd = reinterpret_cast<uint32_t*>(&p->d);
-  dd[side] = ref;
-  dd[2 + side] = count;
+// A finished subtree as its parent sees it: box (unpadded), reference, and
+// the far end of its leaf range (the end the sibling does not know).
+struct Part {
+  float lox, loy, loz, hix, hiy, hiz;
+  uint32_t ref, end;
+};
+static_assert(sizeof(Part) == 32, "part is 32 B");
+
+__device__ __forceinline__ void write_node(FNode* __restrict__ nodes, uint32_t idx, const Part& c0, const Part& c1,
+                                           uint32_t n0, uint32_t n1) {
+  float a0 = c0.lox, a1 = c0.hix, a2 = c0.loy, a3 = c0.hiy, a4 = c0.loz, a5 = c0.hiz;
+  float b0 = c1.lox, b1 = c1.hix, b2 = c1.loy, b3 = c1.hiy, b4 = c1.loz, b5 = c1.hiz;
+  pad_box(a0, a1); pad_box(a2, a3); pad_box(a4, a5);
+  pad_box(b0, b1); pad_box(b2, b3); pad_box(b4, b5);
+  FNode* p = nodes + idx;
+  p->a = make_float4(a0, a1, a2, a3);
+  p->b = make_float4(b0, b1, b2, b3);
+  p->c = make_float4(a4, a5, b4, b5);
+  p->d = make_uint4(c0.ref, c1.ref, n0, n1);
 }
 
-__global__ __launch_bounds__(256) void k_fast_refit(const Box* __restrict__ boxes, const uint32_t* __restrict__ order,
-                                                    uint32_t n, const uint32_t* __restrict__ child,
-                                                    const uint32_t* __restrict__ range_first,
-                                                    const uint32_t* __restrict__ parent_node,
-                                                    const uint32_t* __restrict__ parent_leaf,
-                                                    uint32_t* __restrict__ flags, float* __restrict__ nbox,
-                                                    uint32_t* __restrict__ ncount, FNode* __restrict__ nodes,
-                                                    uint32_t leaf_max) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  Box b = rk::load_box(boxes + order[i]);
-  float lox = b.mn.x, loy = b.mn.y, loz = b.mn.z, hix = b.mx.x, hiy = b.mx.y, hiz = b.mx.z;
+// Max-scan over one workgroup of kLocalThreads threads (waves of 64).
+constexpr uint32_t kLocalThreads = 1024;
+constexpr uint32_t kLocalWaves = kLocalThreads / 64;
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+  uint32_t lo = __shfl_up((uint32_t)v, d, 64), hi = __shfl_up((uint32_t)(v >> 32), d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_down64(uint64_t v, int d) {
+  uint32_t lo = __shfl_down((uint32_t)v, d, 64), hi = __shfl_down((uint32_t)(v >> 32), d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+// before = max of v over threads < t (0 for t = 0), after = max over threads
+// > t (0 for the last thread), t = threadIdx.x.
+__device__ __forceinline__ void block_excl_max(uint64_t v, uint64_t* s_wave, uint64_t& before, uint64_t& after) {
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint64_t up = v, dn = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t u = shfl_up64(up, d), q = shfl_down64(dn, d);
+    if (lane >= (uint32_t)d) up = up > u ? up : u;
+    if (lane + d < 64) dn = dn > q ? dn : q;
+  }
+  if (lane == 63) s_wave[w] = up;  // wave max
+  __syncthreads();
+  uint64_t b = 0, a = 0;
+  for (uint32_t k = 0; k < kLocalWaves; ++k) {
+    if (k < w) b = b > s_wave[k] ? b : s_wave[k];
+    if (k > w) a = a > s_wave[k] ? a : s_wave[k];
+  }
+  uint64_t ub = shfl_up64(up, 1), da = shfl_down64(dn, 1);
+  if (lane > 0) b = b > ub ? b : ub;
+  if (lane < 63) a = a > da ? a : da;
+  before = b;
+  after = a;
+}
+
+constexpr uint32_t kChunk = kLocalThreads;  // sorted positions per workgroup, one per thread
+
+__device__ __forceinline__ Part merge_parts(const Part& lc, const Part& rc) {
+  Part up;
+  up.lox = rk::fmn(lc.lox, rc.lox);
+  up.loy = rk::fmn(lc.loy, rc.loy);
+  up.loz = rk::fmn(lc.loz, rc.loz);
+  up.hix = rk::fmx(lc.hix, rc.hix);
+  up.hiy = rk::fmx(lc.hiy, rc.hiy);
+  up.hiz = rk::fmx(lc.hiz, rc.hiz);
+  return up;
+}
+
+// Parent of the subtree [l, r]: the split at r when split_key(r) <
+// split_key(l - 1) (the subtree is its left child), else the split at l - 1.
+__device__ __forceinline__ bool left_of_parent(bool has_l, bool has_r, uint64_t kl, uint64_t kr) {
+  return !has_l || (has_r && kr < kl);
+}
+
+// Completes split p from its two children: writes its record unless it
+// collapses into a leaf, and returns the merged part (end not set).
+__device__ __forceinline__ Part finish_split(FNode* __restrict__ nodes, uint32_t n, uint32_t leaf_max, uint32_t p,
+                                             uint32_t nl, uint32_t nr, const Part& lc, const Part& rc) {
+  const uint32_t cnt = nr - nl + 1;
+  const bool root = nl == 0 && nr == n - 1;
+  const uint32_t idx = root ? 0u : p + 1;
+  if (root || cnt > leaf_max) write_node(nodes, idx, lc, rc, p - nl + 1, nr - p);
+  Part up = merge_parts(lc, rc);
+  up.ref = (!root && cnt <= leaf_max) ? (kLeafBit | (nl << 4) | (cnt - 1)) : idx;
+  return up;
+}
+
+// 16-B loads and stores that are coherent across XCDs, as relaxed agent-scope
+// atomics are (the same cache policy bit), but one instruction per 16 B.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_part_coherent(Part* p, const Part& v) {
+  const u32x4 a = {__float_as_uint(v.lox), __float_as_uint(v.loy), __float_as_uint(v.loz), __float_as_uint(v.hix)};
+  const u32x4 b = {__float_as_uint(v.hiy), __float_as_uint(v.hiz), v.ref, v.end};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1\n\tglobal_store_dwordx4 %0, %2, off offset:16 sc1"
+               :
+               : "v"(p), "v"(a), "v"(b)
+               : "memory");
+}
+__device__ __forceinline__ Part load_part_coherent(const Part* p) {
+  u32x4 a, b;
+  asm volatile(
+      "global_load_dwordx4 %0, %2, off sc1\n\tglobal_load_dwordx4 %1, %2, off offset:16 sc1\n\ts_waitcnt vmcnt(0)"
+      : "=&v"(a), "=&v"(b)
+      : "v"(p)
+      : "memory");
+  Part v;
+  v.lox = __uint_as_float(a.x);
+  v.loy = __uint_as_float(a.y);
+  v.loz = __uint_as_float(a.z);
+  v.hix = __uint_as_float(a.w);
+  v.hiy = __uint_as_float(b.x);
+  v.hiz = __uint_as_float(b.y);
+  v.ref = b.z;
+  v.end = b.w;
+  return v;
+}
+
+// The hierarchy in one bottom-up pass (agglomerative LBVH, Apetrei 2014): a
+// subtree covering sorted positions [l, r] is a child of the split at r or
+// at l - 1, whichever has the smaller split key; of the two subtrees that
+// reach a split, the second to arrive merges both and climbs on. This gives
+// the binary radix tree of Karras 2012 without per-node binary searches. Each
+// thread starts at its leaf, whose triangle record it also writes.
+//
+// Exchange at a split: both children publish their part before either knows
+// it is second, each in its own slot, the position next to the split (p for
+// the left child, p + 1 for the right one); then a counter decides. A slot is
+// next written by an ancestor of the split, which completes only after the
+// split's reader is done.
+//
+// A workgroup owns kChunk consecutive positions, one per thread. A split
+// whose range lies inside them is reached only by this workgroup, and its
+// exchange goes through LDS. The range of split p ends, on each side, at the
+// nearest split with a larger key, so it lies inside [c0, c1) when such a
+// split exists both in [c0 - 1, p - 1] and in [p + 1, c1 - 1] (prefix and
+// suffix maxima of the chunk's split keys). The few splits that cross chunks
+// exchange through global memory with XCD-coherent loads and stores and an
+// agent-scope counter; the stores are waited for before the counter moves.
+template <typename K>
+__global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __restrict__ keys,
+                                                                  const uint32_t* __restrict__ order,
+                                                                  const uint4* __restrict__ faces,
+                                                                  const float4* __restrict__ verts, uint32_t n,
+                                                                  uint32_t leaf_max, uint32_t* __restrict__ gflags,
+                                                                  Part* __restrict__ gparts,
+                                                                  FNode* __restrict__ nodes,
+                                                                  float4* __restrict__ tris) {
+  __shared__ Part s_part[kChunk];
+  __shared__ uint64_t s_key[kChunk + 1];  // s_key[k] = split_key(c0 - 1 + k), ~0 past either end
+  __shared__ uint32_t s_flag[kChunk];
+  __shared__ uint8_t s_local[kChunk];     // s_local[k]: split c0 + k lies inside the chunk
+  __shared__ uint64_t s_wave[kLocalWaves];
+  const uint32_t t = threadIdx.x;
+  const uint32_t c0 = blockIdx.x * kChunk;
+  const uint32_t c1 = min(c0 + kChunk, n);
+  const uint32_t i = c0 + t;
+  // leaf loads first: their latency overlaps the split-key scan
+  uint32_t fid = 0;
+  uint4 f = make_uint4(0, 0, 0, 0);
+  if (i < c1) {
+    fid = order[i];
+    f = faces[fid];
+  }
+  for (uint32_t k = t; k <= kChunk; k += kLocalThreads) {
+    const int64_t x = (int64_t)c0 - 1 + (int64_t)k;
+    s_key[k] = (x >= 0 && x + 1 < (int64_t)n) ? split_key(keys, (uint32_t)x) : ~0ull;
+  }
+  s_flag[t] = 0u;
+  __syncthreads();
+  {
+    // inside test of split p = c0 - 1 + k for k = t (k >= 1)
+    const uint64_t d = s_key[t];
+    uint64_t before, after;
+    block_excl_max(d, s_wave, before, after);
+    const uint64_t last = s_key[kChunk];
+    after = after > last ? after : last;
+    const bool inside = t >= 1 && c0 + t < c1 && before > d && after > d;
+    if (t >= 1) s_local[t - 1] = inside ? 1 : 0;
+    if (t == kLocalThreads - 1) s_local[kChunk - 1] = 0;  // split c0 + kChunk - 1 always crosses
+  }
+  __syncthreads();
+  if (i >= c1) return;
+  const float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+  const float4 e1 = rk::sub4(v1, v0), e2 = rk::sub4(v2, v0);
+  tris[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
+  tris[3 * (size_t)i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+  tris[3 * (size_t)i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+  Part me;
+  me.lox = rk::fmn(rk::fmn(v0.x, v1.x), v2.x);
+  me.loy = rk::fmn(rk::fmn(v0.y, v1.y), v2.y);
+  me.loz = rk::fmn(rk::fmn(v0.z, v1.z), v2.z);
+  me.hix = rk::fmx(rk::fmx(v0.x, v1.x), v2.x);
+  me.hiy = rk::fmx(rk::fmx(v0.y, v1.y), v2.y);
+  me.hiz = rk::fmx(rk::fmx(v0.z, v1.z), v2.z);
+  me.ref = kLeafBit | (i << 4);  // count - 1 = 0
   if (n == 1) {  // a single face: root with one leaf child and one empty child
-    write_slot(nodes, 0, 0, lox, loy, loz, hix, hiy, hiz, kLeafBit | 0u, 1);
-    FNode* p = nodes;
-    p->b = make_float4(RK_HUGE, -RK_HUGE, RK_HUGE, -RK_HUGE);
-    reinterpret_cast<float*>(&p->c)[2] = RK_HUGE;
-    reinterpret_cast<float*>(&p->c)[3] = -RK_HUGE;
-    reinterpret_cast<uint32_t*>(&p->d)[1] = kLeafBit | 0u;
-    reinterpret_cast<uint32_t*>(&p->d)[3] = 0;
+    Part empty;
+    empty.lox = empty.loy = empty.loz = RK_HUGE;
+    empty.hix = empty.hiy = empty.hiz = -RK_HUGE;
+    empty.ref = kLeafBit;
+    write_node(nodes, 0, me, empty, 1, 0);
     return;
   }
-  uint32_t count = 1;
-  uint32_t ref = kLeafBit | (i << 4);  // count - 1 = 0
-  uint32_t node = parent_leaf[i];
-  uint32_t self = kLeafBit | i;
-  // A Karras tree over n keys with index tie-breaks is at most 64 + 32 deep;
-  // the cap only bounds the loop should the parent links ever be corrupt.
-  // Sibling exchange without release/acquire: on a multi-XCD device an
-  // agent-scope release writes back the whole local L2 and an acquire
-  // invalidates it. Instead the exchanged words go through relaxed
-  // agent-scope atomic stores and loads (coherent across XCDs), and the
-  // counter increment waits until those stores are acknowledged.
-  for (uint32_t guard = 0; guard < 4096u && node < n - 1; ++guard) {
-    const uint32_t side = (child[2 * node] == self) ? 0u : 1u;
-    write_slot(nodes, node, side, lox, loy, loz, hix, hiy, hiz, ref, count);
-    uint32_t* mine = reinterpret_cast<uint32_t*>(nbox) + (size_t)(2 * node + side) * 6;
-    rk::atomic_store_relaxed(mine + 0, __float_as_uint(lox));
-    rk::atomic_store_relaxed(mine + 1, __float_as_uint(loy));
-    rk::atomic_store_relaxed(mine + 2, __float_as_uint(loz));
-    rk::atomic_store_relaxed(mine + 3, __float_as_uint(hix));
-    rk::atomic_store_relaxed(mine + 4, __float_as_uint(hiy));
-    rk::atomic_store_relaxed(mine + 5, __float_as_uint(hiz));
-    rk::atomic_store_relaxed(ncount + 2 * node + side, count);
-    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    const uint32_t old = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;  // the sibling finishes this node
-    const uint32_t* other = reinterpret_cast<const uint32_t*>(nbox) + (size_t)(2 * node + (side ^ 1u)) * 6;
-    lox = rk::fmn(lox, __uint_as_float(rk::atomic_load_relaxed(other + 0)));
-    loy = rk::fmn(loy, __uint_as_float(rk::atomic_load_relaxed(other + 1)));
-    loz = rk::fmn(loz, __uint_as_float(rk::atomic_load_relaxed(other + 2)));
-    hix = rk::fmx(hix, __uint_as_float(rk::atomic_load_relaxed(other + 3)));
-    hiy = rk::fmx(hiy, __uint_as_float(rk::atomic_load_relaxed(other + 4)));
-    hiz = rk::fmx(hiz, __uint_as_float(rk::atomic_load_relaxed(other + 5)));
-    count += rk::atomic_load_relaxed(ncount + 2 * node + (side ^ 1u));
-    if (node == 0) return;
-    ref = (count <= leaf_max) ? (kLeafBit | (range_first[node] << 4) | (count - 1)) : node;
-    self = node;
-    node = parent_node[node];
+  uint32_t l = i, r = i;
+  // Each step widens [l, r] by at least one position: n steps bound it.
+  for (uint32_t guard = 0; guard < n; ++guard) {
+    const bool has_l = l > 0, has_r = r + 1 < n;
+    uint64_t kl = ~0ull, kr = ~0ull;
+    if (has_l) kl = (l >= c0 && l - c0 <= kChunk) ? s_key[l - c0] : split_key(keys, l - 1);
+    if (has_r) kr = (r + 1 >= c0 && r + 1 - c0 <= kChunk) ? s_key[r + 1 - c0] : split_key(keys, r);
+    const bool as_left = left_of_parent(has_l, has_r, kl, kr);
+    const uint32_t p = as_left ? r : l - 1;
+    const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
+    me.end = as_left ? l : r;
+    Part sib;
+    if (p >= c0 && p - c0 < kChunk && s_local[p - c0]) {
+      s_part[mine - c0] = me;  // a local split has p, p + 1 in [c0, c1)
+      const uint32_t old =
+          __hip_atomic_fetch_add(s_flag + (p - c0), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) return;
+      sib = s_part[other - c0];
+    } else {
+      store_part_coherent(gparts + mine, me);
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == 0) return;
+      sib = load_part_coherent(gparts + other);
+    }
+    const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
+    me = as_left ? finish_split(nodes, n, leaf_max, p, nl, nr, me, sib)
+                 : finish_split(nodes, n, leaf_max, p, nl, nr, sib, me);
+    l = nl;
+    r = nr;
+    if (l == 0 && r == n - 1) return;  // the root
   }
-}
-
-__global__ __launch_bounds__(256) void k_fast_tris(const uint4* __restrict__ faces, const float4* __restrict__ verts,
-                                                   const uint32_t* __restrict__ order, uint32_t n,
-                                                   float4* __restrict__ tris) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  uint32_t fid = order[i];
-  uint4 f = faces[fid];
-  float4 v0 = verts[f.x];
-  float4 e1 = rk::sub4(verts[f.y], v0);
-  float4 e2 = rk::sub4(verts[f.z], v0);
-  tris[3 * i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
-  tris[3 * i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-  tris[3 * i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
 }
 
 // ---- trace -------------------------------------------------------------------
@@ -272,7 +382,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
                                                     float* __restrict__ depths, float eps,
                                                     unsigned long long* __restrict__ counters) {
   uint32_t n_box = 0, n_tri = 0;
-  const uint32_t n_nodes = n_tris > 1 ? n_tris - 1 : 1;
+  const uint32_t n_nodes = n_tris;
   const uint32_t max_visits = 4 * n_tris + 64;  // a tree visit touches each node once
   const uint32_t l = blockIdx.x * 256 + threadIdx.x;
   if (l >= n_rays) return;
@@ -358,14 +468,28 @@ extern "C" size_t rk_fast_build_workspace_size(uint32_t face_count, int morton_m
   size_t n = face_count ? face_count : 1;
   size_t kb = morton_mode == RK_MORTON_63 ? 8 : 4;
   size_t b = 0;
-  b += rk_align_up(n * kb);        // keys
-  b += rk_align_up(2 * n * 4);     // child
-  b += rk_align_up(n * 4) * 3;     // range_first, parent_node, parent_leaf
-  b += rk_align_up(n * 4);         // flags
-  b += rk_align_up(2 * n * 6 * 4); // child boxes
-  b += rk_align_up(2 * n * 4);     // child counts
+  b += rk_align_up(n * kb);               // keys
+  b += rk_align_up(n * 4);                // counters of splits that cross chunks
+  b += rk_align_up(n * sizeof(Part));     // parts exchanged at splits that cross chunks
   b += rk_sort_workspace_size((uint32_t)n, (uint32_t)kb);
   return b + 4096;
+}
+
+template <typename K>
+static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v, uint32_t n, float4 rmin,
+                                 float4 rmax, const rk_fast_bvh* bvh, uint32_t leaf_max, K* keys, uint32_t* flags,
+                                 Part* parts, void* sort_ws, size_t sort_bytes, uint32_t* info) {
+  uint32_t* order = bvh->indices;
+  hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
+                     reinterpret_cast<Box*>(bvh->boxes), keys, order);
+  RK_CHECK_LAUNCH();
+  rk_status s = rk_radix_sort_impl<K>(st, keys, order, n, 0, (int)sizeof(K), sort_ws, sort_bytes,
+                                      info + RK_INFO_FLAGS);
+  if (s != RK_SUCCESS) return s;
+  hipLaunchKernelGGL(k_fast_hierarchy<K>, dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st, keys, order, f,
+                     v, n, leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes), static_cast<float4*>(bvh->tris));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
 }
 
 extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
@@ -381,17 +505,11 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
   const bool wide = morton_mode == RK_MORTON_63;
   RkArena ar(ws, ws_bytes);
   void* keys = wide ? (void*)ar.take<uint64_t>(n) : (void*)ar.take<uint32_t>(n);
-  uint32_t* child = ar.take<uint32_t>(2 * (size_t)n);
-  uint32_t* range_first = ar.take<uint32_t>(n);
-  uint32_t* parent_node = ar.take<uint32_t>(n);
-  uint32_t* parent_leaf = ar.take<uint32_t>(n);
   uint32_t* flags = ar.take<uint32_t>(n);
-  float* nbox = ar.take<float>(2 * (size_t)n * 6);
-  uint32_t* ncount = ar.take<uint32_t>(2 * (size_t)n);
+  Part* parts = ar.take<Part>(n);
   size_t sort_bytes = rk_sort_workspace_size(n, wide ? 8 : 4);
   void* sort_ws = ar.take<char>(sort_bytes);
-  if (!keys || !child || !range_first || !parent_node || !parent_leaf || !flags || !nbox || !ncount || !sort_ws)
-    return RK_ERROR_WORKSPACE;
+  if (!keys || !flags || !parts || !sort_ws) return RK_ERROR_WORKSPACE;
   rk_status s;
   if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
   if ((s = rk_fill_u32(stream, flags, n, 0u)) != RK_SUCCESS) return s;
@@ -399,40 +517,11 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
   float4 rmax = make_float4(hitbox->max.x, hitbox->max.y, hitbox->max.z, hitbox->max.w);
   const uint4* f = reinterpret_cast<const uint4*>(faces);
   const float4* v = reinterpret_cast<const float4*>(vertices);
-  uint32_t* order = bvh->indices;
-  if (wide) {
-    hipLaunchKernelGGL(k_fast_prep<uint64_t>, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
-                       reinterpret_cast<Box*>(bvh->boxes), static_cast<uint64_t*>(keys), order);
-    RK_CHECK_LAUNCH();
-    s = rk_radix_sort_impl<uint64_t>(st, static_cast<uint64_t*>(keys), order, n, 0, 8, sort_ws, sort_bytes,
-                                     info + RK_INFO_FLAGS);
-    if (s != RK_SUCCESS) return s;
-    if (n > 1) {
-      hipLaunchKernelGGL(k_karras<uint64_t>, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st,
-                         static_cast<const uint64_t*>(keys), n, child, range_first, parent_node, parent_leaf);
-      RK_CHECK_LAUNCH();
-    }
-  } else {
-    hipLaunchKernelGGL(k_fast_prep<uint32_t>, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
-                       reinterpret_cast<Box*>(bvh->boxes), static_cast<uint32_t*>(keys), order);
-    RK_CHECK_LAUNCH();
-    s = rk_radix_sort_impl<uint32_t>(st, static_cast<uint32_t*>(keys), order, n, 0, 4, sort_ws, sort_bytes,
-                                     info + RK_INFO_FLAGS);
-    if (s != RK_SUCCESS) return s;
-    if (n > 1) {
-      hipLaunchKernelGGL(k_karras<uint32_t>, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st,
-                         static_cast<const uint32_t*>(keys), n, child, range_first, parent_node, parent_leaf);
-      RK_CHECK_LAUNCH();
-    }
-  }
-  hipLaunchKernelGGL(k_fast_refit, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                     reinterpret_cast<const Box*>(bvh->boxes), order, n, child, range_first, parent_node, parent_leaf,
-                     flags, nbox, ncount, static_cast<FNode*>(bvh->nodes), leaf_max);
-  RK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_fast_tris, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, order, n,
-                     static_cast<float4*>(bvh->tris));
-  RK_CHECK_LAUNCH();
-  return RK_SUCCESS;
+  if (wide)
+    return fast_build_keys<uint64_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint64_t*>(keys), flags,
+                                     parts, sort_ws, sort_bytes, info);
+  return fast_build_keys<uint32_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint32_t*>(keys), flags,
+                                   parts, sort_ws, sort_bytes, info);
 }
 
 static inline float det_eps_of(const rk_trace_options* o) {

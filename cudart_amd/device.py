@@ -169,7 +169,7 @@ class FastBuilder:
         self.n = n
         self.mode = morton_mode
         self.ws = workspace(N.lib().rk_fast_build_workspace_size(n, morton_mode))
-        self.nodes = torch.empty((max(1, n - 1), 16), dtype=_U32, device="cuda")
+        self.nodes = torch.empty((max(1, n), 16), dtype=_U32, device="cuda")
         self.tris = torch.empty((n, 12), dtype=torch.float32, device="cuda")
         self.indices = torch.empty(n, dtype=_U32, device="cuda")
         self.boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")

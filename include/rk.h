@@ -165,14 +165,14 @@ rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32
                                 rk_float2* coords, float* depths, int coherency);
 
 /* ---- wide-BVH fast path (no reference counterpart) --------------------------- */
-/* A second tree for throughput: Morton LBVH (Karras topology, atomic bottom-up refit,
- * leaf collapse), stored as 64-B two-child nodes and leaf-ordered triangle records
+/* A second tree for throughput: Morton LBVH (binary radix tree built bottom-up in one
+ * pass with its boxes, leaf collapse), stored as 64-B two-child nodes and leaf-ordered triangle records
  * (v0, v1-v0, v2-v0; 48 B). The trace uses the reference triangle arithmetic, so a
  * hit's t/u/v are bit-identical to the reference's for the same triangle; ties on t
  * go to the smallest original face id.
- *   nodes: capacity max(1, n-1) x 64 B; tris: n x 48 B; indices: n (sorted
+ *   nodes: capacity n x 64 B (root at 0); tris: n x 48 B; indices: n (sorted
  *   position -> original face id); boxes: n (API compatibility, original order).
- * morton_mode is RK_MORTON_30 or RK_MORTON_63; leaf_max in [1, 64]. */
+ * morton_mode is RK_MORTON_30 or RK_MORTON_63; leaf_max in [1, 16]. */
 typedef struct rk_fast_bvh {
   void* nodes;
   void* tris;
