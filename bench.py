@@ -107,6 +107,45 @@ def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
     }
 
 
+def soup_leg(torch, dev, native, timer, steps, warmup):
+    """Random soup of 10M faces (centroids uniform in the unit cube, lognormal
+    edges around 0.002), 63-bit Morton keys sorted as uint64 pairs, and 2^24
+    incoherent rays (origins uniform in the cube, directions uniform on the
+    sphere); one closest-hit frame of all rays per step."""
+    n, nr = 10_000_000, 1 << 24
+    V, F = native.soup(n, seed=0xC3)
+    hb = native.hitbox(V)
+    O, D = native.random_rays(nr, seed=0xC3)
+    Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
+    Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
+    del V, F, O, D
+    builder = dev.FastBuilder(n, native.RK_MORTON_63)
+    build = lambda: builder.build(Ft, Vt, hb, 4)
+    tree = build()
+    fid, uv, dep = dev.new_outputs(nr)
+    f32max = float(np.finfo(np.float32).max)
+
+    def frame():
+        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON)
+
+    for _ in range(warmup):
+        build()
+    build_ms = timer.time(build, steps)
+    for _ in range(warmup):
+        frame()
+    frame_ms = timer.time(frame, steps)
+    hits = int((dep < f32max).sum().item())
+    return {
+        "workload": "soup 10M faces, 63-bit Morton, 2^24 incoherent rays, closest hit",
+        "lbvh63_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
+        "build_ms": round(build_ms, 3),
+        "mrays_s": round(nr / (frame_ms * 1e-3) / 1e6, 2),
+        "frame_ms": round(frame_ms, 3),
+        "hit_fraction": round(hits / nr, 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +153,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref", action="store_true")
+    ap.add_argument("--no-soup", action="store_true")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--profile-only", action="store_true", help="a few builds and frames, no JSON (for rocprofv3)")
     args = ap.parse_args()
@@ -246,6 +286,10 @@ def main():
         result["reference_per_ray_mrays_s"] = round(nr / (ref_ray_ms * 1e-3) / 1e6, 3)
         result["reference_packet_mrays_s"] = round(nr / (ref_pkt_ms * 1e-3) / 1e6, 3)
         result["reference_node_count"] = rtree.node_count
+
+    if not args.no_soup:
+        log("soup 10M faces, 63-bit keys, 2^24 incoherent rays")
+        result["soup"] = soup_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
 
     if not args.no_cpu:
         log("cpu baseline")
