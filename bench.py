@@ -146,6 +146,83 @@ def soup_leg(torch, dev, native, timer, steps, warmup):
     }
 
 
+def batch_leg(torch, dev, native, timer, steps, warmup):
+    """4096 soups (log-uniform 64..65536 faces, 256..16384 rays each): one
+    step builds every mesh and traces every ray. Batched = the autotuned
+    rk_batch_build + rk_batch_trace captured in one graph; plain = one
+    rk_fast_build + rk_fast_trace per mesh, in a loop on one stream."""
+    from cudart_amd import scenes
+    table, F, V, hb, O, D = scenes.soup_batch()
+    nf, nr, m = F.shape[0], O.shape[0], table.shape[0]
+    Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
+    Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
+    b = dev.Batch(table, Ft, Vt, hb, Ot, Dt)
+    out = dev.new_outputs(nr)
+    f32max = float(np.finfo(np.float32).max)
+    config, tuned_ms = b.autotune(out, reps=2, det_epsilon=HEADLINE_DET_EPSILON)
+    s = torch.cuda.Stream()
+    g = dev.Graph(s)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        with g.capture():
+            native.check(native.lib().rk_fill_f32(dev._stream_ptr(s), out[2].data_ptr(), nr, f32max), "fill")
+            b.build(stream=s)
+            b.trace(out, det_epsilon=HEADLINE_DET_EPSILON, stream=s)
+
+    def batched():
+        g.launch()
+
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+
+    def time_on(stream, fn, reps):
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(reps):
+            fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    for _ in range(warmup):
+        batched()
+    batch_ms = time_on(s, batched, steps)
+    hits = int((out[2] < f32max).sum().item())
+
+    builder = dev.FastBuilder(int(table["face_count"].max()), native.RK_MORTON_30)
+    pout = dev.new_outputs(nr)
+    views = []
+    for k in range(m):
+        f0, fc = int(table["face_offset"][k]), int(table["face_count"][k])
+        v0, vc = int(table["vertex_offset"][k]), int(table["vertex_count"][k])
+        r0, rc = int(table["ray_offset"][k]), int(table["ray_count"][k])
+        views.append((Ft[f0:f0 + fc], Vt[v0:v0 + vc], hb[k], Ot[r0:r0 + rc], Dt[r0:r0 + rc],
+                      tuple(x[r0:r0 + rc] for x in pout), rc))
+
+    def plain():
+        for Fm, Vm, hbm, Om, Dm, outm, rc in views:
+            tree = builder.build(Fm, Vm, hbm, 4)
+            native.check(native.lib().rk_fill_f32(None, outm[2].data_ptr(), rc, f32max), "fill")
+            dev.fast_trace(tree, Om, Dm, out=outm, det_epsilon=HEADLINE_DET_EPSILON)
+
+    plain()
+    plain_ms = timer.time(plain, 2)
+    same = bool(torch.equal(pout[2], out[2]))
+    return {
+        "workload": "4096 soups, log-uniform 64..65536 faces and 256..16384 rays per mesh; build all + trace all",
+        "meshes": m, "faces": nf, "rays": nr,
+        "batched_ms": round(batch_ms, 3),
+        "batched_mrays_s": round(nr / (batch_ms * 1e-3) / 1e6, 2),
+        "batched_mtri_s": round(nf / (batch_ms * 1e-3) / 1e6, 2),
+        "plain_ms": round(plain_ms, 3),
+        "speedup_vs_plain": round(plain_ms / batch_ms, 2),
+        "same_depths_as_plain": same,
+        "hit_fraction": round(hits / nr, 4),
+        "autotuned": config.as_dict(),
+        "autotune_best_ms": round(tuned_ms, 3),
+        "graph": "fill + rk_batch_build + rk_batch_trace captured once, replayed per step",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +231,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref", action="store_true")
     ap.add_argument("--no-soup", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--profile-only", action="store_true", help="a few builds and frames, no JSON (for rocprofv3)")
     args = ap.parse_args()
@@ -290,6 +368,10 @@ def main():
     if not args.no_soup:
         log("soup 10M faces, 63-bit keys, 2^24 incoherent rays")
         result["soup"] = soup_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
+
+    if not args.no_batch:
+        log("batch of 4096 meshes: autotune, graph, batched vs plain")
+        result["batch"] = batch_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
 
     if not args.no_cpu:
         log("cpu baseline")

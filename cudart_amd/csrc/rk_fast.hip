@@ -185,11 +185,12 @@ __device__ __forceinline__ bool left_of_parent(bool has_l, bool has_r, uint64_t 
 
 // Completes split p from its two children: writes its record unless it
 // collapses into a leaf, and returns the merged part (end not set).
-__device__ __forceinline__ Part finish_split(FNode* __restrict__ nodes, uint32_t n, uint32_t leaf_max, uint32_t p,
-                                             uint32_t nl, uint32_t nr, const Part& lc, const Part& rc) {
+// The tree's positions are [lo, hi]; its root record goes to lo.
+__device__ __forceinline__ Part finish_split(FNode* __restrict__ nodes, uint32_t lo, uint32_t hi, uint32_t leaf_max,
+                                             uint32_t p, uint32_t nl, uint32_t nr, const Part& lc, const Part& rc) {
   const uint32_t cnt = nr - nl + 1;
-  const bool root = nl == 0 && nr == n - 1;
-  const uint32_t idx = root ? 0u : p + 1;
+  const bool root = nl == lo && nr == hi;
+  const uint32_t idx = root ? lo : p + 1;
   if (root || cnt > leaf_max) write_node(nodes, idx, lc, rc, p - nl + 1, nr - p);
   Part up = merge_parts(lc, rc);
   up.ref = (!root && cnt <= leaf_max) ? (kLeafBit | (nl << 4) | (cnt - 1)) : idx;
@@ -247,7 +248,12 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
 // suffix maxima of the chunk's split keys). The few splits that cross chunks
 // exchange through global memory with XCD-coherent loads and stores and an
 // agent-scope counter; the stores are waited for before the counter moves.
-template <typename K>
+// With kBatch the positions hold many meshes, sorted by (mesh, Morton code):
+// keys are mesh << 30 | 30-bit code, so every split between two meshes ranks
+// above every split inside one, and a climb stops at its mesh's root, whose
+// record goes to the mesh's first position. Mesh m's nodes and triangle
+// records then occupy [face_offset, face_offset + face_count).
+template <typename K, bool kBatch>
 __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __restrict__ keys,
                                                                   const uint32_t* __restrict__ order,
                                                                   const uint4* __restrict__ faces,
@@ -255,7 +261,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
                                                                   uint32_t leaf_max, uint32_t* __restrict__ gflags,
                                                                   Part* __restrict__ gparts,
                                                                   FNode* __restrict__ nodes,
-                                                                  float4* __restrict__ tris) {
+                                                                  float4* __restrict__ tris,
+                                                                  const rk_batch_mesh* __restrict__ meshes) {
   __shared__ Part s_part[kChunk];
   __shared__ uint64_t s_key[kChunk + 1];  // s_key[k] = split_key(c0 - 1 + k), ~0 past either end
   __shared__ uint32_t s_flag[kChunk];
@@ -291,7 +298,14 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
   }
   __syncthreads();
   if (i >= c1) return;
-  const float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+  uint32_t lo = 0, hi = n - 1, voff = 0;  // this leaf's mesh: positions [lo, hi]
+  if (kBatch) {
+    const rk_batch_mesh m = meshes[(uint32_t)((uint64_t)keys[i] >> 30)];
+    lo = m.face_offset;
+    hi = m.face_offset + m.face_count - 1;
+    voff = m.vertex_offset;
+  }
+  const float4 v0 = verts[f.x + voff], v1 = verts[f.y + voff], v2 = verts[f.z + voff];
   const float4 e1 = rk::sub4(v1, v0), e2 = rk::sub4(v2, v0);
   tris[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
   tris[3 * (size_t)i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
@@ -304,12 +318,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
   me.hiy = rk::fmx(rk::fmx(v0.y, v1.y), v2.y);
   me.hiz = rk::fmx(rk::fmx(v0.z, v1.z), v2.z);
   me.ref = kLeafBit | (i << 4);  // count - 1 = 0
-  if (n == 1) {  // a single face: root with one leaf child and one empty child
+  if (lo == hi) {  // a single face: root with one leaf child and one empty child
     Part empty;
     empty.lox = empty.loy = empty.loz = RK_HUGE;
     empty.hix = empty.hiy = empty.hiz = -RK_HUGE;
     empty.ref = kLeafBit;
-    write_node(nodes, 0, me, empty, 1, 0);
+    write_node(nodes, lo, me, empty, 1, 0);
     return;
   }
   uint32_t l = i, r = i;
@@ -338,17 +352,20 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       sib = load_part_coherent(gparts + other);
     }
     const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
-    me = as_left ? finish_split(nodes, n, leaf_max, p, nl, nr, me, sib)
-                 : finish_split(nodes, n, leaf_max, p, nl, nr, sib, me);
+    me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
+                 : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
     l = nl;
     r = nr;
-    if (l == 0 && r == n - 1) return;  // the root
+    if (l == lo && r == hi) return;  // the (mesh) root
   }
 }
 
 // ---- trace -------------------------------------------------------------------
 
-constexpr int kStack = 64;
+// The traversal stack holds at most one entry per tree level, and a radix
+// tree over b-bit keys with position tie-breaks (< 2^27 positions) is at most
+// b + 27 levels deep: 57 for 30-bit keys, 90 for 63-bit keys.
+constexpr int kStack = 96;
 
 // intersectTriRay, CUDA_raytraceMeshTree.cu:57-89, with vert0.w = edge1.w =
 // edge2.w = 0 (:174-176) and the reference's four-term dot products. eps is
@@ -374,32 +391,33 @@ __device__ __forceinline__ float safe_inv(float d) {
   return 1.0f / a;
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
-                                                    uint32_t n_tris, const float4* __restrict__ origins,
-                                                    const float4* __restrict__ dirs, uint32_t n_rays,
-                                                    uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                    float* __restrict__ depths, float eps,
-                                                    unsigned long long* __restrict__ counters) {
-  uint32_t n_box = 0, n_tri = 0;
-  const uint32_t n_nodes = n_tris;
+// Closest hit of one ray in the tree rooted at 'root' (positions and node
+// records share one index space of n_tris). Depth-first, nearer child first,
+// a pending child is skipped once the hit found is nearer than its entry
+// point. Ties on t go to the smallest original face id, so the result does
+// not depend on the tree. Returns the sorted position of the hit or ~0.
+struct RayHit {
+  uint32_t pos;
+  float t, u, v;
+};
+
+template <bool kCount, int kStackN>
+__device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                            uint32_t n_tris, uint32_t root, float4 orig, float4 dir, float depth,
+                                            float eps, uint32_t& n_box, uint32_t& n_tri) {
   const uint32_t max_visits = 4 * n_tris + 64;  // a tree visit touches each node once
-  const uint32_t l = blockIdx.x * 256 + threadIdx.x;
-  if (l >= n_rays) return;
-  const float4 orig = origins[l], dir = dirs[l];
-  float depth = depths[l];
   uint32_t best = 0xFFFFFFFFu, best_oid = 0xFFFFFFFFu;
   float bu = 0.0f, bv = 0.0f;
   const float ix = safe_inv(dir.x), iy = safe_inv(dir.y), iz = safe_inv(dir.z);
   const float ox = orig.x * ix, oy = orig.y * iy, oz = orig.z * iz;
-  uint32_t stack_ref[kStack];
-  float stack_t[kStack];
+  uint32_t stack_ref[kStackN];
+  float stack_t[kStackN];
   int sp = 0;
-  uint32_t cur = 0;  // root
+  uint32_t cur = root;
   uint32_t visits = 0;
   while (++visits <= max_visits) {
     if (!(cur & kLeafBit)) {
-      if (cur >= n_nodes) break;  // never taken for a tree built by rk_fast_build
+      if (cur >= n_tris) break;  // never taken for a tree built here
       const FNode nd = nodes[cur];
       const float a0 = __builtin_fmaf(nd.a.x, ix, -ox), a1 = __builtin_fmaf(nd.a.y, ix, -ox);
       const float a2 = __builtin_fmaf(nd.a.z, iy, -oy), a3 = __builtin_fmaf(nd.a.w, iy, -oy);
@@ -417,7 +435,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
         const bool first0 = tmin0 <= tmin1;
         const uint32_t nearr = first0 ? nd.d.x : nd.d.y, farr = first0 ? nd.d.y : nd.d.x;
         const float fart = first0 ? tmin1 : tmin0;
-        if (sp < kStack) { stack_ref[sp] = farr; stack_t[sp] = fart; ++sp; }
+        if (sp < kStackN) { stack_ref[sp] = farr; stack_t[sp] = fart; ++sp; }
         cur = nearr;
         continue;
       }
@@ -427,10 +445,10 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
       const uint32_t first = (cur & ~kLeafBit) >> 4;
       const uint32_t cnt = first < n_tris ? min((cur & 15u) + 1, n_tris - first) : 0u;
       for (uint32_t k = 0; k < cnt; ++k) {
-        const float4 r0 = tris[3 * (first + k) + 0];
+        const float4 r0 = tris[3 * (size_t)(first + k) + 0];
         float4 v0 = r0;
         v0.w = 0.0f;
-        const float4 e1 = tris[3 * (first + k) + 1], e2 = tris[3 * (first + k) + 2];
+        const float4 e1 = tris[3 * (size_t)(first + k) + 1], e2 = tris[3 * (size_t)(first + k) + 2];
         float t, u, v;
         if (kCount) ++n_tri;
         if (tri_ray(orig, dir, v0, e1, e2, eps, t, u, v)) {
@@ -449,14 +467,29 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
     }
     if (!found) break;
   }
+  return RayHit{best, depth, bu, bv};
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                                    uint32_t n_tris, const float4* __restrict__ origins,
+                                                    const float4* __restrict__ dirs, uint32_t n_rays,
+                                                    uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                    float* __restrict__ depths, float eps,
+                                                    unsigned long long* __restrict__ counters) {
+  const uint32_t l = blockIdx.x * 256 + threadIdx.x;
+  if (l >= n_rays) return;
+  uint32_t n_box = 0, n_tri = 0;
+  const RayHit h = trace_ray<kCount, kStack>(nodes, tris, n_tris, 0u, origins[l], dirs[l], depths[l], eps, n_box,
+                                             n_tri);
   if (kCount) {
     atomicAdd(counters, (unsigned long long)n_box);
     atomicAdd(counters + 1, (unsigned long long)n_tri);
   }
-  if (best != 0xFFFFFFFFu) {
-    faces_id[l] = best;
-    coords[l] = make_float2(bu, bv);
-    depths[l] = depth;
+  if (h.pos != 0xFFFFFFFFu) {
+    faces_id[l] = h.pos;
+    coords[l] = make_float2(h.u, h.v);
+    depths[l] = h.t;
   }
 }
 
@@ -486,8 +519,9 @@ static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v
   rk_status s = rk_radix_sort_impl<K>(st, keys, order, n, 0, (int)sizeof(K), sort_ws, sort_bytes,
                                       info + RK_INFO_FLAGS);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL(k_fast_hierarchy<K>, dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st, keys, order, f,
-                     v, n, leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes), static_cast<float4*>(bvh->tris));
+  hipLaunchKernelGGL((k_fast_hierarchy<K, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st, keys,
+                     order, f, v, n, leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes),
+                     static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -522,6 +556,265 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
                                      parts, sort_ws, sort_bytes, info);
   return fast_build_keys<uint32_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint32_t*>(keys), flags,
                                    parts, sort_ws, sort_bytes, info);
+}
+
+
+// ---- batches ------------------------------------------------------------------
+
+namespace {
+
+// The mesh whose [offset, offset + count) holds g, for offsets in increasing
+// order (mesh_count >= 1): the last mesh with offset <= g.
+__device__ __forceinline__ uint32_t mesh_of_face(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
+                                                 uint32_t g) {
+  uint32_t lo = 0, hi = mesh_count - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (meshes[mid].face_offset <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t mesh_of_ray(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
+                                                uint32_t r) {
+  uint32_t lo = 0, hi = mesh_count - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (meshes[mid].ray_offset <= r) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Per-face box centroid quantised to 10 bits per axis in its mesh's hitbox,
+// keyed (mesh << 30 | code) so one sort orders every mesh at once.
+__global__ __launch_bounds__(256) void k_batch_prep(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
+                                                    const uint4* __restrict__ faces, const float4* __restrict__ verts,
+                                                    uint32_t n, const rk_aabox* __restrict__ hitboxes,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t m = mesh_of_face(meshes, mesh_count, g);
+  const uint32_t voff = meshes[m].vertex_offset;
+  const uint4 f = faces[g];
+  const float4 v0 = verts[f.x + voff], v1 = verts[f.y + voff], v2 = verts[f.z + voff];
+  const float4 mn = rk::min4(rk::min4(v0, v1), v2), mx = rk::max4(rk::max4(v0, v1), v2);
+  const rk_aabox hb = hitboxes[m];
+  const float4 rmin = make_float4(hb.min.x, hb.min.y, hb.min.z, 0.0f);
+  const float4 len = make_float4(hb.max.x - hb.min.x, hb.max.y - hb.min.y, hb.max.z - hb.min.z, 0.0f);
+  const float4 c = rk::muls4(rk::add4(mn, mx), 0.5f);
+  const float sx = 1024.0f / len.x, sy = 1024.0f / len.y, sz = 1024.0f / len.z;
+  const uint32_t qx = min(rk::f2u_sat((c.x - rmin.x) * sx), 1023u);
+  const uint32_t qy = min(rk::f2u_sat((c.y - rmin.y) * sy), 1023u);
+  const uint32_t qz = min(rk::f2u_sat((c.z - rmin.z) * sz), 1023u);
+  keys[g] = ((uint64_t)m << 30) | (uint64_t)((spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz));
+  vals[g] = g;
+}
+
+template <int kStackN>
+__device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
+                                                const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                                uint32_t n_tris, const float4* __restrict__ origins,
+                                                const float4* __restrict__ dirs, uint32_t r,
+                                                uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                float* __restrict__ depths, float eps) {
+  const uint32_t m = mesh_of_ray(meshes, mesh_count, r);
+  uint32_t nb = 0, nt = 0;
+  const RayHit h = trace_ray<false, kStackN>(nodes, tris, n_tris, meshes[m].face_offset, origins[r], dirs[r],
+                                             depths[r], eps, nb, nt);
+  if (h.pos != 0xFFFFFFFFu) {
+    faces_id[r] = h.pos;
+    coords[r] = make_float2(h.u, h.v);
+    depths[r] = h.t;
+  }
+}
+
+// One ray per thread, kWG rays per workgroup.
+template <int kWG, int kStackN>
+__global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* __restrict__ meshes,
+                                                          uint32_t mesh_count, const FNode* __restrict__ nodes,
+                                                          const float4* __restrict__ tris, uint32_t n_tris,
+                                                          const float4* __restrict__ origins,
+                                                          const float4* __restrict__ dirs, uint32_t n_rays,
+                                                          uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                          float* __restrict__ depths, float eps) {
+  const uint32_t r = blockIdx.x * kWG + threadIdx.x;
+  if (r >= n_rays) return;
+  batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths, eps);
+}
+
+// Persistent grid: every wave takes 64-ray tiles from its XCD's queue
+// (workgroups are dealt to the eight XCDs in turn, so blockIdx % 8 is the
+// XCD) and moves to the next queue once one runs dry; a queue that ran dry
+// stays dry, so after eight dry queues in a row the wave has seen all work.
+constexpr uint32_t kQueues = 8, kQueueStride = 32;  // counters 128 B apart
+
+template <int kWG, int kStackN>
+__global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* __restrict__ meshes,
+                                                           uint32_t mesh_count, const FNode* __restrict__ nodes,
+                                                           const float4* __restrict__ tris, uint32_t n_tris,
+                                                           const float4* __restrict__ origins,
+                                                           const float4* __restrict__ dirs, uint32_t n_rays,
+                                                           uint32_t* __restrict__ faces_id,
+                                                           float2* __restrict__ coords, float* __restrict__ depths,
+                                                           float eps, uint32_t* __restrict__ queues) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t n_tiles = (n_rays + 63) / 64;
+  const uint32_t per = (n_tiles + kQueues - 1) / kQueues;
+  uint32_t q = blockIdx.x % kQueues;
+  for (uint32_t dry = 0; dry < kQueues;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(queues + q * kQueueStride, 1u);
+    t = __shfl(t, 0, 64);
+    const uint32_t tile = q * per + t;
+    if (t >= per || tile >= n_tiles) {
+      q = (q + 1) % kQueues;
+      ++dry;
+      continue;
+    }
+    const uint32_t r = tile * 64 + lane;
+    if (r < n_rays)
+      batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths,
+                               eps);
+  }
+}
+
+uint32_t key_passes_for(uint32_t mesh_count) {
+  uint32_t bits = 30;
+  while (bits < 64 && (1ull << (bits - 30)) < mesh_count) ++bits;
+  return (bits + 7) / 8;
+}
+
+bool config_ok(const rk_batch_config* c) {
+  return c && c->leaf_max >= 1 && c->leaf_max <= kMaxLeaf && c->trace_mode <= 1 &&
+         (c->trace_wg == 64 || c->trace_wg == 128 || c->trace_wg == 256) &&
+         (c->stack_depth == 64 || c->stack_depth == 96);
+}
+
+}  // namespace
+
+extern "C" rk_batch_config rk_batch_config_default(void) {
+  rk_batch_config c = {};
+  c.leaf_max = 4;
+  c.trace_mode = 1;
+  c.trace_wg = 256;
+  c.stack_depth = 64;
+  return c;
+}
+
+extern "C" size_t rk_batch_build_workspace_size(uint32_t mesh_count, uint32_t face_count) {
+  (void)mesh_count;
+  size_t n = face_count ? face_count : 1;
+  size_t b = 0;
+  b += rk_align_up(n * 8);             // (mesh, code) keys
+  b += rk_align_up(n * 4);             // counters of splits that cross chunks
+  b += rk_align_up(n * sizeof(Part));  // parts exchanged at splits that cross chunks
+  b += rk_sort_workspace_size((uint32_t)n, 8);
+  return b + 4096;
+}
+
+extern "C" rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count,
+                                    const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                                    const rk_aabox* hitboxes, const rk_batch_config* config, const rk_fast_bvh* bvh,
+                                    uint32_t* info, void* ws, size_t ws_bytes) {
+  const uint32_t n = face_count;
+  if (!meshes || mesh_count == 0 || mesh_count > RK_BATCH_MAX_MESHES || n == 0 || !faces || !vertices ||
+      !hitboxes || !bvh || !info || !config_ok(config))
+    return RK_ERROR_INVALID_VALUE;
+  if (!bvh->nodes || !bvh->tris || !bvh->indices) return RK_ERROR_INVALID_VALUE;
+  if (n >= (1u << 27) || n < mesh_count) return RK_ERROR_UNSUPPORTED;
+  hipStream_t st = rk_hip_stream(stream);
+  RkArena ar(ws, ws_bytes);
+  uint64_t* keys = ar.take<uint64_t>(n);
+  uint32_t* flags = ar.take<uint32_t>(n);
+  Part* parts = ar.take<Part>(n);
+  const size_t sort_bytes = rk_sort_workspace_size(n, 8);
+  void* sort_ws = ar.take<char>(sort_bytes);
+  if (!keys || !flags || !parts || !sort_ws) return RK_ERROR_WORKSPACE;
+  rk_status s;
+  if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
+  if ((s = rk_fill_u32(stream, flags, n, 0u)) != RK_SUCCESS) return s;
+  const uint4* f = reinterpret_cast<const uint4*>(faces);
+  const float4* v = reinterpret_cast<const float4*>(vertices);
+  uint32_t* order = bvh->indices;
+  hipLaunchKernelGGL(k_batch_prep, dim3(blocks_for(n, 256)), dim3(256), 0, st, meshes, mesh_count, f, v, n, hitboxes,
+                     keys, order);
+  RK_CHECK_LAUNCH();
+  s = rk_radix_sort_impl<uint64_t>(st, keys, order, n, 0, (int)key_passes_for(mesh_count), sort_ws, sort_bytes,
+                                   info + RK_INFO_FLAGS);
+  if (s != RK_SUCCESS) return s;
+  hipLaunchKernelGGL((k_fast_hierarchy<uint64_t, true>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
+                     keys, order, f, v, n, config->leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes),
+                     static_cast<float4*>(bvh->tris), meshes);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count) {
+  (void)mesh_count;
+  (void)ray_count;
+  return rk_align_up(kQueues * kQueueStride * 4) + 256;
+}
+
+template <int kWG, int kStackN>
+static void launch_batch_trace(hipStream_t st, uint32_t mode, uint32_t grid_persistent, const rk_batch_mesh* meshes,
+                               uint32_t mesh_count, const FNode* nodes, const float4* tris, uint32_t n_tris,
+                               const float4* o, const float4* d, uint32_t n_rays, uint32_t* fid, float2* uv,
+                               float* dep, float eps, uint32_t* queues) {
+  if (mode == 0)
+    hipLaunchKernelGGL((k_batch_trace_grid<kWG, kStackN>), dim3(blocks_for(n_rays, kWG)), dim3(kWG), 0, st, meshes,
+                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps);
+  else
+    hipLaunchKernelGGL((k_batch_trace_queue<kWG, kStackN>), dim3(grid_persistent), dim3(kWG), 0, st, meshes,
+                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, queues);
+}
+
+static inline float batch_eps_of(const rk_trace_options* o) {
+  return o ? o->det_epsilon : RK_DET_EPSILON_REFERENCE;
+}
+
+extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count,
+                                    const rk_fast_bvh* bvh, const rk_float4* origins, const rk_float4* directions,
+                                    uint32_t ray_count, uint32_t* faces_id, rk_float2* coords, float* depths,
+                                    const rk_trace_options* options, const rk_batch_config* config, void* ws,
+                                    size_t ws_bytes) {
+  if (!meshes || mesh_count == 0 || mesh_count > RK_BATCH_MAX_MESHES || !bvh || !bvh->nodes || !bvh->tris ||
+      bvh->face_count == 0 || !config_ok(config))
+    return RK_ERROR_INVALID_VALUE;
+  if (ray_count == 0) return RK_SUCCESS;
+  if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  RkArena ar(ws, ws_bytes);
+  uint32_t* queues = ar.take<uint32_t>(kQueues * kQueueStride);
+  if (!queues) return RK_ERROR_WORKSPACE;
+  uint32_t grid_persistent = 0;
+  if (config->trace_mode == 1) {
+    rk_status s = rk_fill_u32(stream, queues, kQueues * kQueueStride, 0u);
+    if (s != RK_SUCCESS) return s;
+    int dev = 0, cus = 0;
+    RK_HIP(hipGetDevice(&dev));
+    RK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // 16 waves per CU, a multiple of the eight XCDs
+    grid_persistent = (uint32_t)cus * 16u * 64u / config->trace_wg;
+    grid_persistent = (grid_persistent + kQueues - 1) / kQueues * kQueues;
+  }
+  const FNode* nodes = static_cast<const FNode*>(bvh->nodes);
+  const float4* tris = static_cast<const float4*>(bvh->tris);
+  const float4* o = reinterpret_cast<const float4*>(origins);
+  const float4* d = reinterpret_cast<const float4*>(directions);
+  float2* uv = reinterpret_cast<float2*>(coords);
+  const float eps = batch_eps_of(options);
+  const uint32_t nt = bvh->face_count, mode = config->trace_mode;
+#define RK_BATCH_TRACE(WG, SD)                                                                                     \
+  launch_batch_trace<WG, SD>(st, mode, grid_persistent, meshes, mesh_count, nodes, tris, nt, o, d, ray_count,      \
+                             faces_id, uv, depths, eps, queues)
+  const bool deep = config->stack_depth == 96;
+  switch (config->trace_wg) {
+    case 64: if (deep) RK_BATCH_TRACE(64, 96); else RK_BATCH_TRACE(64, 64); break;
+    case 128: if (deep) RK_BATCH_TRACE(128, 96); else RK_BATCH_TRACE(128, 64); break;
+    default: if (deep) RK_BATCH_TRACE(256, 96); else RK_BATCH_TRACE(256, 64); break;
+  }
+#undef RK_BATCH_TRACE
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
 }
 
 static inline float det_eps_of(const rk_trace_options* o) {

@@ -1,4 +1,6 @@
 // Runtime entry points: device memory, streams, surface fills, primary rays.
+#include <new>
+
 #include "rk_device.h"
 
 extern "C" const char* rk_status_string(rk_status s) {
@@ -143,4 +145,149 @@ extern "C" rk_status rk_calc_primary_rays(rk_stream stream, int width, int heigh
                      make_float4(ray_x.x, ray_x.y, ray_x.z, ray_x.w), make_float4(ray_y.x, ray_y.y, ray_y.z, ray_y.w));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
+}
+
+// ---- graphs ------------------------------------------------------------------
+// A captured sequence of rk_* calls, replayed with one launch.
+struct rk_graph_st {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" rk_status rk_graph_capture_begin(rk_stream stream) {
+  if (!stream) return RK_ERROR_INVALID_VALUE;  // the default stream cannot be captured
+  RK_HIP(hipStreamBeginCapture(rk_hip_stream(stream), hipStreamCaptureModeThreadLocal));
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_graph_capture_end(rk_stream stream, rk_graph* graph) {
+  if (!stream || !graph) return RK_ERROR_INVALID_VALUE;
+  hipGraph_t g = nullptr;
+  RK_HIP(hipStreamEndCapture(rk_hip_stream(stream), &g));
+  hipGraphExec_t e = nullptr;
+  hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return RK_ERROR_LAUNCH;
+  }
+  rk_graph_st* h = new (std::nothrow) rk_graph_st{g, e};
+  if (!h) {
+    (void)hipGraphExecDestroy(e);
+    (void)hipGraphDestroy(g);
+    return RK_ERROR_OUT_OF_MEMORY;
+  }
+  *graph = h;
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_graph_launch(rk_graph graph, rk_stream stream) {
+  if (!graph) return RK_ERROR_INVALID_VALUE;
+  RK_HIP(hipGraphLaunch(graph->exec, rk_hip_stream(stream)));
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_graph_destroy(rk_graph graph) {
+  if (!graph) return RK_SUCCESS;
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return RK_SUCCESS;
+}
+
+// ---- autotuning -----------------------------------------------------------------
+// Candidates: leaf_max 2/4/8 for the build; for the trace, both modes, three
+// workgroup sizes and both exact stack depths. Each candidate is timed on the
+// caller's batch with HIP events on the caller's stream (median of reps).
+namespace {
+float median_of(float* v, uint32_t n) {
+  for (uint32_t i = 1; i < n; ++i)
+    for (uint32_t j = i; j > 0 && v[j - 1] > v[j]; --j) {
+      float t = v[j]; v[j] = v[j - 1]; v[j - 1] = t;
+    }
+  return v[n / 2];
+}
+}  // namespace
+
+extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count,
+                                       const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                                       const rk_aabox* hitboxes, const rk_fast_bvh* bvh, uint32_t* info,
+                                       const rk_float4* origins, const rk_float4* directions, uint32_t ray_count,
+                                       uint32_t* faces_id, rk_float2* coords, float* depths,
+                                       const rk_trace_options* options, uint32_t reps, rk_batch_config* best,
+                                       float* best_ms, void* ws, size_t ws_bytes) {
+  if (!best || reps == 0 || reps > 64) return RK_ERROR_INVALID_VALUE;
+  const size_t build_bytes = rk_batch_build_workspace_size(mesh_count, face_count);
+  if (ws_bytes < build_bytes + rk_batch_trace_workspace_size(mesh_count, ray_count)) return RK_ERROR_WORKSPACE;
+  void* trace_ws = static_cast<char*>(ws) + build_bytes;
+  const size_t trace_bytes = ws_bytes - build_bytes;
+  hipStream_t st = rk_hip_stream(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RK_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return RK_ERROR_DEVICE;
+  }
+  static const uint32_t leaves[] = {2, 4, 8};
+  static const uint32_t wgs[] = {64, 128, 256};
+  static const uint32_t stacks[] = {64, 96};
+  float samples[64];
+  float best_total = 3.4e38f, best_build = 0.0f;
+  rk_batch_config pick = rk_batch_config_default();
+  rk_status s = RK_SUCCESS;
+  auto timed = [&](auto&& call, float* out_ms) -> rk_status {
+    for (uint32_t r = 0; r < reps; ++r) {
+      if (hipEventRecord(e0, st) != hipSuccess) return RK_ERROR_DEVICE;
+      rk_status cs = call();
+      if (cs != RK_SUCCESS) return cs;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return RK_ERROR_DEVICE;
+      if (hipEventElapsedTime(&samples[r], e0, e1) != hipSuccess) return RK_ERROR_DEVICE;
+    }
+    *out_ms = median_of(samples, reps);
+    return RK_SUCCESS;
+  };
+  for (uint32_t li = 0; li < 3 && s == RK_SUCCESS; ++li) {
+    rk_batch_config c = rk_batch_config_default();
+    c.leaf_max = leaves[li];
+    float build_ms = 0.0f;
+    s = timed([&] { return rk_batch_build(stream, meshes, mesh_count, faces, vertices, face_count, hitboxes, &c, bvh,
+                                          info, ws, build_bytes); },
+              &build_ms);
+    for (uint32_t mode = 0; mode < 2 && s == RK_SUCCESS; ++mode)
+      for (uint32_t wi = 0; wi < 3 && s == RK_SUCCESS; ++wi)
+        for (uint32_t si = 0; si < 2 && s == RK_SUCCESS; ++si) {
+          c.trace_mode = mode;
+          c.trace_wg = wgs[wi];
+          c.stack_depth = stacks[si];
+          float trace_ms = 0.0f;
+          s = timed([&] {
+                rk_status fs = rk_fill_f32(stream, depths, ray_count, 3.402823466e38f);
+                if (fs != RK_SUCCESS) return fs;
+                return rk_batch_trace(stream, meshes, mesh_count, bvh, origins, directions, ray_count, faces_id,
+                                      coords, depths, options, &c, trace_ws, trace_bytes);
+              },
+              &trace_ms);
+          if (s == RK_SUCCESS && build_ms + trace_ms < best_total) {
+            best_total = build_ms + trace_ms;
+            best_build = build_ms;
+            pick = c;
+          }
+        }
+  }
+  (void)best_build;
+  // leave the outputs as the chosen configuration produces them
+  if (s == RK_SUCCESS)
+    s = rk_batch_build(stream, meshes, mesh_count, faces, vertices, face_count, hitboxes, &pick, bvh, info, ws,
+                       build_bytes);
+  if (s == RK_SUCCESS) s = rk_fill_f32(stream, depths, ray_count, 3.402823466e38f);
+  if (s == RK_SUCCESS)
+    s = rk_batch_trace(stream, meshes, mesh_count, bvh, origins, directions, ray_count, faces_id, coords, depths,
+                       options, &pick, trace_ws, trace_bytes);
+  if (s == RK_SUCCESS && hipStreamSynchronize(st) != hipSuccess) s = RK_ERROR_DEVICE;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (s == RK_SUCCESS) {
+    *best = pick;
+    if (best_ms) *best_ms = best_total;
+  }
+  return s;
 }

@@ -236,3 +236,95 @@ def build_sah(F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, lbvh: RefTre
     t = RefTree(F, boxes, indices, nodes, info)
     t.node_count = int(t.host_info()[N.RK_INFO_NODE_COUNT])
     return t
+
+
+# ---- batches -------------------------------------------------------------------
+
+def batch_config(leaf_max=4, trace_mode=1, trace_wg=256, stack_depth=64) -> N.BatchConfig:
+    c = N.BatchConfig()
+    c.leaf_max, c.trace_mode, c.trace_wg, c.stack_depth = leaf_max, trace_mode, trace_wg, stack_depth
+    return c
+
+
+class Batch:
+    """Many meshes built and traced together (rk_batch_build / rk_batch_trace).
+    F holds each mesh's faces with mesh-local vertex indices, V the meshes'
+    vertices back to back, O/D their rays back to back; 'table' is from
+    batch_table, 'hitboxes' one (2, 4) box per mesh."""
+
+    def __init__(self, table: np.ndarray, F: torch.Tensor, V: torch.Tensor, hitboxes: np.ndarray,
+                 O: torch.Tensor, D: torch.Tensor):
+        self.m = int(table.shape[0])
+        self.nf, self.nr = int(F.shape[0]), int(O.shape[0])
+        self.table = table
+        self.meshes = torch.from_numpy(table.view(np.uint8).copy()).cuda()
+        self.hitboxes = torch.from_numpy(np.ascontiguousarray(hitboxes, np.float32).reshape(self.m, 2, 4)).cuda()
+        self.F, self.V, self.O, self.D = F, V, O, D
+        self.nodes = torch.empty((self.nf, 16), dtype=_U32, device="cuda")
+        self.tris = torch.empty((self.nf, 12), dtype=torch.float32, device="cuda")
+        self.indices = torch.empty(self.nf, dtype=_U32, device="cuda")
+        self.info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
+        self.bvh = N.FastBVH(self.nodes.data_ptr(), self.tris.data_ptr(), self.indices.data_ptr(), None, self.nf, 0)
+        self.build_bytes = N.lib().rk_batch_build_workspace_size(self.m, self.nf)
+        self.trace_bytes = N.lib().rk_batch_trace_workspace_size(self.m, self.nr)
+        self.ws = workspace(self.build_bytes + self.trace_bytes)
+        self.config = N.lib().rk_batch_config_default()
+
+    def build(self, config: N.BatchConfig | None = None, stream=None):
+        c = config or self.config
+        _check(N.lib().rk_batch_build(_stream_ptr(stream), self.meshes.data_ptr(), self.m, self.F.data_ptr(),
+                                      self.V.data_ptr(), self.nf, self.hitboxes.data_ptr(), C.byref(c),
+                                      C.byref(self.bvh), self.info.data_ptr(), self.ws.data_ptr(), self.build_bytes),
+               "batch_build")
+
+    def trace(self, out, config: N.BatchConfig | None = None, det_epsilon=None, stream=None):
+        c = config or self.config
+        fid, uv, dep = out
+        _check(N.lib().rk_batch_trace(_stream_ptr(stream), self.meshes.data_ptr(), self.m, C.byref(self.bvh),
+                                      self.O.data_ptr(), self.D.data_ptr(), self.nr, fid.data_ptr(), uv.data_ptr(),
+                                      dep.data_ptr(), _opt(det_epsilon), C.byref(c),
+                                      self.ws.data_ptr() + self.build_bytes, self.trace_bytes), "batch_trace")
+        return out
+
+    def autotune(self, out, reps: int = 3, det_epsilon=None, stream=None):
+        fid, uv, dep = out
+        best, ms = N.BatchConfig(), C.c_float(0.0)
+        _check(N.lib().rk_batch_autotune(_stream_ptr(stream), self.meshes.data_ptr(), self.m, self.F.data_ptr(),
+                                         self.V.data_ptr(), self.nf, self.hitboxes.data_ptr(), C.byref(self.bvh),
+                                         self.info.data_ptr(), self.O.data_ptr(), self.D.data_ptr(), self.nr,
+                                         fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon), reps,
+                                         C.byref(best), C.byref(ms), self.ws.data_ptr(), self.ws.numel()),
+               "batch_autotune")
+        self.config = best
+        return best, ms.value
+
+
+class Graph:
+    """rk_* calls issued on 'stream' (a torch.cuda.Stream) inside 'with
+    g.capture():' are recorded once and replayed by g.launch()."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.handle = C.c_void_p()
+
+    def capture(self):
+        g = self
+
+        class _Cap:
+            def __enter__(self_):
+                _check(N.lib().rk_graph_capture_begin(_stream_ptr(g.stream)), "graph_capture_begin")
+                return g
+
+            def __exit__(self_, *exc):
+                _check(N.lib().rk_graph_capture_end(_stream_ptr(g.stream), C.byref(g.handle)), "graph_capture_end")
+                return False
+
+        return _Cap()
+
+    def launch(self):
+        _check(N.lib().rk_graph_launch(self.handle, _stream_ptr(self.stream)), "graph_launch")
+
+    def __del__(self):
+        if self.handle:
+            N.lib().rk_graph_destroy(self.handle)
+            self.handle = C.c_void_p()

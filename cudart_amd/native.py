@@ -52,6 +52,26 @@ def trace_options(det_epsilon: float | None) -> TraceOptions | None:
     return TraceOptions(float(det_epsilon), 0, (C.c_uint32 * 2)(0, 0))
 
 
+class BatchMesh(C.Structure):
+    _fields_ = [("face_offset", C.c_uint32), ("face_count", C.c_uint32), ("vertex_offset", C.c_uint32),
+                ("vertex_count", C.c_uint32), ("ray_offset", C.c_uint32), ("ray_count", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+BATCH_MESH_DTYPE = np.dtype([("face_offset", "<u4"), ("face_count", "<u4"), ("vertex_offset", "<u4"),
+                             ("vertex_count", "<u4"), ("ray_offset", "<u4"), ("ray_count", "<u4"),
+                             ("reserved", "<u4", 2)])
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("leaf_max", C.c_uint32), ("trace_mode", C.c_uint32), ("trace_wg", C.c_uint32),
+                ("stack_depth", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+    def as_dict(self):
+        return {"leaf_max": self.leaf_max, "trace_mode": self.trace_mode, "trace_wg": self.trace_wg,
+                "stack_depth": self.stack_depth}
+
+
 class FastBVH(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("indices", C.c_void_p), ("boxes", C.c_void_p),
                 ("face_count", C.c_uint32), ("reserved", C.c_uint32)]
@@ -97,6 +117,20 @@ _SIGS = {
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
     "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P]),
+    # batches, graphs, autotuning
+    "rk_batch_config_default": (BatchConfig, []),
+    "rk_batch_build_workspace_size": (_SZ, [_U32, _U32]),
+    "rk_batch_build": (_I, [_P, _P, _U32, _P, _P, _U32, _P, C.POINTER(BatchConfig), C.POINTER(FastBVH), _P, _P,
+                            _SZ]),
+    "rk_batch_trace_workspace_size": (_SZ, [_U32, _U32]),
+    "rk_batch_trace": (_I, [_P, _P, _U32, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P,
+                            C.POINTER(BatchConfig), _P, _SZ]),
+    "rk_batch_autotune": (_I, [_P, _P, _U32, _P, _P, _U32, _P, C.POINTER(FastBVH), _P, _P, _P, _U32, _P, _P, _P,
+                               _P, _U32, C.POINTER(BatchConfig), C.POINTER(C.c_float), _P, _SZ]),
+    "rk_graph_capture_begin": (_I, [_P]),
+    "rk_graph_capture_end": (_I, [_P, C.POINTER(_P)]),
+    "rk_graph_launch": (_I, [_P, _P]),
+    "rk_graph_destroy": (_I, [_P]),
     # scenes
     "rk_scene_icosphere": (_I, [_U32, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rk_scene_heightfield": (_I, [_U32, C.c_uint64, C.c_float, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),

@@ -206,6 +206,88 @@ rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const 
                                 const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
                                 float* depths, const rk_trace_options* options, uint64_t* counters);
 
+/* ---- batches: many meshes built and traced together (no reference counterpart) ----
+ * The reference builds and traces one mesh per call (CUDAMeshTree.cpp:88-103,
+ * CUDARaytracer.cpp:130-135); a caller with many meshes loops over them. A batch
+ * holds all meshes in shared arrays: faces of mesh m are faces[face_offset ..
+ * face_offset + face_count) and index the mesh's own vertices, which start at
+ * vertices[vertex_offset]; its rays are origins/directions[ray_offset .. +ray_count).
+ * Meshes are listed in order of face_offset and of ray_offset, with no gaps
+ * (face_offset[m+1] = face_offset[m] + face_count[m], the same for rays), and
+ * face_count >= 1. The batch tree is one rk_fast_bvh over all faces: mesh m's
+ * root is node face_offset[m], its nodes and triangle records lie in
+ * [face_offset, face_offset + face_count), and indices[] maps a sorted position
+ * to the batch face index. At most 4096 meshes and 2^27 faces per batch. */
+typedef struct rk_batch_mesh {
+  uint32_t face_offset, face_count;
+  uint32_t vertex_offset, vertex_count;
+  uint32_t ray_offset, ray_count;
+  uint32_t reserved[2];
+} rk_batch_mesh;
+#define RK_BATCH_MAX_MESHES 4096u
+
+/* Tuning of a batch. leaf_max in [1, 16]; trace_mode 0 = one workgroup per
+ * trace_wg rays, 1 = a persistent grid that takes 64-ray tiles from eight
+ * work queues (one per XCD, stealing when its own runs dry); trace_wg is 64,
+ * 128 or 256; stack_depth is 32 or 64 (the per-ray traversal stack). */
+typedef struct rk_batch_config {
+  uint32_t leaf_max;
+  uint32_t trace_mode;
+  uint32_t trace_wg;
+  uint32_t stack_depth;
+  uint32_t reserved[4];
+} rk_batch_config;
+/* leaf_max 4, persistent queues, 256-thread workgroups, stack 64. */
+rk_batch_config rk_batch_config_default(void);
+
+/* Builds every mesh of the batch: per-mesh 30-bit Morton keys in each mesh's
+ * hitbox (hitboxes: device, one per mesh), one radix sort of (mesh, code)
+ * keys for the whole batch, one hierarchy pass. bvh->nodes and bvh->tris hold
+ * face_count records, bvh->indices face_count entries; bvh->boxes is unused.
+ * meshes: device array of mesh_count entries. */
+size_t rk_batch_build_workspace_size(uint32_t mesh_count, uint32_t face_count);
+rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count, const rk_uint4* faces,
+                         const rk_float4* vertices, uint32_t face_count, const rk_aabox* hitboxes,
+                         const rk_batch_config* config, const rk_fast_bvh* bvh, uint32_t* info, void* ws,
+                         size_t ws_bytes);
+/* Closest hit of every ray of every mesh in that mesh's tree. faces_id receives
+ * the sorted position (original batch face = bvh->indices[position]); depths is
+ * read as tmax; a miss leaves the outputs unchanged. */
+size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count);
+rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count, const rk_fast_bvh* bvh,
+                         const rk_float4* origins, const rk_float4* directions, uint32_t ray_count, uint32_t* faces_id,
+                         rk_float2* coords, float* depths, const rk_trace_options* options,
+                         const rk_batch_config* config, void* ws, size_t ws_bytes);
+
+/* ---- graphs ---------------------------------------------------------------------
+ * Any sequence of rk_* calls issued on one non-NULL stream between begin and
+ * end is recorded into a graph (hipStreamBeginCapture) and replayed by
+ * rk_graph_launch with the same pointers and sizes. The library records no
+ * memset or host copy in the calls that can be captured (the fast and batch
+ * builds and traces, fills, sorts); rk_build_sah and the reference LBVH
+ * (which read results back) cannot be captured. */
+typedef struct rk_graph_st* rk_graph;
+rk_status rk_graph_capture_begin(rk_stream stream);
+rk_status rk_graph_capture_end(rk_stream stream, rk_graph* graph);
+rk_status rk_graph_launch(rk_graph graph, rk_stream stream);
+rk_status rk_graph_destroy(rk_graph graph);
+
+/* ---- autotuning ------------------------------------------------------------------
+ * Times candidate configurations on the caller's own batch: a build for each
+ * leaf_max in {2, 4, 8}, then each trace variant (both modes, workgroups of 64,
+ * 128 and 256, stacks of 64 and 96), each the median of 'reps' runs (1..64)
+ * timed with HIP events on 'stream', and writes the fastest build + trace pair
+ * to *best (its time to *best_ms when not NULL). Each trace starts from
+ * depths = FLT_MAX. On return the tree and outputs are those of the chosen
+ * configuration. ws_bytes >= rk_batch_build_workspace_size +
+ * rk_batch_trace_workspace_size. Synchronises 'stream'. */
+rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count,
+                            const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                            const rk_aabox* hitboxes, const rk_fast_bvh* bvh, uint32_t* info,
+                            const rk_float4* origins, const rk_float4* directions, uint32_t ray_count,
+                            uint32_t* faces_id, rk_float2* coords, float* depths, const rk_trace_options* options,
+                            uint32_t reps, rk_batch_config* best, float* best_ms, void* ws, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif
