@@ -223,6 +223,59 @@ def batch_leg(torch, dev, native, timer, steps, warmup):
     }
 
 
+def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000):
+    """Serving: 32 deforming scenes (log-uniform 1K..2M faces, alternately a
+    terrain and a soup) behind rk_server; Poisson arrivals at 'rate' per
+    second, square frames of log-uniform side 64..1024, refit or rebuild per
+    request. Latency p50/p99 is arrival to observed completion. Saturation:
+    all requests queued at once, 8 slots (continuous batching) against 1 slot
+    (one request at a time, the plain loop)."""
+    import math
+    from cudart_amd import scenes
+    rng = scenes.splitmix64(0xC5)
+    counts = scenes.log_uniform_counts(32, 1000, 2_000_000, rng)
+    pool = []
+    for k, n in enumerate(counts):
+        if k % 2 == 0:
+            V, F = native.heightfield(int(math.sqrt(int(n) / 2)) + 1, seed=0xC5 + k)
+            pool.append((V, F, True))
+        else:
+            V, F = native.soup(int(n), seed=0xC5 + k, mu=float(math.log(0.002 * math.sqrt(1e7 / int(n)))))
+            pool.append((V, F, False))
+    need = sum(native.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in pool)
+    arena = min(budget, int(need * 1.1) + (256 << 20))
+
+    def server(slots):
+        srv = dev.Server(arena, slots=slots, hbm_budget=budget, det_epsilon=HEADLINE_DET_EPSILON)
+        for V, F, z in pool:
+            srv.add_scene(V, F, z)
+        srv.run_poisson(0.0, 64, 64, 1024, 1)  # upload and build every scene once
+        return srv
+
+    srv = server(8)
+    st, _ = srv.run_poisson(rate, count, 64, 1024, 0xC5)
+    sat, _ = srv.run_poisson(0.0, 1000, 64, 1024, 2)
+    srv.close()
+    one = server(1)
+    plain, _ = one.run_poisson(0.0, 1000, 64, 1024, 2)
+    one.close()
+    return {
+        "workload": "32 deforming scenes (1K..2M faces, terrain/soup), Poisson arrivals, frames 64^2..1024^2",
+        "faces_total": int(sum(F.shape[0] for _, F, _ in pool)),
+        "rate_req_s": rate, "requests": int(st.completed),
+        "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "mean_ms": round(st.mean_ms, 3),
+        "max_ms": round(st.max_ms, 3),
+        "mrays_s": round(st.mrays_per_s, 2),
+        "rebuilds": int(st.rebuilds), "refits": int(st.refits),
+        "uploads": int(st.uploads), "evictions": int(st.evictions),
+        "resident_gb": round(st.peak_resident_bytes / 2 ** 30, 3), "budget_gb": budget / 2 ** 30,
+        "slots": 8,
+        "saturated_mrays_s": round(sat.mrays_per_s, 2),
+        "one_slot_mrays_s": round(plain.mrays_per_s, 2),
+        "served_vs_one_at_a_time": round(sat.mrays_per_s / plain.mrays_per_s, 2) if plain.mrays_per_s else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +285,7 @@ def main():
     ap.add_argument("--no-ref", action="store_true")
     ap.add_argument("--no-soup", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--profile-only", action="store_true", help="a few builds and frames, no JSON (for rocprofv3)")
     args = ap.parse_args()
@@ -372,6 +426,10 @@ def main():
     if not args.no_batch:
         log("batch of 4096 meshes: autotune, graph, batched vs plain")
         result["batch"] = batch_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
+
+    if not args.no_serve:
+        log("serving: Poisson 2000 req/s over 32 deforming scenes")
+        result["serve"] = serve_leg(dev, native)
 
     if not args.no_cpu:
         log("cpu baseline")

@@ -63,9 +63,11 @@ __global__ __launch_bounds__(256) void k_fast_prep(const uint4* __restrict__ fac
   Box b;
   b.mn = rk::min4(rk::min4(v0, v1), v2);
   b.mx = rk::max4(rk::max4(v0, v1), v2);
-  float4* o = reinterpret_cast<float4*>(boxes + k);
-  o[0] = b.mn;
-  o[1] = b.mx;
+  if (boxes) {  // the API's per-face boxes, when the caller keeps them
+    float4* o = reinterpret_cast<float4*>(boxes + k);
+    o[0] = b.mn;
+    o[1] = b.mx;
+  }
   float4 len = rk::sub4(rmax, rmin);
   float4 c = rk::muls4(rk::add4(b.mn, b.mx), 0.5f);
   if (sizeof(K) == 4) {
@@ -526,12 +528,13 @@ static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v
   return RK_SUCCESS;
 }
 
-extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+static rk_status fast_build_common(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
                                    uint32_t face_count, const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max,
-                                   const rk_fast_bvh* bvh, uint32_t* info, void* ws, size_t ws_bytes) {
+                                   const rk_fast_bvh* bvh, uint32_t* info, void* sorted_keys, void* ws,
+                                   size_t ws_bytes) {
   const uint32_t n = face_count;
   if (n == 0 || !faces || !vertices || !hitbox || !bvh || !info) return RK_ERROR_INVALID_VALUE;
-  if (!bvh->nodes || !bvh->tris || !bvh->indices || !bvh->boxes) return RK_ERROR_INVALID_VALUE;
+  if (!bvh->nodes || !bvh->tris || !bvh->indices) return RK_ERROR_INVALID_VALUE;
   if (morton_mode != RK_MORTON_30 && morton_mode != RK_MORTON_63) return RK_ERROR_INVALID_VALUE;
   if (leaf_max < 1 || leaf_max > kMaxLeaf) return RK_ERROR_INVALID_VALUE;
   if (n >= (1u << 27)) return RK_ERROR_UNSUPPORTED;
@@ -544,6 +547,7 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
   size_t sort_bytes = rk_sort_workspace_size(n, wide ? 8 : 4);
   void* sort_ws = ar.take<char>(sort_bytes);
   if (!keys || !flags || !parts || !sort_ws) return RK_ERROR_WORKSPACE;
+  if (sorted_keys) keys = sorted_keys;
   rk_status s;
   if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
   if ((s = rk_fill_u32(stream, flags, n, 0u)) != RK_SUCCESS) return s;
@@ -556,6 +560,54 @@ extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, cons
                                      parts, sort_ws, sort_bytes, info);
   return fast_build_keys<uint32_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint32_t*>(keys), flags,
                                    parts, sort_ws, sort_bytes, info);
+}
+
+extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                   uint32_t face_count, const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max,
+                                   const rk_fast_bvh* bvh, uint32_t* info, void* ws, size_t ws_bytes) {
+  return fast_build_common(stream, faces, vertices, face_count, hitbox, morton_mode, leaf_max, bvh, info, nullptr,
+                           ws, ws_bytes);
+}
+
+extern "C" rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                         uint32_t face_count, const rk_aabox* hitbox, int morton_mode,
+                                         uint32_t leaf_max, const rk_fast_bvh* bvh, uint32_t* info, void* sorted_keys,
+                                         void* ws, size_t ws_bytes) {
+  if (!sorted_keys) return RK_ERROR_INVALID_VALUE;
+  return fast_build_common(stream, faces, vertices, face_count, hitbox, morton_mode, leaf_max, bvh, info,
+                           sorted_keys, ws, ws_bytes);
+}
+
+extern "C" rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                   uint32_t face_count, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
+                                   const void* sorted_keys, void* ws, size_t ws_bytes) {
+  const uint32_t n = face_count;
+  if (n == 0 || !faces || !vertices || !bvh || !sorted_keys) return RK_ERROR_INVALID_VALUE;
+  if (!bvh->nodes || !bvh->tris || !bvh->indices) return RK_ERROR_INVALID_VALUE;
+  if (morton_mode != RK_MORTON_30 && morton_mode != RK_MORTON_63) return RK_ERROR_INVALID_VALUE;
+  if (leaf_max < 1 || leaf_max > kMaxLeaf) return RK_ERROR_INVALID_VALUE;
+  if (n >= (1u << 27)) return RK_ERROR_UNSUPPORTED;
+  hipStream_t st = rk_hip_stream(stream);
+  RkArena ar(ws, ws_bytes);
+  uint32_t* flags = ar.take<uint32_t>(n);
+  Part* parts = ar.take<Part>(n);
+  if (!flags || !parts) return RK_ERROR_WORKSPACE;
+  rk_status s = rk_fill_u32(stream, flags, n, 0u);
+  if (s != RK_SUCCESS) return s;
+  const uint4* f = reinterpret_cast<const uint4*>(faces);
+  const float4* v = reinterpret_cast<const float4*>(vertices);
+  FNode* nodes = static_cast<FNode*>(bvh->nodes);
+  float4* tris = static_cast<float4*>(bvh->tris);
+  if (morton_mode == RK_MORTON_63)
+    hipLaunchKernelGGL((k_fast_hierarchy<uint64_t, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
+                       static_cast<const uint64_t*>(sorted_keys), bvh->indices, f, v, n, leaf_max, flags, parts,
+                       nodes, tris, (const rk_batch_mesh*)nullptr);
+  else
+    hipLaunchKernelGGL((k_fast_hierarchy<uint32_t, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
+                       static_cast<const uint32_t*>(sorted_keys), bvh->indices, f, v, n, leaf_max, flags, parts,
+                       nodes, tris, (const rk_batch_mesh*)nullptr);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
 }
 
 
@@ -848,6 +900,137 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
                      reinterpret_cast<unsigned long long*>(counters));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+// ---- camera frames, deformation, frame checksums --------------------------------
+
+namespace {
+
+// Primary ray of pixel (x, y) generated in the trace kernel itself (the
+// reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
+// (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. 16x16
+// pixel tiles keep a wave's rays close together. Every pixel is written:
+// a miss gives face ~0, coords (0, 0), depth FLT_MAX.
+__global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restrict__ nodes,
+                                                           const float4* __restrict__ tris, uint32_t n_tris,
+                                                           float4 ro, float4 rd, float4 rx, float4 ry, int width,
+                                                           int height, uint32_t* __restrict__ faces_id,
+                                                           float2* __restrict__ coords, float* __restrict__ depths,
+                                                           float eps) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= width || y >= height) return;
+  const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
+  uint32_t nb = 0, nt = 0;
+  const RayHit h = trace_ray<false, kStack>(nodes, tris, n_tris, 0u, ro, d, RK_HUGE, eps, nb, nt);
+  const size_t p = (size_t)y * (size_t)width + (size_t)x;
+  const bool hit = h.pos != 0xFFFFFFFFu;
+  faces_id[p] = h.pos;
+  coords[p] = hit ? make_float2(h.u, h.v) : make_float2(0.0f, 0.0f);
+  depths[p] = hit ? h.t : RK_HUGE;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float u01_of(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
+
+// v = base + amplitude * sin(omega * time + phase_v) * axis_v; phase and axis
+// hashed from the vertex index (axis uniform on the sphere, or +z).
+__global__ __launch_bounds__(256) void k_deform(const float4* __restrict__ base, float4* __restrict__ out,
+                                                uint32_t count, float amplitude, float omega, float time,
+                                                uint64_t seed, int along_z) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t h0 = mix64(seed ^ (0x9E3779B97F4A7C15ull * ((uint64_t)i + 1)));
+  const uint64_t h1 = mix64(h0 + 0x632BE59BD9B4E019ull), h2 = mix64(h1 + 0x632BE59BD9B4E019ull);
+  const float phase = 6.2831853f * u01_of(h0);
+  float ax = 0.0f, ay = 0.0f, az = 1.0f;
+  if (!along_z) {
+    const float z = 2.0f * u01_of(h1) - 1.0f, phi = 6.2831853f * u01_of(h2);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    ax = r * cosf(phi);
+    ay = r * sinf(phi);
+    az = z;
+  }
+  const float s = amplitude * sinf(omega * time + phase);
+  const float4 b = base[i];
+  out[i] = make_float4(b.x + s * ax, b.y + s * ay, b.z + s * az, b.w);
+}
+
+// Order-independent frame checksum: the wrapping sum over rays of a hash of
+// (ray, original face id, depth, u, v bits), and the hit count. Original ids
+// make it independent of the tree the frame was traced with.
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
+                                                  const float2* __restrict__ coords,
+                                                  const float* __restrict__ depths, uint32_t count,
+                                                  const uint32_t* __restrict__ indices, uint32_t n_faces,
+                                                  unsigned long long* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint64_t h = 0, hit = 0;
+  if (i < count) {
+    const uint32_t f = faces_id[i];
+    const bool is_hit = f != 0xFFFFFFFFu && f < n_faces;
+    const uint32_t orig = is_hit ? (indices ? indices[f] : f) : 0xFFFFFFFFu;
+    const float2 c = coords[i];
+    h = mix64(((uint64_t)i << 32 | orig) ^ mix64(((uint64_t)__float_as_uint(depths[i]) << 32) ^
+                                                  ((uint64_t)__float_as_uint(c.x) * 0x100000001B3ull) ^
+                                                  __float_as_uint(c.y)));
+    hit = is_hit ? 1 : 0;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {  // lane 0 ends with the wave's sums
+    h += shfl_down64(h, d);
+    hit += shfl_down64(hit, d);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(out, (unsigned long long)h);
+    atomicAdd(out + 1, (unsigned long long)hit);
+  }
+}
+
+}  // namespace
+
+extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
+                                          int width, int height, uint32_t* faces_id, rk_float2* coords, float* depths,
+                                          const rk_trace_options* options) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !camera) return RK_ERROR_INVALID_VALUE;
+  if (width <= 0 || height <= 0 || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
+  auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
+  dim3 grid((unsigned)((width + 15) / 16), (unsigned)((height + 15) / 16));
+  hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(256), 0, rk_hip_stream(stream),
+                     static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
+                     f4(camera->origin), f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
+                     faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_deform_vertices(rk_stream stream, const rk_float4* base, rk_float4* out, uint32_t count,
+                                        float amplitude, float omega, float time, uint64_t seed, int along_z) {
+  if (count == 0) return RK_SUCCESS;
+  if (!base || !out) return RK_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(k_deform, dim3(blocks_for(count, 256)), dim3(256), 0, rk_hip_stream(stream),
+                     reinterpret_cast<const float4*>(base), reinterpret_cast<float4*>(out), count, amplitude, omega,
+                     time, seed, along_z);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords,
+                                       const float* depths, uint32_t count, const uint32_t* indices,
+                                       uint32_t face_count, uint64_t* out) {
+  if (!out) return RK_ERROR_INVALID_VALUE;
+  rk_status s = rk_fill_u32(stream, reinterpret_cast<uint32_t*>(out), 4, 0u);
+  if (s != RK_SUCCESS || count == 0) return s;
+  if (!faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(k_checksum, dim3(blocks_for(count, 256)), dim3(256), 0, rk_hip_stream(stream), faces_id,
+                     reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
+                     reinterpret_cast<unsigned long long*>(out));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

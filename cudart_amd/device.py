@@ -328,3 +328,86 @@ class Graph:
         if self.handle:
             N.lib().rk_graph_destroy(self.handle)
             self.handle = C.c_void_p()
+
+
+# ---- camera frames, deformation, serving ------------------------------------------
+
+def camera_rays(cam: np.ndarray) -> N.CameraRays:
+    """rk_camera_rays from the (4, 4) array of native.camera_lookat."""
+    c = np.ascontiguousarray(cam, np.float32)
+    return N.CameraRays(*(N.Float4(*map(float, c[k])) for k in range(4)))
+
+
+def fast_trace_camera(tree: FastTree, cam: np.ndarray, width: int, height: int, out=None, det_epsilon=None,
+                      stream=None):
+    n = width * height
+    fid, uv, dep = out if out is not None else new_outputs(n)
+    c = camera_rays(cam)
+    _check(N.lib().rk_fast_trace_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
+                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)),
+           "fast_trace_camera")
+    return fid, uv, dep
+
+
+def deform(base: torch.Tensor, amplitude: float, omega: float, time: float, seed: int, along_z: bool,
+           out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    out = torch.empty_like(base) if out is None else out
+    _check(N.lib().rk_deform_vertices(_stream_ptr(stream), base.data_ptr(), out.data_ptr(), base.shape[0], amplitude,
+                                      omega, time, seed, int(along_z)), "deform")
+    return out
+
+
+class Server:
+    """rk_server_*: a request loop over deforming scenes (see include/rk.h)."""
+
+    def __init__(self, arena_bytes: int, **cfg):
+        c = N.lib().rk_server_config_default()
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        self.h = C.c_void_p()
+        _check(N.lib().rk_server_create(C.byref(c), int(arena_bytes), C.byref(self.h)), "server_create")
+        self.scenes = []
+
+    def add_scene(self, V: np.ndarray, F: np.ndarray, along_z: bool) -> int:
+        V = np.ascontiguousarray(V, np.float32)
+        F = np.ascontiguousarray(F, np.uint32)
+        sid = N._U32()
+        _check(N.lib().rk_server_add_scene(self.h, V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0],
+                                           int(along_z), C.byref(sid)), "server_add_scene")
+        self.scenes.append((V, F, along_z))
+        return sid.value
+
+    def submit(self, scene: int, time: float, cam: np.ndarray, width: int, height: int, flags: int = 0,
+               out=None) -> int:
+        r = N.Request()
+        r.scene, r.width, r.height, r.flags, r.time = scene, width, height, flags, time
+        r.camera = camera_rays(cam)
+        r.arrival = -1.0
+        if out is not None:
+            r.faces_id, r.coords, r.depths = (x.data_ptr() for x in out)
+        t = C.c_uint64()
+        _check(N.lib().rk_server_submit(self.h, C.byref(r), C.byref(t)), "server_submit")
+        return t.value
+
+    def wait_all(self):
+        _check(N.lib().rk_server_wait_all(self.h), "server_wait_all")
+
+    def result(self, ticket: int) -> N.Result:
+        r = N.Result()
+        _check(N.lib().rk_server_result(self.h, ticket, C.byref(r)), "server_result")
+        return r
+
+    def run_poisson(self, rate: float, count: int, side_lo: int, side_hi: int, seed: int):
+        st, first = N.ServerStats(), C.c_uint64()
+        _check(N.lib().rk_server_run_poisson(self.h, rate, count, side_lo, side_hi, seed, C.byref(st),
+                                             C.byref(first)), "server_run_poisson")
+        return st, first.value
+
+    def close(self):
+        if self.h:
+            N.lib().rk_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()

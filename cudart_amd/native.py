@@ -72,6 +72,40 @@ class BatchConfig(C.Structure):
                 "stack_depth": self.stack_depth}
 
 
+class CameraRays(C.Structure):
+    _fields_ = [("origin", Float4), ("direction", Float4), ("step_x", Float4), ("step_y", Float4)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("leaf_max", C.c_uint32), ("rebuild_every", C.c_uint32),
+                ("max_faces", C.c_uint32), ("max_pixels", C.c_uint32), ("det_epsilon", C.c_float),
+                ("amplitude_frac", C.c_float), ("omega", C.c_float), ("hbm_budget", C.c_uint64),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32),
+                ("time", C.c_float), ("reserved_f", C.c_float), ("camera", CameraRays), ("arrival", C.c_double),
+                ("faces_id", C.c_void_p), ("coords", C.c_void_p), ("depths", C.c_void_p)]
+
+
+class Result(C.Structure):
+    _fields_ = [("checksum", C.c_uint64), ("hits", C.c_uint64), ("arrival", C.c_double), ("start", C.c_double),
+                ("done", C.c_double), ("scene", C.c_uint32), ("pixels", C.c_uint32), ("rebuilt", C.c_uint32),
+                ("uploaded", C.c_uint32), ("evictions", C.c_uint32), ("slot", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("rebuilds", C.c_uint64), ("refits", C.c_uint64),
+                ("uploads", C.c_uint64), ("evictions", C.c_uint64), ("rays", C.c_uint64),
+                ("first_arrival", C.c_double), ("last_done", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("mrays_per_s", C.c_double), ("peak_resident_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class FastBVH(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("indices", C.c_void_p), ("boxes", C.c_void_p),
                 ("face_count", C.c_uint32), ("reserved", C.c_uint32)]
@@ -117,6 +151,25 @@ _SIGS = {
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
     "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P]),
+    "rk_fast_build_keyed": (_I, [_P, _P, _P, _U32, _P, _I, _U32, C.POINTER(FastBVH), _P, _P, _P, _SZ]),
+    "rk_fast_refit": (_I, [_P, _P, _P, _U32, _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
+    "rk_fast_trace_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, _P]),
+    "rk_deform_vertices": (_I, [_P, _P, _P, _U32, C.c_float, C.c_float, C.c_float, C.c_uint64, _I]),
+    "rk_frame_checksum": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P]),
+    # serving
+    "rk_server_config_default": (ServerConfig, []),
+    "rk_server_create": (_I, [C.POINTER(ServerConfig), C.c_uint64, C.POINTER(_P)]),
+    "rk_server_destroy": (_I, [_P]),
+    "rk_server_scene_bytes": (_SZ, [_U32, _U32]),
+    "rk_server_add_scene": (_I, [_P, _P, _U32, _P, _U32, _I, C.POINTER(_U32)]),
+    "rk_server_now": (C.c_double, [_P]),
+    "rk_server_submit": (_I, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "rk_server_progress": (_I, [_P, C.POINTER(_U32)]),
+    "rk_server_wait_all": (_I, [_P]),
+    "rk_server_result": (_I, [_P, C.c_uint64, C.POINTER(Result)]),
+    "rk_server_stats_get": (_I, [_P, C.c_uint64, C.c_uint64, C.POINTER(ServerStats)]),
+    "rk_server_run_poisson": (_I, [_P, C.c_double, _U32, _U32, _U32, C.c_uint64, C.POINTER(ServerStats),
+                                   C.POINTER(C.c_uint64)]),
     # batches, graphs, autotuning
     "rk_batch_config_default": (BatchConfig, []),
     "rk_batch_build_workspace_size": (_SZ, [_U32, _U32]),

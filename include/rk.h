@@ -171,7 +171,8 @@ rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32
  * hit's t/u/v are bit-identical to the reference's for the same triangle; ties on t
  * go to the smallest original face id.
  *   nodes: capacity n x 64 B (root at 0); tris: n x 48 B; indices: n (sorted
- *   position -> original face id); boxes: n (API compatibility, original order).
+ *   position -> original face id); boxes: n per-face boxes in original order
+ *   (CUDAMeshTree's boxes array), or NULL to skip them.
  * morton_mode is RK_MORTON_30 or RK_MORTON_63; leaf_max in [1, 16]. */
 typedef struct rk_fast_bvh {
   void* nodes;
@@ -205,6 +206,45 @@ rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float
 rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                 const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
                                 float* depths, const rk_trace_options* options, uint64_t* counters);
+
+/* rk_fast_build that also leaves the sorted Morton keys in sorted_keys (device,
+ * face_count keys of 4 B, or 8 B for RK_MORTON_63), for rk_fast_refit. */
+rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                              const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
+                              uint32_t* info, void* sorted_keys, void* ws, size_t ws_bytes);
+/* Refit after the vertices moved: keeps the topology of the rk_fast_build_keyed
+ * that produced sorted_keys and bvh->indices (same faces, same morton_mode and
+ * leaf_max) and recomputes every node box and triangle record from 'vertices'.
+ * Workspace: rk_fast_build_workspace_size. One kernel. */
+rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                        int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh, const void* sorted_keys, void* ws,
+                        size_t ws_bytes);
+
+/* A pinhole camera's primary rays, as CUDA_calcPrimaryRays.cu:43-55 forms them:
+ * pixel (x, y) has origin 'origin' and direction (direction - x*step_x) - y*step_y. */
+typedef struct rk_camera_rays {
+  rk_float4 origin, direction, step_x, step_y;
+} rk_camera_rays;
+/* A camera frame traced without ray arrays: each pixel's primary ray is formed
+ * in the trace kernel (CUDA_calcPrimaryRays.cu:20-55 fused into the trace), tmax
+ * is FLT_MAX, and every pixel is written: a hit gives its sorted position, (u, v)
+ * and t; a miss gives 0xFFFFFFFF, (0, 0) and FLT_MAX. Outputs are row-major
+ * width x height. */
+rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera, int width,
+                               int height, uint32_t* faces_id, rk_float2* coords, float* depths,
+                               const rk_trace_options* options);
+
+/* Deforming meshes: out[v] = base[v] + amplitude * sin(omega * time + phase_v) * axis_v,
+ * with phase_v and axis_v (uniform on the sphere, or +z when along_z != 0) hashed
+ * from seed and v. out.w = base.w. */
+rk_status rk_deform_vertices(rk_stream stream, const rk_float4* base, rk_float4* out, uint32_t count, float amplitude,
+                             float omega, float time, uint64_t seed, int along_z);
+/* out[0] = wrapping sum over rays of a hash of (ray index, original face id,
+ * depth, u, v bits), out[1] = hits (faces_id != 0xFFFFFFFF). The original id is
+ * indices[faces_id] when indices is not NULL, so the sum does not depend on the
+ * tree that produced the frame. out: 2 device uint64. */
+rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords, const float* depths,
+                            uint32_t count, const uint32_t* indices, uint32_t face_count, uint64_t* out);
 
 /* ---- batches: many meshes built and traced together (no reference counterpart) ----
  * The reference builds and traces one mesh per call (CUDAMeshTree.cpp:88-103,
@@ -287,6 +327,94 @@ rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* meshes, uint3
                             const rk_float4* origins, const rk_float4* directions, uint32_t ray_count,
                             uint32_t* faces_id, rk_float2* coords, float* depths, const rk_trace_options* options,
                             uint32_t reps, rk_batch_config* best, float* best_ms, void* ws, size_t ws_bytes);
+
+/* ---- serving (no reference counterpart) --------------------------------------------
+ * A request loop over deforming scenes. Scenes are added once (host arrays, kept
+ * in pinned memory) and live in one device arena of arena_bytes (at most
+ * cfg.hbm_budget); a scene is made resident when a request needs it, evicting
+ * scenes no running request uses, least recently used first. A queued request is
+ * admitted, in arrival order, when one of cfg.slots slots (a stream with its own
+ * workspace and frame buffers) is free and its scene fits; admitted requests run
+ * concurrently. Per request: upload if just made resident, deform the scene's
+ * vertices to the request's time (amplitude cfg.amplitude_frac of the scene's
+ * extent, rk_deform_vertices), rebuild (rk_fast_build_keyed) when the scene has
+ * no tree or has been refitted cfg.rebuild_every times since its last build (0:
+ * always), else refit (rk_fast_refit), trace the camera frame
+ * (rk_fast_trace_camera) and checksum it (rk_frame_checksum). A frame depends
+ * only on the scene, time, camera and size, not on what else runs or on refit vs
+ * rebuild. The host calls are not thread-safe: one thread drives a server. */
+#define RK_REQUEST_FORCE_REBUILD 1u
+#define RK_REQUEST_FORCE_REFIT 2u  /* when the scene has a tree */
+typedef struct rk_server_config {
+  uint32_t slots;          /* requests on the GPU at once, 1..64 */
+  uint32_t leaf_max;       /* 1..16 */
+  uint32_t rebuild_every;  /* refits between rebuilds of a scene */
+  uint32_t max_faces;      /* largest scene that can be added */
+  uint32_t max_pixels;     /* largest frame rendered into the slot buffers */
+  float det_epsilon;       /* triangle test threshold */
+  float amplitude_frac;    /* deformation amplitude / scene extent */
+  float omega;             /* deformation angular rate, radians per unit of request time */
+  uint64_t hbm_budget;     /* cap on arena_bytes */
+  uint32_t reserved[4];
+} rk_server_config;
+/* 8 slots, leaf 4, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
+ * amplitude 1% of extent, omega 2 pi, budget 200 GiB. */
+rk_server_config rk_server_config_default(void);
+typedef struct rk_server_st* rk_server;
+rk_status rk_server_create(const rk_server_config* cfg, uint64_t arena_bytes, rk_server* server);
+rk_status rk_server_destroy(rk_server server);
+/* Device bytes a scene occupies while resident. */
+size_t rk_server_scene_bytes(uint32_t vertex_count, uint32_t face_count);
+/* Host arrays; deform_along_z != 0 moves vertices along +z (terrains), else along
+ * per-vertex random directions. Faces must index vertex_count vertices. */
+rk_status rk_server_add_scene(rk_server server, const rk_float4* vertices, uint32_t vertex_count,
+                              const rk_uint4* faces, uint32_t face_count, int deform_along_z, uint32_t* scene_id);
+typedef struct rk_request {
+  uint32_t scene;
+  uint32_t width, height;
+  uint32_t flags;           /* RK_REQUEST_* */
+  float time;               /* scene time: the deformation phase */
+  float reserved_f;
+  rk_camera_rays camera;
+  double arrival;           /* server clock (rk_server_now); < 0 = at submission */
+  uint32_t* faces_id;       /* optional device outputs of width*height, all three or */
+  rk_float2* coords;        /* none (then the slot's buffers; only the checksum */
+  float* depths;            /* is returned) */
+} rk_request;
+typedef struct rk_result {
+  uint64_t checksum, hits;  /* rk_frame_checksum of the frame */
+  double arrival, start, done;  /* server clock, seconds: done is when completion was observed */
+  uint32_t scene, pixels;
+  uint32_t rebuilt;         /* 1 rebuild, 0 refit */
+  uint32_t uploaded;        /* the scene was uploaded for this request */
+  uint32_t evictions;       /* scenes evicted to admit it */
+  uint32_t slot;
+} rk_result;
+double rk_server_now(rk_server server);
+/* Queues a request; *ticket identifies it. Nothing runs until progress. */
+rk_status rk_server_submit(rk_server server, const rk_request* request, uint64_t* ticket);
+/* Non-blocking: records finished requests, admits queued ones, launches their work. */
+rk_status rk_server_progress(rk_server server, uint32_t* completed);
+/* Progress until nothing is queued or running. */
+rk_status rk_server_wait_all(rk_server server);
+/* RK_ERROR_UNSUPPORTED while the request has not finished. */
+rk_status rk_server_result(rk_server server, uint64_t ticket, rk_result* result);
+typedef struct rk_server_stats {
+  uint64_t completed, rebuilds, refits, uploads, evictions, rays;
+  double first_arrival, last_done;
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* latency = done - arrival */
+  double mrays_per_s;                      /* rays / (last_done - first_arrival) */
+  uint64_t peak_resident_bytes;
+} rk_server_stats;
+/* Statistics over tickets [first_ticket, first_ticket + count). */
+rk_status rk_server_stats_get(rk_server server, uint64_t first_ticket, uint64_t count, rk_server_stats* stats);
+/* Open-loop driver: count requests with Poisson arrivals at 'rate' per second
+ * (rate 0: all arrive at once), each for a scene drawn uniformly, at time =
+ * its arrival, with a square frame of side log-uniform in [side_lo, side_hi] and
+ * a 60-degree camera at 30 degrees elevation on an orbit around the scene. Runs
+ * until all are done and returns their statistics. */
+rk_status rk_server_run_poisson(rk_server server, double rate, uint32_t count, uint32_t side_lo, uint32_t side_hi,
+                                uint64_t seed, rk_server_stats* stats, uint64_t* first_ticket);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,232 @@
+"""Camera frames, deformation + refit, and the serving loop, through the C ABI.
+A served frame must equal the oracle's closest hits for the deformed scene, be
+the same alone and in a mix of concurrent requests (with evictions), and not
+depend on refit vs rebuild."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F32MAX = np.finfo(np.float32).max
+M64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    from cudart_amd import device
+    torch.cuda.init()
+    return device
+
+
+def _t(a):
+    import torch
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    return torch.from_numpy(a).to("cuda")
+
+
+def _h(t, dtype=None):
+    import torch
+    torch.cuda.synchronize()
+    a = t.cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def _mix64(z):
+    z = np.asarray(z, np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _checksum(orig_ids, uv, dep):
+    """Host statement of the frame checksum (wrapping sum of per-ray hashes)."""
+    n = dep.shape[0]
+    i = np.arange(n, dtype=np.uint64)
+    o = np.asarray(orig_ids, np.uint32).astype(np.uint64)
+    d = dep.view(np.uint32).astype(np.uint64)
+    u = uv[:, 0].copy().view(np.uint32).astype(np.uint64)
+    v = uv[:, 1].copy().view(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        inner = _mix64((d << np.uint64(32)) ^ (u * np.uint64(0x100000001B3)) ^ v)
+        h = _mix64(((i << np.uint64(32)) | o) ^ inner)
+        total = int(h.sum(dtype=np.uint64))
+    hits = int((np.asarray(orig_ids, np.uint32) != 0xFFFFFFFF).sum())
+    return total & M64, hits
+
+
+def _camera(rk, V, side, az=0.7):
+    hb = rk.hitbox(V)
+    c = 0.5 * (hb[0, :3] + hb[1, :3])
+    diag = float(np.linalg.norm(hb[1, :3] - hb[0, :3]))
+    el = np.radians(30.0)
+    eye = c + 1.2 * diag * np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    return rk.camera_lookat(eye, c, np.radians(60.0), side, side, up=[0, 0, 1])
+
+
+def test_camera_frame_equals_ray_arrays(dev, rk):
+    V, F = rk.heightfield(128)
+    tree = dev.FastBuilder(F.shape[0], rk.RK_MORTON_30).build(_t(F), _t(V), rk.hitbox(V), 4)
+    cam = _camera(rk, V, 96)
+    O, D = rk.primary_rays_host(cam, 96, 96)
+    fid, uv, dep = (_h(x) for x in dev.fast_trace(tree, _t(O), _t(D), det_epsilon=0.0))
+    cf, cuv, cdep = (_h(x) for x in dev.fast_trace_camera(tree, cam, 96, 96, det_epsilon=0.0))
+    np.testing.assert_array_equal(cdep, dep)
+    hit = dep < F32MAX
+    assert hit.mean() > 0.2
+    np.testing.assert_array_equal(cf[hit], fid[hit])
+    np.testing.assert_array_equal(cuv[hit], uv[hit])
+    assert np.all(cf.view(np.uint32)[~hit] == 0xFFFFFFFF) and np.all(cuv[~hit] == 0)
+
+
+@pytest.mark.parametrize("kind", ["soup", "terrain"])
+def test_refit_after_deform_equals_rebuild_and_oracle(dev, rk, ora, kind):
+    import torch
+    if kind == "soup":
+        V, F = rk.soup(3000, seed=4, mu=float(np.log(0.05)))
+    else:
+        V, F = rk.heightfield(60)
+    n = F.shape[0]
+    hb = rk.hitbox(V)
+    amp = 0.05 * float((hb[1, :3] - hb[0, :3]).max())
+    grow = np.array([[-amp] * 3 + [0], [amp] * 3 + [0]], np.float32)
+    hbg = hb + grow
+    Vt, Ft = _t(V), _t(F)
+    ws = dev.workspace(rk.lib().rk_fast_build_workspace_size(n, rk.RK_MORTON_30))
+    b = dev.FastBuilder(n, rk.RK_MORTON_30)
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(rk.RK_INFO_WORDS, dtype=torch.int32, device="cuda")
+    desc = rk.FastBVH(b.nodes.data_ptr(), b.tris.data_ptr(), b.indices.data_ptr(), None, n, 0)
+    rk.check(rk.lib().rk_fast_build_keyed(None, Ft.data_ptr(), Vt.data_ptr(), n, rk.C.byref(rk.aabox(hbg)),
+                                          rk.RK_MORTON_30, 4, rk.C.byref(desc), info.data_ptr(), keys.data_ptr(),
+                                          ws.data_ptr(), ws.numel()), "build_keyed")
+    Vd = dev.deform(Vt, amp, 6.2831853, 0.37, 99, kind == "terrain")
+    rk.check(rk.lib().rk_fast_refit(None, Ft.data_ptr(), Vd.data_ptr(), n, rk.RK_MORTON_30, 4, rk.C.byref(desc),
+                                    keys.data_ptr(), ws.data_ptr(), ws.numel()), "refit")
+    refit = dev.FastTree(b.nodes, b.tris, b.indices, b.boxes, b.info, desc)
+    cam = _camera(rk, V, 64)
+    rf, ruv, rdep = (_h(x) for x in dev.fast_trace_camera(refit, cam, 64, 64, det_epsilon=0.0))
+    rorder = _h(b.indices, np.uint32).copy()
+    fresh = dev.FastBuilder(n, rk.RK_MORTON_30).build(Ft, Vd, hbg, 4)
+    ff, fuv, fdep = (_h(x) for x in dev.fast_trace_camera(fresh, cam, 64, 64, det_epsilon=0.0))
+    forder = _h(fresh.indices, np.uint32)
+    np.testing.assert_array_equal(rdep, fdep)
+    hit = fdep < F32MAX
+    assert hit.mean() > 0.1
+    np.testing.assert_array_equal(rorder[rf.view(np.uint32)[hit]], forder[ff.view(np.uint32)[hit]])
+    np.testing.assert_array_equal(ruv[hit], fuv[hit])
+    Vh = _h(Vd)
+    O, D = rk.primary_rays_host(cam, 64, 64)
+    of, ouv, od = ora.trace_bruteforce(F, Vh, O, D, det_epsilon=0.0)
+    np.testing.assert_array_equal(fdep, od)
+    np.testing.assert_array_equal(forder[ff.view(np.uint32)[hit]], of[hit])
+
+
+def _scenes(rk, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        if k % 2:
+            V, F = rk.heightfield(int(rng.integers(20, 120)), seed=seed + k)
+            out.append((V, F, True))
+        else:
+            V, F = rk.soup(int(rng.integers(1000, 30000)), seed=seed + k, mu=float(np.log(0.03)))
+            out.append((V, F, False))
+    return out
+
+
+def test_served_frames_match_oracle(dev, rk, ora):
+    import torch
+    scenes = _scenes(rk, 3, 11)
+    srv = dev.Server(1 << 30, slots=2, rebuild_every=2, det_epsilon=0.0)
+    ids = [srv.add_scene(V, F, z) for V, F, z in scenes]
+    jobs = []
+    for k, t in enumerate([0.0, 0.21, 0.45, 0.8, 1.3]):
+        s = ids[k % len(ids)]
+        V, F, z = scenes[s]
+        side = 48
+        cam = _camera(rk, V, side, az=0.3 + k)
+        out = dev.new_outputs(side * side)
+        jobs.append((srv.submit(s, t, cam, side, side, out=out), s, t, cam, side, out))
+    srv.wait_all()
+    for ticket, s, t, cam, side, out in jobs:
+        V, F, z = scenes[s]
+        res = srv.result(ticket)
+        hb = rk.hitbox(V)
+        # the server's float32 product amplitude_frac * extent
+        amp = float(np.float32(srv.cfg.amplitude_frac) * np.float32((hb[1, :3] - hb[0, :3]).max()))
+        Vd = _h(dev.deform(_t(V), amp, float(srv.cfg.omega), t, 0x5EED0000 + s, z))
+        O, D = rk.primary_rays_host(cam, side, side)
+        of, ouv, od = ora.trace_bruteforce(F, Vd, O, D, det_epsilon=0.0)
+        np.testing.assert_array_equal(_h(out[2]), od)
+        orig = np.where(od < F32MAX, of, 0xFFFFFFFF).astype(np.uint32)
+        ouv = np.where((od < F32MAX)[:, None], ouv, 0).astype(np.float32)
+        assert (res.checksum, res.hits) == _checksum(orig, ouv, od)
+        torch.cuda.synchronize()
+    srv.close()
+
+
+def test_served_frames_same_alone_and_in_mix(dev, rk):
+    """60 requests over 8 scenes on 4 slots with an arena too small for all
+    scenes (evictions), then each request alone on a 1-slot server, half of
+    them forced to refit and half to rebuild: every checksum is identical."""
+    scenes = _scenes(rk, 8, 29)
+    sizes = [rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in scenes]
+    arena = max(max(sizes), int(0.5 * sum(sizes)))
+    mix = dev.Server(arena, slots=4, rebuild_every=3, det_epsilon=0.0)
+    alone = dev.Server(2 * sum(sizes), slots=1, rebuild_every=3, det_epsilon=0.0)
+    for V, F, z in scenes:
+        mix.add_scene(V, F, z)
+        alone.add_scene(V, F, z)
+    rng = np.random.default_rng(3)
+    reqs = []
+    for k in range(60):
+        s = int(rng.integers(0, len(scenes)))
+        side = int(rng.integers(16, 96))
+        reqs.append((s, float(rng.uniform(0, 2)), _camera(rk, scenes[s][0], side, az=float(rng.uniform(0, 6.3))),
+                     side))
+    tickets = [mix.submit(s, t, cam, side, side) for s, t, cam, side in reqs]
+    mix.wait_all()
+    got = [mix.result(t) for t in tickets]
+    assert sum(r.evictions for r in got) > 0
+    assert len({r.slot for r in got}) > 1
+    for k, (s, t, cam, side) in enumerate(reqs):
+        flags = 2 if k % 2 else 1  # force refit / force rebuild
+        a = alone.submit(s, t, cam, side, side, flags=flags)
+        alone.wait_all()
+        r = alone.result(a)
+        assert (r.checksum, r.hits) == (got[k].checksum, got[k].hits), k
+    mix.close()
+    alone.close()
+
+
+def test_poisson_driver_statistics(dev, rk):
+    scenes = _scenes(rk, 6, 41)
+    sizes = [rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in scenes]
+    srv = dev.Server(max(max(sizes), int(0.6 * sum(sizes))), slots=4, max_pixels=128 * 128)
+    for V, F, z in scenes:
+        srv.add_scene(V, F, z)
+    st, first = srv.run_poisson(4000.0, 200, 16, 128, 7)
+    assert st.completed == 200 and st.rebuilds + st.refits == 200
+    assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
+    assert st.rays > 200 * 16 * 16 and st.mrays_per_s > 0
+    assert st.uploads >= len(scenes) and st.evictions > 0
+    res = [srv.result(first + k) for k in range(200)]
+    assert all(r.done >= r.start >= r.arrival for r in res)
+    srv.close()
+
+
+def test_server_rejects_what_cannot_fit(dev, rk):
+    V, F = rk.soup(5000, seed=1)
+    need = rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0])
+    srv = dev.Server(need // 2, slots=1, max_pixels=64 * 64)
+    with pytest.raises(rk.RkError):
+        srv.add_scene(V, F, False)
+    V2, F2 = rk.soup(100, seed=2)
+    s = srv.add_scene(V2, F2, False)
+    with pytest.raises(rk.RkError):
+        srv.submit(s, 0.0, _camera(rk, V2, 65), 65, 65)
+    srv.close()
