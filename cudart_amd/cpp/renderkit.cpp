@@ -548,4 +548,27 @@ void CUDARaytracer::raytraceMeshTreeAsync(CUDAMesh* mesh, CUDAMeshTree* tree, bo
                                 fid, uv, dep, coherency ? 1 : 0));
 }
 
+void CUDARaytracer::interpolateOutputSurfaces(CUDAMesh* mesh, bool points, bool normals, bool colors,
+                                              bool texcoords, bool materials) {
+  CUDABuffer* b = current_buffer;
+  const int w = b->getWidth(), h = b->getHeight();
+  if (points && !b->out_points) b->out_points = new CUDASurface<float4>(w, h);
+  if (normals && !b->out_normals) b->out_normals = new CUDASurface<float4>(w, h);
+  if (colors && !b->out_colors) b->out_colors = new CUDASurface<float4>(w, h);
+  if (texcoords && !b->out_texcoords) b->out_texcoords = new CUDASurface<float2>(w, h);
+  if (materials && !b->out_materials) b->out_materials = new CUDASurface<uint>(w, h);
+  auto p4 = [](CUDASurface<float4>* s) { return s ? reinterpret_cast<rk_float4*>(s->getPointer()) : nullptr; };
+  auto a4 = [](CUDAArray<float4>* a) { return a ? reinterpret_cast<const rk_float4*>(a->getPointer()) : nullptr; };
+  RK_CALL(rk_interpolate_data(
+      nullptr, b->getOutputFaceIdSurface()->getPointer(),
+      reinterpret_cast<const rk_float2*>(b->getOutputCoordSurface()->getPointer()), w, h,
+      reinterpret_cast<const rk_uint4*>(mesh->getFacesArray()->getPointer()), (uint32_t)mesh->getFaceCount(),
+      a4(mesh->getVerticesArray()), a4(mesh->getNormalsArray()), a4(mesh->getColorsArray()),
+      mesh->getTexcoordsArray() ? reinterpret_cast<const rk_float2*>(mesh->getTexcoordsArray()->getPointer()) : nullptr,
+      p4(b->out_points), p4(b->out_normals), p4(b->out_colors),
+      b->out_texcoords ? reinterpret_cast<rk_float2*>(b->out_texcoords->getPointer()) : nullptr,
+      b->out_materials ? b->out_materials->getPointer() : nullptr));
+  RK_CALL(rk_stream_synchronize(nullptr));  // the reference returns after the kernel (CUDA_interpolateData.cu:250)
+}
+
 }  // namespace renderkit

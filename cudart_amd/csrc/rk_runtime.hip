@@ -291,3 +291,79 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
   }
   return s;
 }
+
+// ---- attribute interpolation (kernel_interpolateData, CUDA_interpolateData.cu:45-141) ----
+// For each pixel, the hit face's attributes at barycentrics (u, v):
+// a = (1 - u) - v; attribute = (a * x0 + u * x1) + v * x2 per component, the
+// reference's float4 operator order (types/_cuda_float4.h:80-111). Normals are
+// normalised as v * (1 / sqrtf(dot(v, v))) (the reference's device normalize uses
+// the approximate rsqrtf); without vertex normals the face normal
+// normalize(cross(v1 - v0, v2 - v0)) is used. Without colours, (1, 1, 1, 1);
+// without texture coordinates, (u, v); materials are face.w. A pixel whose face
+// id is not a face of the mesh (>= face_count) is left unchanged.
+namespace {
+
+__device__ __forceinline__ float4 bary4(float a, float u, float v, float4 x0, float4 x1, float4 x2) {
+  return rk::add4(rk::add4(rk::muls4(x0, a), rk::muls4(x1, u)), rk::muls4(x2, v));
+}
+
+__device__ __forceinline__ float4 normalize4(float4 v) {
+  const float inv = 1.0f / sqrtf(rk::dot4(v, v));
+  return rk::muls4(v, inv);
+}
+
+__global__ __launch_bounds__(256) void k_interpolate(const uint32_t* __restrict__ faces_id,
+                                                     const float2* __restrict__ coords, int width, int height,
+                                                     const uint4* __restrict__ faces, uint32_t face_count,
+                                                     const float4* __restrict__ verts, const float4* __restrict__ norms,
+                                                     const float4* __restrict__ cols, const float2* __restrict__ texs,
+                                                     float4* __restrict__ out_p, float4* __restrict__ out_n,
+                                                     float4* __restrict__ out_c, float2* __restrict__ out_t,
+                                                     uint32_t* __restrict__ out_m) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= width || y >= height) return;
+  const size_t l = (size_t)y * (size_t)width + (size_t)x;
+  const uint32_t fid = faces_id[l];
+  if (fid >= face_count) return;
+  const float2 c = coords[l];
+  const uint4 f = faces[fid];
+  const float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+  const float a = (1.0f - c.x) - c.y;
+  if (out_p) out_p[l] = bary4(a, c.x, c.y, v0, v1, v2);
+  if (out_n) {
+    if (norms)
+      out_n[l] = normalize4(bary4(a, c.x, c.y, norms[f.x], norms[f.y], norms[f.z]));
+    else
+      out_n[l] = normalize4(rk::cross4(rk::sub4(v1, v0), rk::sub4(v2, v0)));
+  }
+  if (out_c) out_c[l] = cols ? bary4(a, c.x, c.y, cols[f.x], cols[f.y], cols[f.z]) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  if (out_t) {
+    if (texs) {
+      const float2 t0 = texs[f.x], t1 = texs[f.y], t2 = texs[f.z];
+      out_t[l] = make_float2((t0.x * a + t1.x * c.x) + t2.x * c.y, (t0.y * a + t1.y * c.x) + t2.y * c.y);
+    } else {
+      out_t[l] = c;
+    }
+  }
+  if (out_m) out_m[l] = f.w;
+}
+
+}  // namespace
+
+extern "C" rk_status rk_interpolate_data(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords,
+                                         int width, int height, const rk_uint4* faces, uint32_t face_count,
+                                         const rk_float4* vertices, const rk_float4* normals, const rk_float4* colors,
+                                         const rk_float2* texcoords, rk_float4* out_points, rk_float4* out_normals,
+                                         rk_float4* out_colors, rk_float2* out_texcoords, uint32_t* out_materials) {
+  if (width <= 0 || height <= 0 || !faces_id || !coords || !faces || !vertices) return RK_ERROR_INVALID_VALUE;
+  dim3 grid((unsigned)((width + 15) / 16), (unsigned)((height + 15) / 16));
+  hipLaunchKernelGGL(k_interpolate, grid, dim3(256), 0, rk_hip_stream(stream), faces_id,
+                     reinterpret_cast<const float2*>(coords), width, height, reinterpret_cast<const uint4*>(faces),
+                     face_count, reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(normals),
+                     reinterpret_cast<const float4*>(colors), reinterpret_cast<const float2*>(texcoords),
+                     reinterpret_cast<float4*>(out_points), reinterpret_cast<float4*>(out_normals),
+                     reinterpret_cast<float4*>(out_colors), reinterpret_cast<float2*>(out_texcoords), out_materials);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}

@@ -411,3 +411,21 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height: int, F: torch.Tensor,
+                V: torch.Tensor, normals=None, colors=None, texcoords=None, outs=None, stream=None):
+    """rk_interpolate_data; outs = (points, normals, colors, texcoords, materials)
+    tensors (any may be None), allocated zeroed when not given."""
+    n = width * height
+    if outs is None:
+        outs = (torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                torch.zeros((n, 2), dtype=torch.float32, device="cuda"),
+                torch.zeros(n, dtype=_U32, device="cuda"))
+    ptr = lambda t: None if t is None else t.data_ptr()
+    _check(N.lib().rk_interpolate_data(_stream_ptr(stream), faces_id.data_ptr(), coords.data_ptr(), width, height,
+                                       F.data_ptr(), F.shape[0], V.data_ptr(), ptr(normals), ptr(colors),
+                                       ptr(texcoords), *(ptr(t) for t in outs)), "interpolate_data")
+    return outs

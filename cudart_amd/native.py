@@ -170,6 +170,7 @@ _SIGS = {
     "rk_server_stats_get": (_I, [_P, C.c_uint64, C.c_uint64, C.POINTER(ServerStats)]),
     "rk_server_run_poisson": (_I, [_P, C.c_double, _U32, _U32, _U32, C.c_uint64, C.POINTER(ServerStats),
                                    C.POINTER(C.c_uint64)]),
+    "rk_interpolate_data": (_I, [_P, _P, _P, _I, _I, _P, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     # batches, graphs, autotuning
     "rk_batch_config_default": (BatchConfig, []),
     "rk_batch_build_workspace_size": (_SZ, [_U32, _U32]),

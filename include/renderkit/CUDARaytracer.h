@@ -35,6 +35,13 @@ class CUDARaytracer {
   // Stream-ordered variant for callers that manage streams (rk_stream).
   void raytraceMeshTreeAsync(CUDAMesh* mesh, CUDAMeshTree* tree, bool coherency, rk_stream stream);
 
+  // CUDARaytracer.cpp:144-161: allocates the requested output surfaces of the
+  // current buffer when missing, then interpolates the mesh's attributes at
+  // the traced hits (rk_interpolate_data). Pixels whose face id is not a face
+  // of the mesh keep their previous contents.
+  void interpolateOutputSurfaces(CUDAMesh* mesh, bool points = true, bool normals = true, bool colors = false,
+                                 bool texcoords = false, bool materials = false);
+
   // Parallel-ray threshold of the triangle test for fast trees (reference
   // trees always use the reference's 1e-6, CUDA_raytraceMeshTree.cu:36).
   void setDetEpsilon(float eps) { det_epsilon = eps; }

@@ -95,6 +95,23 @@ rk_status rk_fill_f4(rk_stream stream, rk_float4* dst, size_t count, rk_float4 v
 rk_status rk_calc_primary_rays(rk_stream stream, int width, int height, rk_float4 ray_orig, rk_float4 ray_dir,
                                rk_float4 ray_x, rk_float4 ray_y, rk_float4* origins, rk_float4* directions);
 
+/* ---- attribute interpolation --------------------------------------------------- */
+/* void CUDA_interpolateData(CUDABuffer*, CUDAMesh*), CUDARaytracer.h:46 /
+ * kernels/CUDA_interpolateData.cu:217. Per pixel of a traced frame, the hit
+ * face's position, normal, colour, texture coordinates and material at its
+ * barycentrics: a = (1 - u) - v, value = (a*x0 + u*x1) + v*x2 in the reference's
+ * order (kernels/CUDA_interpolateData.cu:45-141). normals/colors/texcoords may be
+ * NULL (mesh without them): the face normal normalize(cross(v1-v0, v2-v0)),
+ * (1,1,1,1) and (u, v) are written instead. Any out_* may be NULL (not wanted).
+ * normalize is v * (1/sqrtf(dot(v, v))), where the reference's device code uses
+ * the approximate rsqrtf. A pixel whose face id is >= face_count (a miss that
+ * kept an out-of-range id) is left unchanged; the reference reads past its faces. */
+rk_status rk_interpolate_data(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords, int width,
+                              int height, const rk_uint4* faces, uint32_t face_count, const rk_float4* vertices,
+                              const rk_float4* normals, const rk_float4* colors, const rk_float2* texcoords,
+                              rk_float4* out_points, rk_float4* out_normals, rk_float4* out_colors,
+                              rk_float2* out_texcoords, uint32_t* out_materials);
+
 /* ---- build pieces ----------------------------------------------------------- */
 /* void CUDA_calcMeshBoxes(aabox*, uint*, uint4*, float4*, uint), CUDA_calcMeshBoxes.cu:65. */
 rk_status rk_calc_mesh_boxes(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t count,

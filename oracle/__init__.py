@@ -41,6 +41,7 @@ _SIGS = {
     "ora_build_sah": (_I, [_P, _P, _U, _P, _P, _U, _U, _P, _P]),
     "ora_reorder_faces": (None, [_P, _P, _U]),
     "ora_num_threads": (_I, []),
+    "ora_interpolate": (None, [_P, _P, _U, _P, _U, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 NODE_DTYPE = np.dtype([("w0", "<u4"), ("w1", "<u4"), ("bmin", "<f4", 3), ("bmax", "<f4", 3)])
@@ -230,6 +231,26 @@ def reorder_faces(F, idx):
     F2 = np.ascontiguousarray(F, np.uint32).copy()
     lib().ora_reorder_faces(_p(F2), _p(np.ascontiguousarray(idx, np.uint32)), F2.shape[0])
     return F2
+
+
+def interpolate(face_id, coords, F, V, normals=None, colors=None, texcoords=None, init=None):
+    """Per-pixel position, normal, colour, texcoords and material of the hit
+    face (CUDA_interpolateData.cu:45-141); 'init' (p, n, c, t, m) are the
+    buffers' contents before the call (pixels without a face keep them)."""
+    n = face_id.shape[0]
+    fid = np.ascontiguousarray(face_id, np.uint32)
+    uv = np.ascontiguousarray(coords, np.float32)
+    p, nn, c, t, m = init if init is not None else (np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32),
+                                                    np.zeros((n, 4), np.float32), np.zeros((n, 2), np.float32),
+                                                    np.zeros(n, np.uint32))
+    p, nn, c, t, m = (np.ascontiguousarray(x).copy() for x in (p, nn, c, t, m))
+    opt = lambda a, dt: None if a is None else _p(np.ascontiguousarray(a, dt))
+    keep = [np.ascontiguousarray(a, np.float32) if a is not None else None for a in (normals, colors, texcoords)]
+    lib().ora_interpolate(_p(fid), _p(uv), n, _p(np.ascontiguousarray(F, np.uint32)), F.shape[0],
+                          _p(np.ascontiguousarray(V, np.float32)), None if keep[0] is None else _p(keep[0]),
+                          None if keep[1] is None else _p(keep[1]), None if keep[2] is None else _p(keep[2]),
+                          _p(p), _p(nn), _p(c), _p(t), _p(m))
+    return p, nn, c, t, m
 
 
 def num_threads() -> int:

@@ -827,6 +827,43 @@ ORA_API void ora_hit_t_f64(const U4* faces, const F4* verts, const F4* origs, co
 }
 
 // ---------------------------------------------------------------------------
+// Attribute interpolation, kernel_interpolateData CUDA_interpolateData.cu:45-141:
+// a = (1 - u) - v; value = (a*x0 + u*x1) + v*x2 componentwise (float4/float2
+// operators, types/_cuda_float4.h:80-111, types/_cuda_float2.h:42-76); normals
+// normalize(...) or, without vertex normals, normalize(cross(v1 - v0, v2 - v0));
+// colours (1,1,1,1) and texcoords (u, v) when the mesh has none; material face.w.
+// DEFINED: normalize is v * (1/sqrtf(dot)) (the device code's rsqrtf is an
+// approximation); pixels with face id >= face_count are left unchanged (the
+// reference reads past its faces).
+ORA_API void ora_interpolate(const uint32_t* faces_id, const float* coords, uint32_t count, const U4* faces,
+                             uint32_t face_count, const F4* verts, const F4* norms, const F4* cols, const float* texs,
+                             F4* out_p, F4* out_n, F4* out_c, float* out_t, uint32_t* out_m) {
+  for (uint32_t l = 0; l < count; ++l) {
+    const uint32_t fid = faces_id[l];
+    if (fid >= face_count) continue;
+    const float u = coords[2 * l], v = coords[2 * l + 1];
+    const U4 f = faces[fid];
+    const F4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+    const float a = (1.0f - u) - v;
+    auto bary = [&](F4 x0, F4 x1, F4 x2) { return add4(add4(muls(x0, a), muls(x1, u)), muls(x2, v)); };
+    if (out_p) out_p[l] = bary(v0, v1, v2);
+    if (out_n) out_n[l] = norms ? normalize4(bary(norms[f.x], norms[f.y], norms[f.z]))
+                                : normalize4(cross4(sub4(v1, v0), sub4(v2, v0)));
+    if (out_c) out_c[l] = cols ? bary(cols[f.x], cols[f.y], cols[f.z]) : F4{1.0f, 1.0f, 1.0f, 1.0f};
+    if (out_t) {
+      if (texs) {
+        out_t[2 * l] = (texs[2 * f.x] * a + texs[2 * f.y] * u) + texs[2 * f.z] * v;
+        out_t[2 * l + 1] = (texs[2 * f.x + 1] * a + texs[2 * f.y + 1] * u) + texs[2 * f.z + 1] * v;
+      } else {
+        out_t[2 * l] = u;
+        out_t[2 * l + 1] = v;
+      }
+    }
+    if (out_m) out_m[l] = f.w;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // A17/A18: binned SAH build, kernel_buildMeshTree SAH.cu:224-645 with the host
 // driver SAH.cu:672-789 and the Hybrid entry kernel_initStack SAH.cu:647-666.
 // DEFINED: items are processed breadth-first in queue order and each internal

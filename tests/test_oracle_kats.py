@@ -444,3 +444,39 @@ def test_sah_two_clusters(ora):
     assert (left["w0"] >> 1) == 20 and (right["w0"] >> 1) == 20
     assert np.all(boxes[idx[:20], 0, 0] < 50) and np.all(boxes[idx[20:], 0, 0] > 50)
     assert left["bmax"][0] == np.float32(1.01) and right["bmin"][0] == np.float32(100)
+
+
+# ---- attribute interpolation (CUDA_interpolateData.cu:45-141) ----------------------
+
+def test_interpolate_known_answers(ora):
+    V = np.array([[0, 0, 0, 0], [4, 0, 0, 0], [0, 8, 0, 0]], np.float32)
+    F = np.array([[0, 1, 2, 7], [0, 2, 1, 9]], np.uint32)  # counter-clockwise, clockwise
+    fid = np.array([0, 0, 0, 0, 1, 5], np.uint32)
+    uv = np.array([[0, 0], [1, 0], [0, 1], [0.25, 0.5], [0.25, 0.5], [0.1, 0.1]], np.float32)
+    init = (np.full((6, 4), 7, np.float32), np.full((6, 4), 7, np.float32), np.full((6, 4), 7, np.float32),
+            np.full((6, 2), 7, np.float32), np.full(6, 3, np.uint32))
+    p, n, c, t, m = ora.interpolate(fid, uv, F, V, init=init)
+    np.testing.assert_array_equal(p[:4], [V[0], V[1], V[2], [1, 4, 0, 0]])  # vertices and an exact interior point
+    np.testing.assert_array_equal(p[4], [2, 2, 0, 0])  # face 1 swaps v1, v2: 0.25*(0,8) + 0.5*(4,0)
+    np.testing.assert_array_equal(n[:4], np.tile([0, 0, 1, 0], (4, 1)))  # face normal of a CCW xy triangle
+    np.testing.assert_array_equal(n[4], [0, 0, -1, 0])
+    np.testing.assert_array_equal(c[:5], np.ones((5, 4)))  # no colours: white
+    np.testing.assert_array_equal(t[:5], uv[:5])  # no texture coordinates: (u, v)
+    np.testing.assert_array_equal(m[:5], [7, 7, 7, 7, 9])
+    for a, b in zip((p, n, c, t, m), init):  # face id out of range: untouched
+        np.testing.assert_array_equal(a[5], b[5])
+
+
+def test_interpolate_attributes(ora):
+    V = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+    F = np.array([[0, 1, 2, 0]], np.uint32)
+    normals = np.array([[0, 0, 2, 0], [0, 0, 2, 0], [0, 0, 2, 0]], np.float32)
+    colors = np.array([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1]], np.float32)
+    tex = np.array([[0, 0], [1, 0], [0, 1]], np.float32)
+    fid = np.zeros(2, np.uint32)
+    uv = np.array([[0.25, 0.25], [0.5, 0.0]], np.float32)
+    p, n, c, t, m = ora.interpolate(fid, uv, F, V, normals, colors, tex)
+    np.testing.assert_array_equal(n, [[0, 0, 1, 0], [0, 0, 1, 0]])  # equal normals normalise to themselves
+    np.testing.assert_array_equal(c, [[0.5, 0.25, 0.25, 1], [0.5, 0.5, 0, 1]])  # barycentric weights of the corners
+    np.testing.assert_array_equal(t, uv)  # texcoords equal to positions: (u, v) again
+    np.testing.assert_array_equal(p[:, :2], uv)
