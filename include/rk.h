@@ -374,8 +374,9 @@ typedef struct rk_server_config {
   uint64_t hbm_budget;     /* cap on arena_bytes */
   uint32_t reserved[4];
 } rk_server_config;
-/* 8 slots, leaf 4, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
- * amplitude 1% of extent, omega 2 pi, budget 200 GiB. */
+/* 16 slots, leaf 4, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
+ * amplitude 1% of extent, omega 2 pi, budget 200 GiB. Slots are streams: they
+ * overlap only on distinct hardware queues (GPU_MAX_HW_QUEUES, 4 by default). */
 rk_server_config rk_server_config_default(void);
 typedef struct rk_server_st* rk_server;
 rk_status rk_server_create(const rk_server_config* cfg, uint64_t arena_bytes, rk_server* server);
