@@ -30,9 +30,10 @@ from pathlib import Path
 import numpy as np
 
 # Each serving slot is a HIP stream; streams only run concurrently on distinct
-# hardware queues, and HIP opens 4 per process unless told otherwise. Set
-# before the first HIP call.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# hardware queues, and HIP opens 4 per process unless told otherwise. Raised
+# to 16 (one per serving slot) before the first HIP call.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))

@@ -964,31 +964,39 @@ __global__ __launch_bounds__(256) void k_deform(const float4* __restrict__ base,
 
 // Order-independent frame checksum: the wrapping sum over rays of a hash of
 // (ray, original face id, depth, u, v bits), and the hit count. Original ids
-// make it independent of the tree the frame was traced with.
+// make it independent of the tree the frame was traced with. A fixed grid of
+// at most kSumBlocks workgroups walks the frame, reduces in registers, waves
+// and LDS, and adds once per workgroup: same-address atomics from every wave
+// would serialise across XCDs.
+constexpr uint32_t kSumBlocks = 256;
+
 __global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
                                                   const float2* __restrict__ coords,
                                                   const float* __restrict__ depths, uint32_t count,
                                                   const uint32_t* __restrict__ indices, uint32_t n_faces,
                                                   unsigned long long* __restrict__ out) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  __shared__ uint64_t s_h[4], s_n[4];
   uint64_t h = 0, hit = 0;
-  if (i < count) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
     const uint32_t f = faces_id[i];
     const bool is_hit = f != 0xFFFFFFFFu && f < n_faces;
     const uint32_t orig = is_hit ? (indices ? indices[f] : f) : 0xFFFFFFFFu;
     const float2 c = coords[i];
-    h = mix64(((uint64_t)i << 32 | orig) ^ mix64(((uint64_t)__float_as_uint(depths[i]) << 32) ^
-                                                  ((uint64_t)__float_as_uint(c.x) * 0x100000001B3ull) ^
-                                                  __float_as_uint(c.y)));
-    hit = is_hit ? 1 : 0;
+    h += mix64(((uint64_t)i << 32 | orig) ^ mix64(((uint64_t)__float_as_uint(depths[i]) << 32) ^
+                                                   ((uint64_t)__float_as_uint(c.x) * 0x100000001B3ull) ^
+                                                   __float_as_uint(c.y)));
+    hit += is_hit ? 1 : 0;
   }
   for (int d = 32; d >= 1; d >>= 1) {  // lane 0 ends with the wave's sums
     h += shfl_down64(h, d);
     hit += shfl_down64(hit, d);
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(out, (unsigned long long)h);
-    atomicAdd(out + 1, (unsigned long long)hit);
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { s_h[w] = h; s_n[w] = hit; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(out, (unsigned long long)(s_h[0] + s_h[1] + s_h[2] + s_h[3]));
+    atomicAdd(out + 1, (unsigned long long)(s_n[0] + s_n[1] + s_n[2] + s_n[3]));
   }
 }
 
@@ -1028,7 +1036,8 @@ extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_i
   rk_status s = rk_fill_u32(stream, reinterpret_cast<uint32_t*>(out), 4, 0u);
   if (s != RK_SUCCESS || count == 0) return s;
   if (!faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_checksum, dim3(blocks_for(count, 256)), dim3(256), 0, rk_hip_stream(stream), faces_id,
+  hipLaunchKernelGGL(k_checksum, dim3(std::min(blocks_for(count, 256), kSumBlocks)), dim3(256), 0,
+                     rk_hip_stream(stream), faces_id,
                      reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
                      reinterpret_cast<unsigned long long*>(out));
   RK_CHECK_LAUNCH();
