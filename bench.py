@@ -57,7 +57,7 @@ def heightfield_scene(native, width=1920, height=1080, grid=708):
     eye = tgt + dist * np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
     cam = native.camera_lookat(eye, tgt, 60 * 0.01745329252, width, height, up=[0, 0, 1])
     O, D = native.primary_rays_host(cam, width, height)
-    return V, F, hb, O, D
+    return V, F, hb, O, D, cam
 
 
 class Timer:
@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--profile-only", action="store_true", help="a few builds and frames, no JSON (for rocprofv3)")
+    ap.add_argument("--profile-soup", action="store_true",
+                    help="the 10M-face soup: two builds and two frames, no JSON (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own "
@@ -306,9 +308,14 @@ def main():
 
     torch.cuda.set_device(0)
     timer = Timer(torch)
+    if args.profile_soup:
+        soup_leg(torch, dev, native, timer, 2, 0)
+        torch.cuda.synchronize()
+        log("profile run done")
+        return
     W, H = 1920, 1080
     log("generating scene")
-    V, F, hb, O, D = heightfield_scene(native, W, H)
+    V, F, hb, O, D, cam = heightfield_scene(native, W, H)
     n, nr = F.shape[0], O.shape[0]
     Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
     Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
@@ -358,6 +365,14 @@ def main():
     trace_ms = max(frame_ms - fill_ms, 1e-6)
     ref_eps_frame_ms = timer.time(lambda: frame(native.DET_EPSILON_REFERENCE), args.steps)
     ref_eps_hits = int((dep < f32max).sum().item())
+    # the same frame with the rays formed inside the trace kernel (8x8 pixels per wave)
+    cout = dev.new_outputs(nr)
+    cam_frame = lambda: dev.fast_trace_camera(tree, cam, W, H, out=cout, det_epsilon=HEADLINE_DET_EPSILON)
+    for _ in range(args.warmup):
+        cam_frame()
+    cam_ms = timer.time(cam_frame, args.steps)
+    frame()
+    cam_same = bool(torch.equal(cout[2], dep))
     dep.fill_(f32max)
     _, _, _, cnt = dev.fast_trace_counted(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON)
     torch.cuda.synchronize()
@@ -387,6 +402,8 @@ def main():
         "hit_fraction": round(hits / nr, 4),
         "trace_kernel_ms": round(trace_ms, 4),
         "mrays_s_reference_epsilon": round(nr / (ref_eps_frame_ms * 1e-3) / 1e6, 2),
+        "camera_fused_mrays_s": round(nr / (cam_ms * 1e-3) / 1e6, 2),
+        "camera_fused_same_depths": cam_same,
         "hit_fraction_reference_epsilon": round(ref_eps_hits / nr, 4),
         "lbvh_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
         "lbvh_build_ms": round(build_ms, 4),

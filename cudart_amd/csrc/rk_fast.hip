@@ -276,10 +276,20 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
   const uint32_t i = c0 + t;
   // leaf loads first: their latency overlaps the split-key scan
   uint32_t fid = 0;
-  uint4 f = make_uint4(0, 0, 0, 0);
+  uint32_t lo = 0, hi = n - 1, voff = 0;  // this leaf's mesh: positions [lo, hi]
+  float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0;
   if (i < c1) {
     fid = order[i];
-    f = faces[fid];
+    if (kBatch) {
+      const rk_batch_mesh m = meshes[(uint32_t)((uint64_t)keys[i] >> 30)];
+      lo = m.face_offset;
+      hi = m.face_offset + m.face_count - 1;
+      voff = m.vertex_offset;
+    }
+    const uint4 f = faces[fid];
+    v0 = verts[f.x + voff];
+    v1 = verts[f.y + voff];
+    v2 = verts[f.z + voff];
   }
   for (uint32_t k = t; k <= kChunk; k += kLocalThreads) {
     const int64_t x = (int64_t)c0 - 1 + (int64_t)k;
@@ -300,14 +310,6 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
   }
   __syncthreads();
   if (i >= c1) return;
-  uint32_t lo = 0, hi = n - 1, voff = 0;  // this leaf's mesh: positions [lo, hi]
-  if (kBatch) {
-    const rk_batch_mesh m = meshes[(uint32_t)((uint64_t)keys[i] >> 30)];
-    lo = m.face_offset;
-    hi = m.face_offset + m.face_count - 1;
-    voff = m.vertex_offset;
-  }
-  const float4 v0 = verts[f.x + voff], v1 = verts[f.y + voff], v2 = verts[f.z + voff];
   const float4 e1 = rk::sub4(v1, v0), e2 = rk::sub4(v2, v0);
   tris[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
   tris[3 * (size_t)i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
@@ -910,8 +912,7 @@ namespace {
 
 // Primary ray of pixel (x, y) generated in the trace kernel itself (the
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
-// (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. 16x16
-// pixel tiles keep a wave's rays close together. Every pixel is written:
+// (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
 // a miss gives face ~0, coords (0, 0), depth FLT_MAX.
 __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restrict__ nodes,
                                                            const float4* __restrict__ tris, uint32_t n_tris,
@@ -919,8 +920,11 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
                                                            int height, uint32_t* __restrict__ faces_id,
                                                            float2* __restrict__ coords, float* __restrict__ depths,
                                                            float eps) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  // each wave takes an 8x8 quadrant of the workgroup's 16x16 tile: rays of a
+  // wave stay close together, so they tend to visit the same nodes
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+  const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
   if (x >= width || y >= height) return;
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
