@@ -142,6 +142,19 @@ def soup_leg(torch, dev, native, timer, steps, warmup):
         frame()
     frame_ms = timer.time(frame, steps)
     hits = int((dep < f32max).sum().item())
+    ref = (fid.clone(), dep.clone())
+    # the same frame with the rays traced in origin-Morton order (key kernel +
+    # 30-bit radix sort + permuted trace, all inside the timed step)
+    sws = dev.workspace(native.lib().rk_fast_trace_sorted_workspace_size(nr))
+
+    def frame_sorted():
+        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        dev.fast_trace_sorted(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, ws=sws)
+
+    for _ in range(warmup):
+        frame_sorted()
+    sorted_ms = timer.time(frame_sorted, steps)
+    same = bool(torch.equal(fid, ref[0]) and torch.equal(dep, ref[1]))
     return {
         "workload": "soup 10M faces, 63-bit Morton, 2^24 incoherent rays, closest hit",
         "lbvh63_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
@@ -149,6 +162,9 @@ def soup_leg(torch, dev, native, timer, steps, warmup):
         "mrays_s": round(nr / (frame_ms * 1e-3) / 1e6, 2),
         "frame_ms": round(frame_ms, 3),
         "hit_fraction": round(hits / nr, 4),
+        "sorted_rays_mrays_s": round(nr / (sorted_ms * 1e-3) / 1e6, 2),
+        "sorted_rays_frame_ms": round(sorted_ms, 3),
+        "sorted_rays_same_results": same,
     }
 
 

@@ -474,14 +474,38 @@ __device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, con
   return RayHit{best, depth, bu, bv};
 }
 
+// Sort key of a ray: the 30-bit Morton code of its origin in the tree's root
+// box (the union of the root record's two child boxes), clamped to the box.
+__global__ __launch_bounds__(256) void k_ray_keys(const FNode* __restrict__ nodes, const float4* __restrict__ origins,
+                                                  uint32_t n, uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const FNode r = nodes[0];
+  const float lx = rk::fmn(r.a.x, r.b.x), hx = rk::fmx(r.a.y, r.b.y);
+  const float ly = rk::fmn(r.a.z, r.b.z), hy = rk::fmx(r.a.w, r.b.w);
+  const float lz = rk::fmn(r.c.x, r.c.z), hz = rk::fmx(r.c.y, r.c.w);
+  const float4 o = origins[i];
+  const uint32_t qx = min(rk::f2u_sat((o.x - lx) * (1024.0f / (hx - lx))), 1023u);
+  const uint32_t qy = min(rk::f2u_sat((o.y - ly) * (1024.0f / (hy - ly))), 1023u);
+  const uint32_t qz = min(rk::f2u_sat((o.z - lz) * (1024.0f / (hz - lz))), 1023u);
+  keys[i] = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
+  vals[i] = i;
+}
+
+// order (may be NULL): thread t traces ray order[t], so rays that start near
+// each other run in the same wave; results still go to the ray's own slot.
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                     uint32_t n_tris, const float4* __restrict__ origins,
                                                     const float4* __restrict__ dirs, uint32_t n_rays,
                                                     uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                     float* __restrict__ depths, float eps,
-                                                    unsigned long long* __restrict__ counters) {
-  const uint32_t l = blockIdx.x * 256 + threadIdx.x;
+                                                    unsigned long long* __restrict__ counters,
+                                                    const uint32_t* __restrict__ order) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n_rays) return;
+  const uint32_t l = order ? order[t] : t;
   if (l >= n_rays) return;
   uint32_t n_box = 0, n_tri = 0;
   const RayHit h = trace_ray<kCount, kStack>(nodes, tris, n_tris, 0u, origins[l], dirs[l], depths[l], eps, n_box,
@@ -885,7 +909,7 @@ extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, con
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
-                     (unsigned long long*)nullptr);
+                     (unsigned long long*)nullptr, (const uint32_t*)nullptr);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -901,7 +925,7 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
-                     reinterpret_cast<unsigned long long*>(counters));
+                     reinterpret_cast<unsigned long long*>(counters), (const uint32_t*)nullptr);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -1044,6 +1068,39 @@ extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_i
                      rk_hip_stream(stream), faces_id,
                      reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
                      reinterpret_cast<unsigned long long*>(out));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" size_t rk_fast_trace_sorted_workspace_size(uint32_t n_rays) {
+  const size_t n = n_rays ? n_rays : 1;
+  return rk_align_up(n * 4) * 2 + rk_sort_workspace_size((uint32_t)n, 4) + 1024;
+}
+
+extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                                          const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id,
+                                          rk_float2* coords, float* depths, const rk_trace_options* options,
+                                          void* ws, size_t ws_bytes) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
+  if (n_rays == 0) return RK_SUCCESS;
+  if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  RkArena ar(ws, ws_bytes);
+  uint32_t* keys = ar.take<uint32_t>(n_rays);
+  uint32_t* order = ar.take<uint32_t>(n_rays);
+  const size_t sort_bytes = rk_sort_workspace_size(n_rays, 4);
+  void* sort_ws = ar.take<char>(sort_bytes);
+  if (!keys || !order || !sort_ws) return RK_ERROR_WORKSPACE;
+  const FNode* nodes = static_cast<const FNode*>(bvh->nodes);
+  const float4* o = reinterpret_cast<const float4*>(origins);
+  hipLaunchKernelGGL(k_ray_keys, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, nodes, o, n_rays, keys, order);
+  RK_CHECK_LAUNCH();
+  rk_status s = rk_radix_sort_impl<uint32_t>(st, keys, order, n_rays, 0, 4, sort_ws, sort_bytes, nullptr);
+  if (s != RK_SUCCESS) return s;
+  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, nodes,
+                     static_cast<const float4*>(bvh->tris), bvh->face_count, o,
+                     reinterpret_cast<const float4*>(directions), n_rays, faces_id, reinterpret_cast<float2*>(coords),
+                     depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

@@ -429,3 +429,18 @@ def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height
                                        F.data_ptr(), F.shape[0], V.data_ptr(), ptr(normals), ptr(colors),
                                        ptr(texcoords), *(ptr(t) for t in outs)), "interpolate_data")
     return outs
+
+
+def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None,
+                      ws: torch.Tensor | None = None):
+    """rk_fast_trace_sorted: the same results as fast_trace, rays traced in
+    origin-Morton order (for incoherent rays)."""
+    n = O.shape[0]
+    fid, uv, dep = out if out is not None else new_outputs(n)
+    need = N.lib().rk_fast_trace_sorted_workspace_size(n)
+    if ws is None or ws.numel() < need:
+        ws = workspace(need)
+    _check(N.lib().rk_fast_trace_sorted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                        ws.data_ptr(), ws.numel()), "fast_trace_sorted")
+    return fid, uv, dep

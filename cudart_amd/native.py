@@ -151,6 +151,8 @@ _SIGS = {
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
     "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P]),
+    "rk_fast_trace_sorted_workspace_size": (_SZ, [_U32]),
+    "rk_fast_trace_sorted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P, _SZ]),
     "rk_fast_build_keyed": (_I, [_P, _P, _P, _U32, _P, _I, _U32, C.POINTER(FastBVH), _P, _P, _P, _SZ]),
     "rk_fast_refit": (_I, [_P, _P, _P, _U32, _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, _P]),

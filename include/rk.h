@@ -218,6 +218,16 @@ typedef struct rk_trace_options {
 rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                         const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
                         float* depths, const rk_trace_options* options);
+/* rk_fast_trace for incoherent rays: the rays are first ordered by the Morton code
+ * of their origin in the tree's root box (a key kernel and a radix sort in ws),
+ * then traced in that order, so rays of a wave start near each other and share
+ * node and triangle fetches. Results are written to each ray's own slot and are
+ * the same as rk_fast_trace's. Pays off when rays start scattered through the
+ * scene (secondary or random rays); camera rays are coherent already. */
+size_t rk_fast_trace_sorted_workspace_size(uint32_t n_rays);
+rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                               const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
+                               float* depths, const rk_trace_options* options, void* ws, size_t ws_bytes);
 /* rk_fast_trace that also adds the number of child-box tests and triangle tests
  * it performs to counters[0] and counters[1] (device, uint64). Same results. */
 rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,

@@ -228,7 +228,7 @@ def _check_fast_boxes(tree, F, V, leaf_max):
     assert seen >= max(1, (n - 1) // leaf_max // 2)
 
 
-def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None, det_epsilon=1e-6):
+def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None, det_epsilon=1e-6, sorted_rays=False):
     hb = rk.hitbox(V)
     b = dev.FastBuilder(F.shape[0], mode)
     tree = b.build(_t(F), _t(V), hb, leaf_max)
@@ -236,7 +236,8 @@ def _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, leaf_max, tmax=None, det
     fid, uv, dep = dev.new_outputs(n)
     if tmax is not None:
         dep.fill_(tmax)
-    dev.fast_trace(tree, _t(O), _t(D), out=(fid, uv, dep), det_epsilon=det_epsilon)
+    trace = dev.fast_trace_sorted if sorted_rays else dev.fast_trace
+    trace(tree, _t(O), _t(D), out=(fid, uv, dep), det_epsilon=det_epsilon)
     bf, buv, bdep = ora.trace_bruteforce(F, V, O, D, depths=None if tmax is None else np.full(n, tmax, np.float32),
                                          det_epsilon=det_epsilon)
     g_dep = _h(dep)
@@ -291,6 +292,12 @@ def test_fast_tree_soup_10m_63bit(dev, rk, ora):
     O, D = rk.random_rays(nr, seed=0xC3)
     tree = dev.FastBuilder(n, rk.RK_MORTON_63).build(_t(F), _t(V), hb, 4)
     fid, uv, dep = dev.fast_trace(tree, _t(O), _t(D))
+    # the same rays traced in origin-Morton order give the same results
+    sf, suv, sdep = dev.fast_trace_sorted(tree, _t(O), _t(D))
+    np.testing.assert_array_equal(_h(sf), _h(fid))
+    np.testing.assert_array_equal(_h(suv), _h(uv))
+    np.testing.assert_array_equal(_h(sdep), _h(dep))
+    del sf, suv, sdep
     order = _h(tree.indices, np.uint32)
     ob, _ = ora.calc_boxes(F, V)
     codes = ora.morton_wide(ob, hb, 21)
@@ -323,6 +330,41 @@ def test_fast_tree_closest_hit_soup(dev, rk, ora, mode):
     V, F = rk.soup(3000, seed=9, mu=float(np.log(0.05)))
     O, D = rk.random_rays(4000, seed=10)
     assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, 4) > 0.05
+
+
+@pytest.mark.parametrize("nr", [1, 255, 257, 4000])
+def test_fast_trace_sorted_closest_hit_soup(dev, rk, ora, nr):
+    """Rays traced in origin-Morton order (a key kernel, a radix sort, then
+    the trace through the permutation) hit what brute force hits, at ragged
+    ray counts."""
+    V, F = rk.soup(3000, seed=9, mu=float(np.log(0.05)))
+    O, D = rk.random_rays(nr, seed=10)
+    _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 2, 4, sorted_rays=True)
+
+
+def test_fast_trace_sorted_tmax_and_single_face(dev, rk, ora):
+    V, F = _scene(rk, "hf128")
+    O, D = _camera_rays(rk, "hf128", 80, 60)
+    _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4, tmax=1.2, sorted_rays=True)
+    V = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+    F = np.array([[0, 1, 2, 0]], np.uint32)
+    O = np.array([[0.25, 0.25, 1, 0], [2, 2, 1, 0], [0.1, 0.1, -1, 0]], np.float32)
+    D = np.array([[0, 0, -1, 0], [0, 0, -1, 0], [0, 0, 1, 0]], np.float32)
+    _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, 4, sorted_rays=True)
+
+
+def test_fast_trace_sorted_workspace_too_small(dev, rk):
+    import torch
+    V, F = rk.soup(100, seed=3)
+    tree = dev.FastBuilder(100, 1).build(_t(F), _t(V), rk.hitbox(V), 4)
+    O, D = rk.random_rays(1000, seed=4)
+    ws = torch.empty(64, dtype=torch.uint8, device="cuda")
+    Ot, Dt = _t(O), _t(D)
+    fid, uv, dep = dev.new_outputs(1000)
+    st = dev.N.lib().rk_fast_trace_sorted(None, dev.C.byref(tree.desc), Ot.data_ptr(), Dt.data_ptr(), 1000,
+                                          fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), None, ws.data_ptr(),
+                                          ws.numel())
+    assert st == 4  # RK_ERROR_WORKSPACE
 
 
 def test_fast_tree_heightfield_and_tmax(dev, rk, ora):
