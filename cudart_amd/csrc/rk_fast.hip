@@ -687,6 +687,33 @@ __global__ __launch_bounds__(256) void k_batch_prep(const rk_batch_mesh* __restr
   vals[g] = g;
 }
 
+// Sort key of ray r: (mesh << 30 | 30-bit Morton code of its origin in the
+// mesh's root box), so one sort orders the rays of every mesh by where they
+// start and keeps each mesh's rays together. A mesh of one face has no root
+// record and keys all its rays alike.
+__global__ __launch_bounds__(256) void k_batch_ray_keys(const rk_batch_mesh* __restrict__ meshes,
+                                                        uint32_t mesh_count, const FNode* __restrict__ nodes,
+                                                        const float4* __restrict__ origins, uint32_t n,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t m = mesh_of_ray(meshes, mesh_count, r);
+  uint32_t code = 0;
+  if (meshes[m].face_count >= 2) {
+    const FNode root = nodes[meshes[m].face_offset];
+    const float lx = rk::fmn(root.a.x, root.b.x), hx = rk::fmx(root.a.y, root.b.y);
+    const float ly = rk::fmn(root.a.z, root.b.z), hy = rk::fmx(root.a.w, root.b.w);
+    const float lz = rk::fmn(root.c.x, root.c.z), hz = rk::fmx(root.c.y, root.c.w);
+    const float4 o = origins[r];
+    const uint32_t qx = min(rk::f2u_sat((o.x - lx) * (1024.0f / (hx - lx))), 1023u);
+    const uint32_t qy = min(rk::f2u_sat((o.y - ly) * (1024.0f / (hy - ly))), 1023u);
+    const uint32_t qz = min(rk::f2u_sat((o.z - lz) * (1024.0f / (hz - lz))), 1023u);
+    code = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
+  }
+  keys[r] = ((uint64_t)m << 30) | code;
+  vals[r] = r;
+}
+
 template <int kStackN>
 __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
                                                 const FNode* __restrict__ nodes, const float4* __restrict__ tris,
@@ -713,8 +740,11 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
                                                           const float4* __restrict__ origins,
                                                           const float4* __restrict__ dirs, uint32_t n_rays,
                                                           uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                          float* __restrict__ depths, float eps) {
-  const uint32_t r = blockIdx.x * kWG + threadIdx.x;
+                                                          float* __restrict__ depths, float eps,
+                                                          const uint32_t* __restrict__ order) {
+  const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+  if (t >= n_rays) return;
+  const uint32_t r = order ? order[t] : t;
   if (r >= n_rays) return;
   batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths, eps);
 }
@@ -733,7 +763,8 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
                                                            const float4* __restrict__ dirs, uint32_t n_rays,
                                                            uint32_t* __restrict__ faces_id,
                                                            float2* __restrict__ coords, float* __restrict__ depths,
-                                                           float eps, uint32_t* __restrict__ queues) {
+                                                           float eps, uint32_t* __restrict__ queues,
+                                                           const uint32_t* __restrict__ order) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t n_tiles = (n_rays + 63) / 64;
   const uint32_t per = (n_tiles + kQueues - 1) / kQueues;
@@ -748,7 +779,8 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
       ++dry;
       continue;
     }
-    const uint32_t r = tile * 64 + lane;
+    const uint32_t p = tile * 64 + lane;
+    const uint32_t r = p < n_rays ? (order ? order[p] : p) : n_rays;
     if (r < n_rays)
       batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths,
                                eps);
@@ -764,7 +796,7 @@ uint32_t key_passes_for(uint32_t mesh_count) {
 bool config_ok(const rk_batch_config* c) {
   return c && c->leaf_max >= 1 && c->leaf_max <= kMaxLeaf && c->trace_mode <= 1 &&
          (c->trace_wg == 64 || c->trace_wg == 128 || c->trace_wg == 256) &&
-         (c->stack_depth == 64 || c->stack_depth == 96);
+         (c->stack_depth == 64 || c->stack_depth == 96) && c->ray_order <= 1;
 }
 
 }  // namespace
@@ -828,21 +860,23 @@ extern "C" rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshe
 
 extern "C" size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count) {
   (void)mesh_count;
-  (void)ray_count;
-  return rk_align_up(kQueues * kQueueStride * 4) + 256;
+  const size_t n = ray_count ? ray_count : 1;
+  // queue counters, then (ray_order 1) the ray keys, the ray order and the sort's own space
+  return rk_align_up(kQueues * kQueueStride * 4) + rk_align_up(n * 8) + rk_align_up(n * 4) +
+         rk_sort_workspace_size((uint32_t)n, 8) + 1024;
 }
 
 template <int kWG, int kStackN>
 static void launch_batch_trace(hipStream_t st, uint32_t mode, uint32_t grid_persistent, const rk_batch_mesh* meshes,
                                uint32_t mesh_count, const FNode* nodes, const float4* tris, uint32_t n_tris,
                                const float4* o, const float4* d, uint32_t n_rays, uint32_t* fid, float2* uv,
-                               float* dep, float eps, uint32_t* queues) {
+                               float* dep, float eps, uint32_t* queues, const uint32_t* order) {
   if (mode == 0)
     hipLaunchKernelGGL((k_batch_trace_grid<kWG, kStackN>), dim3(blocks_for(n_rays, kWG)), dim3(kWG), 0, st, meshes,
-                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps);
+                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, order);
   else
     hipLaunchKernelGGL((k_batch_trace_queue<kWG, kStackN>), dim3(grid_persistent), dim3(kWG), 0, st, meshes,
-                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, queues);
+                       mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, queues, order);
 }
 
 static inline float batch_eps_of(const rk_trace_options* o) {
@@ -881,9 +915,23 @@ extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshe
   float2* uv = reinterpret_cast<float2*>(coords);
   const float eps = batch_eps_of(options);
   const uint32_t nt = bvh->face_count, mode = config->trace_mode;
+  uint32_t* order = nullptr;
+  if (config->ray_order == 1) {
+    uint64_t* keys = ar.take<uint64_t>(ray_count);
+    order = ar.take<uint32_t>(ray_count);
+    const size_t sort_bytes = rk_sort_workspace_size(ray_count, 8);
+    void* sort_ws = ar.take<char>(sort_bytes);
+    if (!keys || !order || !sort_ws) return RK_ERROR_WORKSPACE;
+    hipLaunchKernelGGL(k_batch_ray_keys, dim3(blocks_for(ray_count, 256)), dim3(256), 0, st, meshes, mesh_count,
+                       nodes, o, ray_count, keys, order);
+    RK_CHECK_LAUNCH();
+    rk_status s = rk_radix_sort_impl<uint64_t>(st, keys, order, ray_count, 0, (int)key_passes_for(mesh_count),
+                                               sort_ws, sort_bytes, nullptr);
+    if (s != RK_SUCCESS) return s;
+  }
 #define RK_BATCH_TRACE(WG, SD)                                                                                     \
   launch_batch_trace<WG, SD>(st, mode, grid_persistent, meshes, mesh_count, nodes, tris, nt, o, d, ray_count,      \
-                             faces_id, uv, depths, eps, queues)
+                             faces_id, uv, depths, eps, queues, order)
   const bool deep = config->stack_depth == 96;
   switch (config->trace_wg) {
     case 64: if (deep) RK_BATCH_TRACE(64, 96); else RK_BATCH_TRACE(64, 64); break;

@@ -252,9 +252,11 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
     s = timed([&] { return rk_batch_build(stream, meshes, mesh_count, faces, vertices, face_count, hitboxes, &c, bvh,
                                           info, ws, build_bytes); },
               &build_ms);
+    for (uint32_t oi = 0; oi < 2 && s == RK_SUCCESS; ++oi)
     for (uint32_t mode = 0; mode < 2 && s == RK_SUCCESS; ++mode)
       for (uint32_t wi = 0; wi < 3 && s == RK_SUCCESS; ++wi)
         for (uint32_t si = 0; si < 2 && s == RK_SUCCESS; ++si) {
+          c.ray_order = oi;
           c.trace_mode = mode;
           c.trace_wg = wgs[wi];
           c.stack_depth = stacks[si];

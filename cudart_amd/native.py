@@ -65,11 +65,11 @@ BATCH_MESH_DTYPE = np.dtype([("face_offset", "<u4"), ("face_count", "<u4"), ("ve
 
 class BatchConfig(C.Structure):
     _fields_ = [("leaf_max", C.c_uint32), ("trace_mode", C.c_uint32), ("trace_wg", C.c_uint32),
-                ("stack_depth", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("stack_depth", C.c_uint32), ("ray_order", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self):
         return {"leaf_max": self.leaf_max, "trace_mode": self.trace_mode, "trace_wg": self.trace_wg,
-                "stack_depth": self.stack_depth}
+                "stack_depth": self.stack_depth, "ray_order": self.ray_order}
 
 
 class CameraRays(C.Structure):

@@ -296,15 +296,19 @@ typedef struct rk_batch_mesh {
 /* Tuning of a batch. leaf_max in [1, 16]; trace_mode 0 = one workgroup per
  * trace_wg rays, 1 = a persistent grid that takes 64-ray tiles from eight
  * work queues (one per XCD, stealing when its own runs dry); trace_wg is 64,
- * 128 or 256; stack_depth is 32 or 64 (the per-ray traversal stack). */
+ * 128 or 256; stack_depth is 64 or 96 (the per-ray traversal stack); ray_order
+ * 0 traces the rays as given, 1 first sorts them (within their mesh) by the
+ * Morton code of their origin in the mesh's root box, for incoherent rays; the
+ * results are the same either way. */
 typedef struct rk_batch_config {
   uint32_t leaf_max;
   uint32_t trace_mode;
   uint32_t trace_wg;
   uint32_t stack_depth;
-  uint32_t reserved[4];
+  uint32_t ray_order;
+  uint32_t reserved[3];
 } rk_batch_config;
-/* leaf_max 4, persistent queues, 256-thread workgroups, stack 64. */
+/* leaf_max 4, persistent queues, 256-thread workgroups, stack 64, rays as given. */
 rk_batch_config rk_batch_config_default(void);
 
 /* Builds every mesh of the batch: per-mesh 30-bit Morton keys in each mesh's
@@ -341,8 +345,8 @@ rk_status rk_graph_destroy(rk_graph graph);
 
 /* ---- autotuning ------------------------------------------------------------------
  * Times candidate configurations on the caller's own batch: a build for each
- * leaf_max in {2, 4, 8}, then each trace variant (both modes, workgroups of 64,
- * 128 and 256, stacks of 64 and 96), each the median of 'reps' runs (1..64)
+ * leaf_max in {2, 4, 8}, then each trace variant (rays as given or sorted,
+ * both modes, workgroups of 64, 128 and 256, stacks of 64 and 96), each the median of 'reps' runs (1..64)
  * timed with HIP events on 'stream', and writes the fastest build + trace pair
  * to *best (its time to *best_ms when not NULL). Each trace starts from
  * depths = FLT_MAX. On return the tree and outputs are those of the chosen

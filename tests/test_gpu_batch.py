@@ -87,14 +87,15 @@ def test_batch_every_trace_config_matches_oracle(dev, rk, ora, leaf_max):
     table, meshes, F, V, hb, O, D = _small_batch(rk)
     b = dev.Batch(table, _t(F), _t(V), hb, _t(O), _t(D))
     ref = _oracle_per_mesh(ora, table, meshes, O, D, 0.0)
-    for mode in (0, 1):
-        for wg in (64, 128, 256):
-            for stack in (64, 96):
-                c = dev.batch_config(leaf_max, mode, wg, stack)
-                b.build(c)
-                out = dev.new_outputs(O.shape[0])
-                b.trace(out, c, det_epsilon=0.0)
-                assert _check(b, out, ref) > 0.15, (mode, wg, stack)  # 0.209 of these rays hit
+    for ray_order in (0, 1):
+        for mode in (0, 1):
+            for wg in (64, 128, 256):
+                for stack in (64, 96):
+                    c = dev.batch_config(leaf_max, mode, wg, stack, ray_order)
+                    b.build(c)
+                    out = dev.new_outputs(O.shape[0])
+                    b.trace(out, c, det_epsilon=0.0)
+                    assert _check(b, out, ref) > 0.15, (ray_order, mode, wg, stack)  # 0.209 of these rays hit
 
 
 def test_batch_equals_plain_per_mesh(dev, rk):
@@ -122,13 +123,16 @@ def test_batch_equals_plain_per_mesh(dev, rk):
         torch.cuda.synchronize()
 
 
-def test_batch_graph_replays_match_eager(dev, rk):
+@pytest.mark.parametrize("ray_order", [0, 1])
+def test_batch_graph_replays_match_eager(dev, rk, ray_order):
     """Build + trace captured once; each replay, with eager work and
-    synchronisation in between, reproduces the eager results."""
+    synchronisation in between, reproduces the eager results (with
+    ray_order 1 the ray sort is captured too)."""
     import torch
     from cudart_amd import native, scenes
     table, F, V, hb, O, D = scenes.soup_batch(64, seed=91, faces=(64, 4096), rays=(64, 1024))
     b = dev.Batch(table, _t(F), _t(V), hb, _t(O), _t(D))
+    b.config = dev.batch_config(ray_order=ray_order)
     nr = O.shape[0]
     b.build()
     eager = b.trace(dev.new_outputs(nr), det_epsilon=0.0)
@@ -169,7 +173,7 @@ def test_batch_autotune(dev, rk):
     out = dev.new_outputs(O.shape[0])
     best, ms = b.autotune(out, reps=2, det_epsilon=0.0)
     assert best.leaf_max in (2, 4, 8) and best.trace_mode in (0, 1) and best.trace_wg in (64, 128, 256)
-    assert best.stack_depth in (64, 96) and ms > 0
+    assert best.stack_depth in (64, 96) and best.ray_order in (0, 1) and ms > 0
     got = [_h(x) for x in out]
     np.testing.assert_array_equal(got[2], ref[2])  # depths do not depend on the tree
 
@@ -183,6 +187,12 @@ def test_batch_4096_meshes_sampled(dev, rk, ora):
     b = dev.Batch(table, _t(F), _t(V), hb, _t(O), _t(D))
     b.build()
     fid, uv, dep = (_h(x) for x in b.trace(dev.new_outputs(O.shape[0])))
+    # rays sorted by origin within each mesh: the same results
+    sf, suv, sdep = (_h(x) for x in b.trace(dev.new_outputs(O.shape[0]), dev.batch_config(ray_order=1)))
+    np.testing.assert_array_equal(sf, fid)
+    np.testing.assert_array_equal(suv, uv)
+    np.testing.assert_array_equal(sdep, dep)
+    del sf, suv, sdep
     order = _h(b.indices, np.uint32)
     hits = 0
     for m in range(0, table.shape[0], 128):
