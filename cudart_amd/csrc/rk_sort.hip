@@ -1,12 +1,19 @@
 // Stable LSD radix sort for gfx950, 8-bit digits, onesweep style:
 //   1. one histogram kernel counts the digits of every pass in a single read;
-//   2. one scatter kernel per pass. Each 256-thread workgroup takes a 4096-key
-//      tile from a dynamic tile counter, ranks its keys stably in LDS with
-//      wave64 ballots, publishes its per-digit counts, resolves its global
-//      per-digit prefix by decoupled look-back over earlier tiles, reorders the
-//      tile by digit in LDS and writes it out coalesced.
-// Look-back waits are bounded (kSpinLimit); on expiry the kernel sets an error
-// bit and finishes, so a fault in the protocol cannot hang the GPU.
+//   2. one scatter kernel per pass. Each 512-thread workgroup sorts the
+//      8192-key tile of its block index: it ranks its keys stably with
+//      per-(wave, digit) lane masks in LDS, publishes its per-digit counts,
+//      resolves its global per-digit prefix by decoupled look-back over
+//      earlier tiles, reorders the tile by digit in LDS and writes it out in
+//      per-digit runs, keys first, then values through the same LDS.
+// Tiles follow the block index rather than a counter that every workgroup
+// increments (on MI355X such a counter serialises the workgroups' start
+// across the eight XCDs). A workgroup only waits on lower tiles; workgroups
+// are dispatched in block order on each XCD, so the lowest unfinished tile is
+// always resident or next to be dispatched on its XCD, whose earlier
+// workgroups have all finished. Look-back waits are bounded (kSpinLimit); on
+// expiry the kernel sets an error bit and finishes, so a fault in the
+// protocol cannot hang the GPU.
 //
 // The reference's sorts (CUDA_radixSortGeneral.cu, CUDA_radixSort.cu,
 // CUDA_bitonicSortGeneral.cu, CUDA_bitonicSort.cu, CUDA_bubbleSort512.cu)
@@ -16,18 +23,27 @@
 
 namespace {
 
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;       // 256
-constexpr int kItems = 16;                  // keys per lane
-constexpr int kWaveTile = 64 * kItems;      // 1024
-constexpr int kTile = kThreads * kItems;    // 4096
+#ifndef RK_SCATTER_WAVES
+#define RK_SCATTER_WAVES 8
+#endif
+#ifndef RK_SCATTER_ITEMS
+#define RK_SCATTER_ITEMS 16
+#endif
+constexpr int kWaves = RK_SCATTER_WAVES;    // waves per scatter workgroup
+constexpr int kThreads = 64 * kWaves;
+constexpr int kItems = RK_SCATTER_ITEMS;    // keys per lane
+constexpr int kWaveTile = 64 * kItems;
+constexpr int kTile = kThreads * kItems;    // keys per tile
+static_assert(kThreads >= 256, "one thread per digit");
 constexpr uint32_t kAgg = 1u << 30;
 constexpr uint32_t kInc = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 20;
 constexpr int kMaxPasses = 8;
+constexpr int kLook = 8;  // look-back status loads in flight per digit
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan over the first 256 threads of the block (one value per
+// digit); every thread of the block calls it, threads 256.. get no result.
 __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tmp /*>= 4*/) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t incl = v;
@@ -36,25 +52,53 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_t
     uint32_t y = __shfl_up(incl, off, 64);
     if (lane >= (uint32_t)off) incl += y;
   }
-  if (lane == 63) s_tmp[wave] = incl;
+  if (lane == 63 && wave < 4) s_tmp[wave] = incl;
   __syncthreads();
   uint32_t wprefix = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) wprefix += (w < (int)wave) ? s_tmp[w] : 0u;
+  for (int w = 0; w < 4; ++w) wprefix += (w < (int)wave) ? s_tmp[w] : 0u;
   __syncthreads();
   return wprefix + incl - v;
 }
 
+// Digit counts of every pass in one read. Block b counts keys
+// [b * chunk, (b + 1) * chunk), eight loads in flight per thread; a wave whose
+// 64 keys share a digit (the high digits of spatially ordered keys) adds them
+// with one LDS atomic instead of 64 to the same address.
+constexpr int kHistUnroll = 8;
+constexpr uint32_t kHistMaxBlocks = 1024;
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, uint32_t n, int passes, int first_pass,
-                                                 uint32_t* __restrict__ hist) {
+                                                 uint32_t chunk, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s[kMaxPasses * 256];
   for (int i = threadIdx.x; i < passes * 256; i += 256) s[i] = 0;
   __syncthreads();
-  size_t stride = (size_t)gridDim.x * 256;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    K k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s[p * 256 + (uint32_t)((k >> (8 * (p + first_pass))) & 0xFF)], 1u);
+  const size_t b0 = (size_t)blockIdx.x * chunk;
+  const size_t b1 = b0 + chunk < (size_t)n ? b0 + chunk : (size_t)n;
+  for (size_t i0 = b0; i0 < b1; i0 += 256 * kHistUnroll) {
+    K k[kHistUnroll];
+    bool ok[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const size_t i = i0 + (size_t)u * 256 + threadIdx.x;
+      ok[u] = i < b1;
+      k[u] = ok[u] ? keys[i] : (K)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const uint64_t active = __ballot(ok[u]);
+      if (!ok[u]) continue;
+      for (int p = 0; p < passes; ++p) {
+        const uint32_t d = (uint32_t)(k[u] >> (8 * (p + first_pass))) & 0xFFu;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__ballot(d == d0) == active) {
+          if (__lane_id() == (uint32_t)__builtin_ctzll(active)) atomicAdd(&s[p * 256 + d0], (uint32_t)__popcll(active));
+        } else {
+          atomicAdd(&s[p * 256 + d], 1u);
+        }
+      }
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * 256; i += 256) {
@@ -63,128 +107,183 @@ __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, uin
   }
 }
 
+// One pass over one tile of kTile keys. The tile's keys are ranked stably
+// per digit and moved to their place in the tile's digit order in LDS before
+// the look-back, so their registers are free while it waits; the values are
+// loaded during the look-back and follow through the same LDS buffer once the
+// keys are out, so a workgroup holds kTile keys of LDS, not keys and values.
 template <typename K, bool kVals>
-__global__ __launch_bounds__(256) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                    K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
-                                                    int shift, const uint32_t* __restrict__ hist,
-                                                    uint32_t* __restrict__ tile_ctr, uint32_t* __restrict__ status,
-                                                    uint32_t* __restrict__ err) {
-  __shared__ uint32_t s_tile;
+__global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                         K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                         int shift, const uint32_t* __restrict__ hist,
+                                                         uint32_t* __restrict__ status, uint32_t* __restrict__ err) {
   __shared__ uint32_t s_cnt[kWaves][256];
-  __shared__ uint32_t s_dstart[256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_tmp[8];
-  __shared__ K s_keys[kTile];
-  __shared__ uint32_t s_vals[kVals ? kTile : 1];
+  __shared__ K s_buf[kTile];
 
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-  for (int i = tid; i < kWaves * 256; i += 256) (&s_cnt[0][0])[i] = 0;
+  // While ranking, s_buf holds one 64-bit lane mask per (wave, digit): the
+  // lanes of the current 64 keys that carry that digit.
+  static_assert(sizeof(K) * kTile >= sizeof(uint64_t) * kWaves * 256, "digit masks fit in the tile buffer");
+  uint64_t* s_mask = reinterpret_cast<uint64_t*>(s_buf);
+  for (int i = tid; i < kWaves * 256; i += kThreads) {
+    (&s_cnt[0][0])[i] = 0;
+    s_mask[i] = 0;
+  }
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;
   const size_t base = (size_t)tile * kTile;
   const size_t wbase = base + (size_t)wave * kWaveTile;
 
   K key[kItems];
-  uint32_t val[kItems];
   uint32_t rank[kItems];
-  uint32_t dig[kItems];
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
-    if (idx < n) {
-      key[j] = kin[idx];
-      if (kVals) val[j] = vin[idx];
-    } else {
-      key[j] = 0;
-      if (kVals) val[j] = 0;
-    }
+    key[j] = idx < n ? kin[idx] : (K)0;
   }
   // Stable in-tile ranking: iteration j of wave w covers keys
   // [wbase + 64j, wbase + 64j + 64) in lane order, so (w, j, lane) is memory order.
+  // Each lane ORs its bit into its digit's mask; reading the mask back gives
+  // the lanes that share the digit, so the lane's rank among them is the
+  // count of lower lanes. The group's last lane advances the digit's counter
+  // and clears the mask for the next 64 keys.
+  const uint64_t my_bit = 1ull << lane;
+  const uint64_t lower_lanes = my_bit - 1;
+  uint64_t* wmask = s_mask + wave * 256;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
     bool valid = idx < n;
     uint32_t d = (uint32_t)(key[j] >> shift) & 0xFFu;
-    dig[j] = d;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      uint32_t bit = (d >> b) & 1u;
-      uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
-    uint32_t lower = (uint32_t)__popcll(peers & rk::lanemask_lt());
-    uint32_t cnt = (uint32_t)__popcll(peers);
-    uint32_t old = s_cnt[wave][d];
+    if (valid) __hip_atomic_fetch_or(wmask + d, my_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t peers =
+        valid ? __hip_atomic_load(wmask + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0ull;
+    const uint32_t lower = (uint32_t)__popcll(peers & lower_lanes);
+    const uint32_t cnt = (uint32_t)__popcll(peers);
+    const uint32_t old = s_cnt[wave][d];
     rank[j] = old + lower;
     __builtin_amdgcn_wave_barrier();
-    if (valid && lower + 1 == cnt) s_cnt[wave][d] = old + cnt;
+    if (valid && lower + 1 == cnt) {
+      s_cnt[wave][d] = old + cnt;
+      __hip_atomic_store(wmask + d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
 
+  // Per digit (thread d < 256): the tile's count, published at once for the
+  // look-back of later tiles, and each wave's start within the digit's run.
   const uint32_t d = tid;
-  uint32_t run = 0;
+  uint32_t total = 0;
+  if (tid < 256) {
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    uint32_t c = s_cnt[w][d];
-    s_cnt[w][d] = run;
-    run += c;
-  }
-  const uint32_t total = run;
-  uint32_t* my_status = status + (size_t)tile * 256 + d;
-  rk::atomic_store_relaxed(my_status, (tile == 0 ? kInc : kAgg) | total);
-
-  const uint32_t local_excl = block_excl_scan256(total, s_tmp);
-  const uint32_t global_excl = block_excl_scan256(hist[d], s_tmp);
-
-  uint32_t prefix = 0;
-  if (tile > 0) {
-    int32_t j = (int32_t)tile - 1;
-    uint32_t spins = 0;
-    while (j >= 0) {
-      uint32_t s = rk::atomic_load_relaxed(status + (size_t)j * 256 + d);
-      uint32_t flag = s & ~kCountMask;
-      if (flag == 0) {
-        if (++spins > kSpinLimit) {
-          atomicOr(err, (uint32_t)RK_FLAG_SPIN_LIMIT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      prefix += s & kCountMask;
-      if (flag == kInc) break;
-      --j;
+    for (int w = 0; w < kWaves; ++w) {
+      uint32_t c = s_cnt[w][d];
+      s_cnt[w][d] = total;
+      total += c;
     }
-    rk::atomic_store_relaxed(my_status, kInc | ((prefix + total) & kCountMask));
+    rk::atomic_store_relaxed(status + (size_t)tile * 256 + d, (tile == 0 ? kInc : kAgg) | total);
   }
-  s_dstart[d] = local_excl;
-  s_gbase[d] = global_excl + prefix - local_excl;
+  const uint32_t local_excl = block_excl_scan256(total, s_tmp);
+  if (tid < 256) {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][d] += local_excl;
+  }
   __syncthreads();
-
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
     if (idx < n) {
-      uint32_t dd = dig[j];
-      uint32_t pos = s_dstart[dd] + s_cnt[wave][dd] + rank[j];
-      s_keys[pos] = key[j];
-      if (kVals) s_vals[pos] = val[j];
+      const uint32_t dd = (uint32_t)(key[j] >> shift) & 0xFFu;
+      rank[j] += s_cnt[wave][dd];  // position in the tile's digit order
+      s_buf[rank[j]] = key[j];
     }
+  }
+  uint32_t val[kItems];
+  if (kVals) {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      size_t idx = wbase + (size_t)j * 64 + lane;
+      val[j] = idx < n ? vin[idx] : 0u;
+    }
+  }
+  const uint32_t global_excl = block_excl_scan256(tid < 256 ? hist[tid & 255] : 0u, s_tmp);
+
+  // Decoupled look-back, kLook predecessors' status words per round trip:
+  // add counts from tile j down until an inclusive prefix; stop the round at
+  // a tile that has published nothing yet and reload from there.
+  if (tid < 256) {
+    uint32_t prefix = 0;
+    if (tile > 0) {
+      int32_t j = (int32_t)tile - 1;
+      uint32_t spins = 0;
+      for (;;) {
+        uint32_t sv[kLook];
+#pragma unroll
+        for (int w = 0; w < kLook; ++w)
+          sv[w] = (j - w >= 0) ? rk::atomic_load_relaxed(status + (size_t)(j - w) * 256 + d) : kInc;
+        int used = 0;
+        bool inc = false, stalled = false;
+#pragma unroll
+        for (int w = 0; w < kLook; ++w) {
+          if (inc || stalled) continue;
+          const uint32_t flag = sv[w] & ~kCountMask;
+          if (flag == 0) {
+            stalled = true;
+            continue;
+          }
+          prefix += sv[w] & kCountMask;
+          ++used;
+          if (flag == kInc) inc = true;
+        }
+        if (inc) break;
+        j -= used;
+        if (used == 0) {
+          if (++spins > kSpinLimit) {
+            atomicOr(err, (uint32_t)RK_FLAG_SPIN_LIMIT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      rk::atomic_store_relaxed(status + (size_t)tile * 256 + d, kInc | ((prefix + total) & kCountMask));
+    }
+    s_gbase[d] = global_excl + prefix - local_excl;
   }
   __syncthreads();
 
   const uint32_t tile_n = (n - base) < (size_t)kTile ? (uint32_t)(n - base) : (uint32_t)kTile;
-  for (uint32_t i = tid; i < tile_n; i += 256) {
-    K k = s_keys[i];
-    uint32_t dd = (uint32_t)(k >> shift) & 0xFFu;
-    size_t o = (size_t)s_gbase[dd] + i;
-    if (o < n) {  // always true unless the look-back failed (err is set then)
-      kout[o] = k;
-      if (kVals) vout[o] = s_vals[i];
+  uint32_t opos[kItems];
+#pragma unroll
+  for (int m = 0; m < kItems; ++m) {
+    const uint32_t i = tid + (uint32_t)m * kThreads;
+    opos[m] = 0xFFFFFFFFu;
+    if (i < tile_n) {
+      const K k = s_buf[i];
+      const uint32_t dd = (uint32_t)(k >> shift) & 0xFFu;
+      const size_t o = (size_t)s_gbase[dd] + i;
+      if (o < n) {  // always true unless the look-back failed (err is set then)
+        kout[o] = k;
+        opos[m] = (uint32_t)o;
+      }
+    }
+  }
+  if (kVals) {
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(s_buf);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      size_t idx = wbase + (size_t)j * 64 + lane;
+      if (idx < n) s_val[rank[j]] = val[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kItems; ++m) {
+      const uint32_t i = tid + (uint32_t)m * kThreads;
+      if (i < tile_n && opos[m] != 0xFFFFFFFFu) vout[opos[m]] = s_val[i];
     }
   }
 }
@@ -270,21 +369,21 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
   if (!keys || passes > kMaxPasses) return RK_ERROR_INVALID_VALUE;
   const uint32_t T = tiles_of(n);
   RkArena ar(ws, ws_bytes);
-  const size_t ctl_words = (size_t)passes * 256 + passes + 1 + (size_t)passes * T * 256;
+  const size_t ctl_words = (size_t)passes * 256 + 1 + (size_t)passes * T * 256;
   uint32_t* ctl = ar.take<uint32_t>(ctl_words);
   K* kalt = ar.take<K>(n);
   uint32_t* valt = vals ? ar.take<uint32_t>(n) : nullptr;
   if (!ctl || !kalt || (vals && !valt)) return RK_ERROR_WORKSPACE;
   uint32_t* hist = ctl;
-  uint32_t* ctr = hist + (size_t)passes * 256;
-  uint32_t* werr = ctr + passes;
+  uint32_t* werr = hist + (size_t)passes * 256;
   uint32_t* status = werr + 1;
   if (!err) err = werr;
 
   rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, ctl_words, 0u);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL(k_rs_hist<K>, dim3(grid_for(n, 256 * 16, 256)), dim3(256), 0, st, keys, n, passes, first_pass,
-                     hist);
+  const unsigned hist_blocks = grid_for(n, 256 * kHistUnroll * 2, kHistMaxBlocks);
+  const uint32_t chunk = (uint32_t)(((size_t)n + hist_blocks - 1) / hist_blocks);
+  hipLaunchKernelGGL(k_rs_hist<K>, dim3(hist_blocks), dim3(256), 0, st, keys, n, passes, first_pass, chunk, hist);
   RK_CHECK_LAUNCH();
   K* ks = keys;
   K* kd = kalt;
@@ -293,11 +392,11 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
   for (int p = 0; p < passes; ++p) {
     int shift = 8 * (p + first_pass);
     if (vals) {
-      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(T), dim3(256), 0, st, ks, vs, kd, vd, n, shift,
-                         hist + (size_t)p * 256, ctr + p, status + (size_t)p * T * 256, err);
+      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(T), dim3(kThreads), 0, st, ks, vs, kd, vd, n, shift,
+                         hist + (size_t)p * 256, status + (size_t)p * T * 256, err);
     } else {
-      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(T), dim3(256), 0, st, ks, (const uint32_t*)nullptr, kd,
-                         (uint32_t*)nullptr, n, shift, hist + (size_t)p * 256, ctr + p,
+      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(T), dim3(kThreads), 0, st, ks, (const uint32_t*)nullptr, kd,
+                         (uint32_t*)nullptr, n, shift, hist + (size_t)p * 256,
                          status + (size_t)p * T * 256, err);
     }
     RK_CHECK_LAUNCH();
