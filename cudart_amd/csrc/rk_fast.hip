@@ -8,9 +8,11 @@
 //   k_fast_prep     per-face box (API array) and 30/63-bit Morton key
 //   radix sort      (key, face) pairs
 //   k_fast_hierarchy  topology, node boxes and leaf-ordered triangle records
-//                     (v0, v1-v0, v2-v0; 48 B) in one bottom-up pass
-//                     (agglomerative LBVH, Apetrei 2014); subtrees of
-//                     <= leaf_max faces become leaves
+//                     (v0, v1-v0, v2-v0; 48 B) bottom-up (agglomerative
+//                     LBVH, Apetrei 2014) for the splits inside each
+//                     1024-position chunk; subtrees of <= leaf_max faces
+//                     become leaves
+//   k_fast_hierarchy_top  the same climbs through the splits that cross chunks
 //
 // Node record (64 B): float4 (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y),
 //                     float4 (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y),
@@ -131,7 +133,10 @@ __device__ __forceinline__ void write_node(FNode* __restrict__ nodes, uint32_t i
 }
 
 // Max-scan over one workgroup of kLocalThreads threads (waves of 64).
-constexpr uint32_t kLocalThreads = 1024;
+#ifndef RK_HIER_THREADS
+#define RK_HIER_THREADS 1024
+#endif
+constexpr uint32_t kLocalThreads = RK_HIER_THREADS;
 constexpr uint32_t kLocalWaves = kLocalThreads / 64;
 
 __device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
@@ -167,6 +172,7 @@ __device__ __forceinline__ void block_excl_max(uint64_t v, uint64_t* s_wave, uin
 }
 
 constexpr uint32_t kChunk = kLocalThreads;  // sorted positions per workgroup, one per thread
+constexpr uint32_t kPendLeft = 0x80000000u;  // a handed-on climb whose part is its split's left child
 
 __device__ __forceinline__ Part merge_parts(const Part& lc, const Part& rc) {
   Part up;
@@ -247,9 +253,12 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
 // exchange goes through LDS. The range of split p ends, on each side, at the
 // nearest split with a larger key, so it lies inside [c0, c1) when such a
 // split exists both in [c0 - 1, p - 1] and in [p + 1, c1 - 1] (prefix and
-// suffix maxima of the chunk's split keys). The few splits that cross chunks
-// exchange through global memory with XCD-coherent loads and stores and an
-// agent-scope counter; the stores are waited for before the counter moves.
+// suffix maxima of the chunk's split keys). A climb that reaches a split
+// crossing chunks leaves its part in the split's slot and its position in
+// the chunk's list of handed-on climbs (pend, pend_count), and ends; the
+// second kernel continues these through global memory. The chain of crossing
+// splits above a chunk takes a memory round trip per level, and waiting for
+// it here would hold the workgroup's LDS and keep other chunks from starting.
 // With kBatch the positions hold many meshes, sorted by (mesh, Morton code):
 // keys are mesh << 30 | 30-bit code, so every split between two meshes ranks
 // above every split inside one, and a climb stops at its mesh's root, whose
@@ -264,8 +273,11 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
                                                                   Part* __restrict__ gparts,
                                                                   FNode* __restrict__ nodes,
                                                                   float4* __restrict__ tris,
-                                                                  const rk_batch_mesh* __restrict__ meshes) {
+                                                                  const rk_batch_mesh* __restrict__ meshes,
+                                                                  uint32_t* __restrict__ pend,
+                                                                  uint32_t* __restrict__ pend_count) {
   __shared__ Part s_part[kChunk];
+  __shared__ uint32_t s_npend;
   __shared__ uint64_t s_key[kChunk + 1];  // s_key[k] = split_key(c0 - 1 + k), ~0 past either end
   __shared__ uint32_t s_flag[kChunk];
   __shared__ uint8_t s_local[kChunk];     // s_local[k]: split c0 + k lies inside the chunk
@@ -296,6 +308,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
     s_key[k] = (x >= 0 && x + 1 < (int64_t)n) ? split_key(keys, (uint32_t)x) : ~0ull;
   }
   s_flag[t] = 0u;
+  if (t == 0) s_npend = 0u;
   __syncthreads();
   {
     // inside test of split p = c0 - 1 + k for k = t (k >= 1)
@@ -309,58 +322,122 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
     if (t == kLocalThreads - 1) s_local[kChunk - 1] = 0;  // split c0 + kChunk - 1 always crosses
   }
   __syncthreads();
-  if (i >= c1) return;
-  const float4 e1 = rk::sub4(v1, v0), e2 = rk::sub4(v2, v0);
-  tris[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
-  tris[3 * (size_t)i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-  tris[3 * (size_t)i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
-  Part me;
-  me.lox = rk::fmn(rk::fmn(v0.x, v1.x), v2.x);
-  me.loy = rk::fmn(rk::fmn(v0.y, v1.y), v2.y);
-  me.loz = rk::fmn(rk::fmn(v0.z, v1.z), v2.z);
-  me.hix = rk::fmx(rk::fmx(v0.x, v1.x), v2.x);
-  me.hiy = rk::fmx(rk::fmx(v0.y, v1.y), v2.y);
-  me.hiz = rk::fmx(rk::fmx(v0.z, v1.z), v2.z);
-  me.ref = kLeafBit | (i << 4);  // count - 1 = 0
-  if (lo == hi) {  // a single face: root with one leaf child and one empty child
-    Part empty;
-    empty.lox = empty.loy = empty.loz = RK_HUGE;
-    empty.hix = empty.hiy = empty.hiz = -RK_HUGE;
-    empty.ref = kLeafBit;
-    write_node(nodes, lo, me, empty, 1, 0);
-    return;
-  }
-  uint32_t l = i, r = i;
-  // Each step widens [l, r] by at least one position: n steps bound it.
-  for (uint32_t guard = 0; guard < n; ++guard) {
-    const bool has_l = l > 0, has_r = r + 1 < n;
-    uint64_t kl = ~0ull, kr = ~0ull;
-    if (has_l) kl = (l >= c0 && l - c0 <= kChunk) ? s_key[l - c0] : split_key(keys, l - 1);
-    if (has_r) kr = (r + 1 >= c0 && r + 1 - c0 <= kChunk) ? s_key[r + 1 - c0] : split_key(keys, r);
-    const bool as_left = left_of_parent(has_l, has_r, kl, kr);
-    const uint32_t p = as_left ? r : l - 1;
-    const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
-    me.end = as_left ? l : r;
-    Part sib;
-    if (p >= c0 && p - c0 < kChunk && s_local[p - c0]) {
+  // Climbs from this thread's leaf through the chunk's splits; stops as the
+  // first to arrive at a split, at the (mesh) root, or at a split that
+  // crosses chunks, whose climb k_fast_hierarchy_top continues.
+  auto climb = [&]() {
+    if (i >= c1) return;
+    const float4 e1 = rk::sub4(v1, v0), e2 = rk::sub4(v2, v0);
+    tris[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(fid));
+    tris[3 * (size_t)i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    tris[3 * (size_t)i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    Part me;
+    me.lox = rk::fmn(rk::fmn(v0.x, v1.x), v2.x);
+    me.loy = rk::fmn(rk::fmn(v0.y, v1.y), v2.y);
+    me.loz = rk::fmn(rk::fmn(v0.z, v1.z), v2.z);
+    me.hix = rk::fmx(rk::fmx(v0.x, v1.x), v2.x);
+    me.hiy = rk::fmx(rk::fmx(v0.y, v1.y), v2.y);
+    me.hiz = rk::fmx(rk::fmx(v0.z, v1.z), v2.z);
+    me.ref = kLeafBit | (i << 4);  // count - 1 = 0
+    if (lo == hi) {  // a single face: root with one leaf child and one empty child
+      Part empty;
+      empty.lox = empty.loy = empty.loz = RK_HUGE;
+      empty.hix = empty.hiy = empty.hiz = -RK_HUGE;
+      empty.ref = kLeafBit;
+      write_node(nodes, lo, me, empty, 1, 0);
+      return;
+    }
+    uint32_t l = i, r = i;
+    // Each step widens [l, r] by at least one position: n steps bound it.
+    for (uint32_t guard = 0; guard < n; ++guard) {
+      const bool has_l = l > 0, has_r = r + 1 < n;
+      uint64_t kl = ~0ull, kr = ~0ull;
+      if (has_l) kl = (l >= c0 && l - c0 <= kChunk) ? s_key[l - c0] : split_key(keys, l - 1);
+      if (has_r) kr = (r + 1 >= c0 && r + 1 - c0 <= kChunk) ? s_key[r + 1 - c0] : split_key(keys, r);
+      const bool as_left = left_of_parent(has_l, has_r, kl, kr);
+      const uint32_t p = as_left ? r : l - 1;
+      const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
+      me.end = as_left ? l : r;
+      if (!(p >= c0 && p - c0 < kChunk && s_local[p - c0])) {
+        // crossing split: publish the part in its slot and hand the climb on
+        gparts[mine] = me;
+        const uint32_t k = atomicAdd(&s_npend, 1u);
+        pend[c0 + k] = mine | (as_left ? kPendLeft : 0u);
+        return;
+      }
       s_part[mine - c0] = me;  // a local split has p, p + 1 in [c0, c1)
       const uint32_t old =
           __hip_atomic_fetch_add(s_flag + (p - c0), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0) return;
-      sib = s_part[other - c0];
-    } else {
-      store_part_coherent(gparts + mine, me);
-      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (old == 0) return;
-      sib = load_part_coherent(gparts + other);
+      const Part sib = s_part[other - c0];
+      const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
+      me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
+                   : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
+      l = nl;
+      r = nr;
+      if (l == lo && r == hi) return;  // the (mesh) root
     }
-    const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
-    me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
-                 : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
-    l = nl;
-    r = nr;
-    if (l == lo && r == hi) return;  // the (mesh) root
+  };
+  climb();
+  __syncthreads();
+  if (t == 0) pend_count[blockIdx.x] = s_npend;
+}
+
+// The splits that cross chunks, above the chunks' own: thread k of workgroup
+// c continues the k-th climb that chunk c handed on. Each climb starts with
+// its part already in its slot (written by k_fast_hierarchy) and goes on as
+// there: the counter of the split decides who is second, the second merges
+// and climbs on, publishing its part with XCD-coherent stores that it waits
+// for before the counter moves. Every split above a crossing split crosses
+// chunks too, so these climbs exchange through global memory only.
+template <typename K, bool kBatch>
+__global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__ keys, uint32_t n, uint32_t leaf_max,
+                                                           uint32_t* __restrict__ gflags, Part* __restrict__ gparts,
+                                                           FNode* __restrict__ nodes,
+                                                           const rk_batch_mesh* __restrict__ meshes,
+                                                           const uint32_t* __restrict__ pend,
+                                                           const uint32_t* __restrict__ pend_count) {
+  const uint32_t c0 = blockIdx.x * kChunk;
+  const uint32_t count = pend_count[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < count; k += 64) {
+    const uint32_t e = pend[c0 + k];
+    uint32_t mine = e & ~kPendLeft;
+    bool as_left = (e & kPendLeft) != 0;
+    Part me = load_part_coherent(gparts + mine);
+    uint32_t l = as_left ? me.end : mine, r = as_left ? mine : me.end;
+    uint32_t lo = 0, hi = n - 1;
+    if (kBatch) {
+      const rk_batch_mesh m = meshes[(uint32_t)((uint64_t)keys[mine] >> 30)];
+      lo = m.face_offset;
+      hi = m.face_offset + m.face_count - 1;
+    }
+    bool published = true;  // the first part is in its slot already
+    for (uint32_t guard = 0; guard < n; ++guard) {
+      if (!published) {
+        const bool has_l = l > 0, has_r = r + 1 < n;
+        const uint64_t kl = has_l ? split_key(keys, l - 1) : ~0ull;
+        const uint64_t kr = has_r ? split_key(keys, r) : ~0ull;
+        as_left = left_of_parent(has_l, has_r, kl, kr);
+      }
+      const uint32_t p = as_left ? r : l - 1;
+      mine = as_left ? p : p + 1;
+      const uint32_t other = as_left ? p + 1 : p;
+      if (!published) {
+        me.end = as_left ? l : r;
+        store_part_coherent(gparts + mine, me);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      }
+      published = false;
+      const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == 0) break;
+      const Part sib = load_part_coherent(gparts + other);
+      const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
+      me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
+                   : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
+      l = nl;
+      r = nr;
+      if (l == lo && r == hi) break;  // the (mesh) root
+    }
   }
 }
 
@@ -369,7 +446,16 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
 // The traversal stack holds at most one entry per tree level, and a radix
 // tree over b-bit keys with position tie-breaks (< 2^27 positions) is at most
 // b + 27 levels deep: 57 for 30-bit keys, 90 for 63-bit keys.
-constexpr int kStack = 96;
+#ifndef RK_TRACE_STACK
+#define RK_TRACE_STACK 96
+#endif
+constexpr int kStack = RK_TRACE_STACK;
+#ifndef RK_LDS_STACK
+#define RK_LDS_STACK 8
+#endif
+// Stack entries per lane kept in LDS: 8 x 8 B x 64 lanes x 32 waves = 128 KB,
+// so the traces still run 32 waves per CU.
+constexpr int kLdsStack = RK_LDS_STACK;
 
 // intersectTriRay, CUDA_raytraceMeshTree.cu:57-89, with vert0.w = edge1.w =
 // edge2.w = 0 (:174-176) and the reference's four-term dot products. eps is
@@ -405,17 +491,25 @@ struct RayHit {
   float t, u, v;
 };
 
-template <bool kCount, int kStackN>
+// The traversal stack: its first kLds entries live in LDS (lds: this
+// workgroup's kLds x kWG entries, entry e of thread x at e * kWG + x), the
+// rest in per-lane scratch. Pending entries rarely go deeper than a dozen or
+// two, and an LDS access takes a fraction of the time of a scratch access
+// that misses the L1 (the scratch of thousands of resident lanes does not fit
+// in it).
+template <bool kCount, int kStackN, int kLds = 0, int kWG = 256>
 __device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                             uint32_t n_tris, uint32_t root, float4 orig, float4 dir, float depth,
-                                            float eps, uint32_t& n_box, uint32_t& n_tri) {
+                                            float eps, uint32_t& n_box, uint32_t& n_tri, uint2* lds = nullptr) {
   const uint32_t max_visits = 4 * n_tris + 64;  // a tree visit touches each node once
   uint32_t best = 0xFFFFFFFFu, best_oid = 0xFFFFFFFFu;
   float bu = 0.0f, bv = 0.0f;
   const float ix = safe_inv(dir.x), iy = safe_inv(dir.y), iz = safe_inv(dir.z);
   const float ox = orig.x * ix, oy = orig.y * iy, oz = orig.z * iz;
-  uint32_t stack_ref[kStackN];
-  float stack_t[kStackN];
+  static_assert(kStackN > kLds, "scratch part of the stack");
+  uint32_t stack_ref[kStackN - kLds];
+  float stack_t[kStackN - kLds];
+  uint2* my_lds = kLds > 0 ? lds + threadIdx.x : nullptr;
   int sp = 0;
   uint32_t cur = root;
   uint32_t visits = 0;
@@ -439,7 +533,14 @@ __device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, con
         const bool first0 = tmin0 <= tmin1;
         const uint32_t nearr = first0 ? nd.d.x : nd.d.y, farr = first0 ? nd.d.y : nd.d.x;
         const float fart = first0 ? tmin1 : tmin0;
-        if (sp < kStackN) { stack_ref[sp] = farr; stack_t[sp] = fart; ++sp; }
+        if (sp < kLds) {
+          my_lds[sp * kWG] = make_uint2(farr, __float_as_uint(fart));
+          ++sp;
+        } else if (sp < kStackN) {
+          stack_ref[sp - kLds] = farr;
+          stack_t[sp - kLds] = fart;
+          ++sp;
+        }
         cur = nearr;
         continue;
       }
@@ -467,7 +568,17 @@ __device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, con
     bool found = false;
     while (sp > 0) {
       --sp;
-      if (stack_t[sp] <= depth) { cur = stack_ref[sp]; found = true; break; }
+      uint32_t ref;
+      float t;
+      if (sp < kLds) {
+        const uint2 e = my_lds[sp * kWG];
+        ref = e.x;
+        t = __uint_as_float(e.y);
+      } else {
+        ref = stack_ref[sp - kLds];
+        t = stack_t[sp - kLds];
+      }
+      if (t <= depth) { cur = ref; found = true; break; }
     }
     if (!found) break;
   }
@@ -507,9 +618,10 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
   if (t >= n_rays) return;
   const uint32_t l = order ? order[t] : t;
   if (l >= n_rays) return;
+  __shared__ uint2 s_stack[kLdsStack * 256];
   uint32_t n_box = 0, n_tri = 0;
-  const RayHit h = trace_ray<kCount, kStack>(nodes, tris, n_tris, 0u, origins[l], dirs[l], depths[l], eps, n_box,
-                                             n_tri);
+  const RayHit h = trace_ray<kCount, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, origins[l], dirs[l], depths[l],
+                                                             eps, n_box, n_tri, s_stack);
   if (kCount) {
     atomicAdd(counters, (unsigned long long)n_box);
     atomicAdd(counters + 1, (unsigned long long)n_tri);
@@ -523,6 +635,41 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
 
 inline unsigned blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
+// Workspace of the hierarchy passes over n positions.
+struct HierWs {
+  uint32_t* flags;       // counters of splits that cross chunks (zeroed per build)
+  Part* parts;           // parts exchanged at splits that cross chunks
+  uint32_t* pend;        // per chunk, the climbs handed on to the top pass
+  uint32_t* pend_count;  // per chunk, how many
+};
+inline size_t hier_ws_bytes(size_t n) {
+  return rk_align_up(n * 4) + rk_align_up(n * sizeof(Part)) + rk_align_up(n * 4) +
+         rk_align_up(((n + kChunk - 1) / kChunk) * 4);
+}
+inline bool take_hier_ws(RkArena& ar, uint32_t n, HierWs& w) {
+  w.flags = ar.take<uint32_t>(n);
+  w.parts = ar.take<Part>(n);
+  w.pend = ar.take<uint32_t>(n);
+  w.pend_count = ar.take<uint32_t>(blocks_for(n, kChunk));
+  return w.flags && w.parts && w.pend && w.pend_count;
+}
+// Clears the counters and runs both hierarchy passes.
+template <typename K, bool kBatch>
+rk_status launch_hierarchy(hipStream_t st, const K* keys, const uint32_t* order, const uint4* f, const float4* v,
+                           uint32_t n, uint32_t leaf_max, const HierWs& w, FNode* nodes, float4* tris,
+                           const rk_batch_mesh* meshes) {
+  rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), w.flags, n, 0u);
+  if (s != RK_SUCCESS) return s;
+  const unsigned chunks = blocks_for(n, kChunk);
+  hipLaunchKernelGGL((k_fast_hierarchy<K, kBatch>), dim3(chunks), dim3(kLocalThreads), 0, st, keys, order, f, v, n,
+                     leaf_max, w.flags, w.parts, nodes, tris, meshes, w.pend, w.pend_count);
+  RK_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_fast_hierarchy_top<K, kBatch>), dim3(chunks), dim3(64), 0, st, keys, n, leaf_max, w.flags,
+                     w.parts, nodes, meshes, w.pend, w.pend_count);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" size_t rk_fast_build_workspace_size(uint32_t face_count, int morton_mode) {
@@ -530,16 +677,15 @@ extern "C" size_t rk_fast_build_workspace_size(uint32_t face_count, int morton_m
   size_t kb = morton_mode == RK_MORTON_63 ? 8 : 4;
   size_t b = 0;
   b += rk_align_up(n * kb);               // keys
-  b += rk_align_up(n * 4);                // counters of splits that cross chunks
-  b += rk_align_up(n * sizeof(Part));     // parts exchanged at splits that cross chunks
+  b += hier_ws_bytes(n);
   b += rk_sort_workspace_size((uint32_t)n, (uint32_t)kb);
   return b + 4096;
 }
 
 template <typename K>
 static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v, uint32_t n, float4 rmin,
-                                 float4 rmax, const rk_fast_bvh* bvh, uint32_t leaf_max, K* keys, uint32_t* flags,
-                                 Part* parts, void* sort_ws, size_t sort_bytes, uint32_t* info) {
+                                 float4 rmax, const rk_fast_bvh* bvh, uint32_t leaf_max, K* keys, const HierWs& hw,
+                                 void* sort_ws, size_t sort_bytes, uint32_t* info) {
   uint32_t* order = bvh->indices;
   hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
                      reinterpret_cast<Box*>(bvh->boxes), keys, order);
@@ -547,11 +693,8 @@ static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v
   rk_status s = rk_radix_sort_impl<K>(st, keys, order, n, 0, (int)sizeof(K), sort_ws, sort_bytes,
                                       info + RK_INFO_FLAGS);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL((k_fast_hierarchy<K, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st, keys,
-                     order, f, v, n, leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes),
-                     static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr);
-  RK_CHECK_LAUNCH();
-  return RK_SUCCESS;
+  return launch_hierarchy<K, false>(st, keys, order, f, v, n, leaf_max, hw, static_cast<FNode*>(bvh->nodes),
+                                    static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr);
 }
 
 static rk_status fast_build_common(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
@@ -568,24 +711,23 @@ static rk_status fast_build_common(rk_stream stream, const rk_uint4* faces, cons
   const bool wide = morton_mode == RK_MORTON_63;
   RkArena ar(ws, ws_bytes);
   void* keys = wide ? (void*)ar.take<uint64_t>(n) : (void*)ar.take<uint32_t>(n);
-  uint32_t* flags = ar.take<uint32_t>(n);
-  Part* parts = ar.take<Part>(n);
+  HierWs hw;
+  const bool hw_ok = take_hier_ws(ar, n, hw);
   size_t sort_bytes = rk_sort_workspace_size(n, wide ? 8 : 4);
   void* sort_ws = ar.take<char>(sort_bytes);
-  if (!keys || !flags || !parts || !sort_ws) return RK_ERROR_WORKSPACE;
+  if (!keys || !hw_ok || !sort_ws) return RK_ERROR_WORKSPACE;
   if (sorted_keys) keys = sorted_keys;
   rk_status s;
   if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
-  if ((s = rk_fill_u32(stream, flags, n, 0u)) != RK_SUCCESS) return s;
   float4 rmin = make_float4(hitbox->min.x, hitbox->min.y, hitbox->min.z, hitbox->min.w);
   float4 rmax = make_float4(hitbox->max.x, hitbox->max.y, hitbox->max.z, hitbox->max.w);
   const uint4* f = reinterpret_cast<const uint4*>(faces);
   const float4* v = reinterpret_cast<const float4*>(vertices);
   if (wide)
-    return fast_build_keys<uint64_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint64_t*>(keys), flags,
-                                     parts, sort_ws, sort_bytes, info);
-  return fast_build_keys<uint32_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint32_t*>(keys), flags,
-                                   parts, sort_ws, sort_bytes, info);
+    return fast_build_keys<uint64_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint64_t*>(keys), hw,
+                                     sort_ws, sort_bytes, info);
+  return fast_build_keys<uint32_t>(st, f, v, n, rmin, rmax, bvh, leaf_max, static_cast<uint32_t*>(keys), hw,
+                                   sort_ws, sort_bytes, info);
 }
 
 extern "C" rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
@@ -615,25 +757,17 @@ extern "C" rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, cons
   if (n >= (1u << 27)) return RK_ERROR_UNSUPPORTED;
   hipStream_t st = rk_hip_stream(stream);
   RkArena ar(ws, ws_bytes);
-  uint32_t* flags = ar.take<uint32_t>(n);
-  Part* parts = ar.take<Part>(n);
-  if (!flags || !parts) return RK_ERROR_WORKSPACE;
-  rk_status s = rk_fill_u32(stream, flags, n, 0u);
-  if (s != RK_SUCCESS) return s;
+  HierWs hw;
+  if (!take_hier_ws(ar, n, hw)) return RK_ERROR_WORKSPACE;
   const uint4* f = reinterpret_cast<const uint4*>(faces);
   const float4* v = reinterpret_cast<const float4*>(vertices);
   FNode* nodes = static_cast<FNode*>(bvh->nodes);
   float4* tris = static_cast<float4*>(bvh->tris);
   if (morton_mode == RK_MORTON_63)
-    hipLaunchKernelGGL((k_fast_hierarchy<uint64_t, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
-                       static_cast<const uint64_t*>(sorted_keys), bvh->indices, f, v, n, leaf_max, flags, parts,
-                       nodes, tris, (const rk_batch_mesh*)nullptr);
-  else
-    hipLaunchKernelGGL((k_fast_hierarchy<uint32_t, false>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
-                       static_cast<const uint32_t*>(sorted_keys), bvh->indices, f, v, n, leaf_max, flags, parts,
-                       nodes, tris, (const rk_batch_mesh*)nullptr);
-  RK_CHECK_LAUNCH();
-  return RK_SUCCESS;
+    return launch_hierarchy<uint64_t, false>(st, static_cast<const uint64_t*>(sorted_keys), bvh->indices, f, v, n,
+                                             leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr);
+  return launch_hierarchy<uint32_t, false>(st, static_cast<const uint32_t*>(sorted_keys), bvh->indices, f, v, n,
+                                           leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr);
 }
 
 
@@ -714,17 +848,17 @@ __global__ __launch_bounds__(256) void k_batch_ray_keys(const rk_batch_mesh* __r
   vals[r] = r;
 }
 
-template <int kStackN>
+template <int kStackN, int kWG>
 __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
                                                 const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                 uint32_t n_tris, const float4* __restrict__ origins,
                                                 const float4* __restrict__ dirs, uint32_t r,
                                                 uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                float* __restrict__ depths, float eps) {
+                                                float* __restrict__ depths, float eps, uint2* lds_stack) {
   const uint32_t m = mesh_of_ray(meshes, mesh_count, r);
   uint32_t nb = 0, nt = 0;
-  const RayHit h = trace_ray<false, kStackN>(nodes, tris, n_tris, meshes[m].face_offset, origins[r], dirs[r],
-                                             depths[r], eps, nb, nt);
+  const RayHit h = trace_ray<false, kStackN, kLdsStack, kWG>(nodes, tris, n_tris, meshes[m].face_offset, origins[r],
+                                                             dirs[r], depths[r], eps, nb, nt, lds_stack);
   if (h.pos != 0xFFFFFFFFu) {
     faces_id[r] = h.pos;
     coords[r] = make_float2(h.u, h.v);
@@ -746,7 +880,9 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
   if (t >= n_rays) return;
   const uint32_t r = order ? order[t] : t;
   if (r >= n_rays) return;
-  batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths, eps);
+  __shared__ uint2 s_stack[kLdsStack * kWG];
+  batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths,
+                                eps, s_stack);
 }
 
 // Persistent grid: every wave takes 64-ray tiles from its XCD's queue
@@ -765,6 +901,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
                                                            float2* __restrict__ coords, float* __restrict__ depths,
                                                            float eps, uint32_t* __restrict__ queues,
                                                            const uint32_t* __restrict__ order) {
+  __shared__ uint2 s_stack[kLdsStack * kWG];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t n_tiles = (n_rays + 63) / 64;
   const uint32_t per = (n_tiles + kQueues - 1) / kQueues;
@@ -782,8 +919,8 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
     const uint32_t p = tile * 64 + lane;
     const uint32_t r = p < n_rays ? (order ? order[p] : p) : n_rays;
     if (r < n_rays)
-      batch_trace_one<kStackN>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths,
-                               eps);
+      batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords,
+                                    depths, eps, s_stack);
   }
 }
 
@@ -814,9 +951,8 @@ extern "C" size_t rk_batch_build_workspace_size(uint32_t mesh_count, uint32_t fa
   (void)mesh_count;
   size_t n = face_count ? face_count : 1;
   size_t b = 0;
-  b += rk_align_up(n * 8);             // (mesh, code) keys
-  b += rk_align_up(n * 4);             // counters of splits that cross chunks
-  b += rk_align_up(n * sizeof(Part));  // parts exchanged at splits that cross chunks
+  b += rk_align_up(n * 8);  // (mesh, code) keys
+  b += hier_ws_bytes(n);
   b += rk_sort_workspace_size((uint32_t)n, 8);
   return b + 4096;
 }
@@ -834,14 +970,13 @@ extern "C" rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshe
   hipStream_t st = rk_hip_stream(stream);
   RkArena ar(ws, ws_bytes);
   uint64_t* keys = ar.take<uint64_t>(n);
-  uint32_t* flags = ar.take<uint32_t>(n);
-  Part* parts = ar.take<Part>(n);
+  HierWs hw;
+  const bool hw_ok = take_hier_ws(ar, n, hw);
   const size_t sort_bytes = rk_sort_workspace_size(n, 8);
   void* sort_ws = ar.take<char>(sort_bytes);
-  if (!keys || !flags || !parts || !sort_ws) return RK_ERROR_WORKSPACE;
+  if (!keys || !hw_ok || !sort_ws) return RK_ERROR_WORKSPACE;
   rk_status s;
   if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
-  if ((s = rk_fill_u32(stream, flags, n, 0u)) != RK_SUCCESS) return s;
   const uint4* f = reinterpret_cast<const uint4*>(faces);
   const float4* v = reinterpret_cast<const float4*>(vertices);
   uint32_t* order = bvh->indices;
@@ -851,11 +986,8 @@ extern "C" rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshe
   s = rk_radix_sort_impl<uint64_t>(st, keys, order, n, 0, (int)key_passes_for(mesh_count), sort_ws, sort_bytes,
                                    info + RK_INFO_FLAGS);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL((k_fast_hierarchy<uint64_t, true>), dim3(blocks_for(n, kChunk)), dim3(kLocalThreads), 0, st,
-                     keys, order, f, v, n, config->leaf_max, flags, parts, static_cast<FNode*>(bvh->nodes),
-                     static_cast<float4*>(bvh->tris), meshes);
-  RK_CHECK_LAUNCH();
-  return RK_SUCCESS;
+  return launch_hierarchy<uint64_t, true>(st, keys, order, f, v, n, config->leaf_max, hw,
+                                          static_cast<FNode*>(bvh->nodes), static_cast<float4*>(bvh->tris), meshes);
 }
 
 extern "C" size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count) {
@@ -1000,7 +1132,9 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
   if (x >= width || y >= height) return;
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
-  const RayHit h = trace_ray<false, kStack>(nodes, tris, n_tris, 0u, ro, d, RK_HUGE, eps, nb, nt);
+  __shared__ uint2 s_stack[kLdsStack * 256];
+  const RayHit h = trace_ray<false, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, ro, d, RK_HUGE, eps, nb, nt,
+                                                            s_stack);
   const size_t p = (size_t)y * (size_t)width + (size_t)x;
   const bool hit = h.pos != 0xFFFFFFFFu;
   faces_id[p] = h.pos;

@@ -242,7 +242,8 @@ rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_
 /* Refit after the vertices moved: keeps the topology of the rk_fast_build_keyed
  * that produced sorted_keys and bvh->indices (same faces, same morton_mode and
  * leaf_max) and recomputes every node box and triangle record from 'vertices'.
- * Workspace: rk_fast_build_workspace_size. One kernel. */
+ * Workspace: rk_fast_build_workspace_size. Runs only the hierarchy passes
+ * (no key computation, no sort). */
 rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
                         int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh, const void* sorted_keys, void* ws,
                         size_t ws_bytes);
