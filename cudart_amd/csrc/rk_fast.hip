@@ -235,6 +235,42 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
   return v;
 }
 
+// Climb of the subtree [l, r] (box and ref in me) through splits that cross
+// chunks, starting at its parent on side as_left: publish the part in its
+// slot with XCD-coherent stores, waited for before the split's counter moves
+// (published: already in its slot); the first to arrive stops, the second
+// merges both and climbs on. Every split above a crossing split crosses
+// chunks too, so the whole climb exchanges through global memory.
+template <typename K>
+__device__ __forceinline__ void climb_crossing(const K* __restrict__ keys, uint32_t n, uint32_t leaf_max,
+                                               uint32_t* __restrict__ gflags, Part* __restrict__ gparts,
+                                               FNode* __restrict__ nodes, uint32_t lo, uint32_t hi, Part me,
+                                               uint32_t l, uint32_t r, bool as_left, bool published) {
+  for (uint32_t guard = 0; guard < n; ++guard) {
+    const uint32_t p = as_left ? r : l - 1;
+    const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
+    if (!published) {
+      me.end = as_left ? l : r;
+      store_part_coherent(gparts + mine, me);
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+    published = false;
+    const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    const Part sib = load_part_coherent(gparts + other);
+    const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
+    me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
+                 : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
+    l = nl;
+    r = nr;
+    if (l == lo && r == hi) return;  // the (mesh) root
+    const bool has_l = l > 0, has_r = r + 1 < n;
+    const uint64_t kl = has_l ? split_key(keys, l - 1) : ~0ull;
+    const uint64_t kr = has_r ? split_key(keys, r) : ~0ull;
+    as_left = left_of_parent(has_l, has_r, kl, kr);
+  }
+}
+
 // The hierarchy in one bottom-up pass (agglomerative LBVH, Apetrei 2014): a
 // subtree covering sorted positions [l, r] is a child of the split at r or
 // at l - 1, whichever has the smaller split key; of the two subtrees that
@@ -254,11 +290,12 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
 // nearest split with a larger key, so it lies inside [c0, c1) when such a
 // split exists both in [c0 - 1, p - 1] and in [p + 1, c1 - 1] (prefix and
 // suffix maxima of the chunk's split keys). A climb that reaches a split
-// crossing chunks leaves its part in the split's slot and its position in
-// the chunk's list of handed-on climbs (pend, pend_count), and ends; the
-// second kernel continues these through global memory. The chain of crossing
-// splits above a chunk takes a memory round trip per level, and waiting for
-// it here would hold the workgroup's LDS and keep other chunks from starting.
+// crossing chunks goes on through global memory (climb_crossing) when
+// inline_top is set; otherwise it leaves its part in the split's slot and its
+// position in the chunk's list of handed-on climbs (pend, pend_count), and
+// ends, and k_fast_hierarchy_top continues it. The chain of crossing splits
+// above a chunk takes a few memory round trips per level; on a large tree,
+// waiting for it here would hold the workgroup's LDS while other chunks queue.
 // With kBatch the positions hold many meshes, sorted by (mesh, Morton code):
 // keys are mesh << 30 | 30-bit code, so every split between two meshes ranks
 // above every split inside one, and a climb stops at its mesh's root, whose
@@ -275,7 +312,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
                                                                   float4* __restrict__ tris,
                                                                   const rk_batch_mesh* __restrict__ meshes,
                                                                   uint32_t* __restrict__ pend,
-                                                                  uint32_t* __restrict__ pend_count) {
+                                                                  uint32_t* __restrict__ pend_count,
+                                                                  uint32_t inline_top) {
   __shared__ Part s_part[kChunk];
   __shared__ uint32_t s_npend;
   __shared__ uint64_t s_key[kChunk + 1];  // s_key[k] = split_key(c0 - 1 + k), ~0 past either end
@@ -359,6 +397,10 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
       me.end = as_left ? l : r;
       if (!(p >= c0 && p - c0 < kChunk && s_local[p - c0])) {
+        if (inline_top) {  // small trees: the chain above the chunks is short
+          climb_crossing<K>(keys, n, leaf_max, gflags, gparts, nodes, lo, hi, me, l, r, as_left, false);
+          return;
+        }
         // crossing split: publish the part in its slot and hand the climb on
         gparts[mine] = me;
         const uint32_t k = atomicAdd(&s_npend, 1u);
@@ -384,12 +426,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
 }
 
 // The splits that cross chunks, above the chunks' own: thread k of workgroup
-// c continues the k-th climb that chunk c handed on. Each climb starts with
-// its part already in its slot (written by k_fast_hierarchy) and goes on as
-// there: the counter of the split decides who is second, the second merges
-// and climbs on, publishing its part with XCD-coherent stores that it waits
-// for before the counter moves. Every split above a crossing split crosses
-// chunks too, so these climbs exchange through global memory only.
+// c continues the k-th climb that chunk c handed on, whose part is in its
+// slot already (written by k_fast_hierarchy).
 template <typename K, bool kBatch>
 __global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__ keys, uint32_t n, uint32_t leaf_max,
                                                            uint32_t* __restrict__ gflags, Part* __restrict__ gparts,
@@ -401,43 +439,17 @@ __global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__
   const uint32_t count = pend_count[blockIdx.x];
   for (uint32_t k = threadIdx.x; k < count; k += 64) {
     const uint32_t e = pend[c0 + k];
-    uint32_t mine = e & ~kPendLeft;
-    bool as_left = (e & kPendLeft) != 0;
-    Part me = load_part_coherent(gparts + mine);
-    uint32_t l = as_left ? me.end : mine, r = as_left ? mine : me.end;
+    const uint32_t mine = e & ~kPendLeft;
+    const bool as_left = (e & kPendLeft) != 0;
+    const Part me = load_part_coherent(gparts + mine);
+    const uint32_t l = as_left ? me.end : mine, r = as_left ? mine : me.end;
     uint32_t lo = 0, hi = n - 1;
     if (kBatch) {
       const rk_batch_mesh m = meshes[(uint32_t)((uint64_t)keys[mine] >> 30)];
       lo = m.face_offset;
       hi = m.face_offset + m.face_count - 1;
     }
-    bool published = true;  // the first part is in its slot already
-    for (uint32_t guard = 0; guard < n; ++guard) {
-      if (!published) {
-        const bool has_l = l > 0, has_r = r + 1 < n;
-        const uint64_t kl = has_l ? split_key(keys, l - 1) : ~0ull;
-        const uint64_t kr = has_r ? split_key(keys, r) : ~0ull;
-        as_left = left_of_parent(has_l, has_r, kl, kr);
-      }
-      const uint32_t p = as_left ? r : l - 1;
-      mine = as_left ? p : p + 1;
-      const uint32_t other = as_left ? p + 1 : p;
-      if (!published) {
-        me.end = as_left ? l : r;
-        store_part_coherent(gparts + mine, me);
-        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      }
-      published = false;
-      const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (old == 0) break;
-      const Part sib = load_part_coherent(gparts + other);
-      const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
-      me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
-                   : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
-      l = nl;
-      r = nr;
-      if (l == lo && r == hi) break;  // the (mesh) root
-    }
+    climb_crossing<K>(keys, n, leaf_max, gflags, gparts, nodes, lo, hi, me, l, r, as_left, true);
   }
 }
 
@@ -635,6 +647,16 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
 
 inline unsigned blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
+// Up to this many faces the chunk pass also climbs the splits that cross
+// chunks, in the thread that reaches them: the chain above a few hundred
+// chunks is short, and a second launch would cost more than it saves. Above
+// it, a workgroup waiting on that chain would hold its LDS while thousands of
+// chunks queue, so the climbs go to k_fast_hierarchy_top.
+#ifndef RK_INLINE_TOP_FACES
+#define RK_INLINE_TOP_FACES (1u << 18)
+#endif
+constexpr uint32_t kInlineTopFaces = RK_INLINE_TOP_FACES;
+
 // Workspace of the hierarchy passes over n positions.
 struct HierWs {
   uint32_t* flags;       // counters of splits that cross chunks (zeroed per build)
@@ -661,12 +683,15 @@ rk_status launch_hierarchy(hipStream_t st, const K* keys, const uint32_t* order,
   rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), w.flags, n, 0u);
   if (s != RK_SUCCESS) return s;
   const unsigned chunks = blocks_for(n, kChunk);
+  const uint32_t inline_top = n <= kInlineTopFaces ? 1u : 0u;
   hipLaunchKernelGGL((k_fast_hierarchy<K, kBatch>), dim3(chunks), dim3(kLocalThreads), 0, st, keys, order, f, v, n,
-                     leaf_max, w.flags, w.parts, nodes, tris, meshes, w.pend, w.pend_count);
+                     leaf_max, w.flags, w.parts, nodes, tris, meshes, w.pend, w.pend_count, inline_top);
   RK_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_fast_hierarchy_top<K, kBatch>), dim3(chunks), dim3(64), 0, st, keys, n, leaf_max, w.flags,
-                     w.parts, nodes, meshes, w.pend, w.pend_count);
-  RK_CHECK_LAUNCH();
+  if (!inline_top) {
+    hipLaunchKernelGGL((k_fast_hierarchy_top<K, kBatch>), dim3(chunks), dim3(64), 0, st, keys, n, leaf_max, w.flags,
+                       w.parts, nodes, meshes, w.pend, w.pend_count);
+    RK_CHECK_LAUNCH();
+  }
   return RK_SUCCESS;
 }
 
