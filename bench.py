@@ -1,22 +1,33 @@
 """Benchmark of the mesh-tree build + closest-hit trace path on one MI355X.
 
-Workload: the 708x708 fBm heightfield (999,698 FP32 triangles) and 1920x1080
-primary rays from a camera at 30 degrees elevation, 45 degrees azimuth,
-distance 1.5 from (0.5, 0.5, 0.1), 60 degree fov.
+Headline workload: the 708x708 fBm heightfield (999,698 FP32 triangles) and
+1920x1080 primary rays from a camera at 30 degrees elevation, 45 degrees
+azimuth, distance 1.5 from (0.5, 0.5, 0.1), 60 degree fov.
 
 A step is one frame with the tree resident in HBM: a fill kernel resets tmax
 and rk_fast_trace finds the closest hit of all 2,073,600 rays. `value` is
 Mrays/s over the timed frames. The triangle test's parallel-ray threshold is 0
-for the headline (det_epsilon_headline): the reference's absolute 1e-6
-rejects every hit on this mesh (its edges are ~1.4e-3, so |det| ~ 1e-6), and
-that frame, all misses, is reported beside it (mrays_s_reference_epsilon).
-lbvh_mtri_s times rk_fast_build (30-bit Morton, leaves <= 4 faces). The
-reference tree (rk_build_lbvh, depth 12) and both reference traversals are
-timed once on the same frame. cpu_baseline is the oracle: the reference
-algorithm (its LBVH build and per-ray traversal, reference threshold) on the
-host's cores, on a bounded sample of the frame.
+for the headline (det_epsilon_headline). The reference's absolute 1e-6
+rejects the grazing hits on this mesh, whose edges are ~1.4e-3 so that
+|det| ~ 1e-6; that frame is reported beside it (mrays_s_reference_epsilon,
+hit_fraction_reference_epsilon). The same frame with the rays formed inside
+the trace kernel is camera_fused_mrays_s. lbvh_mtri_s times rk_fast_build
+(30-bit Morton, leaves <= 4 faces). The reference tree (rk_build_lbvh, depth
+12) and both reference traversals are timed once on the same frame.
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-ref] [--profile-only]
+Further legs, each skippable:
+- soup: 10M random faces, 63-bit keys, 2^24 incoherent rays; build and
+  frames with the rays as given and sorted by origin (rk_fast_trace_sorted).
+- batch: 4096 soups built and traced together (autotuned, one captured graph
+  per step) against one rk_fast_build + rk_fast_trace per mesh.
+- serve: the rk_server request loop over 32 deforming scenes, Poisson
+  arrivals; p50/p99 latency and saturated throughput against one slot.
+- cpu_baseline: the oracle, i.e. the reference algorithm (its LBVH build and
+  per-ray traversal, reference threshold) on the host's cores, on a bounded
+  sample of the frame.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-ref]
+                [--no-soup] [--no-batch] [--no-serve] [--profile-only | --profile-soup]
 """
 from __future__ import annotations
 
