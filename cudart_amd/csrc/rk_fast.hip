@@ -597,6 +597,221 @@ __device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, con
   return RayHit{best, depth, bu, bv};
 }
 
+// Wave-shared traversal. Every lane starts on its own ray (has_ray); a lane
+// with nothing left to do takes the newest pending subtree of a lane that
+// still has some (one per donor and step) and traverses it for that lane's
+// ray, culling against the owner's best hit so far. A helper's best hit is
+// merged into its owner's when the subtree is done, by (t, original id) as in
+// trace_ray, so each lane's result equals trace_ray's for its ray. A frame
+// then waits less on its few long rays (grazing rays near a horizon, which
+// would otherwise keep a wave running with one active lane). Every lane of
+// the wave must call it; xchg is this wave's 3 x 64 words of LDS.
+template <bool kCount, int kStackN, int kLds, int kWG>
+__device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                                   uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,
+                                                   float4 dir, float depth0, float eps, uint32_t& n_box,
+                                                   uint32_t& n_tri, uint2* lds, uint32_t* xchg) {
+  const uint32_t lane = __lane_id();
+  const uint32_t max_visits = 4 * n_tris + 64;  // a subtree visit touches each node once
+  // this lane's own ray: its best hit
+  uint32_t o_pos = 0xFFFFFFFFu, o_oid = 0xFFFFFFFFu;
+  float o_t = depth0, o_u = 0.0f, o_v = 0.0f;
+  // the job: a subtree (cur + stack) of the owner's ray, and a helper's own best
+  int owner = has_ray ? (int)lane : -1;
+  float4 jo = orig, jd = dir;
+  float ix = safe_inv(jd.x), iy = safe_inv(jd.y), iz = safe_inv(jd.z);
+  float ox = jo.x * ix, oy = jo.y * iy, oz = jo.z * iz;
+  uint32_t j_pos = 0xFFFFFFFFu, j_oid = 0xFFFFFFFFu;
+  float j_t = 0.0f, j_u = 0.0f, j_v = 0.0f;
+  uint32_t cur = root, visits = 0;
+  static_assert(kStackN > kLds, "scratch part of the stack");
+  uint32_t stack_ref[kStackN - kLds];
+  float stack_t[kStackN - kLds];
+  uint2* my_lds = lds + threadIdx.x;
+  int sp = 0;
+  auto st_get = [&](int i, uint32_t& r, float& t) {
+    if (i < kLds) {
+      const uint2 e = my_lds[i * kWG];
+      r = e.x;
+      t = __uint_as_float(e.y);
+    } else {
+      r = stack_ref[i - kLds];
+      t = stack_t[i - kLds];
+    }
+  };
+  auto st_set = [&](int i, uint32_t r, float t) {
+    if (i < kLds) {
+      my_lds[i * kWG] = make_uint2(r, __float_as_uint(t));
+    } else {
+      stack_ref[i - kLds] = r;
+      stack_t[i - kLds] = t;
+    }
+  };
+  for (;;) {
+    const bool working = owner >= 0;
+    const uint64_t idle = __ballot(!working);
+    if (idle == ~0ull) break;
+    // hand the newest pending subtree of up to popc(idle) lanes to idle lanes
+    const uint64_t donors = __ballot(working && sp > 0);
+    if (idle != 0ull && donors != 0ull) {
+      const uint32_t nd = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
+      const uint64_t lt = rk::lanemask_lt();
+      const uint32_t rd = (uint32_t)__popcll(donors & lt), ri = (uint32_t)__popcll(idle & lt);
+      if (working && sp > 0 && rd < nd) {
+        // the newest entry: taking it is a pop, so the stack stays one
+        // entry per level of the current path at most
+        uint32_t r0;
+        float t0;
+        --sp;
+        st_get(sp, r0, t0);
+        __hip_atomic_store(xchg + rd, r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(xchg + 64 + rd, __float_as_uint(t0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(xchg + 128 + rd, (uint32_t)owner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
+      __builtin_amdgcn_wave_barrier();
+      const bool take = !working && ri < nd;
+      uint32_t g_ref = 0, g_t = 0, g_own = lane;
+      if (take) {
+        g_ref = __hip_atomic_load(xchg + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        g_t = __hip_atomic_load(xchg + 64 + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        g_own = __hip_atomic_load(xchg + 128 + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
+      __builtin_amdgcn_wave_barrier();
+      // every lane takes part in the exchange of the owner's ray
+      const int src = (int)g_own;
+      const float4 no = make_float4(__shfl(orig.x, src, 64), __shfl(orig.y, src, 64), __shfl(orig.z, src, 64),
+                                    __shfl(orig.w, src, 64));
+      const float4 ndr = make_float4(__shfl(dir.x, src, 64), __shfl(dir.y, src, 64), __shfl(dir.z, src, 64),
+                                     __shfl(dir.w, src, 64));
+      const float od = __shfl(o_t, src, 64);
+      if (take && __uint_as_float(g_t) <= od) {
+        owner = src;
+        jo = no;
+        jd = ndr;
+        ix = safe_inv(jd.x);
+        iy = safe_inv(jd.y);
+        iz = safe_inv(jd.z);
+        ox = jo.x * ix;
+        oy = jo.y * iy;
+        oz = jo.z * iz;
+        cur = g_ref;
+        sp = 0;
+        visits = 0;
+        j_pos = 0xFFFFFFFFu;
+        j_oid = 0xFFFFFFFFu;
+      }
+    }
+    const bool own = owner == (int)lane;
+    // the owner's best so far bounds a helper's subtree
+    float dmax = o_t;
+    bool d_hit = o_pos != 0xFFFFFFFFu;
+    if (__ballot(owner >= 0 && !own) != 0ull) {
+      const int src = owner >= 0 ? owner : (int)lane;
+      const float od = __shfl(o_t, src, 64);
+      const uint32_t op = __shfl(o_pos, src, 64);
+      if (!own) {
+        dmax = od;
+        d_hit = op != 0xFFFFFFFFu;
+      }
+    }
+    bool job_end = false;
+    if (owner >= 0) {
+      float bound = own ? o_t : (j_pos != 0xFFFFFFFFu ? fminf(dmax, j_t) : dmax);
+      bool need_pop = false;
+      if (++visits > max_visits) {
+        job_end = true;
+      } else if (!(cur & kLeafBit)) {
+        if (cur >= n_tris) {
+          job_end = true;  // never taken for a tree built here
+        } else {
+          const FNode nd = nodes[cur];
+          const float a0 = __builtin_fmaf(nd.a.x, ix, -ox), a1 = __builtin_fmaf(nd.a.y, ix, -ox);
+          const float a2 = __builtin_fmaf(nd.a.z, iy, -oy), a3 = __builtin_fmaf(nd.a.w, iy, -oy);
+          const float a4 = __builtin_fmaf(nd.c.x, iz, -oz), a5 = __builtin_fmaf(nd.c.y, iz, -oz);
+          const float b0 = __builtin_fmaf(nd.b.x, ix, -ox), b1 = __builtin_fmaf(nd.b.y, ix, -ox);
+          const float b2 = __builtin_fmaf(nd.b.z, iy, -oy), b3 = __builtin_fmaf(nd.b.w, iy, -oy);
+          const float b4 = __builtin_fmaf(nd.c.z, iz, -oz), b5 = __builtin_fmaf(nd.c.w, iz, -oz);
+          const float tmin0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
+          const float tmax0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), bound));
+          const float tmin1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
+          const float tmax1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), bound));
+          const bool h0 = tmin0 <= tmax0, h1 = tmin1 <= tmax1;
+          if (kCount) n_box += 2;
+          if (h0 && h1) {
+            const bool first0 = tmin0 <= tmin1;
+            if (sp < kStackN) {
+              st_set(sp, first0 ? nd.d.y : nd.d.x, first0 ? tmin1 : tmin0);
+              ++sp;
+            }
+            cur = first0 ? nd.d.x : nd.d.y;
+          } else if (h0) {
+            cur = nd.d.x;
+          } else if (h1) {
+            cur = nd.d.y;
+          } else {
+            need_pop = true;
+          }
+        }
+      } else {
+        const uint32_t first = (cur & ~kLeafBit) >> 4;
+        const uint32_t cnt = first < n_tris ? min((cur & 15u) + 1, n_tris - first) : 0u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+          const float4 r0 = tris[3 * (size_t)(first + k) + 0];
+          float4 v0 = r0;
+          v0.w = 0.0f;
+          const float4 e1 = tris[3 * (size_t)(first + k) + 1], e2 = tris[3 * (size_t)(first + k) + 2];
+          float t, u, v;
+          if (kCount) ++n_tri;
+          if (tri_ray(jo, jd, v0, e1, e2, eps, t, u, v)) {
+            const uint32_t oid = __float_as_uint(r0.w);
+            if (own) {
+              if (t < o_t || (o_pos != 0xFFFFFFFFu && t == o_t && oid < o_oid)) {
+                o_t = t; o_pos = first + k; o_oid = oid; o_u = u; o_v = v;
+              }
+            } else if ((t < dmax || (d_hit && t == dmax)) &&
+                       (j_pos == 0xFFFFFFFFu || t < j_t || (t == j_t && oid < j_oid))) {
+              j_t = t; j_pos = first + k; j_oid = oid; j_u = u; j_v = v;
+            }
+          }
+        }
+        need_pop = true;
+      }
+      if (need_pop) {
+        // the nearest pending entry that can still hold a closer hit
+        bound = own ? o_t : (j_pos != 0xFFFFFFFFu ? fminf(dmax, j_t) : dmax);
+        bool found = false;
+        while (sp > 0) {
+          --sp;
+          uint32_t r;
+          float t;
+          st_get(sp, r, t);
+          if (t <= bound) {
+            cur = r;
+            found = true;
+            break;
+          }
+        }
+        if (!found) job_end = true;
+      }
+    }
+    // merge finished helpers' hits into their owners, one at a time
+    uint64_t m = __ballot(job_end && !own && j_pos != 0xFFFFFFFFu);
+    while (m != 0ull) {
+      const int e = __builtin_ctzll(m);
+      m &= m - 1;
+      const int ow = __shfl(owner, e, 64);
+      const float ct = __shfl(j_t, e, 64);
+      const uint32_t coid = __shfl(j_oid, e, 64), cpos = __shfl(j_pos, e, 64);
+      const float cu = __shfl(j_u, e, 64), cv = __shfl(j_v, e, 64);
+      if ((int)lane == ow && (ct < o_t || (o_pos != 0xFFFFFFFFu && ct == o_t && coid < o_oid))) {
+        o_t = ct; o_pos = cpos; o_oid = coid; o_u = cu; o_v = cv;
+      }
+    }
+    if (job_end) owner = -1;
+  }
+  return RayHit{o_pos, o_t, o_u, o_v};
+}
+
 // Sort key of a ray: the 30-bit Morton code of its origin in the tree's root
 // box (the union of the root record's two child boxes), clamped to the box.
 __global__ __launch_bounds__(256) void k_ray_keys(const FNode* __restrict__ nodes, const float4* __restrict__ origins,
@@ -627,18 +842,20 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
                                                     unsigned long long* __restrict__ counters,
                                                     const uint32_t* __restrict__ order) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= n_rays) return;
-  const uint32_t l = order ? order[t] : t;
-  if (l >= n_rays) return;
+  uint32_t l = t < n_rays ? (order ? order[t] : t) : n_rays;
+  const bool has_ray = l < n_rays;
   __shared__ uint2 s_stack[kLdsStack * 256];
+  __shared__ uint32_t s_xchg[4][3 * 64];
   uint32_t n_box = 0, n_tri = 0;
-  const RayHit h = trace_ray<kCount, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, origins[l], dirs[l], depths[l],
-                                                             eps, n_box, n_tri, s_stack);
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const RayHit h = trace_ray_shared<kCount, kStack, kLdsStack, 256>(
+      nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
+      has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
     atomicAdd(counters, (unsigned long long)n_box);
     atomicAdd(counters + 1, (unsigned long long)n_tri);
   }
-  if (h.pos != 0xFFFFFFFFu) {
+  if (has_ray && h.pos != 0xFFFFFFFFu) {
     faces_id[l] = h.pos;
     coords[l] = make_float2(h.u, h.v);
     depths[l] = h.t;
@@ -873,18 +1090,24 @@ __global__ __launch_bounds__(256) void k_batch_ray_keys(const rk_batch_mesh* __r
   vals[r] = r;
 }
 
+// Closest hit of ray r (r >= n_rays: none, the lane only helps the others of
+// its wave), in its mesh's tree. Every lane of the wave calls it.
 template <int kStackN, int kWG>
 __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
                                                 const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                 uint32_t n_tris, const float4* __restrict__ origins,
-                                                const float4* __restrict__ dirs, uint32_t r,
+                                                const float4* __restrict__ dirs, uint32_t r, uint32_t n_rays,
                                                 uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                float* __restrict__ depths, float eps, uint2* lds_stack) {
-  const uint32_t m = mesh_of_ray(meshes, mesh_count, r);
+                                                float* __restrict__ depths, float eps, uint2* lds_stack,
+                                                uint32_t* xchg) {
+  const bool has_ray = r < n_rays;
+  const uint32_t root = has_ray ? meshes[mesh_of_ray(meshes, mesh_count, r)].face_offset : 0u;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t nb = 0, nt = 0;
-  const RayHit h = trace_ray<false, kStackN, kLdsStack, kWG>(nodes, tris, n_tris, meshes[m].face_offset, origins[r],
-                                                             dirs[r], depths[r], eps, nb, nt, lds_stack);
-  if (h.pos != 0xFFFFFFFFu) {
+  const RayHit h = trace_ray_shared<false, kStackN, kLdsStack, kWG>(
+      nodes, tris, n_tris, root, has_ray, has_ray ? origins[r] : zero4, has_ray ? dirs[r] : zero4,
+      has_ray ? depths[r] : 0.0f, eps, nb, nt, lds_stack, xchg);
+  if (has_ray && h.pos != 0xFFFFFFFFu) {
     faces_id[r] = h.pos;
     coords[r] = make_float2(h.u, h.v);
     depths[r] = h.t;
@@ -902,12 +1125,11 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
                                                           float* __restrict__ depths, float eps,
                                                           const uint32_t* __restrict__ order) {
   const uint32_t t = blockIdx.x * kWG + threadIdx.x;
-  if (t >= n_rays) return;
-  const uint32_t r = order ? order[t] : t;
-  if (r >= n_rays) return;
+  const uint32_t r = t < n_rays ? (order ? order[t] : t) : n_rays;
   __shared__ uint2 s_stack[kLdsStack * kWG];
-  batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords, depths,
-                                eps, s_stack);
+  __shared__ uint32_t s_xchg[kWG / 64][3 * 64];
+  batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
+                                depths, eps, s_stack, s_xchg[threadIdx.x >> 6]);
 }
 
 // Persistent grid: every wave takes 64-ray tiles from its XCD's queue
@@ -927,6 +1149,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
                                                            float eps, uint32_t* __restrict__ queues,
                                                            const uint32_t* __restrict__ order) {
   __shared__ uint2 s_stack[kLdsStack * kWG];
+  __shared__ uint32_t s_xchg[kWG / 64][3 * 64];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t n_tiles = (n_rays + 63) / 64;
   const uint32_t per = (n_tiles + kQueues - 1) / kQueues;
@@ -943,9 +1166,8 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
     }
     const uint32_t p = tile * 64 + lane;
     const uint32_t r = p < n_rays ? (order ? order[p] : p) : n_rays;
-    if (r < n_rays)
-      batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, faces_id, coords,
-                                    depths, eps, s_stack);
+    batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
+                                  depths, eps, s_stack, s_xchg[threadIdx.x >> 6]);
   }
 }
 
@@ -1154,12 +1376,14 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
   const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-  if (x >= width || y >= height) return;
+  const bool has_ray = x < width && y < height;
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
   __shared__ uint2 s_stack[kLdsStack * 256];
-  const RayHit h = trace_ray<false, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, ro, d, RK_HUGE, eps, nb, nt,
-                                                            s_stack);
+  __shared__ uint32_t s_xchg[4][3 * 64];
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
+                                                                   eps, nb, nt, s_stack, s_xchg[wave]);
+  if (!has_ray) return;
   const size_t p = (size_t)y * (size_t)width + (size_t)x;
   const bool hit = h.pos != 0xFFFFFFFFu;
   faces_id[p] = h.pos;
