@@ -493,119 +493,33 @@ __device__ __forceinline__ float safe_inv(float d) {
   return 1.0f / a;
 }
 
-// Closest hit of one ray in the tree rooted at 'root' (positions and node
-// records share one index space of n_tris). Depth-first, nearer child first,
-// a pending child is skipped once the hit found is nearer than its entry
-// point. Ties on t go to the smallest original face id, so the result does
-// not depend on the tree. Returns the sorted position of the hit or ~0.
+// A traversal's result: the sorted position of the closest hit (~0: none),
+// its t (the tmax it started with when there is no hit), u and v.
 struct RayHit {
   uint32_t pos;
   float t, u, v;
 };
 
-// The traversal stack: its first kLds entries live in LDS (lds: this
-// workgroup's kLds x kWG entries, entry e of thread x at e * kWG + x), the
-// rest in per-lane scratch. Pending entries rarely go deeper than a dozen or
-// two, and an LDS access takes a fraction of the time of a scratch access
-// that misses the L1 (the scratch of thousands of resident lanes does not fit
-// in it).
-template <bool kCount, int kStackN, int kLds = 0, int kWG = 256>
-__device__ __forceinline__ RayHit trace_ray(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
-                                            uint32_t n_tris, uint32_t root, float4 orig, float4 dir, float depth,
-                                            float eps, uint32_t& n_box, uint32_t& n_tri, uint2* lds = nullptr) {
-  const uint32_t max_visits = 4 * n_tris + 64;  // a tree visit touches each node once
-  uint32_t best = 0xFFFFFFFFu, best_oid = 0xFFFFFFFFu;
-  float bu = 0.0f, bv = 0.0f;
-  const float ix = safe_inv(dir.x), iy = safe_inv(dir.y), iz = safe_inv(dir.z);
-  const float ox = orig.x * ix, oy = orig.y * iy, oz = orig.z * iz;
-  static_assert(kStackN > kLds, "scratch part of the stack");
-  uint32_t stack_ref[kStackN - kLds];
-  float stack_t[kStackN - kLds];
-  uint2* my_lds = kLds > 0 ? lds + threadIdx.x : nullptr;
-  int sp = 0;
-  uint32_t cur = root;
-  uint32_t visits = 0;
-  while (++visits <= max_visits) {
-    if (!(cur & kLeafBit)) {
-      if (cur >= n_tris) break;  // never taken for a tree built here
-      const FNode nd = nodes[cur];
-      const float a0 = __builtin_fmaf(nd.a.x, ix, -ox), a1 = __builtin_fmaf(nd.a.y, ix, -ox);
-      const float a2 = __builtin_fmaf(nd.a.z, iy, -oy), a3 = __builtin_fmaf(nd.a.w, iy, -oy);
-      const float a4 = __builtin_fmaf(nd.c.x, iz, -oz), a5 = __builtin_fmaf(nd.c.y, iz, -oz);
-      const float b0 = __builtin_fmaf(nd.b.x, ix, -ox), b1 = __builtin_fmaf(nd.b.y, ix, -ox);
-      const float b2 = __builtin_fmaf(nd.b.z, iy, -oy), b3 = __builtin_fmaf(nd.b.w, iy, -oy);
-      const float b4 = __builtin_fmaf(nd.c.z, iz, -oz), b5 = __builtin_fmaf(nd.c.w, iz, -oz);
-      const float tmin0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
-      const float tmax0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), depth));
-      const float tmin1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
-      const float tmax1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), depth));
-      const bool h0 = tmin0 <= tmax0, h1 = tmin1 <= tmax1;
-      if (kCount) n_box += 2;
-      if (h0 && h1) {
-        const bool first0 = tmin0 <= tmin1;
-        const uint32_t nearr = first0 ? nd.d.x : nd.d.y, farr = first0 ? nd.d.y : nd.d.x;
-        const float fart = first0 ? tmin1 : tmin0;
-        if (sp < kLds) {
-          my_lds[sp * kWG] = make_uint2(farr, __float_as_uint(fart));
-          ++sp;
-        } else if (sp < kStackN) {
-          stack_ref[sp - kLds] = farr;
-          stack_t[sp - kLds] = fart;
-          ++sp;
-        }
-        cur = nearr;
-        continue;
-      }
-      if (h0) { cur = nd.d.x; continue; }
-      if (h1) { cur = nd.d.y; continue; }
-    } else {
-      const uint32_t first = (cur & ~kLeafBit) >> 4;
-      const uint32_t cnt = first < n_tris ? min((cur & 15u) + 1, n_tris - first) : 0u;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const float4 r0 = tris[3 * (size_t)(first + k) + 0];
-        float4 v0 = r0;
-        v0.w = 0.0f;
-        const float4 e1 = tris[3 * (size_t)(first + k) + 1], e2 = tris[3 * (size_t)(first + k) + 2];
-        float t, u, v;
-        if (kCount) ++n_tri;
-        if (tri_ray(orig, dir, v0, e1, e2, eps, t, u, v)) {
-          const uint32_t oid = __float_as_uint(r0.w);
-          if (t < depth || (best != 0xFFFFFFFFu && t == depth && oid < best_oid)) {
-            depth = t; best = first + k; best_oid = oid; bu = u; bv = v;
-          }
-        }
-      }
-    }
-    // pop the nearest pending entry that can still hold a closer hit
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      uint32_t ref;
-      float t;
-      if (sp < kLds) {
-        const uint2 e = my_lds[sp * kWG];
-        ref = e.x;
-        t = __uint_as_float(e.y);
-      } else {
-        ref = stack_ref[sp - kLds];
-        t = stack_t[sp - kLds];
-      }
-      if (t <= depth) { cur = ref; found = true; break; }
-    }
-    if (!found) break;
-  }
-  return RayHit{best, depth, bu, bv};
-}
-
-// Wave-shared traversal. Every lane starts on its own ray (has_ray); a lane
-// with nothing left to do takes the newest pending subtree of a lane that
-// still has some (one per donor and step) and traverses it for that lane's
-// ray, culling against the owner's best hit so far. A helper's best hit is
-// merged into its owner's when the subtree is done, by (t, original id) as in
-// trace_ray, so each lane's result equals trace_ray's for its ray. A frame
-// then waits less on its few long rays (grazing rays near a horizon, which
-// would otherwise keep a wave running with one active lane). Every lane of
-// the wave must call it; xchg is this wave's 3 x 64 words of LDS.
+// Closest hit of each lane's ray in the tree rooted at 'root' (positions and
+// node records share one index space of n_tris). Depth-first, nearer child
+// first; a pending child is skipped once the hit found is nearer than its
+// entry point. Ties on t go to the smallest original face id, so the result
+// does not depend on the tree or on the order of the visits.
+//
+// The wave shares the work. Every lane starts on its own ray (has_ray); a
+// lane with nothing left to do takes the newest pending subtree of a lane
+// that still has some (one per donor and step) and traverses it for that
+// lane's ray, culling against the owner's best hit so far. A helper's best
+// hit is merged into its owner's when the subtree is done, by (t, original
+// id). A frame then waits less on its few long rays (grazing rays near a
+// horizon), which would otherwise keep a wave running with one active lane.
+// Every lane of the wave must call it; xchg is this wave's 3 x 64 words of LDS.
+//
+// The stack: its first kLds entries live in LDS (lds: this workgroup's
+// kLds x kWG entries, entry e of thread x at e * kWG + x), the rest in
+// per-lane scratch. Pending entries rarely go deeper than a dozen or two, and
+// an LDS access takes a fraction of the time of a scratch access that misses
+// the L1 (the scratch of thousands of resident lanes does not fit in it).
 template <bool kCount, int kStackN, int kLds, int kWG>
 __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                    uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,

@@ -332,6 +332,54 @@ def test_fast_tree_closest_hit_soup(dev, rk, ora, mode):
     assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, 4) > 0.05
 
 
+def _flat_grid(g):
+    """A flat (g-1)x(g-1) grid in z = 0 on [0, 1]^2 with spacing 1/(g-1), two
+    faces per cell; with g - 1 a power of two every vertex is exact."""
+    xs = np.arange(g, dtype=np.float32) / np.float32(g - 1)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    V = np.stack([X.ravel(), Y.ravel(), np.zeros(g * g, np.float32), np.zeros(g * g, np.float32)], 1)
+    F = []
+    for j in range(g - 1):
+        for i in range(g - 1):
+            a, b, c, d = j * g + i, j * g + i + 1, (j + 1) * g + i, (j + 1) * g + i + 1
+            F.append((a, b, d, 0))
+            F.append((a, d, c, 0))
+    return np.ascontiguousarray(V, np.float32), np.asarray(F, np.uint32)
+
+
+@pytest.mark.parametrize("leaf_max", [1, 2, 4])
+def test_fast_trace_exact_ties_in_shared_waves(dev, rk, ora, leaf_max):
+    """Rays through grid vertices hit up to six faces at exactly the same t;
+    grazing rays cross the whole grid and keep their lanes busy while the
+    others finish and help them. Within each wave the three kinds are mixed,
+    so subtrees move between lanes and ties are resolved across lanes: every
+    hit must equal brute force's, ties to the smallest original face id."""
+    g = 65
+    V, F = _flat_grid(g)
+    rng = np.random.default_rng(17)
+    n = 4096
+    O = np.zeros((n, 4), np.float32)
+    D = np.zeros((n, 4), np.float32)
+    vi = rng.integers(0, g, size=(n, 2)).astype(np.float32) / np.float32(g - 1)
+    kind = np.arange(n) % 4
+    # straight down through a vertex: t = 1 exactly for every face at it
+    O[:, 0], O[:, 1], O[:, 2] = vi[:, 0], vi[:, 1], 1.0
+    D[:, 2] = -1.0
+    # tilted through a vertex
+    tilt = kind == 1
+    O[tilt, 0] = vi[tilt, 0] + 0.25
+    O[tilt, 1] = vi[tilt, 1] - 0.125
+    D[tilt, 0], D[tilt, 1], D[tilt, 2] = -0.25, 0.125, -1.0
+    # grazing: across the grid just above it, descending slowly
+    graze = kind >= 2
+    O[graze, 0] = -0.25
+    O[graze, 1] = rng.random(graze.sum()).astype(np.float32)
+    O[graze, 2] = 0.0005
+    D[graze, 0], D[graze, 1], D[graze, 2] = 1.0, 0.0, -0.0005
+    hit = _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 1, leaf_max, det_epsilon=0.0)
+    assert hit > 0.9
+
+
 @pytest.mark.parametrize("nr", [1, 255, 257, 4000])
 def test_fast_trace_sorted_closest_hit_soup(dev, rk, ora, nr):
     """Rays traced in origin-Morton order (a key kernel, a radix sort, then
