@@ -54,12 +54,17 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v) {
   return v;
 }
 
+// Also clears the build's info words and the hierarchy's split counters
+// (flags, n words), which saves two fill launches per build.
 template <typename K>
 __global__ __launch_bounds__(256) void k_fast_prep(const uint4* __restrict__ faces, const float4* __restrict__ verts,
                                                    uint32_t n, float4 rmin, float4 rmax, Box* __restrict__ boxes,
-                                                   K* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                   K* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ info, uint32_t* __restrict__ flags) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k < RK_INFO_WORDS) info[k] = 0u;
   if (k >= n) return;
+  flags[k] = 0u;
   uint4 f = faces[k];
   float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
   Box b;
@@ -810,9 +815,11 @@ inline bool take_hier_ws(RkArena& ar, uint32_t n, HierWs& w) {
 template <typename K, bool kBatch>
 rk_status launch_hierarchy(hipStream_t st, const K* keys, const uint32_t* order, const uint4* f, const float4* v,
                            uint32_t n, uint32_t leaf_max, const HierWs& w, FNode* nodes, float4* tris,
-                           const rk_batch_mesh* meshes) {
-  rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), w.flags, n, 0u);
-  if (s != RK_SUCCESS) return s;
+                           const rk_batch_mesh* meshes, bool flags_clear = false) {
+  if (!flags_clear) {
+    rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), w.flags, n, 0u);
+    if (s != RK_SUCCESS) return s;
+  }
   const unsigned chunks = blocks_for(n, kChunk);
   const uint32_t inline_top = n <= kInlineTopFaces ? 1u : 0u;
   hipLaunchKernelGGL((k_fast_hierarchy<K, kBatch>), dim3(chunks), dim3(kLocalThreads), 0, st, keys, order, f, v, n,
@@ -843,14 +850,14 @@ static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v
                                  float4 rmax, const rk_fast_bvh* bvh, uint32_t leaf_max, K* keys, const HierWs& hw,
                                  void* sort_ws, size_t sort_bytes, uint32_t* info) {
   uint32_t* order = bvh->indices;
-  hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
-                     reinterpret_cast<Box*>(bvh->boxes), keys, order);
+  hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(n < RK_INFO_WORDS ? RK_INFO_WORDS : n, 256)), dim3(256), 0, st, f,
+                     v, n, rmin, rmax, reinterpret_cast<Box*>(bvh->boxes), keys, order, info, hw.flags);
   RK_CHECK_LAUNCH();
   rk_status s = rk_radix_sort_impl<K>(st, keys, order, n, 0, (int)sizeof(K), sort_ws, sort_bytes,
                                       info + RK_INFO_FLAGS);
   if (s != RK_SUCCESS) return s;
   return launch_hierarchy<K, false>(st, keys, order, f, v, n, leaf_max, hw, static_cast<FNode*>(bvh->nodes),
-                                    static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr);
+                                    static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr, true);
 }
 
 static rk_status fast_build_common(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
@@ -873,8 +880,6 @@ static rk_status fast_build_common(rk_stream stream, const rk_uint4* faces, cons
   void* sort_ws = ar.take<char>(sort_bytes);
   if (!keys || !hw_ok || !sort_ws) return RK_ERROR_WORKSPACE;
   if (sorted_keys) keys = sorted_keys;
-  rk_status s;
-  if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
   float4 rmin = make_float4(hitbox->min.x, hitbox->min.y, hitbox->min.z, hitbox->min.w);
   float4 rmax = make_float4(hitbox->max.x, hitbox->max.y, hitbox->max.z, hitbox->max.w);
   const uint4* f = reinterpret_cast<const uint4*>(faces);
