@@ -1106,7 +1106,7 @@ bool config_ok(const rk_batch_config* c) {
 
 extern "C" rk_batch_config rk_batch_config_default(void) {
   rk_batch_config c = {};
-  c.leaf_max = 2;
+  c.leaf_max = 1;
   c.trace_mode = 1;
   c.trace_wg = 256;
   c.stack_depth = 64;

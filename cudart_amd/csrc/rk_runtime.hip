@@ -227,7 +227,7 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
     (void)hipEventDestroy(e0);
     return RK_ERROR_DEVICE;
   }
-  static const uint32_t leaves[] = {2, 4, 8};
+  static const uint32_t leaves[] = {1, 2, 4, 8};
   static const uint32_t wgs[] = {64, 128, 256};
   static const uint32_t stacks[] = {64, 96};
   float samples[64];
@@ -245,7 +245,7 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
     *out_ms = median_of(samples, reps);
     return RK_SUCCESS;
   };
-  for (uint32_t li = 0; li < 3 && s == RK_SUCCESS; ++li) {
+  for (uint32_t li = 0; li < 4 && s == RK_SUCCESS; ++li) {
     rk_batch_config c = rk_batch_config_default();
     c.leaf_max = leaves[li];
     float build_ms = 0.0f;

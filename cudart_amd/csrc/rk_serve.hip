@@ -275,7 +275,7 @@ struct rk_server_st {
 extern "C" rk_server_config rk_server_config_default(void) {
   rk_server_config c = {};
   c.slots = 16;
-  c.leaf_max = 2;
+  c.leaf_max = 1;
   c.rebuild_every = 8;
   c.max_faces = 2u << 20;
   c.max_pixels = 1024u * 1024u;

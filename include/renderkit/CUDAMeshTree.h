@@ -35,7 +35,7 @@ class CUDAMeshTree {
   // The throughput tree: Karras LBVH with 30-bit (or 63-bit) Morton keys and
   // leaves of <= leaf_max faces. Faces are reordered in leaf order, like the
   // reference builds; isFast() trees are traced by rk_fast_trace.
-  void buildMeshTreeFast(uint leaf_max = 2, bool morton63 = false);
+  void buildMeshTreeFast(uint leaf_max = 1, bool morton63 = false);
   bool isFast() const { return fast_nodes != nullptr; }
   void* getFastNodes() const { return fast_nodes ? fast_nodes->getPointer() : nullptr; }
   void* getFastTriangles() const { return fast_tris ? fast_tris->getPointer() : nullptr; }

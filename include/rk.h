@@ -309,7 +309,7 @@ typedef struct rk_batch_config {
   uint32_t ray_order;
   uint32_t reserved[3];
 } rk_batch_config;
-/* leaf_max 2, persistent queues, 256-thread workgroups, stack 64, rays as given. */
+/* leaf_max 1, persistent queues, 256-thread workgroups, stack 64, rays as given. */
 rk_batch_config rk_batch_config_default(void);
 
 /* Builds every mesh of the batch: per-mesh 30-bit Morton keys in each mesh's
@@ -346,7 +346,7 @@ rk_status rk_graph_destroy(rk_graph graph);
 
 /* ---- autotuning ------------------------------------------------------------------
  * Times candidate configurations on the caller's own batch: a build for each
- * leaf_max in {2, 4, 8}, then each trace variant (rays as given or sorted,
+ * leaf_max in {1, 2, 4, 8}, then each trace variant (rays as given or sorted,
  * both modes, workgroups of 64, 128 and 256, stacks of 64 and 96), each the median of 'reps' runs (1..64)
  * timed with HIP events on 'stream', and writes the fastest build + trace pair
  * to *best (its time to *best_ms when not NULL). Each trace starts from
@@ -389,7 +389,7 @@ typedef struct rk_server_config {
   uint64_t hbm_budget;     /* cap on arena_bytes */
   uint32_t reserved[4];
 } rk_server_config;
-/* 16 slots, leaf 2, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
+/* 16 slots, leaf 1, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
  * amplitude 1% of extent, omega 2 pi, budget 200 GiB. Slots are streams: they
  * overlap only on distinct hardware queues (GPU_MAX_HW_QUEUES, 4 by default). */
 rk_server_config rk_server_config_default(void);
