@@ -380,6 +380,27 @@ def test_fast_trace_exact_ties_in_shared_waves(dev, rk, ora, leaf_max):
     assert hit > 0.9
 
 
+@pytest.mark.parametrize("dist", [1e2, 1e3, 1e4])
+def test_fast_trace_far_origins(dev, rk, ora, dist):
+    """Rays from far away (dist times the scene's size) aimed at points of a
+    small soup: the padded box test stays conservative while the rounding of
+    the origin term (|o| * 2^-24) is small against the padding, so every hit
+    equals brute force's. At 1e6 times the scene's size it is not (half the
+    hits are lost at box boundaries): the documented limit of the fast trace."""
+    V, F = rk.soup(2000, seed=21, mu=float(np.log(0.05)))
+    rng = np.random.default_rng(int(dist))
+    n = 2048
+    target = rng.random((n, 3)).astype(np.float64)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    O = np.zeros((n, 4), np.float32)
+    D = np.zeros((n, 4), np.float32)
+    O[:, :3] = (target - d * dist).astype(np.float32)
+    D[:, :3] = d.astype(np.float32)
+    hit = _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, 2, 1, det_epsilon=0.0)
+    assert hit > 0.02
+
+
 @pytest.mark.parametrize("nr", [1, 255, 257, 4000])
 def test_fast_trace_sorted_closest_hit_soup(dev, rk, ora, nr):
     """Rays traced in origin-Morton order (a key kernel, a radix sort, then
