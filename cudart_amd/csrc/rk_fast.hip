@@ -18,8 +18,11 @@
 //                     float4 (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y),
 //                     float4 (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z),
 //                     uint4  (ref0, ref1, count0, count1)
-// ref: node index (root 0, the split between sorted positions p and p+1 is
-// p + 1), or kLeafBit | first << 4 | (count - 1). n records for n faces.
+// ref: node index, or kLeafBit | first << 4 | (count - 1). The root is
+// record 0 (a batch mesh's root: its first position); a node covering sorted
+// positions [l, r] is record r when it is its parent's left child and record
+// l when it is the right child, so the children of the split between p and
+// p + 1 are records p and p + 1. n records for n faces.
 #include "rk_device.h"
 
 template <typename K>
@@ -198,12 +201,16 @@ __device__ __forceinline__ bool left_of_parent(bool has_l, bool has_r, uint64_t 
 
 // Completes split p from its two children: writes its record unless it
 // collapses into a leaf, and returns the merged part (end not set).
-// The tree's positions are [lo, hi]; its root record goes to lo.
+// The tree's positions are [lo, hi]; its root record goes to lo. Any other
+// node [nl, nr] goes to nr when it is its parent's left child (next_left)
+// and to nl when it is the right child, so the two children of a split p sit
+// in records p and p + 1, next to each other (the numbering of Karras 2012).
 __device__ __forceinline__ Part finish_split(FNode* __restrict__ nodes, uint32_t lo, uint32_t hi, uint32_t leaf_max,
-                                             uint32_t p, uint32_t nl, uint32_t nr, const Part& lc, const Part& rc) {
+                                             uint32_t p, uint32_t nl, uint32_t nr, const Part& lc, const Part& rc,
+                                             bool next_left) {
   const uint32_t cnt = nr - nl + 1;
   const bool root = nl == lo && nr == hi;
-  const uint32_t idx = root ? lo : p + 1;
+  const uint32_t idx = root ? lo : (next_left ? nr : nl);
   if (root || cnt > leaf_max) write_node(nodes, idx, lc, rc, p - nl + 1, nr - p);
   Part up = merge_parts(lc, rc);
   up.ref = (!root && cnt <= leaf_max) ? (kLeafBit | (nl << 4) | (cnt - 1)) : idx;
@@ -264,15 +271,20 @@ __device__ __forceinline__ void climb_crossing(const K* __restrict__ keys, uint3
     if (old == 0) return;
     const Part sib = load_part_coherent(gparts + other);
     const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
-    me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
-                 : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
+    const bool root = nl == lo && nr == hi;  // the (mesh) root
+    bool next_left = false;
+    if (!root) {
+      const bool has_l = nl > 0, has_r = nr + 1 < n;
+      const uint64_t kl = has_l ? split_key(keys, nl - 1) : ~0ull;
+      const uint64_t kr = has_r ? split_key(keys, nr) : ~0ull;
+      next_left = left_of_parent(has_l, has_r, kl, kr);
+    }
+    me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib, next_left)
+                 : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me, next_left);
+    if (root) return;
     l = nl;
     r = nr;
-    if (l == lo && r == hi) return;  // the (mesh) root
-    const bool has_l = l > 0, has_r = r + 1 < n;
-    const uint64_t kl = has_l ? split_key(keys, l - 1) : ~0ull;
-    const uint64_t kr = has_r ? split_key(keys, r) : ~0ull;
-    as_left = left_of_parent(has_l, has_r, kl, kr);
+    as_left = next_left;
   }
 }
 
@@ -390,14 +402,18 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       write_node(nodes, lo, me, empty, 1, 0);
       return;
     }
-    uint32_t l = i, r = i;
-    // Each step widens [l, r] by at least one position: n steps bound it.
-    for (uint32_t guard = 0; guard < n; ++guard) {
+    // the side of [l, r] under its parent, from the split keys at its ends
+    auto side = [&](uint32_t l, uint32_t r) {
       const bool has_l = l > 0, has_r = r + 1 < n;
       uint64_t kl = ~0ull, kr = ~0ull;
       if (has_l) kl = (l >= c0 && l - c0 <= kChunk) ? s_key[l - c0] : split_key(keys, l - 1);
       if (has_r) kr = (r + 1 >= c0 && r + 1 - c0 <= kChunk) ? s_key[r + 1 - c0] : split_key(keys, r);
-      const bool as_left = left_of_parent(has_l, has_r, kl, kr);
+      return left_of_parent(has_l, has_r, kl, kr);
+    };
+    uint32_t l = i, r = i;
+    bool as_left = side(l, r);
+    // Each step widens [l, r] by at least one position: n steps bound it.
+    for (uint32_t guard = 0; guard < n; ++guard) {
       const uint32_t p = as_left ? r : l - 1;
       const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
       me.end = as_left ? l : r;
@@ -418,11 +434,14 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       if (old == 0) return;
       const Part sib = s_part[other - c0];
       const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
-      me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib)
-                   : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me);
+      const bool root = nl == lo && nr == hi;  // the (mesh) root
+      const bool next_left = root ? false : side(nl, nr);
+      me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib, next_left)
+                   : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me, next_left);
+      if (root) return;
       l = nl;
       r = nr;
-      if (l == lo && r == hi) return;  // the (mesh) root
+      as_left = next_left;
     }
   };
   climb();
