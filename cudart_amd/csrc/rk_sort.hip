@@ -40,7 +40,10 @@ constexpr uint32_t kInc = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 20;
 constexpr int kMaxPasses = 8;
-constexpr int kLook = 8;  // look-back status loads in flight per digit
+#ifndef RK_SORT_LOOK
+#define RK_SORT_LOOK 8
+#endif
+constexpr int kLook = RK_SORT_LOOK;  // look-back status loads in flight per digit
 
 // Exclusive scan over the first 256 threads of the block (one value per
 // digit); every thread of the block calls it, threads 256.. get no result.
