@@ -12,8 +12,10 @@ rejects the grazing hits on this mesh, whose edges are ~1.4e-3 so that
 |det| ~ 1e-6; that frame is reported beside it (mrays_s_reference_epsilon,
 hit_fraction_reference_epsilon). The same frame with the rays formed inside
 the trace kernel is camera_fused_mrays_s. lbvh_mtri_s times rk_fast_build
-(30-bit Morton, leaves <= 4 faces). The reference tree (rk_build_lbvh, depth
-12) and both reference traversals are timed once on the same frame.
+(30-bit Morton, one face per leaf). The reference tree (rk_build_lbvh, depth
+12) and both reference traversals are timed once on the same frame, and so
+are the reference's 32-bin SAH build (rk_build_sah), its per-ray traversal of
+the SAH tree, and the Hybrid build (depth-12 LBVH, each leaf refined by SAH).
 
 Further legs, each skippable:
 - soup: 10M random faces, 63-bit keys, 2^24 incoherent rays; build and
@@ -54,6 +56,10 @@ FP32_PEAK_TFLOPS = 157.3
 OPS_PER_BOX_TEST = 24   # algorithmic FP32 ops of one slab test
 OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 0.0
+# HBM bytes per launch of the headline k_fast_trace, from two rocprofv3 --pmc
+# runs of `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
+# FETCH_SIZE 146,000 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,634 KiB.
+TRACE_TRAFFIC_BYTES = (146_000 * 2 + 3_634) * 1024
 
 
 def log(*a):
@@ -444,9 +450,12 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": TRACE_TRAFFIC_BYTES,
+            "algorithmic_bytes": 52 * nr,
             "note": "algorithmic FP32 ops = 24 per slab test + 53 per triangle test, counted on this frame; "
-                    "52 B/ray of compulsory HBM traffic makes this kernel VALU/latency-bound, not HBM-bound",
+                    "52 B/ray of compulsory HBM traffic makes this kernel VALU/latency-bound, not HBM-bound; "
+                    "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs (rays, outputs "
+                    "and node/triangle records fetched past the L2)",
         },
     }
 
@@ -468,6 +477,37 @@ def main():
         result["reference_per_ray_mrays_s"] = round(nr / (ref_ray_ms * 1e-3) / 1e6, 3)
         result["reference_packet_mrays_s"] = round(nr / (ref_pkt_ms * 1e-3) / 1e6, 3)
         result["reference_node_count"] = rtree.node_count
+
+        log("reference 32-bin SAH and Hybrid builds")
+        Fs = Ft.clone()
+        sb = dev.SAHBuilder(n)
+        sah = lambda: sb.build(Fs, Vt, hb)  # rk_build_sah waits for each level, so it returns built
+        sah()
+        sah_ms = timer.time(sah, 5)
+        stree = sb.build(Ft.clone(), Vt, hb)
+        torch.cuda.synchronize()
+        stree.node_count = int(stree.host_info()[native.RK_INFO_NODE_COUNT])
+        sf, suv, sdep = dev.new_outputs(nr)
+        sah_ray_ms = timer.time(lambda: dev.trace_ref(stree, Vt, Ot, Dt, W, H, False, out=(sf, suv, sdep)), 1)
+        hl = dev.LBVHBuilder(n)
+        hs = dev.SAHBuilder(n, 1 << 13)  # a depth-12 LBVH has at most 2^13 - 1 nodes
+        Fh = Ft.clone()
+
+        def hybrid():
+            lb = hl.build(Fh, Vt, hb, 12, False)
+            torch.cuda.synchronize()
+            lb.node_count = int(lb.host_info()[native.RK_INFO_NODE_COUNT])
+            return hs.build(Fh, Vt, hb, lbvh=lb)
+
+        hybrid()
+        hybrid_ms = timer.time(hybrid, 5)
+        result["reference_sah_mtri_s"] = round(n / (sah_ms * 1e-3) / 1e6, 2)
+        result["reference_sah_build_ms"] = round(sah_ms, 3)
+        result["reference_sah_node_count"] = stree.node_count
+        result["reference_sah_per_ray_mrays_s"] = round(nr / (sah_ray_ms * 1e-3) / 1e6, 3)
+        result["reference_sah_same_depths"] = bool(torch.equal(sdep, rdep))
+        result["reference_hybrid_mtri_s"] = round(n / (hybrid_ms * 1e-3) / 1e6, 2)
+        result["reference_hybrid_build_ms"] = round(hybrid_ms, 3)
 
     if not args.no_soup:
         log("soup 10M faces, 63-bit keys, 2^24 incoherent rays")

@@ -7,18 +7,22 @@
 //   k_sah_prep       chunk offsets per item (one workgroup scan), accumulator reset
 //   k_sah_bounds     centroid bounds per item (ordered-int atomics)
 //   k_sah_bins       32 bins per item: count + primitive-box bounds (LDS, then atomics)
-//   k_sah_decide     one lane per item: the reference sweep (:424-474), leaf
-//                    test (:479-496), children boxes (:501-530), node record,
-//                    next-level items; child ids in breadth-first order
+//   k_sah_decide     one half-wave per item, one lane per bin: the reference
+//                    sweep (:424-474), leaf test (:479-496), children boxes
+//                    (:501-530)
+//   k_sah_emit       1024 items per workgroup: scan of the internal items,
+//                    node records, next-level items; child ids in
+//                    breadth-first order
 //   k_sah_count      per-chunk left counts
-//   k_sah_offsets    per-item exclusive scan over chunks
+//   k_sah_offsets    per-item exclusive scan over chunks, one wave per item
 //   k_sah_scatter    stable partition (bin <= m_min goes left, :572-594)
 //   k_sah_copyback   partitioned ranges back into indices
-// Min/max are exact in any order, so the atomics reproduce the reference's
-// serial merges; the sweep itself runs in the reference's order on one lane.
-// Node numbering: items are processed level by level in queue order, and the
-// k-th internal item of a level takes children (count + 2k, count + 2k + 1),
-// the breadth-first equivalent of the reference's atomicAdd(node_count, 2).
+// Min/max are exact in any order, so the atomics and the sweep's scans
+// reproduce the reference's serial merges; each cost is the reference's
+// expression on the same values. Node numbering: items are processed level by
+// level in queue order, and the k-th internal item of a level takes children
+// (count + 2k, count + 2k + 1), the breadth-first equivalent of the
+// reference's atomicAdd(node_count, 2).
 #include "rk_device.h"
 
 namespace {
@@ -27,6 +31,14 @@ using rk::Box;
 using rk::Node;
 
 constexpr uint32_t kChunk = 1024;
+// threads per workgroup of the per-chunk kernels (bounds, bins, count,
+// scatter, copyback): one chunk each
+#ifndef RK_SAH_CHUNK_THREADS
+#define RK_SAH_CHUNK_THREADS 128
+#endif
+constexpr int kCT = RK_SAH_CHUNK_THREADS;
+static_assert(kCT % 64 == 0 && kCT <= 256, "whole waves, at most 4");
+constexpr uint32_t kEmitGroup = 1024;  // items per workgroup of k_sah_emit
 constexpr int kBins = 32;
 constexpr uint32_t kLeafMin = 16;    // MIN_LEAF_SIZE, :55
 constexpr uint32_t kLeafMax = 224;   // MAX_LEAF_SIZE, :56
@@ -49,7 +61,9 @@ struct Decision {
   int m_min;        // -1: leaf
   uint32_t lcount;
   uint32_t pad[3];
+  float child[12];  // internal items: left min xyz, left max xyz, right min xyz, right max xyz
 };
+static_assert(sizeof(Decision) == 80, "decision is 80 B");
 
 __device__ __forceinline__ int f2o(float f) {  // order-preserving float -> int
   int i = __float_as_int(f);
@@ -93,14 +107,23 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
 }
 
 // ctl words: [0] items this level (m), [1] chunks this level (T), [2] items next level,
-// [3] node count, [4] flags
+// [3] node count, [4] flags, [5] node count after this level. k_sah_emit writes
+// [2] and [5]; the next level's k_sah_prep (advance) moves them to [0] and [3].
 __global__ __launch_bounds__(1024) void k_sah_prep(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
-                                                   uint32_t* __restrict__ chunk_base, int* __restrict__ acc) {
+                                                   uint32_t* __restrict__ chunk_base, int* __restrict__ acc,
+                                                   uint32_t* __restrict__ group_internal, uint32_t advance) {
   __shared__ uint32_t s_tmp[16];
   __shared__ uint32_t s_carry;
-  const uint32_t m = ctl[0];
-  if (threadIdx.x == 0) s_carry = 0;
+  if (threadIdx.x == 0) {
+    s_carry = 0;
+    if (advance) {
+      ctl[0] = ctl[2];
+      ctl[3] = ctl[5];
+    }
+  }
   __syncthreads();
+  const uint32_t m = ctl[0];
+  for (uint32_t g = threadIdx.x; g < (m + kEmitGroup - 1) / kEmitGroup; g += 1024) group_internal[g] = 0u;
   for (uint32_t c0 = 0; c0 < m; c0 += 1024) {
     uint32_t i = c0 + threadIdx.x;
     uint32_t nch = 0;
@@ -144,7 +167,7 @@ __device__ __forceinline__ float center_axis(const Box& b, int a) {  // 0.5f * (
   return a == 0 ? (b.mn.x + b.mx.x) * 0.5f : (a == 1 ? (b.mn.y + b.mx.y) * 0.5f : (b.mn.z + b.mx.z) * 0.5f);
 }
 
-__global__ __launch_bounds__(256) void k_sah_bounds(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+__global__ __launch_bounds__(kCT) void k_sah_bounds(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
                                                     const uint32_t* __restrict__ chunk_base,
                                                     const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                     int* __restrict__ acc) {
@@ -152,13 +175,13 @@ __global__ __launch_bounds__(256) void k_sah_bounds(const Item* __restrict__ ite
   if (!r.ok) return;
   if (blockIdx.x == chunk_base[r.item]) {  // the item's first chunk resets its bins for k_sah_bins
     int* bins = acc + (size_t)r.item * kAccWords + 6;
-    for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+    for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
       int k = w % 7;
       bins[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
     }
   }
   float mn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, mx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
-  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) {
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
     Box b = rk::load_box(boxes + indices[l]);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -174,11 +197,20 @@ __global__ __launch_bounds__(256) void k_sah_bounds(const Item* __restrict__ ite
       mn[a] = rk::fmn(mn[a], __shfl_xor(mn[a], off, 64));
       mx[a] = rk::fmx(mx[a], __shfl_xor(mx[a], off, 64));
     }
-  if ((threadIdx.x & 63) == 0) {
+  // one set of atomics per workgroup: an item's chunks all update the same
+  // six words, and same-address atomics serialise
+  __shared__ float s_red[kCT / 64][6];
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 3; ++k) { s_red[w][k] = mn[k]; s_red[w][3 + k] = mx[k]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
     int* a = acc + (size_t)r.item * kAccWords;
     for (int k = 0; k < 3; ++k) {
-      atomicMin(a + k, f2o(mn[k]));
-      atomicMax(a + 3 + k, f2o(mx[k]));
+      float lo = s_red[0][k], hi = s_red[0][3 + k];
+      for (int q = 1; q < kCT / 64; ++q) { lo = rk::fmn(lo, s_red[q][k]); hi = rk::fmx(hi, s_red[q][3 + k]); }
+      atomicMin(a + k, f2o(lo));
+      atomicMax(a + 3 + k, f2o(hi));
     }
   }
 }
@@ -200,14 +232,14 @@ __device__ __forceinline__ int bin_of(const Box& b, int axis, float scale, float
   return m;
 }
 
-__global__ __launch_bounds__(256) void k_sah_bins(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+__global__ __launch_bounds__(kCT) void k_sah_bins(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
                                                   const uint32_t* __restrict__ chunk_base,
                                                   const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                   int* __restrict__ acc) {
   __shared__ int s_bin[kBins * 7];
   const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
   if (!r.ok) return;
-  for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+  for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
     int k = w % 7;
     s_bin[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
   }
@@ -216,7 +248,7 @@ __global__ __launch_bounds__(256) void k_sah_bins(const Item* __restrict__ items
   int axis;
   float scale, bmin0;
   item_axis(a, axis, scale, bmin0);
-  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) {
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
     Box b = rk::load_box(boxes + indices[l]);
     int m = bin_of(b, axis, scale, bmin0);
     int* s = s_bin + m * 7;
@@ -225,7 +257,7 @@ __global__ __launch_bounds__(256) void k_sah_bins(const Item* __restrict__ items
     atomicMax(s + 4, f2o(b.mx.x)); atomicMax(s + 5, f2o(b.mx.y)); atomicMax(s + 6, f2o(b.mx.z));
   }
   __syncthreads();
-  for (int w = threadIdx.x; w < kBins * 7; w += 256) {
+  for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
     int k = w % 7, v = s_bin[w];
     int* g = a + 6 + w;
     if (k == 0) { if (v) atomicAdd(g, v); }
@@ -249,113 +281,182 @@ __device__ __forceinline__ Node make_leaf(uint32_t first, uint32_t count, const 
   return n;
 }
 
-// One lane per item: the reference's per-node decision, in the reference's order.
-__global__ __launch_bounds__(1024) void k_sah_decide(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
-                                                     const int* __restrict__ acc, Decision* __restrict__ dec,
-                                                     Item* __restrict__ next, Node* __restrict__ nodes,
-                                                     uint32_t nodes_cap) {
+// Items per workgroup of k_sah_decide (two per wave, one lane per bin) and
+// per workgroup of k_sah_emit.
+constexpr uint32_t kDecideItems = 8;
+constexpr uint32_t kEmitItems = kEmitGroup;
+static_assert(kEmitItems % kDecideItems == 0, "a decide workgroup's items lie in one emit group");
+
+// The reference's per-node decision (:424-530), one 32-lane half-wave per
+// item and one lane per bin. Lane k holds bin k. The left sweep's running box
+// and count after bin k are the inclusive prefix at lane k, and the right
+// sweep's after bin k the inclusive suffix; min, max and integer sums give the
+// same values in any order, and each lane then evaluates the reference's area
+// and cost expressions on those values. The reference keeps the first strict
+// minimum of its sweep from bin 31 down, i.e. the largest split index among
+// the minimal costs. The children's boxes are the prefix at the chosen split
+// and the suffix after it.
+__global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                    const int* __restrict__ acc, Decision* __restrict__ dec,
+                                                    uint32_t* __restrict__ group_internal) {
+  __shared__ uint32_t s_internal;
+  const uint32_t m = ctl[0];
+  const uint32_t k = threadIdx.x & 31;
+  const uint32_t i = blockIdx.x * kDecideItems + (threadIdx.x >> 5);
+  if (threadIdx.x == 0) s_internal = 0;
+  __syncthreads();
+  Item it;
+  uint32_t size = 0;
+  if (i < m) {
+    it = items[i];
+    size = it.hi - it.lo;
+  }
+  const bool split_test = i < m && size > kLeafMin;
+  int axis = 0, cnt = 0;
+  float scale = 0.f, bmin0 = 0.f;
+  float bmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, bmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+  if (split_test) {
+    const int* a = acc + (size_t)i * kAccWords;
+    item_axis(a, axis, scale, bmin0);
+    const int* b = a + 6 + k * 7;
+    cnt = b[0];
+    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(b[1 + q]); bmx[q] = o2f(b[4 + q]); }
+  }
+  // inclusive prefix (left) and suffix (right) over the 32 bins of this half
+  int lc = cnt, rc = cnt;
+  float lmn[3], lmx[3], rmn[3], rmx[3];
+  for (int q = 0; q < 3; ++q) { lmn[q] = rmn[q] = bmn[q]; lmx[q] = rmx[q] = bmx[q]; }
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) {
+    const int ylc = __shfl_up(lc, off, 32), yrc = __shfl_down(rc, off, 32);
+    float ylmn[3], ylmx[3], yrmn[3], yrmx[3];
+    for (int q = 0; q < 3; ++q) {
+      ylmn[q] = __shfl_up(lmn[q], off, 32); ylmx[q] = __shfl_up(lmx[q], off, 32);
+      yrmn[q] = __shfl_down(rmn[q], off, 32); yrmx[q] = __shfl_down(rmx[q], off, 32);
+    }
+    if (k >= (uint32_t)off) {
+      lc += ylc;
+      for (int q = 0; q < 3; ++q) { lmn[q] = rk::fmn(lmn[q], ylmn[q]); lmx[q] = rk::fmx(lmx[q], ylmx[q]); }
+    }
+    if (k + off < 32) {
+      rc += yrc;
+      for (int q = 0; q < 3; ++q) { rmn[q] = rk::fmn(rmn[q], yrmn[q]); rmx[q] = rk::fmx(rmx[q], yrmx[q]); }
+    }
+  }
+  // cost of the split after bin k (k <= 30): prefix at k, suffix at k + 1
+  const float rarea = 0.f + box_area(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2]);
+  const float rarea_next = __shfl_down(rarea, 1, 32);
+  const int rc_next = __shfl_down(rc, 1, 32);
+  float pa = 0.f;
+  if (split_test) pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
+  const float larea = 0.f + box_area(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2]);
+  const int lcount = 0 + lc;
+  const float T = 1.0f * 2.f + 0.125f * ((larea / pa) * (float)lcount + (rarea_next / pa) * (float)(0 + rc_next));
+  float bt = T;
+  int bj = (k < 31 && T < RK_HUGE) ? (int)k : -1;  // NaN and +huge never win a strict <
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) {
+    const float ot = __shfl_xor(bt, off, 32);
+    const int oj = __shfl_xor(bj, off, 32);
+    if (oj >= 0 && (bj < 0 || ot < bt || (ot == bt && oj > bj))) { bt = ot; bj = oj; }
+  }
+  const int m_min = bj;
+  const float T_min = m_min >= 0 ? bt : RK_HUGE;
+  // children: prefix at m_min, suffix at m_min + 1
+  const int src_l = m_min >= 0 ? m_min : 0, src_r = m_min >= 0 ? m_min + 1 : 0;
+  float child[12];
+  for (int q = 0; q < 3; ++q) {
+    child[q] = __shfl(lmn[q], src_l, 32);
+    child[3 + q] = __shfl(lmx[q], src_l, 32);
+    child[6 + q] = __shfl(rmn[q], src_r, 32);
+    child[9 + q] = __shfl(rmx[q], src_r, 32);
+  }
+  const int lcount_at = __shfl(lc, src_l, 32);
+  if (i < m && k == 0) {
+    Decision d;
+    d.axis = 0; d.scale = 0.f; d.bmin0 = 0.f; d.m_min = -1; d.lcount = 0;
+    d.pad[0] = d.pad[1] = d.pad[2] = 0;
+    for (int q = 0; q < 12; ++q) d.child[q] = 0.f;
+    if (split_test) {
+      d.axis = axis; d.scale = scale; d.bmin0 = bmin0;
+      bool leaf = size < kLeafMax && T_min > (float)(size + 1) * 0.125f;
+      if (!leaf && m_min < 0) {
+        leaf = true;
+        atomicOr(ctl + 4, (uint32_t)RK_FLAG_SAH_NO_CUT);
+      }
+      if (!leaf) {
+        d.m_min = m_min;
+        d.lcount = (uint32_t)lcount_at;
+        for (int q = 0; q < 12; ++q) d.child[q] = child[q];
+        atomicAdd(&s_internal, 1u);
+      }
+    }
+    dec[i] = d;
+  }
+  __syncthreads();
+  // kEmitItems is a multiple of kDecideItems: the workgroup's items share a group
+  if (threadIdx.x == 0 && s_internal)
+    atomicAdd(group_internal + blockIdx.x * kDecideItems / kEmitItems, s_internal);
+}
+
+// Node records and the next level's items, in item order: the k-th internal
+// item of the level takes children (count + 2k, count + 2k + 1). Workgroup g
+// handles items [g, g + 1) * kEmitItems and starts after the internal items
+// counted in the groups before it. The grid may hold more workgroups than the
+// level has groups; the one with the last group (workgroup 0 when the level
+// is empty) writes the new counts to ctl[2] and ctl[5], as ctl[3] is read by
+// every workgroup.
+__global__ __launch_bounds__(1024) void k_sah_emit(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                   const Decision* __restrict__ dec,
+                                                   const uint32_t* __restrict__ group_internal,
+                                                   Item* __restrict__ next, Node* __restrict__ nodes,
+                                                   uint32_t nodes_cap) {
   __shared__ uint32_t s_tmp[16];
-  __shared__ uint32_t s_carry, s_flags;
   const uint32_t m = ctl[0];
   const uint32_t count0 = ctl[3];
-  if (threadIdx.x == 0) { s_carry = 0; s_flags = 0; }
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < m; c0 += 1024) {
-    const uint32_t i = c0 + threadIdx.x;
-    uint32_t internal = 0;
-    Item it;
-    Decision d;
-    float L[6], R[6];
-    if (i < m) {
-      it = items[i];
-      const uint32_t size = it.hi - it.lo;
-      d.axis = 0; d.scale = 0.f; d.bmin0 = 0.f; d.m_min = -1; d.lcount = 0;
-      if (size > kLeafMin) {
-        const int* a = acc + (size_t)i * kAccWords;
-        item_axis(a, d.axis, d.scale, d.bmin0);
-        const int* bins = a + 6;
-        float pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
-        // left sweep :427-441
-        float larea[kBins];
-        int lcnt[kBins];
-        float lmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, lmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
-        int lc = 0;
-        for (int k = 0; k < kBins - 1; ++k) {
-          const int* b = bins + k * 7;
-          lc += b[0];
-          for (int q = 0; q < 3; ++q) { lmn[q] = rk::fmn(lmn[q], o2f(b[1 + q])); lmx[q] = rk::fmx(lmx[q], o2f(b[4 + q])); }
-          larea[k] = 0.f + box_area(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2]);
-          lcnt[k] = 0 + lc;
-        }
-        // right sweep and costs :447-473 (box_cost 1.0, obj_cost 0.125, strict <)
-        float T_min = RK_HUGE;
-        int m_min = -1;
-        float rmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, rmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
-        int rc = 0;
-        for (int k = kBins - 1; k > 0; --k) {
-          const int* b = bins + k * 7;
-          rc += b[0];
-          for (int q = 0; q < 3; ++q) { rmn[q] = rk::fmn(rmn[q], o2f(b[1 + q])); rmx[q] = rk::fmx(rmx[q], o2f(b[4 + q])); }
-          float rarea = 0.f + box_area(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2]);
-          int rcount = 0 + rc;
-          float T = 1.0f * 2.f + 0.125f * ((larea[k - 1] / pa) * (float)lcnt[k - 1] + (rarea / pa) * (float)rcount);
-          if (T < T_min) { T_min = T; m_min = k - 1; }
-        }
-        bool leaf = size < kLeafMax && T_min > (float)(size + 1) * 0.125f;
-        if (!leaf && m_min < 0) { leaf = true; atomicOr(&s_flags, (uint32_t)RK_FLAG_SAH_NO_CUT); }
-        if (!leaf) {
-          internal = 1;
-          d.m_min = m_min;
-          for (int q = 0; q < 3; ++q) { L[q] = RK_HUGE; L[3 + q] = -RK_HUGE; R[q] = RK_HUGE; R[3 + q] = -RK_HUGE; }
-          int lcount = 0;
-          for (int k = 0; k <= m_min; ++k) {
-            const int* b = bins + k * 7;
-            lcount += b[0];
-            for (int q = 0; q < 3; ++q) { L[q] = rk::fmn(L[q], o2f(b[1 + q])); L[3 + q] = rk::fmx(L[3 + q], o2f(b[4 + q])); }
-          }
-          for (int k = m_min + 1; k < kBins; ++k) {
-            const int* b = bins + k * 7;
-            for (int q = 0; q < 3; ++q) { R[q] = rk::fmn(R[q], o2f(b[1 + q])); R[3 + q] = rk::fmx(R[3 + q], o2f(b[4 + q])); }
-          }
-          d.lcount = (uint32_t)lcount;
-        }
-      }
-    }
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(internal, s_tmp, &total, 1024);
-    if (i < m) {
-      dec[i] = d;
-      if (internal) {
-        const uint32_t k = s_carry + ex;
-        const uint32_t c = count0 + 2 * k;
-        if (it.node < nodes_cap) nodes[it.node] = make_node((uint32_t)d.axis, c, it.bmin, it.bmax);
-        Item li, ri;
-        li.node = c; li.lo = it.lo; li.hi = it.lo + d.lcount; li.pad = 0;
-        ri.node = c + 1; ri.lo = it.lo + d.lcount; ri.hi = it.hi; ri.pad = 0;
-        for (int q = 0; q < 3; ++q) {
-          li.bmin[q] = L[q]; li.bmax[q] = L[3 + q];
-          ri.bmin[q] = R[q]; ri.bmax[q] = R[3 + q];
-        }
-        li.pad2[0] = li.pad2[1] = ri.pad2[0] = ri.pad2[1] = 0.f;
-        next[2 * k] = li;
-        next[2 * k + 1] = ri;
-      } else if (it.node < nodes_cap) {
-        nodes[it.node] = make_leaf(it.lo, it.hi - it.lo, it.bmin, it.bmax);
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += total;
-    __syncthreads();
+  const uint32_t g = blockIdx.x;
+  uint32_t before = 0, before_total;
+  for (uint32_t h = threadIdx.x; h < g; h += 1024) before += group_internal[h];
+  block_excl_scan(before, s_tmp, &before_total, 1024);
+  const uint32_t i = g * kEmitItems + threadIdx.x;
+  uint32_t internal = 0;
+  Item it;
+  Decision d;
+  if (i < m) {
+    it = items[i];
+    d = dec[i];
+    internal = d.m_min >= 0 ? 1u : 0u;
   }
-  if (threadIdx.x == 0) {
-    ctl[2] = 2 * s_carry;
-    ctl[3] = count0 + 2 * s_carry;
-    ctl[4] |= s_flags;
-    if (count0 + 2 * s_carry > nodes_cap) ctl[4] |= 0x20u;
+  uint32_t total;
+  const uint32_t ex = block_excl_scan(internal, s_tmp, &total, 1024);
+  if (i < m) {
+    if (internal) {
+      const uint32_t k = before_total + ex;
+      const uint32_t c = count0 + 2 * k;
+      if (it.node < nodes_cap) nodes[it.node] = make_node((uint32_t)d.axis, c, it.bmin, it.bmax);
+      Item li, ri;
+      li.node = c; li.lo = it.lo; li.hi = it.lo + d.lcount; li.pad = 0;
+      ri.node = c + 1; ri.lo = it.lo + d.lcount; ri.hi = it.hi; ri.pad = 0;
+      for (int q = 0; q < 3; ++q) {
+        li.bmin[q] = d.child[q]; li.bmax[q] = d.child[3 + q];
+        ri.bmin[q] = d.child[6 + q]; ri.bmax[q] = d.child[9 + q];
+      }
+      li.pad2[0] = li.pad2[1] = ri.pad2[0] = ri.pad2[1] = 0.f;
+      next[2 * k] = li;
+      next[2 * k + 1] = ri;
+    } else if (it.node < nodes_cap) {
+      nodes[it.node] = make_leaf(it.lo, it.hi - it.lo, it.bmin, it.bmax);
+    }
+  }
+  if (threadIdx.x == 0 && g == (m ? (m - 1) / kEmitItems : 0u)) {
+    const uint32_t all = before_total + total;
+    ctl[2] = 2 * all;
+    ctl[5] = count0 + 2 * all;
+    if (count0 + 2 * all > nodes_cap) atomicOr(ctl + 4, 0x20u);
   }
 }
 
-__global__ __launch_bounds__(256) void k_sah_count(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+__global__ __launch_bounds__(kCT) void k_sah_count(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
                                                    const uint32_t* __restrict__ chunk_base,
                                                    const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                    const Decision* __restrict__ dec, uint32_t* __restrict__ chunk_left) {
@@ -367,7 +468,7 @@ __global__ __launch_bounds__(256) void k_sah_count(const Item* __restrict__ item
   __syncthreads();
   if (d.m_min >= 0) {
     uint32_t c = 0;
-    for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256)
+    for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT)
       c += bin_of(rk::load_box(boxes + indices[l]), d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
     atomicAdd(&s_cnt, c);
   }
@@ -375,25 +476,34 @@ __global__ __launch_bounds__(256) void k_sah_count(const Item* __restrict__ item
   if (threadIdx.x == 0) chunk_left[blockIdx.x] = s_cnt;
 }
 
-// Exclusive scan of chunk_left within each item (an item's chunks are contiguous).
-__global__ __launch_bounds__(1024) void k_sah_offsets(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
-                                                      const uint32_t* __restrict__ chunk_base,
-                                                      const uint32_t* __restrict__ chunk_left,
-                                                      uint32_t* __restrict__ chunk_loff) {
+// Exclusive scan of chunk_left within each item (an item's chunks are
+// contiguous), one wave per item.
+__global__ __launch_bounds__(256) void k_sah_offsets(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                     const uint32_t* __restrict__ chunk_base,
+                                                     const uint32_t* __restrict__ chunk_left,
+                                                     uint32_t* __restrict__ chunk_loff) {
   const uint32_t m = ctl[0];
-  for (uint32_t i = threadIdx.x; i < m; i += 1024) {
-    const uint32_t size = items[i].hi - items[i].lo;
-    if (size <= kLeafMin) continue;
-    const uint32_t b = chunk_base[i], nch = (size + kChunk - 1) / kChunk;
-    uint32_t run = 0;
-    for (uint32_t c = 0; c < nch; ++c) {
-      chunk_loff[b + c] = run;
-      run += chunk_left[b + c];
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= m) return;  // wave-uniform
+  const uint32_t size = items[i].hi - items[i].lo;
+  if (size <= kLeafMin) return;
+  const uint32_t b = chunk_base[i], nch = (size + kChunk - 1) / kChunk;
+  uint32_t run = 0;
+  for (uint32_t c0 = 0; c0 < nch; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    const uint32_t v = c < nch ? chunk_left[b + c] : 0u;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= (uint32_t)off) incl += y;
     }
+    if (c < nch) chunk_loff[b + c] = run + incl - v;
+    run += __shfl(incl, 63, 64);
   }
 }
 
-__global__ __launch_bounds__(256) void k_sah_scatter(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+__global__ __launch_bounds__(kCT) void k_sah_scatter(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
                                                      const uint32_t* __restrict__ chunk_base,
                                                      const uint32_t* __restrict__ indices,
                                                      const Box* __restrict__ boxes, const Decision* __restrict__ dec,
@@ -408,7 +518,7 @@ __global__ __launch_bounds__(256) void k_sah_scatter(const Item* __restrict__ it
   const uint32_t loff = chunk_loff[blockIdx.x];
   const uint32_t chunk_start = r.lo - it.lo;  // elements of this item before the chunk
   uint32_t left_before = 0;                    // left elements of this chunk before the current step
-  for (uint32_t s = r.lo; s < r.hi; s += 256) {
+  for (uint32_t s = r.lo; s < r.hi; s += kCT) {
     const uint32_t l = s + threadIdx.x;
     uint32_t id = 0, is_left = 0;
     const bool in = l < r.hi;
@@ -417,7 +527,7 @@ __global__ __launch_bounds__(256) void k_sah_scatter(const Item* __restrict__ it
       is_left = bin_of(rk::load_box(boxes + id), d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
     }
     uint32_t total;
-    const uint32_t ex = block_excl_scan(is_left, s_tmp, &total, 256);
+    const uint32_t ex = block_excl_scan(is_left, s_tmp, &total, kCT);
     if (in) {
       const uint32_t pos_in_item = chunk_start + (l - r.lo);
       const uint32_t lrank = loff + left_before + ex;            // left elements before this one
@@ -428,14 +538,14 @@ __global__ __launch_bounds__(256) void k_sah_scatter(const Item* __restrict__ it
   }
 }
 
-__global__ __launch_bounds__(256) void k_sah_copyback(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+__global__ __launch_bounds__(kCT) void k_sah_copyback(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
                                                       const uint32_t* __restrict__ chunk_base,
                                                       const Decision* __restrict__ dec,
                                                       const uint32_t* __restrict__ tmp, uint32_t* __restrict__ indices) {
   const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
   if (!r.ok) return;
   if (dec[r.item].m_min < 0) return;
-  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += 256) indices[l] = tmp[l];
+  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) indices[l] = tmp[l];
 }
 
 // Hybrid entry (kernel_initStack, :647-666): every leaf of the existing tree
@@ -467,7 +577,7 @@ __global__ __launch_bounds__(1024) void k_sah_init_from_leaves(const Node* __res
     if (threadIdx.x == 0) s_carry += total;
     __syncthreads();
   }
-  if (threadIdx.x == 0) { ctl[0] = s_carry; ctl[3] = existing; ctl[4] = 0; }
+  if (threadIdx.x == 0) { ctl[0] = s_carry; ctl[3] = ctl[5] = existing; ctl[4] = 0; }
 }
 
 __global__ void k_sah_init_root(Item* __restrict__ items, uint32_t* __restrict__ ctl, uint32_t n, float4 rmin,
@@ -478,13 +588,11 @@ __global__ void k_sah_init_root(Item* __restrict__ items, uint32_t* __restrict__
   it.bmax[0] = rmax.x; it.bmax[1] = rmax.y; it.bmax[2] = rmax.z;
   it.pad2[0] = it.pad2[1] = 0.f;
   items[0] = it;
-  ctl[0] = 1; ctl[3] = 1; ctl[4] = 0;
+  ctl[0] = 1; ctl[3] = ctl[5] = 1; ctl[4] = 0;
 }
 
-__global__ void k_sah_next_level(uint32_t* __restrict__ ctl) { ctl[0] = ctl[2]; }
-
 __global__ void k_sah_finish(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ info) {
-  info[RK_INFO_NODE_COUNT] = ctl[3];
+  info[RK_INFO_NODE_COUNT] = ctl[5];
   info[RK_INFO_FLAGS] |= ctl[4];
 }
 
@@ -525,6 +633,7 @@ extern "C" size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t exis
   b += rk_align_up(n * 4);                         // partition temp
   b += rk_align_up(n * 16);                        // face reorder temp
   b += rk_align_up(64);
+  b += rk_align_up((m / kEmitGroup + 1) * 4);      // internal items per emit group
   return b + 4096;
 }
 
@@ -557,7 +666,9 @@ extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_fl
   uint32_t* tmp = ar.take<uint32_t>(n);
   uint4* ftmp = ar.take<uint4>(n);
   uint32_t* ctl = ar.take<uint32_t>(16);
-  if (!items_a || !items_b || !chunk_base || !acc || !dec || !chunk_left || !chunk_loff || !tmp || !ftmp || !ctl)
+  uint32_t* group_internal = ar.take<uint32_t>(m_cap / kEmitGroup + 1);
+  if (!items_a || !items_b || !chunk_base || !acc || !dec || !chunk_left || !chunk_loff || !tmp || !ftmp || !ctl ||
+      !group_internal)
     return RK_ERROR_WORKSPACE;
   rk_status s;
   if ((s = rk_fill_u32(stream, ctl, 16, 0u)) != RK_SUCCESS) return s;
@@ -574,29 +685,47 @@ extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_fl
                        existing_nodes, items_a, ctl);
   }
   RK_CHECK_LAUNCH();
-  uint32_t m = 0;
+  uint32_t m = 0;  // this level's item count, or an upper bound of it while !exact
   RK_HIP(hipMemcpyAsync(&m, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   RK_HIP(hipStreamSynchronize(st));
+  bool exact = true;
+  // Levels whose item count is bounded by a few thousand run without reading
+  // it back: a level has at most twice the items of the one before, and every
+  // kernel reads the exact count on the device, so a grid sized by the bound
+  // is only larger than needed. Deeper levels read the count back to size
+  // their grids; an empty level ends the build.
+  const uint32_t async_items = std::min<uint32_t>(4096u, n / 256);
   Item* cur = items_a;
   Item* nxt = items_b;
   const Box* bx = reinterpret_cast<const Box*>(boxes);
   Node* nd = reinterpret_cast<Node*>(nodes);
-  for (uint32_t level = 0; m > 0; ++level) {
+  for (uint32_t level = 0; !(exact && m == 0); ++level) {
     if (level > 4096) return RK_ERROR_LAUNCH;
     const unsigned grid = (unsigned)(n / kChunk + m + 1);
-    hipLaunchKernelGGL(k_sah_prep, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, acc);
-    hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, acc);
-    hipLaunchKernelGGL(k_sah_bins, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, acc);
-    hipLaunchKernelGGL(k_sah_decide, dim3(1), dim3(1024), 0, st, cur, ctl, acc, dec, nxt, nd, nodes_capacity);
-    hipLaunchKernelGGL(k_sah_count, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_left);
-    hipLaunchKernelGGL(k_sah_offsets, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, chunk_left, chunk_loff);
-    hipLaunchKernelGGL(k_sah_scatter, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_loff,
+    hipLaunchKernelGGL(k_sah_prep, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, acc, group_internal,
+                       level > 0 ? 1u : 0u);
+    hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, acc);
+    hipLaunchKernelGGL(k_sah_bins, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, acc);
+    hipLaunchKernelGGL(k_sah_decide, dim3((m + kDecideItems - 1) / kDecideItems), dim3(256), 0, st, cur, ctl, acc,
+                       dec, group_internal);
+    hipLaunchKernelGGL(k_sah_emit, dim3(m ? (m + kEmitItems - 1) / kEmitItems : 1u), dim3(1024), 0, st, cur, ctl,
+                       dec, group_internal, nxt, nd, nodes_capacity);
+    hipLaunchKernelGGL(k_sah_count, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_left);
+    hipLaunchKernelGGL(k_sah_offsets, dim3((m + 3) / 4), dim3(256), 0, st, cur, ctl, chunk_base, chunk_left,
+                       chunk_loff);
+    hipLaunchKernelGGL(k_sah_scatter, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_loff,
                        tmp);
-    hipLaunchKernelGGL(k_sah_copyback, dim3(grid), dim3(256), 0, st, cur, ctl, chunk_base, dec, tmp, indices);
-    hipLaunchKernelGGL(k_sah_next_level, dim3(1), dim3(1), 0, st, ctl);
+    hipLaunchKernelGGL(k_sah_copyback, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, dec, tmp, indices);
     RK_CHECK_LAUNCH();
-    RK_HIP(hipMemcpyAsync(&m, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    RK_HIP(hipStreamSynchronize(st));
+    const uint64_t bound = std::min<uint64_t>(2 * (uint64_t)m, m_cap);
+    if (bound <= async_items) {
+      m = (uint32_t)bound;
+      exact = false;
+    } else {
+      RK_HIP(hipMemcpyAsync(&m, ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      RK_HIP(hipStreamSynchronize(st));
+      exact = true;
+    }
     std::swap(cur, nxt);
   }
   hipLaunchKernelGGL(k_sah_finish, dim3(1), dim3(1), 0, st, ctl, info);

@@ -213,27 +213,46 @@ def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=Non
     return fid, uv, dep, cnt
 
 
+class SAHBuilder:
+    """Reusable buffers for rk_build_sah on meshes of up to n faces, on top of
+    an LBVH of up to 'existing' nodes (0: pure SAH trees only)."""
+
+    def __init__(self, n: int, existing: int = 0):
+        self.n, self.existing = n, existing
+        self.cap = existing + 2 * n + 1
+        self.ws = workspace(N.lib().rk_build_sah_workspace_size(n, existing))
+        self.boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+        self.indices = torch.empty(n, dtype=_U32, device="cuda")
+        self.nodes = torch.zeros((self.cap, 8), dtype=_U32, device="cuda")
+        self.info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
+
+    def build(self, F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, lbvh: RefTree | None = None,
+              stream=None) -> RefTree:
+        """A pure SAH tree (lbvh None) or the Hybrid refinement of an LBVH built
+        with complete=False (its boxes and indices are used in place)."""
+        n = F.shape[0]
+        assert n <= self.n
+        existing = 0 if lbvh is None else int(lbvh.node_count)
+        assert existing <= self.existing
+        if lbvh is None:
+            boxes, indices = self.boxes[:n], self.indices[:n]
+        else:
+            boxes, indices = lbvh.boxes, lbvh.indices
+            self.nodes[:existing] = lbvh.nodes[:existing]
+        hb = N.aabox(hitbox)
+        _check(N.lib().rk_build_sah(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, C.byref(hb),
+                                    boxes.data_ptr(), indices.data_ptr(), self.nodes.data_ptr(), existing,
+                                    existing + 2 * n + 1, self.info.data_ptr(), self.ws.data_ptr(),
+                                    self.ws.numel()), "build_sah")
+        return RefTree(F, boxes, indices, self.nodes, self.info)
+
+
 def build_sah(F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, lbvh: RefTree | None = None, stream=None):
     """rk_build_sah: a pure SAH tree (lbvh None) or the Hybrid refinement of an
     LBVH built with complete=False (its boxes, indices and nodes are reused)."""
-    n = F.shape[0]
     existing = 0 if lbvh is None else int(lbvh.node_count)
-    cap = existing + 2 * n + 1
-    nodes = torch.zeros((cap, 8), dtype=_U32, device="cuda")
-    if lbvh is None:
-        boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
-        indices = torch.empty(n, dtype=_U32, device="cuda")
-    else:
-        boxes, indices = lbvh.boxes, lbvh.indices
-        nodes[:existing] = lbvh.nodes[:existing]
-    info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
-    ws = workspace(N.lib().rk_build_sah_workspace_size(n, existing))
-    hb = N.aabox(hitbox)
-    _check(N.lib().rk_build_sah(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, C.byref(hb), boxes.data_ptr(),
-                                indices.data_ptr(), nodes.data_ptr(), existing, cap, info.data_ptr(), ws.data_ptr(),
-                                ws.numel()), "build_sah")
+    t = SAHBuilder(F.shape[0], existing).build(F, V, hitbox, lbvh, stream)
     torch.cuda.synchronize()
-    t = RefTree(F, boxes, indices, nodes, info)
     t.node_count = int(t.host_info()[N.RK_INFO_NODE_COUNT])
     return t
 

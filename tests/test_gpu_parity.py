@@ -478,14 +478,7 @@ def test_fast_vs_reference_trace_c1(dev, rk, ora):
     assert np.allclose(gt, rt, rtol=1e-6)
 
 
-@pytest.mark.parametrize("name", ["tiny", "ico3", "ico5", "hf128", "soup50k"])
-def test_sah_bit_exact(dev, rk, ora, name):
-    V, F = _scene(rk, name)
-    hb = rk.hitbox(V)
-    boxes, idx = ora.calc_boxes(F, V)
-    onodes, oidx, oflags = ora.build_sah(boxes, idx, hb)
-    Ft = _t(F)
-    tree = dev.build_sah(Ft, _t(V), hb)
+def _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags):
     M = tree.node_count
     assert M == onodes.shape[0]
     assert int(tree.host_info()[rk.RK_INFO_FLAGS]) == oflags
@@ -497,7 +490,19 @@ def test_sah_bit_exact(dev, rk, ora, name):
     assert np.array_equal(nodes["bmin"], onodes["bmin"]) and np.array_equal(nodes["bmax"], onodes["bmax"])
 
 
-@pytest.mark.parametrize("name,depth", [("ico5", 12), ("hf128", 6), ("soup50k", 12)])
+@pytest.mark.parametrize("name", ["tiny", "ico3", "ico5", "hf128", "soup50k", "hf708"])
+def test_sah_bit_exact(dev, rk, ora, name):
+    """hf708 is the full-size heightfield (999,698 faces) of the benchmark."""
+    V, F = _scene(rk, name)
+    hb = rk.hitbox(V)
+    boxes, idx = ora.calc_boxes(F, V)
+    onodes, oidx, oflags = ora.build_sah(boxes, idx, hb)
+    Ft = _t(F)
+    tree = dev.build_sah(Ft, _t(V), hb)
+    _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
+
+
+@pytest.mark.parametrize("name,depth", [("ico5", 12), ("hf128", 6), ("soup50k", 12), ("hf708", 12)])
 def test_hybrid_bit_exact(dev, rk, ora, name, depth):
     V, F = _scene(rk, name)
     hb = rk.hitbox(V)
@@ -507,14 +512,29 @@ def test_hybrid_bit_exact(dev, rk, ora, name, depth):
     lb = dev.build_lbvh(Ft, _t(V), hb, depth, False)
     assert lb.node_count == ref.node_count
     tree = dev.build_sah(Ft, _t(V), hb, lbvh=lb)
-    M = tree.node_count
-    assert M == onodes.shape[0]
-    np.testing.assert_array_equal(_h(tree.indices, np.uint32), oidx)
-    np.testing.assert_array_equal(_h(Ft, np.uint32), ora.reorder_faces(F, oidx))
-    nodes = _h(tree.nodes, np.uint32)[:M].copy().view(ora.NODE_DTYPE).reshape(M)
-    np.testing.assert_array_equal(nodes["w0"], onodes["w0"])
-    np.testing.assert_array_equal(nodes["w1"], onodes["w1"])
-    assert np.array_equal(nodes["bmin"], onodes["bmin"]) and np.array_equal(nodes["bmax"], onodes["bmax"])
+    _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
+
+
+def test_sah_builder_reuse(dev, rk, ora):
+    """One SAHBuilder's buffers reused: a larger mesh, then a smaller one, then
+    a Hybrid refinement; each tree equals the oracle's."""
+    sb = dev.SAHBuilder(50_000, 1 << 13)
+    for name in ["soup50k", "ico5"]:
+        V, F = _scene(rk, name)
+        hb = rk.hitbox(V)
+        boxes, idx = ora.calc_boxes(F, V)
+        onodes, oidx, oflags = ora.build_sah(boxes, idx, hb)
+        Ft = _t(F)
+        tree = sb.build(Ft, _t(V), hb)
+        tree.node_count = int(tree.host_info()[rk.RK_INFO_NODE_COUNT])
+        _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
+    ref = ora.build_lbvh_ref(F, V, hb, 12, False)
+    onodes, oidx, oflags = ora.build_sah(ref.boxes, ref.indices, hb, ref.nodes, ref.node_count)
+    Ft = _t(F)
+    lb = dev.build_lbvh(Ft, _t(V), hb, 12, False)
+    tree = sb.build(Ft, _t(V), hb, lbvh=lb)
+    tree.node_count = int(tree.host_info()[rk.RK_INFO_NODE_COUNT])
+    _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
 
 
 @pytest.mark.parametrize("coherency", [False, True])
