@@ -10,6 +10,9 @@
 //   k_sah_decide     one half-wave per item, one lane per bin: the reference
 //                    sweep (:424-474), leaf test (:479-496), children boxes
 //                    (:501-530)
+//   k_sah_small      items of 17 to 1024 faces, one wave each: all of the
+//                    above and the partition, in LDS; the chunk kernels take
+//                    the items of more than 1024 faces
 //   k_sah_emit       1024 items per workgroup: scan of the internal items,
 //                    node records, next-level items; child ids in
 //                    breadth-first order
@@ -50,9 +53,13 @@ struct Item {
 };
 static_assert(sizeof(Item) == 48, "item is 48 B");
 
-// Per-item accumulators: centroid bounds (6 ordered ints) and 32 bins of
-// (count, 3 min, 3 max ordered ints).
-constexpr int kAccWords = 6 + kBins * 7;
+// Accumulators of an item of more than kChunk faces: its centroid bounds
+// (6 ordered ints) in kBoundSlots copies, which its chunks update in turn so
+// that fewer atomics meet on one address, then 32 bins of (count, 3 min,
+// 3 max ordered ints).
+constexpr int kBoundSlots = 8;
+constexpr int kBinBase = 6 * kBoundSlots;
+constexpr int kAccWords = kBinBase + kBins * 7;
 
 // Per-item decision written by k_sah_decide.
 struct Decision {
@@ -70,15 +77,6 @@ __device__ __forceinline__ int f2o(float f) {  // order-preserving float -> int
   return i >= 0 ? i : (i ^ 0x7FFFFFFF);
 }
 __device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : (i ^ 0x7FFFFFFF)); }
-
-__device__ __forceinline__ uint32_t find_item(const uint32_t* __restrict__ chunk_base, uint32_t m, uint32_t c) {
-  uint32_t lo = 0, hi = m;  // last item with chunk_base <= c
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (chunk_base[mid] <= c) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 __device__ __forceinline__ float box_area(float mnx, float mny, float mnz, float mxx, float mxy, float mxz) {
   float vx = mxx - mnx, vy = mxy - mny, vz = mxz - mnz;  // CUDABox.h:35-44
@@ -106,59 +104,63 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
   return wprefix + incl - v;
 }
 
-// ctl words: [0] items this level (m), [1] chunks this level (T), [2] items next level,
-// [3] node count, [4] flags, [5] node count after this level. k_sah_emit writes
-// [2] and [5]; the next level's k_sah_prep (advance) moves them to [0] and [3].
-__global__ __launch_bounds__(1024) void k_sah_prep(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
-                                                   uint32_t* __restrict__ chunk_base, int* __restrict__ acc,
-                                                   uint32_t* __restrict__ group_internal, uint32_t advance) {
-  __shared__ uint32_t s_tmp[16];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) {
-    s_carry = 0;
-    if (advance) {
-      ctl[0] = ctl[2];
-      ctl[3] = ctl[5];
-    }
+// ctl words: [0] flags; per level parity p (level & 1), at kCtlLevel(p):
+// +0 items (m), +1 items of more than kChunk faces, +2 chunks (T), +3 slots
+// taken, +4 node count before the level. A level of parity p reads its own
+// words and writes the next level's m, big-item count and node count; its
+// k_sah_prep clears the next level's counters, which nothing of this level
+// uses.
+constexpr int kCtlFlags = 0, kCtlWords = 32;
+__device__ __host__ constexpr int kCtlLevel(uint32_t p) { return 8 + 8 * (int)p; }
+constexpr int kLvM = 0, kLvBig = 1, kLvT = 2, kLvSlot = 3, kLvCount = 4;
+
+// Items of more than kChunk faces get a slot of accumulators (kept in the
+// item's pad word) and a run of chunks, in any order among items; chunk_item
+// maps a chunk back to its item. Smaller items are k_sah_small's.
+__global__ __launch_bounds__(256) void k_sah_prep(Item* __restrict__ items, uint32_t* __restrict__ ctl, uint32_t par,
+                                                  uint32_t* __restrict__ chunk_base,
+                                                  uint32_t* __restrict__ chunk_item, int* __restrict__ acc,
+                                                  uint32_t* __restrict__ group_internal) {
+  uint32_t* lv = ctl + kCtlLevel(par);
+  const uint32_t m = lv[kLvM];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) {
+    uint32_t* nx = ctl + kCtlLevel(par ^ 1u);
+    nx[kLvBig] = 0u;
+    nx[kLvT] = 0u;
+    nx[kLvSlot] = 0u;
   }
-  __syncthreads();
-  const uint32_t m = ctl[0];
-  for (uint32_t g = threadIdx.x; g < (m + kEmitGroup - 1) / kEmitGroup; g += 1024) group_internal[g] = 0u;
-  for (uint32_t c0 = 0; c0 < m; c0 += 1024) {
-    uint32_t i = c0 + threadIdx.x;
-    uint32_t nch = 0;
-    if (i < m) {
-      uint32_t size = items[i].hi - items[i].lo;
-      nch = size > kLeafMin ? (size + kChunk - 1) / kChunk : 0u;
-    }
-    uint32_t total;
-    uint32_t ex = block_excl_scan(nch, s_tmp, &total, 1024);
-    if (i < m) {
-      chunk_base[i] = s_carry + ex;
-      if (nch) {  // centroid-bound accumulators; the bins are reset by k_sah_bounds
-        int* a = acc + (size_t)i * kAccWords;
-        a[0] = a[1] = a[2] = f2o(RK_HUGE);
-        a[3] = a[4] = a[5] = f2o(-RK_HUGE);
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += total;
-    __syncthreads();
+  if (i < (m + kEmitGroup - 1) / kEmitGroup) group_internal[i] = 0u;
+  if (i >= m) return;
+  const uint32_t size = items[i].hi - items[i].lo;
+  if (size <= kChunk) return;
+  const uint32_t nch = (size + kChunk - 1) / kChunk;
+  const uint32_t slot = atomicAdd(lv + kLvSlot, 1u);
+  const uint32_t base = atomicAdd(lv + kLvT, nch);
+  items[i].pad = slot;
+  chunk_base[i] = base;
+  for (uint32_t j = 0; j < nch; ++j) chunk_item[base + j] = i;
+  int* a = acc + (size_t)slot * kAccWords;  // the bins are reset by k_sah_bounds
+  for (int q = 0; q < kBoundSlots; ++q) {
+    a[6 * q + 0] = a[6 * q + 1] = a[6 * q + 2] = f2o(RK_HUGE);
+    a[6 * q + 3] = a[6 * q + 4] = a[6 * q + 5] = f2o(-RK_HUGE);
   }
-  if (threadIdx.x == 0) ctl[1] = s_carry;
 }
 
-struct ChunkRange { uint32_t item, lo, hi; bool ok; };
+struct ChunkRange { uint32_t item, slot, lo, hi; bool first, ok; };
 
 __device__ __forceinline__ ChunkRange chunk_range(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
-                                                  const uint32_t* __restrict__ chunk_base, uint32_t c) {
+                                                  uint32_t par, const uint32_t* __restrict__ chunk_base,
+                                                  const uint32_t* __restrict__ chunk_item, uint32_t c) {
   ChunkRange r;
-  r.ok = c < ctl[1];
+  r.ok = c < ctl[kCtlLevel(par) + kLvT];
   if (!r.ok) return r;
-  const uint32_t m = ctl[0];
-  r.item = find_item(chunk_base, m, c);
-  const Item& it = items[r.item];
-  r.lo = it.lo + (c - chunk_base[r.item]) * kChunk;
+  r.item = chunk_item[c];
+  const Item it = items[r.item];
+  const uint32_t j = c - chunk_base[r.item];
+  r.slot = it.pad;
+  r.first = j == 0;
+  r.lo = it.lo + j * kChunk;
   r.hi = min(it.hi, r.lo + kChunk);
   return r;
 }
@@ -168,55 +170,64 @@ __device__ __forceinline__ float center_axis(const Box& b, int a) {  // 0.5f * (
 }
 
 __global__ __launch_bounds__(kCT) void k_sah_bounds(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
-                                                    const uint32_t* __restrict__ chunk_base,
+                                                    uint32_t par, const uint32_t* __restrict__ chunk_base,
+                                                    const uint32_t* __restrict__ chunk_item,
                                                     const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                     int* __restrict__ acc) {
-  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  const ChunkRange r = chunk_range(items, ctl, par, chunk_base, chunk_item, blockIdx.x);
   if (!r.ok) return;
-  if (blockIdx.x == chunk_base[r.item]) {  // the item's first chunk resets its bins for k_sah_bins
-    int* bins = acc + (size_t)r.item * kAccWords + 6;
+  int* a = acc + (size_t)r.slot * kAccWords;
+  if (r.first) {  // the item's first chunk resets its bins for k_sah_bins
+    int* bins = a + kBinBase;
     for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
       int k = w % 7;
       bins[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
     }
   }
-  float mn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, mx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+  // ordered ints, as the atomics compare them
+  int mn[3] = {f2o(RK_HUGE), f2o(RK_HUGE), f2o(RK_HUGE)}, mx[3] = {f2o(-RK_HUGE), f2o(-RK_HUGE), f2o(-RK_HUGE)};
   for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
     Box b = rk::load_box(boxes + indices[l]);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      float c = center_axis(b, a);
-      mn[a] = rk::fmn(mn[a], c);
-      mx[a] = rk::fmx(mx[a], c);
+    for (int q = 0; q < 3; ++q) {
+      const int c = f2o(center_axis(b, q));
+      mn[q] = min(mn[q], c);
+      mx[q] = max(mx[q], c);
     }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = rk::fmn(mn[a], __shfl_xor(mn[a], off, 64));
-      mx[a] = rk::fmx(mx[a], __shfl_xor(mx[a], off, 64));
+    for (int q = 0; q < 3; ++q) {
+      mn[q] = min(mn[q], __shfl_xor(mn[q], off, 64));
+      mx[q] = max(mx[q], __shfl_xor(mx[q], off, 64));
     }
-  // one set of atomics per workgroup: an item's chunks all update the same
-  // six words, and same-address atomics serialise
-  __shared__ float s_red[kCT / 64][6];
+  // one set of atomics per workgroup, into one of the item's bound slots
+  __shared__ int s_red[kCT / 64][6];
   const uint32_t w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 3; ++k) { s_red[w][k] = mn[k]; s_red[w][3 + k] = mx[k]; }
+    for (int q = 0; q < 3; ++q) { s_red[w][q] = mn[q]; s_red[w][3 + q] = mx[q]; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int* a = acc + (size_t)r.item * kAccWords;
-    for (int k = 0; k < 3; ++k) {
-      float lo = s_red[0][k], hi = s_red[0][3 + k];
-      for (int q = 1; q < kCT / 64; ++q) { lo = rk::fmn(lo, s_red[q][k]); hi = rk::fmx(hi, s_red[q][3 + k]); }
-      atomicMin(a + k, f2o(lo));
-      atomicMax(a + 3 + k, f2o(hi));
+    int* bs = a + 6 * (blockIdx.x % kBoundSlots);
+    for (int q = 0; q < 3; ++q) {
+      int lo = s_red[0][q], hi = s_red[0][3 + q];
+      for (int v = 1; v < kCT / 64; ++v) { lo = min(lo, s_red[v][q]); hi = max(hi, s_red[v][3 + q]); }
+      atomicMin(bs + q, lo);
+      atomicMax(bs + 3 + q, hi);
     }
   }
 }
 
+// An item's centroid bounds from its bound slots.
+__device__ __forceinline__ void acc_bounds(const int* __restrict__ a, int (&cb)[6]) {
+  for (int q = 0; q < 3; ++q) { cb[q] = a[q]; cb[3 + q] = a[3 + q]; }
+  for (int v = 1; v < kBoundSlots; ++v)
+    for (int q = 0; q < 3; ++q) { cb[q] = min(cb[q], a[6 * v + q]); cb[3 + q] = max(cb[3 + q], a[6 * v + 3 + q]); }
+}
+
 // Axis and bin scale of an item from its centroid bounds (:302-342).
-__device__ __forceinline__ void item_axis(const int* __restrict__ a, int& axis, float& scale, float& bmin0) {
+__device__ __forceinline__ void item_axis(const int (&a)[6], int& axis, float& scale, float& bmin0) {
   float ex = o2f(a[3]) - o2f(a[0]), ey = o2f(a[4]) - o2f(a[1]), ez = o2f(a[5]) - o2f(a[2]);
   if (ex > ey) axis = (ex > ez) ? 0 : 2;
   else axis = (ey > ez) ? 1 : 2;
@@ -233,21 +244,24 @@ __device__ __forceinline__ int bin_of(const Box& b, int axis, float scale, float
 }
 
 __global__ __launch_bounds__(kCT) void k_sah_bins(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
-                                                  const uint32_t* __restrict__ chunk_base,
+                                                  uint32_t par, const uint32_t* __restrict__ chunk_base,
+                                                  const uint32_t* __restrict__ chunk_item,
                                                   const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                   int* __restrict__ acc) {
   __shared__ int s_bin[kBins * 7];
-  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  const ChunkRange r = chunk_range(items, ctl, par, chunk_base, chunk_item, blockIdx.x);
   if (!r.ok) return;
   for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
     int k = w % 7;
     s_bin[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
   }
   __syncthreads();
-  int* a = acc + (size_t)r.item * kAccWords;
+  int* a = acc + (size_t)r.slot * kAccWords;
+  int cb[6];
+  acc_bounds(a, cb);
   int axis;
   float scale, bmin0;
-  item_axis(a, axis, scale, bmin0);
+  item_axis(cb, axis, scale, bmin0);
   for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
     Box b = rk::load_box(boxes + indices[l]);
     int m = bin_of(b, axis, scale, bmin0);
@@ -259,7 +273,7 @@ __global__ __launch_bounds__(kCT) void k_sah_bins(const Item* __restrict__ items
   __syncthreads();
   for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
     int k = w % 7, v = s_bin[w];
-    int* g = a + 6 + w;
+    int* g = a + kBinBase + w;
     if (k == 0) { if (v) atomicAdd(g, v); }
     else if (k < 4) { if (v != f2o(RK_HUGE)) atomicMin(g, v); }
     else { if (v != f2o(-RK_HUGE)) atomicMax(g, v); }
@@ -287,42 +301,25 @@ constexpr uint32_t kDecideItems = 8;
 constexpr uint32_t kEmitItems = kEmitGroup;
 static_assert(kEmitItems % kDecideItems == 0, "a decide workgroup's items lie in one emit group");
 
-// The reference's per-node decision (:424-530), one 32-lane half-wave per
-// item and one lane per bin. Lane k holds bin k. The left sweep's running box
-// and count after bin k are the inclusive prefix at lane k, and the right
-// sweep's after bin k the inclusive suffix; min, max and integer sums give the
-// same values in any order, and each lane then evaluates the reference's area
-// and cost expressions on those values. The reference keeps the first strict
-// minimum of its sweep from bin 31 down, i.e. the largest split index among
-// the minimal costs. The children's boxes are the prefix at the chosen split
-// and the suffix after it.
-__global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
-                                                    const int* __restrict__ acc, Decision* __restrict__ dec,
-                                                    uint32_t* __restrict__ group_internal) {
-  __shared__ uint32_t s_internal;
-  const uint32_t m = ctl[0];
-  const uint32_t k = threadIdx.x & 31;
-  const uint32_t i = blockIdx.x * kDecideItems + (threadIdx.x >> 5);
-  if (threadIdx.x == 0) s_internal = 0;
-  __syncthreads();
-  Item it;
-  uint32_t size = 0;
-  if (i < m) {
-    it = items[i];
-    size = it.hi - it.lo;
-  }
-  const bool split_test = i < m && size > kLeafMin;
-  int axis = 0, cnt = 0;
-  float scale = 0.f, bmin0 = 0.f;
-  float bmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, bmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
-  if (split_test) {
-    const int* a = acc + (size_t)i * kAccWords;
-    item_axis(a, axis, scale, bmin0);
-    const int* b = a + 6 + k * 7;
-    cnt = b[0];
-    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(b[1 + q]); bmx[q] = o2f(b[4 + q]); }
-  }
-  // inclusive prefix (left) and suffix (right) over the 32 bins of this half
+// The reference's sweep over one item's bins (:424-474) and children boxes
+// (:501-530), one lane per bin: lane k of each 32-lane half holds bin k
+// (count, ordered-int box) of its half's item, and every lane of the wave
+// calls this. The left sweep's running box and count after bin k are the
+// inclusive prefix at lane k, and the right sweep's after bin k the inclusive
+// suffix; min, max and integer sums give the same values in any order, and
+// each lane then evaluates the reference's area and cost expressions on those
+// values. The reference keeps the first strict minimum of its sweep from bin
+// 31 down, i.e. the largest split index among the minimal costs. The
+// children's boxes are the prefix at the chosen split and the suffix after it.
+struct Sweep {
+  int m_min;  // -1: no finite cost below +huge
+  float T_min;
+  int lcount;
+  float child[12];  // left min xyz, left max xyz, right min xyz, right max xyz
+};
+
+__device__ __forceinline__ Sweep sweep_bins(int cnt, const float (&bmn)[3], const float (&bmx)[3], float pa,
+                                            uint32_t k) {
   int lc = cnt, rc = cnt;
   float lmn[3], lmx[3], rmn[3], rmx[3];
   for (int q = 0; q < 3; ++q) { lmn[q] = rmn[q] = bmn[q]; lmx[q] = rmx[q] = bmx[q]; }
@@ -347,8 +344,6 @@ __global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ ite
   const float rarea = 0.f + box_area(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2]);
   const float rarea_next = __shfl_down(rarea, 1, 32);
   const int rc_next = __shfl_down(rc, 1, 32);
-  float pa = 0.f;
-  if (split_test) pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
   const float larea = 0.f + box_area(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2]);
   const int lcount = 0 + lc;
   const float T = 1.0f * 2.f + 0.125f * ((larea / pa) * (float)lcount + (rarea_next / pa) * (float)(0 + rc_next));
@@ -360,37 +355,86 @@ __global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ ite
     const int oj = __shfl_xor(bj, off, 32);
     if (oj >= 0 && (bj < 0 || ot < bt || (ot == bt && oj > bj))) { bt = ot; bj = oj; }
   }
-  const int m_min = bj;
-  const float T_min = m_min >= 0 ? bt : RK_HUGE;
-  // children: prefix at m_min, suffix at m_min + 1
-  const int src_l = m_min >= 0 ? m_min : 0, src_r = m_min >= 0 ? m_min + 1 : 0;
-  float child[12];
+  Sweep r;
+  r.m_min = bj;
+  r.T_min = bj >= 0 ? bt : RK_HUGE;
+  const int src_l = bj >= 0 ? bj : 0, src_r = bj >= 0 ? bj + 1 : 0;
   for (int q = 0; q < 3; ++q) {
-    child[q] = __shfl(lmn[q], src_l, 32);
-    child[3 + q] = __shfl(lmx[q], src_l, 32);
-    child[6 + q] = __shfl(rmn[q], src_r, 32);
-    child[9 + q] = __shfl(rmx[q], src_r, 32);
+    r.child[q] = __shfl(lmn[q], src_l, 32);
+    r.child[3 + q] = __shfl(lmx[q], src_l, 32);
+    r.child[6 + q] = __shfl(rmn[q], src_r, 32);
+    r.child[9 + q] = __shfl(rmx[q], src_r, 32);
   }
-  const int lcount_at = __shfl(lc, src_l, 32);
-  if (i < m && k == 0) {
-    Decision d;
-    d.axis = 0; d.scale = 0.f; d.bmin0 = 0.f; d.m_min = -1; d.lcount = 0;
-    d.pad[0] = d.pad[1] = d.pad[2] = 0;
-    for (int q = 0; q < 12; ++q) d.child[q] = 0.f;
-    if (split_test) {
-      d.axis = axis; d.scale = scale; d.bmin0 = bmin0;
-      bool leaf = size < kLeafMax && T_min > (float)(size + 1) * 0.125f;
-      if (!leaf && m_min < 0) {
-        leaf = true;
-        atomicOr(ctl + 4, (uint32_t)RK_FLAG_SAH_NO_CUT);
-      }
-      if (!leaf) {
-        d.m_min = m_min;
-        d.lcount = (uint32_t)lcount_at;
-        for (int q = 0; q < 12; ++q) d.child[q] = child[q];
-        atomicAdd(&s_internal, 1u);
-      }
-    }
+  r.lcount = __shfl(lc, src_l, 32);
+  return r;
+}
+
+// The leaf test (:479-496) on a sweep's result; an item with no finite cut
+// that is too large for a leaf becomes one anyway and is flagged in the flags word
+// (when ctl is given).
+__device__ __forceinline__ Decision make_decision(uint32_t size, int axis, float scale, float bmin0, const Sweep& sw,
+                                                  uint32_t* __restrict__ ctl) {
+  Decision d;
+  d.axis = axis; d.scale = scale; d.bmin0 = bmin0; d.m_min = -1; d.lcount = 0;
+  d.pad[0] = d.pad[1] = d.pad[2] = 0;
+  for (int q = 0; q < 12; ++q) d.child[q] = 0.f;
+  bool leaf = size < kLeafMax && sw.T_min > (float)(size + 1) * 0.125f;
+  if (!leaf && sw.m_min < 0) {
+    leaf = true;
+    if (ctl) atomicOr(ctl + kCtlFlags, (uint32_t)RK_FLAG_SAH_NO_CUT);
+  }
+  if (!leaf) {
+    d.m_min = sw.m_min;
+    d.lcount = (uint32_t)sw.lcount;
+    for (int q = 0; q < 12; ++q) d.child[q] = sw.child[q];
+  }
+  return d;
+}
+
+__device__ __forceinline__ Decision leaf_decision() {
+  Decision d;
+  d.axis = 0; d.scale = 0.f; d.bmin0 = 0.f; d.m_min = -1; d.lcount = 0;
+  d.pad[0] = d.pad[1] = d.pad[2] = 0;
+  for (int q = 0; q < 12; ++q) d.child[q] = 0.f;
+  return d;
+}
+
+// Items of more than kChunk faces, decided from their chunks' accumulators;
+// k_sah_small decides the rest. One 32-lane half-wave per item.
+__global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                    uint32_t par, const int* __restrict__ acc,
+                                                    Decision* __restrict__ dec,
+                                                    uint32_t* __restrict__ group_internal) {
+  __shared__ uint32_t s_internal;
+  const uint32_t m = ctl[kCtlLevel(par) + kLvM];
+  const uint32_t k = threadIdx.x & 31;
+  const uint32_t i = blockIdx.x * kDecideItems + (threadIdx.x >> 5);
+  if (threadIdx.x == 0) s_internal = 0;
+  __syncthreads();
+  Item it;
+  uint32_t size = 0;
+  if (i < m) {
+    it = items[i];
+    size = it.hi - it.lo;
+  }
+  const bool split_test = i < m && size > kChunk;
+  int axis = 0, cnt = 0;
+  float scale = 0.f, bmin0 = 0.f, pa = 0.f;
+  float bmn[3] = {RK_HUGE, RK_HUGE, RK_HUGE}, bmx[3] = {-RK_HUGE, -RK_HUGE, -RK_HUGE};
+  if (split_test) {
+    const int* a = acc + (size_t)it.pad * kAccWords;
+    int cb[6];
+    acc_bounds(a, cb);
+    item_axis(cb, axis, scale, bmin0);
+    const int* b = a + kBinBase + k * 7;
+    cnt = b[0];
+    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(b[1 + q]); bmx[q] = o2f(b[4 + q]); }
+    pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
+  }
+  const Sweep sw = sweep_bins(cnt, bmn, bmx, pa, k);
+  if (k == 0 && split_test) {
+    const Decision d = make_decision(size, axis, scale, bmin0, sw, ctl);
+    if (d.m_min >= 0) atomicAdd(&s_internal, 1u);
     dec[i] = d;
   }
   __syncthreads();
@@ -399,21 +443,135 @@ __global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ ite
     atomicAdd(group_internal + blockIdx.x * kDecideItems / kEmitItems, s_internal);
 }
 
+// Items of kLeafMin + 1 to kChunk faces, one wave each, start to end: centroid
+// bounds, bins, the sweep and the stable partition, with the item's face ids
+// held in LDS so that the partition writes back in place. Every step reduces
+// or scans the same values as the chunk kernels, so the decision is the same.
+// Items of at most kLeafMin faces are leaves (:239-255).
+constexpr uint32_t kSmallWaves = 4;
+static_assert(kEmitItems % kSmallWaves == 0, "a small-item workgroup's items lie in one emit group");
+__global__ __launch_bounds__(64 * kSmallWaves) void k_sah_small(const Item* __restrict__ items,
+                                                                uint32_t* __restrict__ ctl, uint32_t par,
+                                                                uint32_t* __restrict__ indices,
+                                                                const Box* __restrict__ boxes,
+                                                                Decision* __restrict__ dec,
+                                                                uint32_t* __restrict__ group_internal) {
+  __shared__ uint32_t s_ids[kSmallWaves][kChunk];
+  __shared__ uint8_t s_binof[kSmallWaves][kChunk];
+  __shared__ int s_bins[kSmallWaves][kBins * 7];
+  __shared__ uint32_t s_internal;
+  const uint32_t m = ctl[kCtlLevel(par) + kLvM];
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t i = blockIdx.x * kSmallWaves + w;
+  if (threadIdx.x == 0) s_internal = 0;
+  __syncthreads();
+  Item it;
+  uint32_t size = 0;
+  if (i < m) {
+    it = items[i];
+    size = it.hi - it.lo;
+  }
+  if (i < m && size <= kLeafMin) {  // wave-uniform
+    if (lane == 0) dec[i] = leaf_decision();
+  } else if (size > kLeafMin && size <= kChunk) {  // wave-uniform
+    uint32_t* ids = s_ids[w];
+    uint8_t* binof = s_binof[w];
+    int* bins = s_bins[w];
+    // centroid bounds, ordered ints as in k_sah_bounds' atomics; the first
+    // face of each lane stays in registers for the binning pass
+    int cmn[3] = {f2o(RK_HUGE), f2o(RK_HUGE), f2o(RK_HUGE)}, cmx[3] = {f2o(-RK_HUGE), f2o(-RK_HUGE), f2o(-RK_HUGE)};
+    Box b0;
+    for (uint32_t l = lane; l < size; l += 64) {
+      const uint32_t id = indices[it.lo + l];
+      ids[l] = id;
+      const Box b = rk::load_box(boxes + id);
+      if (l == lane) b0 = b;
+      for (int a = 0; a < 3; ++a) {
+        const int c = f2o(center_axis(b, a));
+        cmn[a] = min(cmn[a], c);
+        cmx[a] = max(cmx[a], c);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      for (int a = 0; a < 3; ++a) {
+        cmn[a] = min(cmn[a], __shfl_xor(cmn[a], off, 64));
+        cmx[a] = max(cmx[a], __shfl_xor(cmx[a], off, 64));
+      }
+    int cb[6] = {cmn[0], cmn[1], cmn[2], cmx[0], cmx[1], cmx[2]};
+    int axis;
+    float scale, bmin0;
+    item_axis(cb, axis, scale, bmin0);
+    for (int q = lane; q < kBins * 7; q += 64) {
+      const int kq = q % 7;
+      bins[q] = kq == 0 ? 0 : (kq < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (uint32_t l = lane; l < size; l += 64) {
+      const Box b = l == lane ? b0 : rk::load_box(boxes + ids[l]);
+      const int bn = bin_of(b, axis, scale, bmin0);
+      binof[l] = (uint8_t)bn;
+      int* sb = bins + bn * 7;
+      atomicAdd(sb, 1);
+      atomicMin(sb + 1, f2o(b.mn.x)); atomicMin(sb + 2, f2o(b.mn.y)); atomicMin(sb + 3, f2o(b.mn.z));
+      atomicMax(sb + 4, f2o(b.mx.x)); atomicMax(sb + 5, f2o(b.mx.y)); atomicMax(sb + 6, f2o(b.mx.z));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t k = lane & 31;
+    const int* bk = bins + k * 7;
+    const int cnt = bk[0];
+    float bmn[3], bmx[3];
+    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(bk[1 + q]); bmx[q] = o2f(bk[4 + q]); }
+    const float pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
+    const Sweep sw = sweep_bins(cnt, bmn, bmx, pa, k);
+    const Decision d = make_decision(size, axis, scale, bmin0, sw, lane == 0 ? ctl : nullptr);
+    if (d.m_min >= 0) {
+      // stable partition of the item's ids, bin <= m_min first (:534-610)
+      uint32_t left_before = 0, right_before = 0;
+      for (uint32_t l0 = 0; l0 < size; l0 += 64) {
+        const uint32_t l = l0 + lane;
+        uint32_t id = 0;
+        bool is_left = false;
+        if (l < size) {
+          id = ids[l];
+          is_left = (int)binof[l] <= d.m_min;
+        }
+        const uint64_t lm = __ballot(l < size && is_left), rm = __ballot(l < size && !is_left);
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (l < size) {
+          const uint32_t out = is_left ? it.lo + left_before + (uint32_t)__popcll(lm & below)
+                                       : it.lo + d.lcount + right_before + (uint32_t)__popcll(rm & below);
+          indices[out] = id;
+        }
+        left_before += (uint32_t)__popcll(lm);
+        right_before += (uint32_t)__popcll(rm);
+      }
+      if (lane == 0) atomicAdd(&s_internal, 1u);
+    }
+    if (lane == 0) dec[i] = d;
+  }
+  __syncthreads();
+  // kEmitItems is a multiple of kSmallWaves: the workgroup's items share a group
+  if (threadIdx.x == 0 && s_internal)
+    atomicAdd(group_internal + blockIdx.x * kSmallWaves / kEmitItems, s_internal);
+}
+
 // Node records and the next level's items, in item order: the k-th internal
 // item of the level takes children (count + 2k, count + 2k + 1). Workgroup g
 // handles items [g, g + 1) * kEmitItems and starts after the internal items
 // counted in the groups before it. The grid may hold more workgroups than the
 // level has groups; the one with the last group (workgroup 0 when the level
-// is empty) writes the new counts to ctl[2] and ctl[5], as ctl[3] is read by
-// every workgroup.
+// is empty) writes the next level's item and node counts.
 __global__ __launch_bounds__(1024) void k_sah_emit(const Item* __restrict__ items, uint32_t* __restrict__ ctl,
-                                                   const Decision* __restrict__ dec,
+                                                   uint32_t par, const Decision* __restrict__ dec,
                                                    const uint32_t* __restrict__ group_internal,
                                                    Item* __restrict__ next, Node* __restrict__ nodes,
                                                    uint32_t nodes_cap) {
   __shared__ uint32_t s_tmp[16];
-  const uint32_t m = ctl[0];
-  const uint32_t count0 = ctl[3];
+  const uint32_t m = ctl[kCtlLevel(par) + kLvM];
+  const uint32_t count0 = ctl[kCtlLevel(par) + kLvCount];
   const uint32_t g = blockIdx.x;
   uint32_t before = 0, before_total;
   for (uint32_t h = threadIdx.x; h < g; h += 1024) before += group_internal[h];
@@ -429,6 +587,12 @@ __global__ __launch_bounds__(1024) void k_sah_emit(const Item* __restrict__ item
   }
   uint32_t total;
   const uint32_t ex = block_excl_scan(internal, s_tmp, &total, 1024);
+  // children of more than kChunk faces, for the next level's launches
+  uint32_t big = 0, big_total;
+  if (internal) big = (d.lcount > kChunk ? 1u : 0u) + (it.hi - it.lo - d.lcount > kChunk ? 1u : 0u);
+  block_excl_scan(big, s_tmp, &big_total, 1024);
+  uint32_t* nx = ctl + kCtlLevel(par ^ 1u);
+  if (threadIdx.x == 0 && big_total) atomicAdd(nx + kLvBig, big_total);
   if (i < m) {
     if (internal) {
       const uint32_t k = before_total + ex;
@@ -450,18 +614,19 @@ __global__ __launch_bounds__(1024) void k_sah_emit(const Item* __restrict__ item
   }
   if (threadIdx.x == 0 && g == (m ? (m - 1) / kEmitItems : 0u)) {
     const uint32_t all = before_total + total;
-    ctl[2] = 2 * all;
-    ctl[5] = count0 + 2 * all;
-    if (count0 + 2 * all > nodes_cap) atomicOr(ctl + 4, 0x20u);
+    nx[kLvM] = 2 * all;
+    nx[kLvCount] = count0 + 2 * all;
+    if (count0 + 2 * all > nodes_cap) atomicOr(ctl + kCtlFlags, 0x20u);
   }
 }
 
 __global__ __launch_bounds__(kCT) void k_sah_count(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                   uint32_t par, const uint32_t* __restrict__ chunk_item,
                                                    const uint32_t* __restrict__ chunk_base,
                                                    const uint32_t* __restrict__ indices, const Box* __restrict__ boxes,
                                                    const Decision* __restrict__ dec, uint32_t* __restrict__ chunk_left) {
   __shared__ uint32_t s_cnt;
-  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  const ChunkRange r = chunk_range(items, ctl, par, chunk_base, chunk_item, blockIdx.x);
   if (!r.ok) return;
   const Decision d = dec[r.item];
   if (threadIdx.x == 0) s_cnt = 0;
@@ -479,14 +644,14 @@ __global__ __launch_bounds__(kCT) void k_sah_count(const Item* __restrict__ item
 // Exclusive scan of chunk_left within each item (an item's chunks are
 // contiguous), one wave per item.
 __global__ __launch_bounds__(256) void k_sah_offsets(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
-                                                     const uint32_t* __restrict__ chunk_base,
+                                                     uint32_t par, const uint32_t* __restrict__ chunk_base,
                                                      const uint32_t* __restrict__ chunk_left,
                                                      uint32_t* __restrict__ chunk_loff) {
-  const uint32_t m = ctl[0];
+  const uint32_t m = ctl[kCtlLevel(par) + kLvM];
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= m) return;  // wave-uniform
   const uint32_t size = items[i].hi - items[i].lo;
-  if (size <= kLeafMin) return;
+  if (size <= kChunk) return;  // no chunks
   const uint32_t b = chunk_base[i], nch = (size + kChunk - 1) / kChunk;
   uint32_t run = 0;
   for (uint32_t c0 = 0; c0 < nch; c0 += 64) {
@@ -504,13 +669,14 @@ __global__ __launch_bounds__(256) void k_sah_offsets(const Item* __restrict__ it
 }
 
 __global__ __launch_bounds__(kCT) void k_sah_scatter(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                     uint32_t par, const uint32_t* __restrict__ chunk_item,
                                                      const uint32_t* __restrict__ chunk_base,
                                                      const uint32_t* __restrict__ indices,
                                                      const Box* __restrict__ boxes, const Decision* __restrict__ dec,
                                                      const uint32_t* __restrict__ chunk_loff,
                                                      uint32_t* __restrict__ tmp) {
   __shared__ uint32_t s_tmp[8];
-  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  const ChunkRange r = chunk_range(items, ctl, par, chunk_base, chunk_item, blockIdx.x);
   if (!r.ok) return;
   const Decision d = dec[r.item];
   if (d.m_min < 0) return;
@@ -539,10 +705,11 @@ __global__ __launch_bounds__(kCT) void k_sah_scatter(const Item* __restrict__ it
 }
 
 __global__ __launch_bounds__(kCT) void k_sah_copyback(const Item* __restrict__ items, const uint32_t* __restrict__ ctl,
+                                                      uint32_t par, const uint32_t* __restrict__ chunk_item,
                                                       const uint32_t* __restrict__ chunk_base,
                                                       const Decision* __restrict__ dec,
                                                       const uint32_t* __restrict__ tmp, uint32_t* __restrict__ indices) {
-  const ChunkRange r = chunk_range(items, ctl, chunk_base, blockIdx.x);
+  const ChunkRange r = chunk_range(items, ctl, par, chunk_base, chunk_item, blockIdx.x);
   if (!r.ok) return;
   if (dec[r.item].m_min < 0) return;
   for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) indices[l] = tmp[l];
@@ -553,8 +720,8 @@ __global__ __launch_bounds__(kCT) void k_sah_copyback(const Item* __restrict__ i
 __global__ __launch_bounds__(1024) void k_sah_init_from_leaves(const Node* __restrict__ nodes, uint32_t existing,
                                                                Item* __restrict__ items, uint32_t* __restrict__ ctl) {
   __shared__ uint32_t s_tmp[16];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
+  __shared__ uint32_t s_carry, s_big;
+  if (threadIdx.x == 0) s_carry = s_big = 0;
   __syncthreads();
   for (uint32_t c0 = 0; c0 < existing; c0 += 1024) {
     const uint32_t k = c0 + threadIdx.x;
@@ -564,8 +731,9 @@ __global__ __launch_bounds__(1024) void k_sah_init_from_leaves(const Node* __res
       nd = nodes[k];
       leaf = rk::node_is_leaf(nd.w0);
     }
-    uint32_t total;
+    uint32_t total, big_total;
     const uint32_t ex = block_excl_scan(leaf, s_tmp, &total, 1024);
+    block_excl_scan(leaf && rk::leaf_count(nd.w0) > kChunk ? 1u : 0u, s_tmp, &big_total, 1024);
     if (leaf) {
       Item it;
       it.node = k; it.lo = nd.w1; it.hi = nd.w1 + rk::leaf_count(nd.w0); it.pad = 0;
@@ -574,10 +742,19 @@ __global__ __launch_bounds__(1024) void k_sah_init_from_leaves(const Node* __res
       items[s_carry + ex] = it;
     }
     __syncthreads();
-    if (threadIdx.x == 0) s_carry += total;
+    if (threadIdx.x == 0) {
+      s_carry += total;
+      s_big += big_total;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { ctl[0] = s_carry; ctl[3] = ctl[5] = existing; ctl[4] = 0; }
+  if (threadIdx.x == 0) {
+    uint32_t* lv = ctl + kCtlLevel(0);
+    lv[kLvM] = s_carry;
+    lv[kLvBig] = s_big;
+    lv[kLvCount] = existing;
+    ctl[kCtlFlags] = 0;
+  }
 }
 
 __global__ void k_sah_init_root(Item* __restrict__ items, uint32_t* __restrict__ ctl, uint32_t n, float4 rmin,
@@ -588,12 +765,17 @@ __global__ void k_sah_init_root(Item* __restrict__ items, uint32_t* __restrict__
   it.bmax[0] = rmax.x; it.bmax[1] = rmax.y; it.bmax[2] = rmax.z;
   it.pad2[0] = it.pad2[1] = 0.f;
   items[0] = it;
-  ctl[0] = 1; ctl[3] = ctl[5] = 1; ctl[4] = 0;
+  uint32_t* lv = ctl + kCtlLevel(0);
+  lv[kLvM] = 1;
+  lv[kLvBig] = n > kChunk ? 1u : 0u;
+  lv[kLvCount] = 1;
+  ctl[kCtlFlags] = 0;
 }
 
-__global__ void k_sah_finish(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ info) {
-  info[RK_INFO_NODE_COUNT] = ctl[5];
-  info[RK_INFO_FLAGS] |= ctl[4];
+// par: the parity of the first level that did not run, whose node count is the total.
+__global__ void k_sah_finish(const uint32_t* __restrict__ ctl, uint32_t par, uint32_t* __restrict__ info) {
+  info[RK_INFO_NODE_COUNT] = ctl[kCtlLevel(par) + kLvCount];
+  info[RK_INFO_FLAGS] |= ctl[kCtlFlags];
 }
 
 __global__ __launch_bounds__(256) void k_boxes_iota(const uint4* __restrict__ faces, const float4* __restrict__ verts,
@@ -623,16 +805,17 @@ __global__ __launch_bounds__(256) void k_copy_faces2(const uint4* __restrict__ s
 extern "C" size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t existing_nodes) {
   size_t n = face_count ? face_count : 1;
   size_t m = n + existing_nodes + 1;   // items per level
-  size_t T = n / kChunk + m + 1;       // chunks per level (upper bound)
+  size_t big = n / kChunk + 1;         // items of more than kChunk faces per level
+  size_t T = n / kChunk + big + 1;     // chunks per level: their ceil(size / kChunk)
   size_t b = 0;
   b += rk_align_up(m * sizeof(Item)) * 2;
   b += rk_align_up(m * 4);                         // chunk_base
-  b += rk_align_up(m * kAccWords * 4);             // accumulators
+  b += rk_align_up(big * kAccWords * 4);           // accumulators
   b += rk_align_up(m * sizeof(Decision));
-  b += rk_align_up(T * 4) * 2;                     // chunk_left, chunk_loff
+  b += rk_align_up(T * 4) * 3;                     // chunk_item, chunk_left, chunk_loff
   b += rk_align_up(n * 4);                         // partition temp
   b += rk_align_up(n * 16);                        // face reorder temp
-  b += rk_align_up(64);
+  b += rk_align_up(kCtlWords * 4);
   b += rk_align_up((m / kEmitGroup + 1) * 4);      // internal items per emit group
   return b + 4096;
 }
@@ -654,24 +837,26 @@ extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_fl
   if ((size_t)nodes_capacity < (size_t)existing_nodes + 2 * (size_t)n - 1) return RK_ERROR_INVALID_VALUE;
   hipStream_t st = rk_hip_stream(stream);
   const size_t m_cap = (size_t)n + existing_nodes + 1;
-  const size_t T_cap = n / kChunk + m_cap + 1;
+  const size_t big_cap = n / kChunk + 1;
+  const size_t T_cap = n / kChunk + big_cap + 1;
   RkArena ar(ws, ws_bytes);
   Item* items_a = ar.take<Item>(m_cap);
   Item* items_b = ar.take<Item>(m_cap);
   uint32_t* chunk_base = ar.take<uint32_t>(m_cap);
-  int* acc = ar.take<int>(m_cap * kAccWords);
+  int* acc = ar.take<int>(big_cap * kAccWords);
   Decision* dec = ar.take<Decision>(m_cap);
+  uint32_t* chunk_item = ar.take<uint32_t>(T_cap);
   uint32_t* chunk_left = ar.take<uint32_t>(T_cap);
   uint32_t* chunk_loff = ar.take<uint32_t>(T_cap);
   uint32_t* tmp = ar.take<uint32_t>(n);
   uint4* ftmp = ar.take<uint4>(n);
-  uint32_t* ctl = ar.take<uint32_t>(16);
+  uint32_t* ctl = ar.take<uint32_t>(kCtlWords);
   uint32_t* group_internal = ar.take<uint32_t>(m_cap / kEmitGroup + 1);
-  if (!items_a || !items_b || !chunk_base || !acc || !dec || !chunk_left || !chunk_loff || !tmp || !ftmp || !ctl ||
-      !group_internal)
+  if (!items_a || !items_b || !chunk_base || !acc || !dec || !chunk_item || !chunk_left || !chunk_loff || !tmp ||
+      !ftmp || !ctl || !group_internal)
     return RK_ERROR_WORKSPACE;
   rk_status s;
-  if ((s = rk_fill_u32(stream, ctl, 16, 0u)) != RK_SUCCESS) return s;
+  if ((s = rk_fill_u32(stream, ctl, kCtlWords, 0u)) != RK_SUCCESS) return s;
   if ((s = rk_fill_u32(stream, info, RK_INFO_WORDS, 0u)) != RK_SUCCESS) return s;
   if (existing_nodes == 0) {
     hipLaunchKernelGGL(k_boxes_iota, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<const uint4*>(faces),
@@ -685,50 +870,73 @@ extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_fl
                        existing_nodes, items_a, ctl);
   }
   RK_CHECK_LAUNCH();
-  uint32_t m = 0;  // this level's item count, or an upper bound of it while !exact
-  RK_HIP(hipMemcpyAsync(&m, ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  // this level's item count and items of more than kChunk faces, or upper
+  // bounds of them while !exact
+  uint32_t lvh[2] = {0u, 0u};
+  RK_HIP(hipMemcpyAsync(lvh, ctl + kCtlLevel(0), sizeof(lvh), hipMemcpyDeviceToHost, st));
   RK_HIP(hipStreamSynchronize(st));
+  uint32_t m = lvh[0], big = lvh[1];
   bool exact = true;
-  // Levels whose item count is bounded by a few thousand run without reading
-  // it back: a level has at most twice the items of the one before, and every
-  // kernel reads the exact count on the device, so a grid sized by the bound
-  // is only larger than needed. Deeper levels read the count back to size
-  // their grids; an empty level ends the build.
+  // The first levels, while their item count is bounded by a few thousand,
+  // run without reading it back: a level has at most twice the items of the
+  // one before, and every kernel reads the exact counts on the device, so a
+  // grid sized by a bound is only larger than needed. From the first level
+  // beyond that bound on, the counts are read back after every level to size
+  // the next level's grids (and to skip the chunk kernels when no item has
+  // more than kChunk faces), and an empty level ends the build.
   const uint32_t async_items = std::min<uint32_t>(4096u, n / 256);
+  const uint32_t big_bound = n / (kChunk + 1);
+  bool shallow = true;
   Item* cur = items_a;
   Item* nxt = items_b;
   const Box* bx = reinterpret_cast<const Box*>(boxes);
   Node* nd = reinterpret_cast<Node*>(nodes);
-  for (uint32_t level = 0; !(exact && m == 0); ++level) {
+  uint32_t level = 0;
+  for (; !(exact && m == 0); ++level) {
     if (level > 4096) return RK_ERROR_LAUNCH;
-    const unsigned grid = (unsigned)(n / kChunk + m + 1);
-    hipLaunchKernelGGL(k_sah_prep, dim3(1), dim3(1024), 0, st, cur, ctl, chunk_base, acc, group_internal,
-                       level > 0 ? 1u : 0u);
-    hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, acc);
-    hipLaunchKernelGGL(k_sah_bins, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, acc);
-    hipLaunchKernelGGL(k_sah_decide, dim3((m + kDecideItems - 1) / kDecideItems), dim3(256), 0, st, cur, ctl, acc,
-                       dec, group_internal);
+    const uint32_t par = level & 1u;
+    const unsigned grid = (unsigned)(n / kChunk + std::min(big, big_bound) + 1);  // chunks: see the workspace
+    hipLaunchKernelGGL(k_sah_prep, dim3((m + 255) / 256), dim3(256), 0, st, cur, ctl, par, chunk_base, chunk_item,
+                       acc, group_internal);
+    if (big) {
+      hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_base, chunk_item, indices,
+                         bx, acc);
+      hipLaunchKernelGGL(k_sah_bins, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_base, chunk_item, indices, bx,
+                         acc);
+      hipLaunchKernelGGL(k_sah_decide, dim3((m + kDecideItems - 1) / kDecideItems), dim3(256), 0, st, cur, ctl, par,
+                         acc, dec, group_internal);
+    }
+    hipLaunchKernelGGL(k_sah_small, dim3((m + kSmallWaves - 1) / kSmallWaves), dim3(64 * kSmallWaves), 0, st, cur,
+                       ctl, par, indices, bx, dec, group_internal);
     hipLaunchKernelGGL(k_sah_emit, dim3(m ? (m + kEmitItems - 1) / kEmitItems : 1u), dim3(1024), 0, st, cur, ctl,
-                       dec, group_internal, nxt, nd, nodes_capacity);
-    hipLaunchKernelGGL(k_sah_count, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_left);
-    hipLaunchKernelGGL(k_sah_offsets, dim3((m + 3) / 4), dim3(256), 0, st, cur, ctl, chunk_base, chunk_left,
-                       chunk_loff);
-    hipLaunchKernelGGL(k_sah_scatter, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, indices, bx, dec, chunk_loff,
-                       tmp);
-    hipLaunchKernelGGL(k_sah_copyback, dim3(grid), dim3(kCT), 0, st, cur, ctl, chunk_base, dec, tmp, indices);
+                       par, dec, group_internal, nxt, nd, nodes_capacity);
+    if (big) {
+      hipLaunchKernelGGL(k_sah_count, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_item, chunk_base, indices,
+                         bx, dec, chunk_left);
+      hipLaunchKernelGGL(k_sah_offsets, dim3((m + 3) / 4), dim3(256), 0, st, cur, ctl, par, chunk_base, chunk_left,
+                         chunk_loff);
+      hipLaunchKernelGGL(k_sah_scatter, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_item, chunk_base, indices,
+                         bx, dec, chunk_loff, tmp);
+      hipLaunchKernelGGL(k_sah_copyback, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_item, chunk_base, dec, tmp,
+                         indices);
+    }
     RK_CHECK_LAUNCH();
     const uint64_t bound = std::min<uint64_t>(2 * (uint64_t)m, m_cap);
-    if (bound <= async_items) {
+    if (shallow && bound <= async_items) {
       m = (uint32_t)bound;
+      big = (uint32_t)std::min<uint64_t>(bound, big_bound);
       exact = false;
     } else {
-      RK_HIP(hipMemcpyAsync(&m, ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      shallow = false;
+      RK_HIP(hipMemcpyAsync(lvh, ctl + kCtlLevel(par ^ 1u), sizeof(lvh), hipMemcpyDeviceToHost, st));
       RK_HIP(hipStreamSynchronize(st));
+      m = lvh[0];
+      big = lvh[1];
       exact = true;
     }
     std::swap(cur, nxt);
   }
-  hipLaunchKernelGGL(k_sah_finish, dim3(1), dim3(1), 0, st, ctl, info);
+  hipLaunchKernelGGL(k_sah_finish, dim3(1), dim3(1), 0, st, ctl, level & 1u, info);
   RK_CHECK_LAUNCH();
   // CUDA_reorderMeshFaces, :778-784
   uint4* f = reinterpret_cast<uint4*>(faces);
