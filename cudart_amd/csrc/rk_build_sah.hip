@@ -55,11 +55,13 @@ static_assert(sizeof(Item) == 48, "item is 48 B");
 
 // Accumulators of an item of more than kChunk faces: its centroid bounds
 // (6 ordered ints) in kBoundSlots copies, which its chunks update in turn so
-// that fewer atomics meet on one address, then 32 bins of (count, 3 min,
-// 3 max ordered ints).
+// that fewer atomics meet on one address, then kBinSlots copies of 32 bins
+// of (count, 3 min, 3 max ordered ints).
 constexpr int kBoundSlots = 8;
+constexpr int kBinSlots = 4;  // copies of the bins, likewise
 constexpr int kBinBase = 6 * kBoundSlots;
-constexpr int kAccWords = kBinBase + kBins * 7;
+constexpr int kBinWords = kBins * 7;
+constexpr int kAccWords = kBinBase + kBinSlots * kBinWords;
 
 // Per-item decision written by k_sah_decide.
 struct Decision {
@@ -116,35 +118,42 @@ constexpr int kLvM = 0, kLvBig = 1, kLvT = 2, kLvSlot = 3, kLvCount = 4;
 
 // Items of more than kChunk faces get a slot of accumulators (kept in the
 // item's pad word) and a run of chunks, in any order among items; chunk_item
-// maps a chunk back to its item. Smaller items are k_sah_small's.
-__global__ __launch_bounds__(256) void k_sah_prep(Item* __restrict__ items, uint32_t* __restrict__ ctl, uint32_t par,
-                                                  uint32_t* __restrict__ chunk_base,
-                                                  uint32_t* __restrict__ chunk_item, int* __restrict__ acc,
-                                                  uint32_t* __restrict__ group_internal) {
+// maps a chunk back to its item. Smaller items are k_sah_small's. One wave
+// per item, launched over the level's items (with a bound, kPrepItems per
+// workgroup), and the grid's threads also clear the emit groups' counters.
+constexpr uint32_t kPrepItems = 4;
+__global__ __launch_bounds__(64 * kPrepItems) void k_sah_prep(Item* __restrict__ items, uint32_t* __restrict__ ctl,
+                                                              uint32_t par, uint32_t* __restrict__ chunk_base,
+                                                              uint32_t* __restrict__ chunk_item,
+                                                              int* __restrict__ acc,
+                                                              uint32_t* __restrict__ group_internal) {
   uint32_t* lv = ctl + kCtlLevel(par);
   const uint32_t m = lv[kLvM];
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) {
+  const uint32_t t = blockIdx.x * (64 * kPrepItems) + threadIdx.x, lane = threadIdx.x & 63;
+  if (t == 0) {
     uint32_t* nx = ctl + kCtlLevel(par ^ 1u);
     nx[kLvBig] = 0u;
     nx[kLvT] = 0u;
     nx[kLvSlot] = 0u;
   }
-  if (i < (m + kEmitGroup - 1) / kEmitGroup) group_internal[i] = 0u;
-  if (i >= m) return;
+  if (t < (m + kEmitGroup - 1) / kEmitGroup) group_internal[t] = 0u;
+  const uint32_t i = t >> 6;
+  if (i >= m) return;  // wave-uniform
   const uint32_t size = items[i].hi - items[i].lo;
   if (size <= kChunk) return;
   const uint32_t nch = (size + kChunk - 1) / kChunk;
-  const uint32_t slot = atomicAdd(lv + kLvSlot, 1u);
-  const uint32_t base = atomicAdd(lv + kLvT, nch);
-  items[i].pad = slot;
-  chunk_base[i] = base;
-  for (uint32_t j = 0; j < nch; ++j) chunk_item[base + j] = i;
-  int* a = acc + (size_t)slot * kAccWords;  // the bins are reset by k_sah_bounds
-  for (int q = 0; q < kBoundSlots; ++q) {
-    a[6 * q + 0] = a[6 * q + 1] = a[6 * q + 2] = f2o(RK_HUGE);
-    a[6 * q + 3] = a[6 * q + 4] = a[6 * q + 5] = f2o(-RK_HUGE);
+  uint32_t slot = 0, base = 0;
+  if (lane == 0) {
+    slot = atomicAdd(lv + kLvSlot, 1u);
+    base = atomicAdd(lv + kLvT, nch);
+    items[i].pad = slot;
+    chunk_base[i] = base;
   }
+  slot = __shfl(slot, 0, 64);
+  base = __shfl(base, 0, 64);
+  for (uint32_t j = lane; j < nch; j += 64) chunk_item[base + j] = i;
+  int* a = acc + (size_t)slot * kAccWords;  // the bins are reset by k_sah_bounds
+  if (lane < 6 * kBoundSlots) a[lane] = (lane % 6) < 3 ? f2o(RK_HUGE) : f2o(-RK_HUGE);
 }
 
 struct ChunkRange { uint32_t item, slot, lo, hi; bool first, ok; };
@@ -165,6 +174,23 @@ __device__ __forceinline__ ChunkRange chunk_range(const Item* __restrict__ items
   return r;
 }
 
+// The boxes of a chunk's faces, kPer per thread, with every gather in flight
+// at once: the face ids first, then the boxes.
+constexpr int kPer = kChunk / kCT;
+__device__ __forceinline__ void chunk_boxes(const ChunkRange& r, const uint32_t* __restrict__ indices,
+                                            const Box* __restrict__ boxes, Box (&b)[kPer], bool (&in)[kPer]) {
+  uint32_t id[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const uint32_t l = r.lo + threadIdx.x + (uint32_t)u * kCT;
+    in[u] = l < r.hi;
+    id[u] = in[u] ? indices[l] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u)
+    if (in[u]) b[u] = rk::load_box(boxes + id[u]);
+}
+
 __device__ __forceinline__ float center_axis(const Box& b, int a) {  // 0.5f * (min + max), :292
   return a == 0 ? (b.mn.x + b.mx.x) * 0.5f : (a == 1 ? (b.mn.y + b.mx.y) * 0.5f : (b.mn.z + b.mx.z) * 0.5f);
 }
@@ -179,18 +205,22 @@ __global__ __launch_bounds__(kCT) void k_sah_bounds(const Item* __restrict__ ite
   int* a = acc + (size_t)r.slot * kAccWords;
   if (r.first) {  // the item's first chunk resets its bins for k_sah_bins
     int* bins = a + kBinBase;
-    for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
-      int k = w % 7;
+    for (int w = threadIdx.x; w < kBinSlots * kBinWords; w += kCT) {
+      int k = (w % kBinWords) % 7;
       bins[w] = k == 0 ? 0 : (k < 4 ? f2o(RK_HUGE) : f2o(-RK_HUGE));
     }
   }
   // ordered ints, as the atomics compare them
   int mn[3] = {f2o(RK_HUGE), f2o(RK_HUGE), f2o(RK_HUGE)}, mx[3] = {f2o(-RK_HUGE), f2o(-RK_HUGE), f2o(-RK_HUGE)};
-  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
-    Box b = rk::load_box(boxes + indices[l]);
+  Box bb[kPer];
+  bool in[kPer];
+  chunk_boxes(r, indices, boxes, bb, in);
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (!in[u]) continue;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const int c = f2o(center_axis(b, q));
+      const int c = f2o(center_axis(bb[u], q));
       mn[q] = min(mn[q], c);
       mx[q] = max(mx[q], c);
     }
@@ -262,8 +292,13 @@ __global__ __launch_bounds__(kCT) void k_sah_bins(const Item* __restrict__ items
   int axis;
   float scale, bmin0;
   item_axis(cb, axis, scale, bmin0);
-  for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT) {
-    Box b = rk::load_box(boxes + indices[l]);
+  Box bb[kPer];
+  bool in[kPer];
+  chunk_boxes(r, indices, boxes, bb, in);
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (!in[u]) continue;
+    const Box& b = bb[u];
     int m = bin_of(b, axis, scale, bmin0);
     int* s = s_bin + m * 7;
     atomicAdd(s, 1);
@@ -271,9 +306,10 @@ __global__ __launch_bounds__(kCT) void k_sah_bins(const Item* __restrict__ items
     atomicMax(s + 4, f2o(b.mx.x)); atomicMax(s + 5, f2o(b.mx.y)); atomicMax(s + 6, f2o(b.mx.z));
   }
   __syncthreads();
+  int* slot_bins = a + kBinBase + (blockIdx.x % kBinSlots) * kBinWords;
   for (int w = threadIdx.x; w < kBins * 7; w += kCT) {
     int k = w % 7, v = s_bin[w];
-    int* g = a + kBinBase + w;
+    int* g = slot_bins + w;
     if (k == 0) { if (v) atomicAdd(g, v); }
     else if (k < 4) { if (v != f2o(RK_HUGE)) atomicMin(g, v); }
     else { if (v != f2o(-RK_HUGE)) atomicMax(g, v); }
@@ -426,9 +462,16 @@ __global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ ite
     int cb[6];
     acc_bounds(a, cb);
     item_axis(cb, axis, scale, bmin0);
-    const int* b = a + kBinBase + k * 7;
-    cnt = b[0];
-    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(b[1 + q]); bmx[q] = o2f(b[4 + q]); }
+    int bi[7];
+    for (int q = 0; q < 7; ++q) bi[q] = a[kBinBase + k * 7 + q];
+    for (int v = 1; v < kBinSlots; ++v) {
+      const int* b = a + kBinBase + v * kBinWords + k * 7;
+      bi[0] += b[0];
+      for (int q = 1; q < 4; ++q) bi[q] = min(bi[q], b[q]);
+      for (int q = 4; q < 7; ++q) bi[q] = max(bi[q], b[q]);
+    }
+    cnt = bi[0];
+    for (int q = 0; q < 3; ++q) { bmn[q] = o2f(bi[1 + q]); bmx[q] = o2f(bi[4 + q]); }
     pa = box_area(it.bmin[0], it.bmin[1], it.bmin[2], it.bmax[0], it.bmax[1], it.bmax[2]);
   }
   const Sweep sw = sweep_bins(cnt, bmn, bmx, pa, k);
@@ -632,9 +675,13 @@ __global__ __launch_bounds__(kCT) void k_sah_count(const Item* __restrict__ item
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   if (d.m_min >= 0) {
+    Box bb[kPer];
+    bool in[kPer];
+    chunk_boxes(r, indices, boxes, bb, in);
     uint32_t c = 0;
-    for (uint32_t l = r.lo + threadIdx.x; l < r.hi; l += kCT)
-      c += bin_of(rk::load_box(boxes + indices[l]), d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (in[u]) c += bin_of(bb[u], d.axis, d.scale, d.bmin0) <= d.m_min ? 1u : 0u;
     atomicAdd(&s_cnt, c);
   }
   __syncthreads();
@@ -896,8 +943,8 @@ extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_fl
     if (level > 4096) return RK_ERROR_LAUNCH;
     const uint32_t par = level & 1u;
     const unsigned grid = (unsigned)(n / kChunk + std::min(big, big_bound) + 1);  // chunks: see the workspace
-    hipLaunchKernelGGL(k_sah_prep, dim3((m + 255) / 256), dim3(256), 0, st, cur, ctl, par, chunk_base, chunk_item,
-                       acc, group_internal);
+    hipLaunchKernelGGL(k_sah_prep, dim3(m ? (m + kPrepItems - 1) / kPrepItems : 1u), dim3(64 * kPrepItems), 0, st,
+                       cur, ctl, par, chunk_base, chunk_item, acc, group_internal);
     if (big) {
       hipLaunchKernelGGL(k_sah_bounds, dim3(grid), dim3(kCT), 0, st, cur, ctl, par, chunk_base, chunk_item, indices,
                          bx, acc);
