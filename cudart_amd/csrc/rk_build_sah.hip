@@ -492,6 +492,7 @@ __global__ __launch_bounds__(256) void k_sah_decide(const Item* __restrict__ ite
 // or scans the same values as the chunk kernels, so the decision is the same.
 // Items of at most kLeafMin faces are leaves (:239-255).
 constexpr uint32_t kSmallWaves = 4;
+constexpr int kSmallPer = 4;  // faces per lane in flight
 static_assert(kEmitItems % kSmallWaves == 0, "a small-item workgroup's items lie in one emit group");
 __global__ __launch_bounds__(64 * kSmallWaves) void k_sah_small(const Item* __restrict__ items,
                                                                 uint32_t* __restrict__ ctl, uint32_t par,
@@ -520,19 +521,33 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_sah_small(const Item* __re
     uint32_t* ids = s_ids[w];
     uint8_t* binof = s_binof[w];
     int* bins = s_bins[w];
-    // centroid bounds, ordered ints as in k_sah_bounds' atomics; the first
-    // face of each lane stays in registers for the binning pass
+    // centroid bounds, ordered ints as in k_sah_bounds' atomics. Faces go
+    // kSmallPer per lane at a time, every gather in flight at once; the first
+    // 64 * kSmallPer boxes stay in registers for the binning pass.
     int cmn[3] = {f2o(RK_HUGE), f2o(RK_HUGE), f2o(RK_HUGE)}, cmx[3] = {f2o(-RK_HUGE), f2o(-RK_HUGE), f2o(-RK_HUGE)};
-    Box b0;
-    for (uint32_t l = lane; l < size; l += 64) {
-      const uint32_t id = indices[it.lo + l];
-      ids[l] = id;
-      const Box b = rk::load_box(boxes + id);
-      if (l == lane) b0 = b;
-      for (int a = 0; a < 3; ++a) {
-        const int c = f2o(center_axis(b, a));
-        cmn[a] = min(cmn[a], c);
-        cmx[a] = max(cmx[a], c);
+    Box keep[kSmallPer];
+    for (uint32_t l0 = 0; l0 < size; l0 += 64 * kSmallPer) {
+      uint32_t id[kSmallPer];
+      Box b[kSmallPer];
+#pragma unroll
+      for (int u = 0; u < kSmallPer; ++u) {
+        const uint32_t l = l0 + lane + 64u * u;
+        id[u] = l < size ? indices[it.lo + l] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kSmallPer; ++u)
+        if (l0 + lane + 64u * u < size) b[u] = rk::load_box(boxes + id[u]);
+#pragma unroll
+      for (int u = 0; u < kSmallPer; ++u) {
+        const uint32_t l = l0 + lane + 64u * u;
+        if (l >= size) continue;
+        ids[l] = id[u];
+        if (l0 == 0) keep[u] = b[u];
+        for (int a = 0; a < 3; ++a) {
+          const int c = f2o(center_axis(b[u], a));
+          cmn[a] = min(cmn[a], c);
+          cmx[a] = max(cmx[a], c);
+        }
       }
     }
 #pragma unroll
@@ -551,14 +566,24 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_sah_small(const Item* __re
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (uint32_t l = lane; l < size; l += 64) {
-      const Box b = l == lane ? b0 : rk::load_box(boxes + ids[l]);
-      const int bn = bin_of(b, axis, scale, bmin0);
-      binof[l] = (uint8_t)bn;
-      int* sb = bins + bn * 7;
-      atomicAdd(sb, 1);
-      atomicMin(sb + 1, f2o(b.mn.x)); atomicMin(sb + 2, f2o(b.mn.y)); atomicMin(sb + 3, f2o(b.mn.z));
-      atomicMax(sb + 4, f2o(b.mx.x)); atomicMax(sb + 5, f2o(b.mx.y)); atomicMax(sb + 6, f2o(b.mx.z));
+    for (uint32_t l0 = 0; l0 < size; l0 += 64 * kSmallPer) {
+      Box b[kSmallPer];
+#pragma unroll
+      for (int u = 0; u < kSmallPer; ++u) {
+        const uint32_t l = l0 + lane + 64u * u;
+        if (l < size) b[u] = l0 == 0 ? keep[u] : rk::load_box(boxes + ids[l]);
+      }
+#pragma unroll
+      for (int u = 0; u < kSmallPer; ++u) {
+        const uint32_t l = l0 + lane + 64u * u;
+        if (l >= size) continue;
+        const int bn = bin_of(b[u], axis, scale, bmin0);
+        binof[l] = (uint8_t)bn;
+        int* sb = bins + bn * 7;
+        atomicAdd(sb, 1);
+        atomicMin(sb + 1, f2o(b[u].mn.x)); atomicMin(sb + 2, f2o(b[u].mn.y)); atomicMin(sb + 3, f2o(b[u].mn.z));
+        atomicMax(sb + 4, f2o(b[u].mx.x)); atomicMax(sb + 5, f2o(b[u].mx.y)); atomicMax(sb + 6, f2o(b[u].mx.z));
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_wave_barrier();
