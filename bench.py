@@ -122,7 +122,8 @@ def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
         "unit": "Mrays/s",
         "cores": int(threads),
         "kind": "port",
-        "sample": f"every {stride}th ray of the 1920x1080 frame ({rays} rays): reference LBVH (depth 12) and "
+        "sample": (f"every {stride}th ray of the 1920x1080 frame ({rays} rays)" if stride > 1 else
+                   f"all {rays} rays of the 1920x1080 frame") + ": reference LBVH (depth 12) and "
                   f"per-ray traversal (CUDA_raytraceMeshTree.cu:111-219) with the reference threshold; the "
                   f"reference build of all {F.shape[0]} faces took {build_s:.3f} s",
         "lbvh_mtri_s": round(F.shape[0] / build_s / 1e6, 4),

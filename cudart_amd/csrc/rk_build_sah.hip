@@ -897,8 +897,9 @@ extern "C" size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t exis
 // (re)computed and the root is the mesh hitbox; existing_nodes > 0 (Hybrid):
 // boxes/indices come from the LBVH pass and every leaf among nodes[0,
 // existing_nodes) is refined. Faces are reordered by the final indices.
-// Synchronises 'stream' once per tree level (the level's item count sizes the
-// next launch), so it returns after the build has finished.
+// Synchronises 'stream' once per tree level once a level can hold more than
+// min(4096, n/256) items (the level's counts size the next launches); the
+// info words and the face reorder are queued on 'stream' when it returns.
 extern "C" rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
                                   const rk_aabox* hitbox, rk_aabox* boxes, uint32_t* indices, rk_bvnode* nodes,
                                   uint32_t existing_nodes, uint32_t nodes_capacity, uint32_t* info, void* ws,

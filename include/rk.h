@@ -164,7 +164,9 @@ rk_status rk_build_lbvh(rk_stream stream, rk_uint4* faces, const rk_float4* vert
  * existing_nodes entries of 'nodes' are an LBVH built with tree_complete = 0 (boxes and
  * indices from that build) and each of its leaves is refined. Node ids are breadth-first
  * (the reference's are in atomicAdd order). nodes_capacity >= existing_nodes + 2n - 1.
- * Faces are reordered in place. Synchronises 'stream' once per tree level. */
+ * Faces are reordered in place. Synchronises 'stream' once per tree level, except on the first
+ * levels (while a level can hold at most min(4096, n/256) nodes); the info words and the face
+ * reorder are still queued on 'stream' when it returns. */
 size_t rk_build_sah_workspace_size(uint32_t face_count, uint32_t existing_nodes);
 rk_status rk_build_sah(rk_stream stream, rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
                        const rk_aabox* hitbox, rk_aabox* boxes, uint32_t* indices, rk_bvnode* nodes,
