@@ -515,6 +515,34 @@ def test_hybrid_bit_exact(dev, rk, ora, name, depth):
     _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
 
 
+def _degenerate_scene(rk, case):
+    """Faces that share one centroid, where the reference's sweep finds no
+    finite cost (a NaN for every split): alone, in two clusters, and mixed
+    into a soup. Sizes put such items on both sides of 1024 faces."""
+    V, F = rk.soup(4000, seed=7)
+    if case == "one300":
+        F = np.repeat(F[:1], 300, axis=0)
+    elif case == "one5000":
+        F = np.repeat(F[:1], 5000, axis=0)
+    elif case == "two_clusters":
+        F = np.concatenate([np.repeat(F[:1], 2000, axis=0), np.repeat(F[1:2], 3000, axis=0)])
+    elif case == "soup_plus_dups":
+        F = np.concatenate([F, np.repeat(F[5:6], 1500, axis=0), np.repeat(F[9:10], 230, axis=0)])
+    return V, np.ascontiguousarray(F)
+
+
+@pytest.mark.parametrize("case", ["one300", "one5000", "two_clusters", "soup_plus_dups"])
+def test_sah_degenerate_no_cut(dev, rk, ora, case):
+    V, F = _degenerate_scene(rk, case)
+    hb = rk.hitbox(V)
+    boxes, idx = ora.calc_boxes(F, V)
+    onodes, oidx, oflags = ora.build_sah(boxes, idx, hb)
+    assert oflags & rk.RK_FLAG_SAH_NO_CUT
+    Ft = _t(F)
+    tree = dev.build_sah(Ft, _t(V), hb)
+    _compare_sah(rk, ora, F, Ft, tree, onodes, oidx, oflags)
+
+
 def test_sah_builder_reuse(dev, rk, ora):
     """One SAHBuilder's buffers reused: a larger mesh, then a smaller one, then
     a Hybrid refinement; each tree equals the oracle's."""
