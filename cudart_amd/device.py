@@ -175,7 +175,7 @@ class FastBuilder:
         self.boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
         self.info = torch.zeros(N.RK_INFO_WORDS, dtype=_U32, device="cuda")
 
-    def build(self, F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, leaf_max: int = 4,
+    def build(self, F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, leaf_max: int = 1,
               stream=None) -> FastTree:
         n = F.shape[0]
         assert n <= self.n

@@ -260,13 +260,15 @@ def test_fast_tree_det_epsilon_zero(dev, rk, ora, mode):
     assert _fast_vs_bruteforce(dev, rk, ora, V, F, O, D, mode, 4, det_epsilon=0.0) > 0.05
 
 
-def test_fast_tree_full_size_heightfield(dev, rk, ora):
+@pytest.mark.parametrize("leaf_max", [1, 4])
+def test_fast_tree_full_size_heightfield(dev, rk, ora, leaf_max):
     """1M faces, 1920x1080 camera: every 37th ray of the frame against the
-    oracle's closest hit through its own depth-31 tree, det_epsilon 0."""
+    oracle's closest hit through its own depth-31 tree, det_epsilon 0. Leaf 1
+    is the benchmark's tree."""
     V, F = _scene(rk, "hf708")
     hb = rk.hitbox(V)
     O, D = _camera_rays(rk, "hf708", 1920, 1080)
-    tree = dev.FastBuilder(F.shape[0], 1).build(_t(F), _t(V), hb, 4)
+    tree = dev.FastBuilder(F.shape[0], 1).build(_t(F), _t(V), hb, leaf_max)
     fid, uv, dep = dev.fast_trace(tree, _t(O), _t(D), det_epsilon=0.0)
     sel = np.arange(0, O.shape[0], 37)
     Os, Ds = np.ascontiguousarray(O[sel]), np.ascontiguousarray(D[sel])
@@ -281,16 +283,18 @@ def test_fast_tree_full_size_heightfield(dev, rk, ora):
     np.testing.assert_array_equal(_h(uv)[sel][hit], ouv[hit])
 
 
-def test_fast_tree_soup_10m_63bit(dev, rk, ora):
+@pytest.mark.parametrize("leaf_max", [1, 4])
+def test_fast_tree_soup_10m_63bit(dev, rk, ora, leaf_max):
     """10M-face soup (lognormal edges around 0.002), 63-bit Morton keys sorted
     as uint64 pairs, 16.7M incoherent rays: the sort order equals the
     oracle's stable sort of its 21/21/21 codes, and every 4099th ray's
-    closest hit equals the oracle's through its own tree."""
+    closest hit equals the oracle's through its own tree. Leaf 1 is the
+    benchmark's tree."""
     n, nr = 10_000_000, 1 << 24
     V, F = rk.soup(n, seed=0xC3)
     hb = rk.hitbox(V)
     O, D = rk.random_rays(nr, seed=0xC3)
-    tree = dev.FastBuilder(n, rk.RK_MORTON_63).build(_t(F), _t(V), hb, 4)
+    tree = dev.FastBuilder(n, rk.RK_MORTON_63).build(_t(F), _t(V), hb, leaf_max)
     fid, uv, dep = dev.fast_trace(tree, _t(O), _t(D))
     # the same rays traced in origin-Morton order give the same results
     sf, suv, sdep = dev.fast_trace_sorted(tree, _t(O), _t(D))
