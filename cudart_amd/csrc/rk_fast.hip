@@ -769,6 +769,32 @@ __global__ __launch_bounds__(256) void k_ray_keys(const FNode* __restrict__ node
   vals[i] = i;
 }
 
+// Rays of one frame differ in cost by orders of magnitude: on a terrain seen
+// at a low angle, the rays that graze the horizon make hundreds of node
+// visits and the rays that leave the root box make one, and the grazing rays
+// fill a few rows. A wave lasts as long as its costliest rays; its idle lanes
+// help (trace_ray_shared), but a wave of 64 grazing rays has none idle. So a
+// wave does not take 64 consecutive rays: it takes runs of kSpreadRun
+// consecutive rays (a 128-B line of origins and one of directions) from
+// 64 / kSpreadRun bands of the ray array, the run at the same place in each
+// band. Runs of 8 measured best on the 1920x1080 heightfield frame (0.224 ms
+// against 0.257 ms for 64 consecutive rays; 0.276 ms for runs of 4, 0.231
+// for runs of 16). 0 traces the rays in array order. Sorted traces keep
+// their order.
+#ifndef RK_TRACE_SPREAD
+#define RK_TRACE_SPREAD 8
+#endif
+constexpr uint32_t kSpreadRun = RK_TRACE_SPREAD;
+static_assert(kSpreadRun == 0 || 64 % kSpreadRun == 0, "runs tile a wave");
+
+// The ray of thread t (n_rays when none).
+__device__ __forceinline__ uint32_t spread_ray(uint32_t t, uint32_t n_rays) {
+  if (kSpreadRun == 0) return t < n_rays ? t : n_rays;
+  const uint32_t waves = (n_rays + 63) / 64, w = t >> 6, q = t & 63;
+  const uint32_t r = ((q / kSpreadRun) * waves + w) * kSpreadRun + q % kSpreadRun;  // < 64 * waves
+  return (w < waves && r < n_rays) ? r : n_rays;
+}
+
 // order (may be NULL): thread t traces ray order[t], so rays that start near
 // each other run in the same wave; results still go to the ray's own slot.
 template <bool kCount>
@@ -780,7 +806,7 @@ __global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ no
                                                     unsigned long long* __restrict__ counters,
                                                     const uint32_t* __restrict__ order) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  uint32_t l = t < n_rays ? (order ? order[t] : t) : n_rays;
+  const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : spread_ray(t, n_rays);
   const bool has_ray = l < n_rays;
   __shared__ uint2 s_stack[kLdsStack * 256];
   __shared__ uint32_t s_xchg[4][3 * 64];
@@ -1299,6 +1325,15 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
 
 namespace {
 
+// Bands of the camera kernel: 16 bands of 4-pixel runs measured best on the
+// 1920x1080 heightfield frame (0.212 ms; 8 bands 0.216, 32 bands 0.218,
+// 64 bands of single pixels 0.231, and 8x8 tiles per wave 0.249).
+#ifndef RK_CAM_BANDS
+#define RK_CAM_BANDS 16
+#endif
+constexpr int kCamBands = RK_CAM_BANDS, kCamRun = 64 / RK_CAM_BANDS;
+static_assert(kCamBands >= 1 && 64 % kCamBands == 0, "runs tile a wave");
+
 // Primary ray of pixel (x, y) generated in the trace kernel itself (the
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
@@ -1309,12 +1344,16 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
                                                            int height, uint32_t* __restrict__ faces_id,
                                                            float2* __restrict__ coords, float* __restrict__ depths,
                                                            float eps) {
-  // each wave takes an 8x8 quadrant of the workgroup's 16x16 tile: rays of a
-  // wave stay close together, so they tend to visit the same nodes
+  // Wave g takes a row segment of kCamRun pixels in each of kCamBands
+  // horizontal bands of the image, at the same place in every band, for the
+  // reason given at spread_ray.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  const int y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-  const bool has_ray = x < width && y < height;
+  const int g = blockIdx.x * 4 + wave;
+  const int segs = (width + kCamRun - 1) / kCamRun, band_h = (height + kCamBands - 1) / kCamBands;
+  const int row_in_band = g / segs;
+  const int x = (g % segs) * kCamRun + lane % kCamRun;
+  const int y = (lane / kCamRun) * band_h + row_in_band;
+  const bool has_ray = row_in_band < band_h && x < width && y < height;
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
   __shared__ uint2 s_stack[kLdsStack * 256];
@@ -1366,12 +1405,13 @@ __global__ __launch_bounds__(256) void k_deform(const float4* __restrict__ base,
 // and LDS, and adds once per workgroup: same-address atomics from every wave
 // would serialise across XCDs.
 constexpr uint32_t kSumBlocks = 256;
+static_assert(kSumBlocks <= 256, "k_checksum_to_host's last workgroup reads one share per thread");
 
-__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
-                                                  const float2* __restrict__ coords,
-                                                  const float* __restrict__ depths, uint32_t count,
-                                                  const uint32_t* __restrict__ indices, uint32_t n_faces,
-                                                  unsigned long long* __restrict__ out) {
+// This workgroup's share of the frame checksum: (hash sum, hits), valid in thread 0.
+__device__ __forceinline__ void block_checksum(const uint32_t* __restrict__ faces_id,
+                                               const float2* __restrict__ coords, const float* __restrict__ depths,
+                                               uint32_t count, const uint32_t* __restrict__ indices,
+                                               uint32_t n_faces, uint64_t& bh, uint64_t& bn) {
   __shared__ uint64_t s_h[4], s_n[4];
   uint64_t h = 0, hit = 0;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
@@ -1391,9 +1431,68 @@ __global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ f
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) { s_h[w] = h; s_n[w] = hit; }
   __syncthreads();
+  bh = s_h[0] + s_h[1] + s_h[2] + s_h[3];
+  bn = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+}
+
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
+                                                  const float2* __restrict__ coords,
+                                                  const float* __restrict__ depths, uint32_t count,
+                                                  const uint32_t* __restrict__ indices, uint32_t n_faces,
+                                                  unsigned long long* __restrict__ out) {
+  uint64_t bh, bn;
+  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
   if (threadIdx.x == 0) {
-    atomicAdd(out, (unsigned long long)(s_h[0] + s_h[1] + s_h[2] + s_h[3]));
-    atomicAdd(out + 1, (unsigned long long)(s_n[0] + s_n[1] + s_n[2] + s_n[3]));
+    atomicAdd(out, (unsigned long long)bh);
+    atomicAdd(out + 1, (unsigned long long)bn);
+  }
+}
+
+// The same checksum in one launch, with no clearing before and no copy
+// after: each workgroup stores its share in partials (2 words each), and
+// the last one to count itself in adds the shares, writes the total to
+// out[0..1] (host memory mapped for the device), sets the counter back to
+// zero for the next launch, and then writes done_value to out[2]: the host
+// sees the frame finished without a runtime query.
+__global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __restrict__ faces_id,
+                                                          const float2* __restrict__ coords,
+                                                          const float* __restrict__ depths, uint32_t count,
+                                                          const uint32_t* __restrict__ indices, uint32_t n_faces,
+                                                          uint64_t* __restrict__ partials,
+                                                          uint32_t* __restrict__ counter, uint64_t* out,
+                                                          uint64_t done_value) {
+  __shared__ uint32_t s_last;
+  uint64_t bh, bn;
+  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = bh;
+    partials[2 * blockIdx.x + 1] = bn;
+    __threadfence();
+    s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;  // workgroup-uniform
+  __threadfence();
+  // thread b takes workgroup b's share (at most 256 workgroups)
+  uint64_t h = 0, n = 0;
+  if (threadIdx.x < gridDim.x) {
+    h = __hip_atomic_load(partials + 2 * threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    n = __hip_atomic_load(partials + 2 * threadIdx.x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    h += shfl_down64(h, d);
+    n += shfl_down64(n, d);
+  }
+  __shared__ uint64_t s_h[4], s_n[4];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { s_h[w] = h; s_n[w] = n; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = s_h[0] + s_h[1] + s_h[2] + s_h[3];
+    out[1] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+    *counter = 0u;
+    __threadfence_system();  // the total lands before the done word
+    __hip_atomic_store(out + 2, done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1406,7 +1505,9 @@ extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* b
   if (width <= 0 || height <= 0 || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
   auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
-  dim3 grid((unsigned)((width + 15) / 16), (unsigned)((height + 15) / 16));
+  // one wave per row segment of each band, four waves per workgroup
+  const uint64_t waves = (uint64_t)((height + kCamBands - 1) / kCamBands) * (uint64_t)((width + kCamRun - 1) / kCamRun);
+  dim3 grid((unsigned)((waves + 3) / 4));
   hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(256), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      f4(camera->origin), f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
@@ -1437,6 +1538,23 @@ extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_i
                      rk_hip_stream(stream), faces_id,
                      reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
                      reinterpret_cast<unsigned long long*>(out));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+// The frame checksum for the request loop: one launch, the total written to
+// out_host[0..1], a device-writable mapping of pinned coherent host memory,
+// then done_value to out_host[2]. partials holds 2 * kSumBlocks words and
+// counter one word that is zero between launches (zero it once before the
+// first).
+rk_status rk_frame_checksum_to_host(hipStream_t st, const uint32_t* faces_id, const rk_float2* coords,
+                                    const float* depths, uint32_t count, const uint32_t* indices,
+                                    uint32_t face_count, uint64_t* partials, uint32_t* counter, uint64_t* out_host,
+                                    uint64_t done_value) {
+  if (!faces_id || !coords || !depths || !partials || !counter || !out_host) return RK_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(k_checksum_to_host, dim3(std::max(1u, std::min(blocks_for(count, 256), kSumBlocks))), dim3(256),
+                     0, st, faces_id, reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
+                     partials, counter, out_host, done_value);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

@@ -26,9 +26,16 @@
 #include "rk_device.h"
 #include "rk_scenes.h"
 
+// rk_fast.hip
+rk_status rk_frame_checksum_to_host(hipStream_t st, const uint32_t* faces_id, const rk_float2* coords,
+                                    const float* depths, uint32_t count, const uint32_t* indices,
+                                    uint32_t face_count, uint64_t* partials, uint32_t* counter, uint64_t* out_host,
+                                    uint64_t done_value);
+
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr size_t kSumParts = 2 * 256;  // two words per checksum workgroup, at most 256 of them
 
 struct Scene {
   rk_float4* h_verts = nullptr;  // pinned host copies, the source of uploads
@@ -60,9 +67,12 @@ struct Slot {
   uint32_t* fid = nullptr;
   rk_float2* uv = nullptr;
   float* dep = nullptr;
-  uint64_t* sum_dev = nullptr;
-  uint64_t* sum_host = nullptr;  // pinned
+  uint64_t* sum_parts = nullptr;   // per-workgroup shares of the checksum, then a zeroed counter
+  uint64_t* sum_host = nullptr;    // pinned, coherent: checksum, hits, then ticket + 1 once the frame is done
+  uint64_t* sum_host_dev = nullptr;  // the device's address of sum_host
   int64_t ticket = -1;
+  bool recorded = false;  // 'done' marks the end of the running request
+  uint32_t polls = 0;
 };
 
 struct Pending {
@@ -180,8 +190,16 @@ struct rk_server_st {
     res.evictions = evicted;
     res.slot = (uint32_t)slot_index;
     rk_stream st = reinterpret_cast<rk_stream>(sl.st);
-    if (s.in_use > 0 && s.last_slot >= 0 && s.last_slot != slot_index)
-      RK_HIP(hipStreamWaitEvent(sl.st, slots[s.last_slot].done, 0));
+    if (s.in_use > 0 && s.last_slot >= 0 && s.last_slot != slot_index) {
+      // the scene's latest request still runs on another slot: mark its end
+      // there (that slot holds nothing else) and wait for it
+      Slot& prev = slots[s.last_slot];
+      if (!prev.recorded) {
+        RK_HIP(hipEventRecord(prev.done, prev.st));
+        prev.recorded = true;
+      }
+      RK_HIP(hipStreamWaitEvent(sl.st, prev.done, 0));
+    }
     if (!s.uploaded) {  // a scene made resident for this request starts from its host copy
       RK_HIP(hipMemcpyAsync(s.base, s.h_verts, 16ull * s.nv, hipMemcpyHostToDevice, sl.st));
       RK_HIP(hipMemcpyAsync(s.faces, s.h_faces, 16ull * s.nf, hipMemcpyHostToDevice, sl.st));
@@ -219,11 +237,13 @@ struct rk_server_st {
     if ((e = rk_fast_trace_camera(st, &bvh, &r.camera, (int)r.width, (int)r.height, fid, uv, dep, &opt)) !=
         RK_SUCCESS)
       return e;
-    if ((e = rk_frame_checksum(st, fid, uv, dep, r.width * r.height, s.indices, s.nf, sl.sum_dev)) != RK_SUCCESS)
+    if ((e = rk_frame_checksum_to_host(sl.st, fid, uv, dep, r.width * r.height, s.indices, s.nf, sl.sum_parts,
+                                       reinterpret_cast<uint32_t*>(sl.sum_parts + kSumParts), sl.sum_host_dev,
+                                       ticket + 1)) != RK_SUCCESS)
       return e;
-    RK_HIP(hipMemcpyAsync(sl.sum_host, sl.sum_dev, 16, hipMemcpyDeviceToHost, sl.st));
-    RK_HIP(hipEventRecord(sl.done, sl.st));
     sl.ticket = (int64_t)ticket;
+    sl.recorded = false;
+    sl.polls = 0;
     s.in_use++;
     s.last_slot = slot_index;
     s.last_use = ++tick;
@@ -234,9 +254,14 @@ struct rk_server_st {
     uint32_t done_now = 0;
     for (auto& sl : slots) {
       if (sl.ticket < 0) continue;
-      hipError_t q = hipEventQuery(sl.done);
-      if (q == hipErrorNotReady) continue;
-      if (q != hipSuccess) return RK_ERROR_DEVICE;
+      // the checksum kernel, the request's last, writes ticket + 1 last
+      if (__atomic_load_n(&sl.sum_host[2], __ATOMIC_ACQUIRE) != (uint64_t)sl.ticket + 1) {
+        if ((++sl.polls & 1023u) == 0) {  // now and then, ask the runtime whether the stream failed
+          hipError_t q = hipStreamQuery(sl.st);
+          if (q != hipSuccess && q != hipErrorNotReady) return RK_ERROR_DEVICE;
+        }
+        continue;
+      }
       rk_result& res = results[(size_t)sl.ticket];
       res.done = now();
       res.checksum = sl.sum_host[0];
@@ -311,10 +336,15 @@ extern "C" rk_status rk_server_create(const rk_server_config* cfg, uint64_t aren
     if (hipMalloc(reinterpret_cast<void**>(&sl.fid), 4ull * cfg->max_pixels) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&sl.uv), 8ull * cfg->max_pixels) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&sl.dep), 4ull * cfg->max_pixels) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&sl.sum_dev), 32) != hipSuccess)
+        hipMalloc(reinterpret_cast<void**>(&sl.sum_parts), 8 * (kSumParts + 1)) != hipSuccess)
       return fail(RK_ERROR_OUT_OF_MEMORY);
-    if (hipHostMalloc(reinterpret_cast<void**>(&sl.sum_host), 32, hipHostMallocDefault) != hipSuccess)
+    if (hipMemset(sl.sum_parts, 0, 8 * (kSumParts + 1)) != hipSuccess) return fail(RK_ERROR_DEVICE);
+    if (hipHostMalloc(reinterpret_cast<void**>(&sl.sum_host), 32, hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
       return fail(RK_ERROR_OUT_OF_MEMORY);
+    std::memset(sl.sum_host, 0, 32);
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.sum_host_dev), sl.sum_host, 0) != hipSuccess)
+      return fail(RK_ERROR_DEVICE);
   }
   *out = s;
   return RK_SUCCESS;
@@ -330,7 +360,7 @@ extern "C" rk_status rk_server_destroy(rk_server s) {
     (void)hipFree(sl.fid);
     (void)hipFree(sl.uv);
     (void)hipFree(sl.dep);
-    (void)hipFree(sl.sum_dev);
+    (void)hipFree(sl.sum_parts);
     if (sl.sum_host) (void)hipHostFree(sl.sum_host);
   }
   for (auto& sc : s->scenes) {
