@@ -952,9 +952,33 @@ extern "C" rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces
                            sorted_keys, ws, ws_bytes);
 }
 
+static rk_status fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                            int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh, const void* sorted_keys,
+                            void* ws, size_t ws_bytes, bool counters_clear);
+
 extern "C" rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
                                    uint32_t face_count, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
                                    const void* sorted_keys, void* ws, size_t ws_bytes) {
+  return fast_refit(stream, faces, vertices, face_count, morton_mode, leaf_max, bvh, sorted_keys, ws, ws_bytes,
+                    false);
+}
+
+// For the request loop: where a refit over n faces keeps its split counters
+// in ws (n words), and the refit that takes them as already zero.
+uint32_t* rk_fast_refit_counters(void* ws, size_t ws_bytes, uint32_t n) {
+  RkArena ar(ws, ws_bytes);
+  HierWs hw;
+  return take_hier_ws(ar, n, hw) ? hw.flags : nullptr;
+}
+rk_status rk_fast_refit_precleared(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                   uint32_t face_count, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
+                                   const void* sorted_keys, void* ws, size_t ws_bytes) {
+  return fast_refit(stream, faces, vertices, face_count, morton_mode, leaf_max, bvh, sorted_keys, ws, ws_bytes, true);
+}
+
+static rk_status fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
+                            int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh, const void* sorted_keys,
+                            void* ws, size_t ws_bytes, bool counters_clear) {
   const uint32_t n = face_count;
   if (n == 0 || !faces || !vertices || !bvh || !sorted_keys) return RK_ERROR_INVALID_VALUE;
   if (!bvh->nodes || !bvh->tris || !bvh->indices) return RK_ERROR_INVALID_VALUE;
@@ -971,9 +995,10 @@ extern "C" rk_status rk_fast_refit(rk_stream stream, const rk_uint4* faces, cons
   float4* tris = static_cast<float4*>(bvh->tris);
   if (morton_mode == RK_MORTON_63)
     return launch_hierarchy<uint64_t, false>(st, static_cast<const uint64_t*>(sorted_keys), bvh->indices, f, v, n,
-                                             leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr);
+                                             leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr,
+                                             counters_clear);
   return launch_hierarchy<uint32_t, false>(st, static_cast<const uint32_t*>(sorted_keys), bvh->indices, f, v, n,
-                                           leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr);
+                                           leaf_max, hw, nodes, tris, (const rk_batch_mesh*)nullptr, counters_clear);
 }
 
 
@@ -1334,6 +1359,154 @@ namespace {
 constexpr int kCamBands = RK_CAM_BANDS, kCamRun = 64 / RK_CAM_BANDS;
 static_assert(kCamBands >= 1 && 64 % kCamBands == 0, "runs tile a wave");
 
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float u01_of(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
+
+// v = base + amplitude * sin(omega * time + phase_v) * axis_v; phase and axis
+// hashed from the vertex index (axis uniform on the sphere, or +z).
+// Threads below clear_n also zero clear[i] (the request loop clears a
+// refit's split counters here instead of in a launch of its own).
+__global__ __launch_bounds__(256) void k_deform(const float4* __restrict__ base, float4* __restrict__ out,
+                                                uint32_t count, float amplitude, float omega, float time,
+                                                uint64_t seed, int along_z, uint32_t* __restrict__ clear,
+                                                uint32_t clear_n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < clear_n) clear[i] = 0u;
+  if (i >= count) return;
+  const uint64_t h0 = mix64(seed ^ (0x9E3779B97F4A7C15ull * ((uint64_t)i + 1)));
+  const uint64_t h1 = mix64(h0 + 0x632BE59BD9B4E019ull), h2 = mix64(h1 + 0x632BE59BD9B4E019ull);
+  const float phase = 6.2831853f * u01_of(h0);
+  float ax = 0.0f, ay = 0.0f, az = 1.0f;
+  if (!along_z) {
+    const float z = 2.0f * u01_of(h1) - 1.0f, phi = 6.2831853f * u01_of(h2);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    ax = r * cosf(phi);
+    ay = r * sinf(phi);
+    az = z;
+  }
+  const float s = amplitude * sinf(omega * time + phase);
+  const float4 b = base[i];
+  out[i] = make_float4(b.x + s * ax, b.y + s * ay, b.z + s * az, b.w);
+}
+
+// Order-independent frame checksum: the wrapping sum over rays of a hash of
+// (ray, original face id, depth, u, v bits), and the hit count. Original ids
+// make it independent of the tree the frame was traced with. A fixed grid of
+// at most kSumBlocks workgroups walks the frame, reduces in registers, waves
+// and LDS, and adds once per workgroup: same-address atomics from every wave
+// would serialise across XCDs.
+constexpr uint32_t kSumBlocks = 256;
+static_assert(kSumBlocks <= 256, "k_checksum_to_host's last workgroup reads one share per thread");
+
+// One ray's term of the frame checksum.
+__device__ __forceinline__ uint64_t ray_hash(uint32_t i, uint32_t orig, float depth, float2 c) {
+  return mix64(((uint64_t)i << 32 | orig) ^ mix64(((uint64_t)__float_as_uint(depth) << 32) ^
+                                                  ((uint64_t)__float_as_uint(c.x) * 0x100000001B3ull) ^
+                                                  __float_as_uint(c.y)));
+}
+
+// Wave and workgroup sums of (h, n), valid in thread 0 of the workgroup.
+__device__ __forceinline__ void block_sum2(uint64_t& h, uint64_t& n) {
+  __shared__ uint64_t s_h[4], s_n[4];
+  for (int d = 32; d >= 1; d >>= 1) {  // lane 0 ends with the wave's sums
+    h += shfl_down64(h, d);
+    n += shfl_down64(n, d);
+  }
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { s_h[w] = h; s_n[w] = n; }
+  __syncthreads();
+  h = s_h[0] + s_h[1] + s_h[2] + s_h[3];
+  n = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+}
+
+// This workgroup's share of the frame checksum: (hash sum, hits), valid in thread 0.
+__device__ __forceinline__ void block_checksum(const uint32_t* __restrict__ faces_id,
+                                               const float2* __restrict__ coords, const float* __restrict__ depths,
+                                               uint32_t count, const uint32_t* __restrict__ indices,
+                                               uint32_t n_faces, uint64_t& bh, uint64_t& bn) {
+  uint64_t h = 0, hit = 0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
+    const uint32_t f = faces_id[i];
+    const bool is_hit = f != 0xFFFFFFFFu && f < n_faces;
+    h += ray_hash(i, is_hit ? (indices ? indices[f] : f) : 0xFFFFFFFFu, depths[i], coords[i]);
+    hit += is_hit ? 1 : 0;
+  }
+  block_sum2(h, hit);
+  bh = h;
+  bn = hit;
+}
+
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
+                                                  const float2* __restrict__ coords,
+                                                  const float* __restrict__ depths, uint32_t count,
+                                                  const uint32_t* __restrict__ indices, uint32_t n_faces,
+                                                  unsigned long long* __restrict__ out) {
+  uint64_t bh, bn;
+  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
+  if (threadIdx.x == 0) {
+    atomicAdd(out, (unsigned long long)bh);
+    atomicAdd(out + 1, (unsigned long long)bn);
+  }
+}
+
+// The end of a one-launch checksum, called by every thread of every
+// workgroup with the workgroup's share in thread 0 (bh, bn): each workgroup
+// stores its share in partials (2 words each), and the last one to count
+// itself in adds the shares, writes the total to out[0..1] (host memory
+// mapped for the device), sets the counter back to zero for the next launch,
+// and then writes done_value to out[2]: the host sees the frame finished
+// without a runtime query.
+__device__ __forceinline__ void finish_checksum(uint64_t bh, uint64_t bn, uint64_t* __restrict__ partials,
+                                                uint32_t* __restrict__ counter, uint64_t* out, uint64_t done_value) {
+  __shared__ uint32_t s_last;
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = bh;
+    partials[2 * blockIdx.x + 1] = bn;
+    __threadfence();
+    s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;  // workgroup-uniform
+  __threadfence();
+  uint64_t h = 0, n = 0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    h += __hip_atomic_load(partials + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    n += __hip_atomic_load(partials + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    h += shfl_down64(h, d);
+    n += shfl_down64(n, d);
+  }
+  __shared__ uint64_t s_h[4], s_n[4];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { s_h[w] = h; s_n[w] = n; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = s_h[0] + s_h[1] + s_h[2] + s_h[3];
+    out[1] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+    *counter = 0u;
+    __threadfence_system();  // the total lands before the done word
+    __hip_atomic_store(out + 2, done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// The checksum of a frame in memory in one launch (finish_checksum).
+__global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __restrict__ faces_id,
+                                                          const float2* __restrict__ coords,
+                                                          const float* __restrict__ depths, uint32_t count,
+                                                          const uint32_t* __restrict__ indices, uint32_t n_faces,
+                                                          uint64_t* __restrict__ partials,
+                                                          uint32_t* __restrict__ counter, uint64_t* out,
+                                                          uint64_t done_value) {
+  uint64_t bh, bn;
+  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
+  finish_checksum(bh, bn, partials, counter, out, done_value);
+}
+
 // Primary ray of pixel (x, y) generated in the trace kernel itself (the
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
@@ -1368,135 +1541,9 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
   depths[p] = hit ? h.t : RK_HUGE;
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__device__ __forceinline__ float u01_of(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
-
-// v = base + amplitude * sin(omega * time + phase_v) * axis_v; phase and axis
-// hashed from the vertex index (axis uniform on the sphere, or +z).
-__global__ __launch_bounds__(256) void k_deform(const float4* __restrict__ base, float4* __restrict__ out,
-                                                uint32_t count, float amplitude, float omega, float time,
-                                                uint64_t seed, int along_z) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= count) return;
-  const uint64_t h0 = mix64(seed ^ (0x9E3779B97F4A7C15ull * ((uint64_t)i + 1)));
-  const uint64_t h1 = mix64(h0 + 0x632BE59BD9B4E019ull), h2 = mix64(h1 + 0x632BE59BD9B4E019ull);
-  const float phase = 6.2831853f * u01_of(h0);
-  float ax = 0.0f, ay = 0.0f, az = 1.0f;
-  if (!along_z) {
-    const float z = 2.0f * u01_of(h1) - 1.0f, phi = 6.2831853f * u01_of(h2);
-    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    ax = r * cosf(phi);
-    ay = r * sinf(phi);
-    az = z;
-  }
-  const float s = amplitude * sinf(omega * time + phase);
-  const float4 b = base[i];
-  out[i] = make_float4(b.x + s * ax, b.y + s * ay, b.z + s * az, b.w);
-}
-
-// Order-independent frame checksum: the wrapping sum over rays of a hash of
-// (ray, original face id, depth, u, v bits), and the hit count. Original ids
-// make it independent of the tree the frame was traced with. A fixed grid of
-// at most kSumBlocks workgroups walks the frame, reduces in registers, waves
-// and LDS, and adds once per workgroup: same-address atomics from every wave
-// would serialise across XCDs.
-constexpr uint32_t kSumBlocks = 256;
-static_assert(kSumBlocks <= 256, "k_checksum_to_host's last workgroup reads one share per thread");
-
-// This workgroup's share of the frame checksum: (hash sum, hits), valid in thread 0.
-__device__ __forceinline__ void block_checksum(const uint32_t* __restrict__ faces_id,
-                                               const float2* __restrict__ coords, const float* __restrict__ depths,
-                                               uint32_t count, const uint32_t* __restrict__ indices,
-                                               uint32_t n_faces, uint64_t& bh, uint64_t& bn) {
-  __shared__ uint64_t s_h[4], s_n[4];
-  uint64_t h = 0, hit = 0;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
-    const uint32_t f = faces_id[i];
-    const bool is_hit = f != 0xFFFFFFFFu && f < n_faces;
-    const uint32_t orig = is_hit ? (indices ? indices[f] : f) : 0xFFFFFFFFu;
-    const float2 c = coords[i];
-    h += mix64(((uint64_t)i << 32 | orig) ^ mix64(((uint64_t)__float_as_uint(depths[i]) << 32) ^
-                                                   ((uint64_t)__float_as_uint(c.x) * 0x100000001B3ull) ^
-                                                   __float_as_uint(c.y)));
-    hit += is_hit ? 1 : 0;
-  }
-  for (int d = 32; d >= 1; d >>= 1) {  // lane 0 ends with the wave's sums
-    h += shfl_down64(h, d);
-    hit += shfl_down64(hit, d);
-  }
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) { s_h[w] = h; s_n[w] = hit; }
-  __syncthreads();
-  bh = s_h[0] + s_h[1] + s_h[2] + s_h[3];
-  bn = s_n[0] + s_n[1] + s_n[2] + s_n[3];
-}
-
-__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ faces_id,
-                                                  const float2* __restrict__ coords,
-                                                  const float* __restrict__ depths, uint32_t count,
-                                                  const uint32_t* __restrict__ indices, uint32_t n_faces,
-                                                  unsigned long long* __restrict__ out) {
-  uint64_t bh, bn;
-  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
-  if (threadIdx.x == 0) {
-    atomicAdd(out, (unsigned long long)bh);
-    atomicAdd(out + 1, (unsigned long long)bn);
-  }
-}
-
-// The same checksum in one launch, with no clearing before and no copy
-// after: each workgroup stores its share in partials (2 words each), and
-// the last one to count itself in adds the shares, writes the total to
-// out[0..1] (host memory mapped for the device), sets the counter back to
-// zero for the next launch, and then writes done_value to out[2]: the host
-// sees the frame finished without a runtime query.
-__global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __restrict__ faces_id,
-                                                          const float2* __restrict__ coords,
-                                                          const float* __restrict__ depths, uint32_t count,
-                                                          const uint32_t* __restrict__ indices, uint32_t n_faces,
-                                                          uint64_t* __restrict__ partials,
-                                                          uint32_t* __restrict__ counter, uint64_t* out,
-                                                          uint64_t done_value) {
-  __shared__ uint32_t s_last;
-  uint64_t bh, bn;
-  block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = bh;
-    partials[2 * blockIdx.x + 1] = bn;
-    __threadfence();
-    s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;  // workgroup-uniform
-  __threadfence();
-  // thread b takes workgroup b's share (at most 256 workgroups)
-  uint64_t h = 0, n = 0;
-  if (threadIdx.x < gridDim.x) {
-    h = __hip_atomic_load(partials + 2 * threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    n = __hip_atomic_load(partials + 2 * threadIdx.x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  for (int d = 32; d >= 1; d >>= 1) {
-    h += shfl_down64(h, d);
-    n += shfl_down64(n, d);
-  }
-  __shared__ uint64_t s_h[4], s_n[4];
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) { s_h[w] = h; s_n[w] = n; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    out[0] = s_h[0] + s_h[1] + s_h[2] + s_h[3];
-    out[1] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
-    *counter = 0u;
-    __threadfence_system();  // the total lands before the done word
-    __hip_atomic_store(out + 2, done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
 }  // namespace
+
+uint32_t rk_camera_blocks(int width, int height);
 
 extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
                                           int width, int height, uint32_t* faces_id, rk_float2* coords, float* depths,
@@ -1506,8 +1553,7 @@ extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* b
   if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
   auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
   // one wave per row segment of each band, four waves per workgroup
-  const uint64_t waves = (uint64_t)((height + kCamBands - 1) / kCamBands) * (uint64_t)((width + kCamRun - 1) / kCamRun);
-  dim3 grid((unsigned)((waves + 3) / 4));
+  dim3 grid(rk_camera_blocks(width, height));
   hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(256), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      f4(camera->origin), f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
@@ -1522,9 +1568,30 @@ extern "C" rk_status rk_deform_vertices(rk_stream stream, const rk_float4* base,
   if (!base || !out) return RK_ERROR_INVALID_VALUE;
   hipLaunchKernelGGL(k_deform, dim3(blocks_for(count, 256)), dim3(256), 0, rk_hip_stream(stream),
                      reinterpret_cast<const float4*>(base), reinterpret_cast<float4*>(out), count, amplitude, omega,
-                     time, seed, along_z);
+                     time, seed, along_z, (uint32_t*)nullptr, 0u);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
+}
+
+// For the request loop (rk_serve.hip): rk_deform_vertices that also zeroes
+// clear[0, clear_n).
+rk_status rk_deform_and_clear(hipStream_t st, const rk_float4* base, rk_float4* out, uint32_t count, float amplitude,
+                              float omega, float time, uint64_t seed, int along_z, uint32_t* clear,
+                              uint32_t clear_n) {
+  if (!base || !out || (clear_n && !clear)) return RK_ERROR_INVALID_VALUE;
+  const uint32_t n = std::max(count, clear_n);
+  if (n == 0) return RK_SUCCESS;
+  hipLaunchKernelGGL(k_deform, dim3(blocks_for(n, 256)), dim3(256), 0, st, reinterpret_cast<const float4*>(base),
+                     reinterpret_cast<float4*>(out), count, amplitude, omega, time, seed, along_z, clear, clear_n);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+// Workgroups of k_fast_trace_camera for a width x height frame.
+uint32_t rk_camera_blocks(int width, int height) {
+  const uint64_t waves =
+      (uint64_t)((height + kCamBands - 1) / kCamBands) * (uint64_t)((width + kCamRun - 1) / kCamRun);
+  return (uint32_t)((waves + 3) / 4);
 }
 
 extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords,

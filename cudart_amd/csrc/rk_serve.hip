@@ -7,9 +7,13 @@
 // Admitted requests run concurrently, one per slot: the GPU overlaps their
 // kernels while the host keeps admitting. Per request, on its slot's stream:
 // upload the scene if it was not resident, deform its vertices for the
-// request's time, rebuild or refit its tree, trace the camera frame, and
-// checksum the frame; an event marks completion. Requests of one scene are
-// ordered through that scene's last event, since they share its buffers.
+// request's time (the same launch clears a refit's split counters), rebuild
+// or refit its tree, trace the camera frame, and checksum it; the checksum
+// launch's last workgroup writes the checksum and then the request's ticket
+// to pinned host memory, which is how the host sees the request finish (four
+// launches per refit request). Requests of one scene
+// are ordered, since they share its buffers: a request whose scene's latest
+// request still runs on another slot waits for an event recorded there.
 //
 // Nothing here allocates per request: the arena, the slot workspaces and the
 // pinned result words are allocated at creation or when a scene is added.
@@ -31,11 +35,19 @@ rk_status rk_frame_checksum_to_host(hipStream_t st, const uint32_t* faces_id, co
                                     const float* depths, uint32_t count, const uint32_t* indices,
                                     uint32_t face_count, uint64_t* partials, uint32_t* counter, uint64_t* out_host,
                                     uint64_t done_value);
+rk_status rk_deform_and_clear(hipStream_t st, const rk_float4* base, rk_float4* out, uint32_t count, float amplitude,
+                              float omega, float time, uint64_t seed, int along_z, uint32_t* clear,
+                              uint32_t clear_n);
+uint32_t* rk_fast_refit_counters(void* ws, size_t ws_bytes, uint32_t n);
+rk_status rk_fast_refit_precleared(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                   uint32_t face_count, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
+                                   const void* sorted_keys, void* ws, size_t ws_bytes);
 
 namespace {
 
 constexpr size_t kAlign = 256;
 constexpr size_t kSumParts = 2 * 256;  // two words per checksum workgroup, at most 256 of them
+
 
 struct Scene {
   rk_float4* h_verts = nullptr;  // pinned host copies, the source of uploads
@@ -70,6 +82,8 @@ struct Slot {
   uint64_t* sum_parts = nullptr;   // per-workgroup shares of the checksum, then a zeroed counter
   uint64_t* sum_host = nullptr;    // pinned, coherent: checksum, hits, then ticket + 1 once the frame is done
   uint64_t* sum_host_dev = nullptr;  // the device's address of sum_host
+  rk_float4 *rays_o = nullptr, *rays_d = nullptr;  // plain path: primary rays
+  uint64_t* sum_dev = nullptr;                      // plain path: checksum in device memory
   int64_t ticket = -1;
   bool recorded = false;  // 'done' marks the end of the running request
   uint32_t polls = 0;
@@ -183,6 +197,50 @@ struct rk_server_st {
     return true;
   }
 
+  void mark_running(Slot& sl, Scene& s, int slot_index, uint64_t ticket) {
+    sl.ticket = (int64_t)ticket;
+    sl.recorded = false;
+    sl.polls = 0;
+    s.in_use++;
+    s.last_slot = slot_index;
+    s.last_use = ++tick;
+  }
+
+  // The plain per-request path (rk_server_config.plain): what a caller of the
+  // plain entry points does per frame, in the same order of results.
+  rk_status launch_plain(Slot& sl, Scene& s, const rk_request& r, uint64_t ticket, rk_result& res,
+                         rk_fast_bvh& bvh) {
+    rk_stream st = reinterpret_cast<rk_stream>(sl.st);
+    rk_status e;
+    if ((e = rk_deform_vertices(st, s.base, s.verts, s.nv, s.amplitude, cfg.omega, r.time, 0x5EED0000ull + r.scene,
+                                s.along_z)) != RK_SUCCESS)
+      return e;
+    if ((e = rk_fast_build(st, s.faces, s.verts, s.nf, &s.hitbox, RK_MORTON_30, cfg.leaf_max, &bvh, s.info, sl.ws,
+                           sl.ws_bytes)) != RK_SUCCESS)
+      return e;
+    s.built = false;  // no keys kept: a later request of the request loop rebuilds
+    ++rebuilds;
+    res.rebuilt = 1;
+    const uint32_t n = r.width * r.height;
+    uint32_t* fid = r.faces_id ? r.faces_id : sl.fid;
+    rk_float2* uv = r.coords ? r.coords : sl.uv;
+    float* dep = r.depths ? r.depths : sl.dep;
+    // the frame as the camera kernel leaves misses: face ~0, coords 0, depth FLT_MAX
+    if ((e = rk_fill_u32(st, fid, n, 0xFFFFFFFFu)) != RK_SUCCESS || (e = rk_fill_f2(st, uv, n, {0.0f, 0.0f})) !=
+        RK_SUCCESS || (e = rk_fill_f32(st, dep, n, RK_HUGE)) != RK_SUCCESS)
+      return e;
+    if ((e = rk_calc_primary_rays(st, (int)r.width, (int)r.height, r.camera.origin, r.camera.direction,
+                                  r.camera.step_x, r.camera.step_y, sl.rays_o, sl.rays_d)) != RK_SUCCESS)
+      return e;
+    rk_trace_options opt = {cfg.det_epsilon, 0, {0, 0}};
+    if ((e = rk_fast_trace(st, &bvh, sl.rays_o, sl.rays_d, n, fid, uv, dep, &opt)) != RK_SUCCESS) return e;
+    if ((e = rk_frame_checksum(st, fid, uv, dep, n, s.indices, s.nf, sl.sum_dev)) != RK_SUCCESS) return e;
+    RK_HIP(hipMemcpyAsync(sl.sum_host, sl.sum_dev, 16, hipMemcpyDeviceToHost, sl.st));
+    RK_HIP(hipStreamSynchronize(sl.st));  // one request at a time, finished before the next
+    sl.sum_host[2] = ticket + 1;
+    return RK_SUCCESS;
+  }
+
   rk_status launch(Slot& sl, int slot_index, const rk_request& r, uint64_t ticket, uint32_t evicted) {
     Scene& s = scenes[r.scene];
     rk_result& res = results[ticket];
@@ -209,14 +267,23 @@ struct rk_server_st {
       ++uploads;
     }
     rk_status e;
-    if ((e = rk_deform_vertices(st, s.base, s.verts, s.nv, s.amplitude, cfg.omega, r.time, 0x5EED0000ull + r.scene,
-                                s.along_z)) != RK_SUCCESS)
-      return e;
     rk_fast_bvh bvh = {s.nodes, s.tris, s.indices, nullptr, s.nf, 0};
+    if (cfg.plain) {
+      if ((e = launch_plain(sl, s, r, ticket, res, bvh)) != RK_SUCCESS) return e;
+      mark_running(sl, s, slot_index, ticket);
+      return RK_SUCCESS;
+    }
     const bool force_rebuild = (r.flags & RK_REQUEST_FORCE_REBUILD) != 0;
     const bool force_refit = (r.flags & RK_REQUEST_FORCE_REFIT) != 0;
     const bool rebuild = !s.built || force_rebuild ||
                          (!force_refit && (cfg.rebuild_every == 0 || s.refits >= cfg.rebuild_every));
+    // A refit's split counters are cleared by the deform launch; a rebuild
+    // clears its own in its key kernel.
+    uint32_t* counters = rebuild ? nullptr : rk_fast_refit_counters(sl.ws, sl.ws_bytes, s.nf);
+    if (!rebuild && !counters) return RK_ERROR_WORKSPACE;
+    if ((e = rk_deform_and_clear(sl.st, s.base, s.verts, s.nv, s.amplitude, cfg.omega, r.time,
+                                 0x5EED0000ull + r.scene, s.along_z, counters, rebuild ? 0u : s.nf)) != RK_SUCCESS)
+      return e;
     if (rebuild) {
       e = rk_fast_build_keyed(st, s.faces, s.verts, s.nf, &s.hitbox, RK_MORTON_30, cfg.leaf_max, &bvh, s.info, s.keys,
                               sl.ws, sl.ws_bytes);
@@ -224,7 +291,8 @@ struct rk_server_st {
       s.refits = 0;
       ++rebuilds;
     } else {
-      e = rk_fast_refit(st, s.faces, s.verts, s.nf, RK_MORTON_30, cfg.leaf_max, &bvh, s.keys, sl.ws, sl.ws_bytes);
+      e = rk_fast_refit_precleared(st, s.faces, s.verts, s.nf, RK_MORTON_30, cfg.leaf_max, &bvh, s.keys, sl.ws,
+                                   sl.ws_bytes);
       ++s.refits;
       ++refits;
     }
@@ -241,12 +309,7 @@ struct rk_server_st {
                                        reinterpret_cast<uint32_t*>(sl.sum_parts + kSumParts), sl.sum_host_dev,
                                        ticket + 1)) != RK_SUCCESS)
       return e;
-    sl.ticket = (int64_t)ticket;
-    sl.recorded = false;
-    sl.polls = 0;
-    s.in_use++;
-    s.last_slot = slot_index;
-    s.last_use = ++tick;
+    mark_running(sl, s, slot_index, ticket);
     return RK_SUCCESS;
   }
 
@@ -335,16 +398,22 @@ extern "C" rk_status rk_server_create(const rk_server_config* cfg, uint64_t aren
     if (hipMalloc(&sl.ws, ws) != hipSuccess) return fail(RK_ERROR_OUT_OF_MEMORY);
     if (hipMalloc(reinterpret_cast<void**>(&sl.fid), 4ull * cfg->max_pixels) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&sl.uv), 8ull * cfg->max_pixels) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&sl.dep), 4ull * cfg->max_pixels) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&sl.sum_parts), 8 * (kSumParts + 1)) != hipSuccess)
+        hipMalloc(reinterpret_cast<void**>(&sl.dep), 4ull * cfg->max_pixels) != hipSuccess)
       return fail(RK_ERROR_OUT_OF_MEMORY);
-    if (hipMemset(sl.sum_parts, 0, 8 * (kSumParts + 1)) != hipSuccess) return fail(RK_ERROR_DEVICE);
+    const size_t parts_bytes = 8 * (kSumParts + 1);
+    if (hipMalloc(reinterpret_cast<void**>(&sl.sum_parts), parts_bytes) != hipSuccess)
+      return fail(RK_ERROR_OUT_OF_MEMORY);
+    if (hipMemset(sl.sum_parts, 0, parts_bytes) != hipSuccess) return fail(RK_ERROR_DEVICE);
     if (hipHostMalloc(reinterpret_cast<void**>(&sl.sum_host), 32, hipHostMallocMapped | hipHostMallocCoherent) !=
         hipSuccess)
       return fail(RK_ERROR_OUT_OF_MEMORY);
     std::memset(sl.sum_host, 0, 32);
     if (hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.sum_host_dev), sl.sum_host, 0) != hipSuccess)
       return fail(RK_ERROR_DEVICE);
+    if (cfg->plain && (hipMalloc(reinterpret_cast<void**>(&sl.rays_o), 16ull * cfg->max_pixels) != hipSuccess ||
+                       hipMalloc(reinterpret_cast<void**>(&sl.rays_d), 16ull * cfg->max_pixels) != hipSuccess ||
+                       hipMalloc(reinterpret_cast<void**>(&sl.sum_dev), 32) != hipSuccess))
+      return fail(RK_ERROR_OUT_OF_MEMORY);
   }
   *out = s;
   return RK_SUCCESS;
@@ -361,6 +430,9 @@ extern "C" rk_status rk_server_destroy(rk_server s) {
     (void)hipFree(sl.uv);
     (void)hipFree(sl.dep);
     (void)hipFree(sl.sum_parts);
+    (void)hipFree(sl.rays_o);
+    (void)hipFree(sl.rays_d);
+    (void)hipFree(sl.sum_dev);
     if (sl.sum_host) (void)hipHostFree(sl.sum_host);
   }
   for (auto& sc : s->scenes) {

@@ -80,7 +80,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("leaf_max", C.c_uint32), ("rebuild_every", C.c_uint32),
                 ("max_faces", C.c_uint32), ("max_pixels", C.c_uint32), ("det_epsilon", C.c_float),
                 ("amplitude_frac", C.c_float), ("omega", C.c_float), ("hbm_budget", C.c_uint64),
-                ("reserved", C.c_uint32 * 4)]
+                ("plain", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Request(C.Structure):

@@ -389,11 +389,16 @@ typedef struct rk_server_config {
   float amplitude_frac;    /* deformation amplitude / scene extent */
   float omega;             /* deformation angular rate, radians per unit of request time */
   uint64_t hbm_budget;     /* cap on arena_bytes */
-  uint32_t reserved[4];
+  uint32_t plain;          /* != 0: the plain per-request path, a baseline (below) */
+  uint32_t reserved[3];
 } rk_server_config;
 /* 16 slots, leaf 1, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
- * amplitude 1% of extent, omega 2 pi, budget 200 GiB. Slots are streams: they
- * overlap only on distinct hardware queues (GPU_MAX_HW_QUEUES, 4 by default). */
+ * amplitude 1% of extent, omega 2 pi, budget 200 GiB, plain 0. Slots are streams: they
+ * overlap only on distinct hardware queues (GPU_MAX_HW_QUEUES, 4 by default).
+ * plain != 0 runs every request the way a caller of the plain entry points would,
+ * as a baseline for the request loop: deform, rk_fast_build from scratch, clear the
+ * frame, rk_calc_primary_rays into ray arrays, rk_fast_trace, rk_frame_checksum and a
+ * copy of it to the host. Frames and checksums equal the request loop's. */
 rk_server_config rk_server_config_default(void);
 typedef struct rk_server_st* rk_server;
 rk_status rk_server_create(const rk_server_config* cfg, uint64_t arena_bytes, rk_server* server);

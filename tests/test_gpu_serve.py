@@ -203,6 +203,36 @@ def test_served_frames_same_alone_and_in_mix(dev, rk):
     alone.close()
 
 
+def test_plain_path_same_frames(dev, rk):
+    """The plain per-request path (rebuild from scratch, ray arrays, separate
+    checksum, one request at a time) gives every request of a concurrent mix
+    the same checksum, frame by frame."""
+    scenes = _scenes(rk, 5, 53)
+    sizes = [rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in scenes]
+    mix = dev.Server(2 * sum(sizes), slots=4, rebuild_every=3, det_epsilon=0.0)
+    plain = dev.Server(2 * sum(sizes), slots=1, det_epsilon=0.0, plain=1)
+    for V, F, z in scenes:
+        mix.add_scene(V, F, z)
+        plain.add_scene(V, F, z)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(30):
+        s = int(rng.integers(0, len(scenes)))
+        side = int(rng.integers(16, 96))
+        reqs.append((s, float(rng.uniform(0, 2)), _camera(rk, scenes[s][0], side, az=float(rng.uniform(0, 6.3))),
+                     side))
+    tickets = [mix.submit(s, t, cam, side, side) for s, t, cam, side in reqs]
+    mix.wait_all()
+    ptickets = [plain.submit(s, t, cam, side, side) for s, t, cam, side in reqs]
+    plain.wait_all()
+    for k, (a, b) in enumerate(zip(tickets, ptickets)):
+        ra, rb = mix.result(a), plain.result(b)
+        assert rb.rebuilt == 1
+        assert (ra.checksum, ra.hits) == (rb.checksum, rb.hits), k
+    mix.close()
+    plain.close()
+
+
 def test_poisson_driver_statistics(dev, rk):
     scenes = _scenes(rk, 6, 41)
     sizes = [rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in scenes]
