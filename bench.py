@@ -58,8 +58,9 @@ OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 0.0
 # HBM bytes per launch of the headline k_fast_trace, from two rocprofv3 --pmc
 # runs of `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
-# FETCH_SIZE 146,000 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,634 KiB.
-TRACE_TRAFFIC_BYTES = (146_000 * 2 + 3_634) * 1024
+# FETCH_SIZE 123,207 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 4,115 KiB
+# (means of the three det_epsilon 0 frames).
+TRACE_TRAFFIC_BYTES = (123_207 * 2 + 4_115) * 1024
 
 
 def log(*a):
