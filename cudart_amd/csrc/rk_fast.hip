@@ -797,22 +797,33 @@ __device__ __forceinline__ uint32_t spread_ray(uint32_t t, uint32_t n_rays) {
 
 // order (may be NULL): thread t traces ray order[t], so rays that start near
 // each other run in the same wave; results still go to the ray's own slot.
+// Threads per workgroup of k_fast_trace and k_fast_trace_camera. One wave per
+// workgroup measured best on the 1920x1080 heightfield frame (rays 0.211 ms,
+// camera 0.199 ms; 128: 0.218 / 0.200; 256: 0.228 / 0.209; 512: 0.251 /
+// 0.230) and within 1% on the 10M soup's incoherent rays: a finished wave
+// frees its slot at once, where a larger workgroup waits for its slowest wave.
+#ifndef RK_TRACE_WG
+#define RK_TRACE_WG 64
+#endif
+constexpr int kTraceWG = RK_TRACE_WG;
+static_assert(kTraceWG % 64 == 0 && kTraceWG <= 1024, "whole waves");
+
 template <bool kCount>
-__global__ __launch_bounds__(256) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+__global__ __launch_bounds__(kTraceWG) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                     uint32_t n_tris, const float4* __restrict__ origins,
                                                     const float4* __restrict__ dirs, uint32_t n_rays,
                                                     uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                     float* __restrict__ depths, float eps,
                                                     unsigned long long* __restrict__ counters,
                                                     const uint32_t* __restrict__ order) {
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t t = blockIdx.x * kTraceWG + threadIdx.x;
   const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : spread_ray(t, n_rays);
   const bool has_ray = l < n_rays;
-  __shared__ uint2 s_stack[kLdsStack * 256];
-  __shared__ uint32_t s_xchg[4][3 * 64];
+  __shared__ uint2 s_stack[kLdsStack * kTraceWG];
+  __shared__ uint32_t s_xchg[kTraceWG / 64][3 * 64];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const RayHit h = trace_ray_shared<kCount, kStack, kLdsStack, 256>(
+  const RayHit h = trace_ray_shared<kCount, kStack, kLdsStack, kTraceWG>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
       has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
@@ -1321,7 +1332,7 @@ extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, con
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
@@ -1337,7 +1348,7 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !counters) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_fast_trace<true>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL(k_fast_trace<true>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
@@ -1511,7 +1522,7 @@ __global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __rest
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
 // a miss gives face ~0, coords (0, 0), depth FLT_MAX.
-__global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restrict__ nodes,
+__global__ __launch_bounds__(kTraceWG) void k_fast_trace_camera(const FNode* __restrict__ nodes,
                                                            const float4* __restrict__ tris, uint32_t n_tris,
                                                            float4 ro, float4 rd, float4 rx, float4 ry, int width,
                                                            int height, uint32_t* __restrict__ faces_id,
@@ -1521,7 +1532,7 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
   // horizontal bands of the image, at the same place in every band, for the
   // reason given at spread_ray.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = blockIdx.x * 4 + wave;
+  const int g = blockIdx.x * (kTraceWG / 64) + wave;
   const int segs = (width + kCamRun - 1) / kCamRun, band_h = (height + kCamBands - 1) / kCamBands;
   const int row_in_band = g / segs;
   const int x = (g % segs) * kCamRun + lane % kCamRun;
@@ -1529,9 +1540,9 @@ __global__ __launch_bounds__(256) void k_fast_trace_camera(const FNode* __restri
   const bool has_ray = row_in_band < band_h && x < width && y < height;
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
-  __shared__ uint2 s_stack[kLdsStack * 256];
-  __shared__ uint32_t s_xchg[4][3 * 64];
-  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, 256>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
+  __shared__ uint2 s_stack[kLdsStack * kTraceWG];
+  __shared__ uint32_t s_xchg[kTraceWG / 64][3 * 64];
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kTraceWG>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
                                                                    eps, nb, nt, s_stack, s_xchg[wave]);
   if (!has_ray) return;
   const size_t p = (size_t)y * (size_t)width + (size_t)x;
@@ -1552,9 +1563,9 @@ extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* b
   if (width <= 0 || height <= 0 || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
   auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
-  // one wave per row segment of each band, four waves per workgroup
+  // one wave per row segment of each band
   dim3 grid(rk_camera_blocks(width, height));
-  hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(256), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(kTraceWG), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      f4(camera->origin), f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
@@ -1591,7 +1602,7 @@ rk_status rk_deform_and_clear(hipStream_t st, const rk_float4* base, rk_float4* 
 uint32_t rk_camera_blocks(int width, int height) {
   const uint64_t waves =
       (uint64_t)((height + kCamBands - 1) / kCamBands) * (uint64_t)((width + kCamRun - 1) / kCamRun);
-  return (uint32_t)((waves + 3) / 4);
+  return (uint32_t)((waves + kTraceWG / 64 - 1) / (kTraceWG / 64));
 }
 
 extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords,
@@ -1651,7 +1662,7 @@ extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* b
   RK_CHECK_LAUNCH();
   rk_status s = rk_radix_sort_impl<uint32_t>(st, keys, order, n_rays, 0, 4, sort_ws, sort_bytes, nullptr);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, nodes,
+  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, st, nodes,
                      static_cast<const float4*>(bvh->tris), bvh->face_count, o,
                      reinterpret_cast<const float4*>(directions), n_rays, faces_id, reinterpret_cast<float2*>(coords),
                      depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order);
