@@ -371,12 +371,13 @@ rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* meshes, uint3
  * workspace and frame buffers) is free and its scene fits; admitted requests run
  * concurrently. Per request: upload if just made resident, deform the scene's
  * vertices to the request's time (amplitude cfg.amplitude_frac of the scene's
- * extent, rk_deform_vertices), rebuild (rk_fast_build_keyed) when the scene has
+ * extent, as rk_deform_vertices), rebuild (rk_fast_build_keyed) when the scene has
  * no tree or has been refitted cfg.rebuild_every times since its last build (0:
- * always), else refit (rk_fast_refit), trace the camera frame
- * (rk_fast_trace_camera) and checksum it (rk_frame_checksum). A frame depends
- * only on the scene, time, camera and size, not on what else runs or on refit vs
- * rebuild. The host calls are not thread-safe: one thread drives a server. */
+ * always), else refit (as rk_fast_refit), trace the camera frame
+ * (rk_fast_trace_camera) and checksum it (as rk_frame_checksum, the result written
+ * to pinned host memory with the request's completion). A frame depends only on
+ * the scene, time, camera and size, not on what else runs or on refit vs rebuild.
+ * The host calls are not thread-safe: one thread drives a server. */
 #define RK_REQUEST_FORCE_REBUILD 1u
 #define RK_REQUEST_FORCE_REFIT 2u  /* when the scene has a tree */
 typedef struct rk_server_config {
