@@ -80,6 +80,19 @@ __device__ __forceinline__ void atomic_store_relaxed(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Ray of thread t when each wave takes runs of kRun consecutive rays from
+// 64 / kRun equal bands of an array of n_rays, the run at the same place in
+// each band (n_rays when thread t has none; kRun 0: ray t). A bijection of
+// [0, n_rays) onto the threads below 64 * ceil(n_rays / 64).
+template <uint32_t kRun>
+__device__ __forceinline__ uint32_t band_ray(uint32_t t, uint32_t n_rays) {
+  static_assert(kRun == 0 || 64 % kRun == 0, "runs tile a wave");
+  if (kRun == 0) return t < n_rays ? t : n_rays;
+  const uint32_t waves = (n_rays + 63) / 64, w = t >> 6, q = t & 63;
+  const uint32_t r = ((q / kRun) * waves + w) * kRun + q % kRun;  // < 64 * waves
+  return (w < waves && r < n_rays) ? r : n_rays;
+}
+
 }  // namespace rk
 
 // Host-side launch check: every entry point returns RK_ERROR_LAUNCH when a

@@ -785,14 +785,10 @@ __global__ __launch_bounds__(256) void k_ray_keys(const FNode* __restrict__ node
 #define RK_TRACE_SPREAD 8
 #endif
 constexpr uint32_t kSpreadRun = RK_TRACE_SPREAD;
-static_assert(kSpreadRun == 0 || 64 % kSpreadRun == 0, "runs tile a wave");
 
 // The ray of thread t (n_rays when none).
 __device__ __forceinline__ uint32_t spread_ray(uint32_t t, uint32_t n_rays) {
-  if (kSpreadRun == 0) return t < n_rays ? t : n_rays;
-  const uint32_t waves = (n_rays + 63) / 64, w = t >> 6, q = t & 63;
-  const uint32_t r = ((q / kSpreadRun) * waves + w) * kSpreadRun + q % kSpreadRun;  // < 64 * waves
-  return (w < waves && r < n_rays) ? r : n_rays;
+  return rk::band_ray<kSpreadRun>(t, n_rays);
 }
 
 // order (may be NULL): thread t traces ray order[t], so rays that start near
