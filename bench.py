@@ -23,7 +23,9 @@ Further legs, each skippable:
 - batch: 4096 soups built and traced together (autotuned, one captured graph
   per step) against one rk_fast_build + rk_fast_trace per mesh.
 - serve: the rk_server request loop over 32 deforming scenes, Poisson
-  arrivals; p50/p99 latency and saturated throughput against one slot.
+  arrivals; p50/p99 latency, and saturated throughput against the same loop
+  on one slot and against the plain per-request path (rebuild, ray arrays,
+  trace, checksum, one request at a time).
 - cpu_baseline: the oracle, i.e. the reference algorithm (its LBVH build and
   per-ray traversal, reference threshold) on the host's cores, on a bounded
   sample of the frame.
