@@ -565,13 +565,18 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   static_assert(kStackN > kLds, "scratch part of the stack");
   uint32_t stack_ref[kStackN - kLds];
   float stack_t[kStackN - kLds];
-  uint2* my_lds = lds + threadIdx.x;
+  // An LDS-typed pointer (an entry is ref | t << 32, the layout of a uint2):
+  // with a generic one the compiler merges the two branches of st_get into one
+  // flat load through a selected address, which costs a flat access (and a
+  // wait on every memory counter) per pop.
+  typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+  lds_u64* my_lds = (lds_u64*)(lds + threadIdx.x);
   int sp = 0;
   auto st_get = [&](int i, uint32_t& r, float& t) {
     if (i < kLds) {
-      const uint2 e = my_lds[i * kWG];
-      r = e.x;
-      t = __uint_as_float(e.y);
+      const unsigned long long e = my_lds[i * kWG];
+      r = (uint32_t)e;
+      t = __uint_as_float((uint32_t)(e >> 32));
     } else {
       r = stack_ref[i - kLds];
       t = stack_t[i - kLds];
@@ -579,7 +584,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   };
   auto st_set = [&](int i, uint32_t r, float t) {
     if (i < kLds) {
-      my_lds[i * kWG] = make_uint2(r, __float_as_uint(t));
+      my_lds[i * kWG] = (unsigned long long)r | ((unsigned long long)__float_as_uint(t) << 32);
     } else {
       stack_ref[i - kLds] = r;
       stack_t[i - kLds] = t;
