@@ -537,13 +537,18 @@ struct RayHit {
 // hit is merged into its owner's when the subtree is done, by (t, original
 // id). A frame then waits less on its few long rays (grazing rays near a
 // horizon), which would otherwise keep a wave running with one active lane.
-// Every lane of the wave must call it; xchg is this wave's 3 x 64 words of LDS.
+// Every lane of the wave must call it; xchg is this wave's kXchgWords words of
+// LDS.
 //
 // The stack: its first kLds entries live in LDS (lds: this workgroup's
 // kLds x kWG entries, entry e of thread x at e * kWG + x), the rest in
 // per-lane scratch. Pending entries rarely go deeper than a dozen or two, and
 // an LDS access takes a fraction of the time of a scratch access that misses
 // the L1 (the scratch of thousands of resident lanes does not fit in it).
+// LDS words per wave for the exchange of pending subtrees: reference, entry
+// distance, owner lane and donor lane of each of up to 64 donations.
+constexpr int kXchgWords = 4 * 64;
+
 template <bool kCount, int kStackN, int kLds, int kWG>
 __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                    uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,
@@ -610,33 +615,39 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
         __hip_atomic_store(xchg + rd, r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         __hip_atomic_store(xchg + 64 + rd, __float_as_uint(t0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         __hip_atomic_store(xchg + 128 + rd, (uint32_t)owner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(xchg + 192 + rd, lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       __builtin_amdgcn_wave_barrier();
       const bool take = !working && ri < nd;
-      uint32_t g_ref = 0, g_t = 0, g_own = lane;
+      uint32_t g_ref = 0, g_t = 0, g_own = lane, g_don = lane;
       if (take) {
         g_ref = __hip_atomic_load(xchg + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         g_t = __hip_atomic_load(xchg + 64 + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         g_own = __hip_atomic_load(xchg + 128 + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        g_don = __hip_atomic_load(xchg + 192 + ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
       __builtin_amdgcn_wave_barrier();
-      // every lane takes part in the exchange of the owner's ray
-      const int src = (int)g_own;
-      const float4 no = make_float4(__shfl(orig.x, src, 64), __shfl(orig.y, src, 64), __shfl(orig.z, src, 64),
-                                    __shfl(orig.w, src, 64));
-      const float4 ndr = make_float4(__shfl(dir.x, src, 64), __shfl(dir.y, src, 64), __shfl(dir.z, src, 64),
-                                     __shfl(dir.w, src, 64));
+      // Every lane takes part in the exchange. The donor's job is a subtree of
+      // the owner's ray, so the donor holds that ray and its slab terms (the
+      // same bits the owner computed): copying them saves three divisions.
+      const int src = (int)g_own, don = (int)g_don;
+      const float4 no = make_float4(__shfl(jo.x, don, 64), __shfl(jo.y, don, 64), __shfl(jo.z, don, 64),
+                                    __shfl(jo.w, don, 64));
+      const float4 ndr = make_float4(__shfl(jd.x, don, 64), __shfl(jd.y, don, 64), __shfl(jd.z, don, 64),
+                                     __shfl(jd.w, don, 64));
+      const float nix = __shfl(ix, don, 64), niy = __shfl(iy, don, 64), niz = __shfl(iz, don, 64);
+      const float nox = __shfl(ox, don, 64), noy = __shfl(oy, don, 64), noz = __shfl(oz, don, 64);
       const float od = __shfl(o_t, src, 64);
       if (take && __uint_as_float(g_t) <= od) {
         owner = src;
         jo = no;
         jd = ndr;
-        ix = safe_inv(jd.x);
-        iy = safe_inv(jd.y);
-        iz = safe_inv(jd.z);
-        ox = jo.x * ix;
-        oy = jo.y * iy;
-        oz = jo.z * iz;
+        ix = nix;
+        iy = niy;
+        iz = niz;
+        ox = nox;
+        oy = noy;
+        oz = noz;
         cur = g_ref;
         sp = 0;
         visits = 0;
@@ -821,7 +832,7 @@ __global__ __launch_bounds__(kTraceWG) void k_fast_trace(const FNode* __restrict
   const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : spread_ray(t, n_rays);
   const bool has_ray = l < n_rays;
   __shared__ uint2 s_stack[kLdsStack * kTraceWG];
-  __shared__ uint32_t s_xchg[kTraceWG / 64][3 * 64];
+  __shared__ uint32_t s_xchg[kTraceWG / 64][kXchgWords];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const RayHit h = trace_ray_shared<kCount, kStack, kLdsStack, kTraceWG>(
@@ -1128,7 +1139,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
   const uint32_t t = blockIdx.x * kWG + threadIdx.x;
   const uint32_t r = t < n_rays ? (order ? order[t] : t) : n_rays;
   __shared__ uint2 s_stack[kLdsStack * kWG];
-  __shared__ uint32_t s_xchg[kWG / 64][3 * 64];
+  __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
                                 depths, eps, s_stack, s_xchg[threadIdx.x >> 6]);
 }
@@ -1150,7 +1161,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
                                                            float eps, uint32_t* __restrict__ queues,
                                                            const uint32_t* __restrict__ order) {
   __shared__ uint2 s_stack[kLdsStack * kWG];
-  __shared__ uint32_t s_xchg[kWG / 64][3 * 64];
+  __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t n_tiles = (n_rays + 63) / 64;
   const uint32_t per = (n_tiles + kQueues - 1) / kQueues;
@@ -1542,7 +1553,7 @@ __global__ __launch_bounds__(kTraceWG) void k_fast_trace_camera(const FNode* __r
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
   __shared__ uint2 s_stack[kLdsStack * kTraceWG];
-  __shared__ uint32_t s_xchg[kTraceWG / 64][3 * 64];
+  __shared__ uint32_t s_xchg[kTraceWG / 64][kXchgWords];
   const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kTraceWG>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
                                                                    eps, nb, nt, s_stack, s_xchg[wave]);
   if (!has_ray) return;
