@@ -60,9 +60,9 @@ OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 0.0
 # HBM bytes per launch of the headline k_fast_trace, from two rocprofv3 --pmc
 # runs of `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
-# FETCH_SIZE 115,231 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,512 KiB
+# FETCH_SIZE 117,868 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,512 KiB
 # (means of the three det_epsilon 0 frames).
-TRACE_TRAFFIC_BYTES = (115_231 * 2 + 3_512) * 1024
+TRACE_TRAFFIC_BYTES = (117_868 * 2 + 3_512) * 1024
 
 
 def log(*a):
