@@ -414,7 +414,7 @@ def main():
     trace_ms = max(frame_ms - fill_ms, 1e-6)
     ref_eps_frame_ms = timer.time(lambda: frame(native.DET_EPSILON_REFERENCE), args.steps)
     ref_eps_hits = int((dep < f32max).sum().item())
-    # the same frame with the rays formed inside the trace kernel (4-pixel runs from 16 bands per wave)
+    # the same frame with the rays formed inside the trace kernel (8-pixel runs from 8 bands per wave)
     cout = dev.new_outputs(nr)
     cam_frame = lambda: dev.fast_trace_camera(tree, cam, W, H, out=cout, det_epsilon=HEADLINE_DET_EPSILON)
     for _ in range(args.warmup):

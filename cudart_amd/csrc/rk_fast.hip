@@ -1373,11 +1373,13 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
 
 namespace {
 
-// Bands of the camera kernel: 16 bands of 4-pixel runs measured best on the
-// 1920x1080 heightfield frame (0.212 ms; 8 bands 0.216, 32 bands 0.218,
-// 64 bands of single pixels 0.231, and 8x8 tiles per wave 0.249).
+// Bands of the camera kernel: 8 bands of 8-pixel runs measured best on the
+// 1920x1080 heightfield frame with the 64-VGPR traversal (0.181 ms; 16 bands
+// of 4-pixel runs 0.184, 4 bands 0.192). At 79 VGPRs 16 bands had been best
+// (0.212 ms; 8 bands 0.216, 32 bands 0.218, 64 bands of single pixels 0.231,
+// and 8x8 tiles per wave 0.249).
 #ifndef RK_CAM_BANDS
-#define RK_CAM_BANDS 16
+#define RK_CAM_BANDS 8
 #endif
 constexpr int kCamBands = RK_CAM_BANDS, kCamRun = 64 / RK_CAM_BANDS;
 static_assert(kCamBands >= 1 && 64 % kCamBands == 0, "runs tile a wave");
