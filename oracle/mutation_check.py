@@ -1,0 +1,61 @@
+"""Mutation check of the oracle (test infrastructure).
+
+Applies each one-line mutation below to a scratch copy of oracle.cpp, rebuilds
+the oracle there and runs the CPU test suite against it. Every mutation must
+make at least one test fail; a mutation that survives names a rule of the
+reference that no known answer pins.
+
+    python oracle/mutation_check.py [name ...]
+"""
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+REPO = Path(__file__).resolve().parent.parent
+MUT = [
+    ("scale 31->32", "scale = 31.0f / ext.x", "scale = 32.0f / ext.x"),
+    ("sweep < -> <=", "if (T < T_min) { T_min = T; m_min = m - 1; }", "if (T <= T_min) { T_min = T; m_min = m - 1; }"),
+    ("axis x>y -> >=", "if (ext.x > ext.y) axis", "if (ext.x >= ext.y) axis"),
+    ("axis x>z -> >=", "axis = (ext.x > ext.z) ? 0 : 2;", "axis = (ext.x >= ext.z) ? 0 : 2;"),
+    ("axis y>z -> >=", "else axis = (ext.y > ext.z) ? 1 : 2;", "else axis = (ext.y >= ext.z) ? 1 : 2;"),
+    ("leaf size<224 -> <=", "if (size < 224 && T_min", "if (size <= 224 && T_min"),
+    ("leaf (size+1) -> size", "T_min > (float)(size + 1) * 0.125f", "T_min > (float)(size) * 0.125f"),
+    ("leaf T> -> >=", "T_min > (float)(size + 1) * 0.125f", "T_min >= (float)(size + 1) * 0.125f"),
+    ("box cost 2 -> 1", "float T = 1.0f * 2.f", "float T = 1.0f * 1.f"),
+    ("min leaf <=16 -> <16", "if (size <= 16) { leaf_node(); continue; }", "if (size < 16) { leaf_node(); continue; }"),
+    ("partition <= -> <", "if (bin_of(boxes[id]) <= m_min)", "if (bin_of(boxes[id]) < m_min)"),
+    ("hybrid root box", "queue.push_back({b, k, nd.w1", "queue.push_back({*root, k, nd.w1"),
+    ("hybrid ids", "    count = n_existing;\n", "    count = n_existing + 2;\n"),
+    ("refit left >0 -> >=", "if (l.bmax[2] - l.bmin[2] > 0.0f) {", "if (l.bmax[2] - l.bmin[2] >= 0.0f) {"),
+    ("refit right >0 -> >=", "if (r.bmax[2] - r.bmin[2] > 0.0f) {", "if (r.bmax[2] - r.bmin[2] >= 0.0f) {"),
+    ("tri u>1 -> >=", "if (u < 0.0f || u > 1.0f) return false;", "if (u < 0.0f || u >= 1.0f) return false;"),
+    ("tri u<0 -> <=", "if (u < 0.0f || u > 1.0f) return false;", "if (u <= 0.0f || u > 1.0f) return false;"),
+    ("tri v<0 -> <=", "if (v < 0.0f || u + v > 1.0f) return false;", "if (v <= 0.0f || u + v > 1.0f) return false;"),
+    ("tri u+v>1 -> >=", "if (v < 0.0f || u + v > 1.0f) return false;", "if (v < 0.0f || u + v >= 1.0f) return false;"),
+    ("tri t>=0 -> >", "return t >= 0.0f;\n}\n\n// intersectBoxRay", "return t > 0.0f;\n}\n\n// intersectBoxRay"),
+]
+only = sys.argv[1:]
+base = Path(tempfile.mkdtemp(prefix="oracle_mut_"))
+shutil.rmtree(base)
+shutil.copytree(REPO, base, ignore=shutil.ignore_patterns(".git", "_work", "gpurun_out", "__pycache__", "_build"))
+src = (REPO / "oracle/oracle.cpp").read_text()
+res = []
+for name, a, b in MUT:
+    if only and name not in only:
+        continue
+    n = src.count(a)
+    assert n >= 1, (name, "pattern not found")
+    (base / "oracle/oracle.cpp").write_text(src.replace(a, b))
+    subprocess.run(["make", "-s", "-C", str(base / "oracle")], check=True)
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-m", "not gpu", "-p", "no:cacheprovider",
+                        "tests"], cwd=base, capture_output=True, text=True)
+    last = [l for l in r.stdout.splitlines() if l.strip()][-1]
+    failed = [l for l in r.stdout.splitlines() if l.startswith("FAILED")]
+    res.append((name, r.returncode != 0, failed[:1], last))
+    print(f"{name:28s} caught={r.returncode != 0}  {failed[:1]} {last}", flush=True)
+(base / "oracle/oracle.cpp").write_text(src)
+shutil.rmtree(base)
+ok = all(c for _, c, _, _ in res)
+print("ALL CAUGHT" if ok else "SOME SURVIVED")
+sys.exit(0 if ok else 1)

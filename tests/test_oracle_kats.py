@@ -480,3 +480,230 @@ def test_interpolate_attributes(ora):
     np.testing.assert_array_equal(c, [[0.5, 0.25, 0.25, 1], [0.5, 0.5, 0, 1]])  # barycentric weights of the corners
     np.testing.assert_array_equal(t, uv)  # texcoords equal to positions: (u, v) again
     np.testing.assert_array_equal(p[:, :2], uv)
+
+
+# ---- SAH rules, hand-computed (CUDA_buildMeshTreeSAH.cu:224-645, CUDABin.h:44-111) ----
+
+def _sah_node(nd):
+    if nd["w0"] & 1:
+        return ("L", int(nd["w1"]), int(nd["w0"] >> 1))
+    return ("N", int((nd["w0"] >> 1) & 3), int(nd["w1"]))
+
+
+def _boxes(lo, hi):
+    n = len(lo)
+    b = np.zeros((n, 2, 4), np.float32)
+    b[:, 0, :3] = np.asarray(lo, np.float32)
+    b[:, 1, :3] = np.asarray(hi, np.float32)
+    return b
+
+
+def _points(centres):
+    return _boxes(centres, centres)
+
+
+def test_sah_line_bins_and_tie_to_larger_split():
+    """64 boxes [i-0.5, i+0.5] x [0,1] x [0,1], i = 0..63, fed in shuffled order.
+
+    Centroid extent 63 along x only: axis 0, scale = 31/63, bin(i) =
+    (int)(i * fl(31/63)) = 0 for i = 0,1,2, then two per bin, 63 alone in bin 31
+    (SAH.cu:312-325, 373-385). The cut after bin 14 leaves 31 boxes on the left
+    and 33 on the right; the cut after bin 15 leaves 33 and 31. The group of k
+    adjacent boxes has area 2(2k+1), exact in float, so the two costs are
+    2 + 0.125 * (a + b) and 2 + 0.125 * (b + a) with a = fl(fl(126/258) * 31),
+    b = fl(fl(134/258) * 33): an exact tie, and no other cut is within 0.7%.
+    The right sweep keeps the first strict minimum from bin 31 down
+    (SAH.cu:452-472), so the cut after bin 15 wins: 33 left, 31 right. A scale
+    of 32/63 would make a 32/32 cut available; '<=' would pick 31/33."""
+    import oracle as ora
+    i = np.arange(64, dtype=np.float32)
+    boxes = _boxes(np.stack([i - 0.5, 0 * i, 0 * i], 1), np.stack([i + 0.5, 0 * i + 1, 0 * i + 1], 1))
+    perm = np.random.default_rng(3).permutation(64).astype(np.uint32)
+    root = np.array([[-0.5, 0, 0, 0], [63.5, 1, 1, 0]], np.float32)
+    nodes, idx, fl = ora.build_sah(boxes, perm, root)
+    assert fl == 0
+    assert _sah_node(nodes[0]) == ("N", 0, 1)
+    assert sorted(idx[:33].tolist()) == list(range(33))  # left subtree: boxes 0..32
+    assert sorted(idx[33:].tolist()) == list(range(33, 64))
+    # children boxes are the bin unions (SAH.cu:501-523); the node keeps the item box
+    np.testing.assert_array_equal(nodes[1]["bmin"], [-0.5, 0, 0])
+    np.testing.assert_array_equal(nodes[1]["bmax"], [32.5, 1, 1])
+    np.testing.assert_array_equal(nodes[2]["bmin"], [32.5, 0, 0])
+    np.testing.assert_array_equal(nodes[2]["bmax"], [63.5, 1, 1])
+    np.testing.assert_array_equal(nodes[0]["bmin"], root[0, :3])
+    np.testing.assert_array_equal(nodes[0]["bmax"], root[1, :3])
+
+
+@pytest.mark.parametrize("a,b,axis", [
+    ((0, 0, 0), (8, 8, 0), 1),    # x == y > z: 'x > y' fails, 'y > z' holds -> 1
+    ((0, 0, 0), (8, 3, 8), 2),    # x > y, x == z: 'x > z' fails -> 2
+    ((0, 0, 0), (3, 8, 8), 2),    # y > x, y == z: 'y > z' fails -> 2
+    ((0, 0, 0), (8, 8, 8), 2),    # all equal -> 2
+    ((0, 0, 0), (8, 7, 7), 0),
+    ((0, 0, 0), (7, 8, 7), 1),
+])
+def test_sah_axis_ties(a, b, axis):
+    """Two clusters of 10 point boxes (SAH.cu:312-325): ties between centroid
+    extents go to the later axis. Point boxes have area 0, so every cut between
+    the clusters costs exactly 2 and the 20 boxes split 10/10 into two leaves."""
+    import oracle as ora
+    boxes = _points([a] * 10 + [b] * 10)
+    idx = np.array([k // 2 + (k % 2) * 10 for k in range(20)], np.uint32)  # interleaved
+    root = np.array([[-1, -1, -1, 0], [9, 9, 9, 0]], np.float32)
+    nodes, out, fl = ora.build_sah(boxes, idx, root)
+    assert [_sah_node(nd) for nd in nodes] == [("N", axis, 1), ("L", 0, 10), ("L", 10, 10)]
+    assert out.tolist() == list(range(20))  # stable partition in index order
+    np.testing.assert_array_equal(nodes[1]["bmin"], a)
+    np.testing.assert_array_equal(nodes[2]["bmax"], b)
+
+
+@pytest.mark.parametrize("n,leaf", [(16, True), (17, False)])
+def test_sah_min_leaf_size(n, leaf):
+    """At most 16 objects is a leaf before any binning (SAH.cu:239-255)."""
+    import oracle as ora
+    boxes = _points([(0, 0, 0)] * (n // 2) + [(5, 0, 0)] * (n - n // 2))
+    root = np.array([[-1, -1, -1, 0], [6, 1, 1, 0]], np.float32)
+    nodes, _, _ = ora.build_sah(boxes, np.arange(n, dtype=np.uint32), root)
+    if leaf:
+        assert [_sah_node(nd) for nd in nodes] == [("L", 0, n)]
+    else:
+        assert [_sah_node(nd) for nd in nodes] == [("N", 0, 1), ("L", 0, n // 2), ("L", n // 2, n - n // 2)]
+
+
+@pytest.mark.parametrize("root_x,leaf", [(6.0, False), (4.0, True), (5.5, False)])
+def test_sah_leaf_cost_rule(root_x, leaf):
+    """10 unit boxes [0,1]^3 and 10 boxes [9,10]x[0,1]^2 under an item box
+    [0,root_x] x [0,1]^2 (area 4*root_x + 2). Each child has area 6, so the best
+    cost is T = 2 + 0.125 * 20 * 6 / (4*root_x + 2) (CUDABin.h:104-111): 2.577
+    for root_x = 6 and 2.833 for root_x = 4. The node is a leaf when it has fewer
+    than 224 objects and T > (20 + 1) * 0.125 = 2.625 (SAH.cu:479-496), so the
+    first splits and the second does not (a threshold of 20 * 0.125 would make
+    both leaves, a box cost of 1 instead of 2 both splits). With root_x = 5.5 the
+    area is exactly 24 and T = 2 + 0.125 * (0.25 * 10 + 0.25 * 10) = 2.625, equal
+    to the threshold in float: the strict '>' splits."""
+    import oracle as ora
+    boxes = _boxes([(0, 0, 0)] * 10 + [(9, 0, 0)] * 10, [(1, 1, 1)] * 10 + [(10, 1, 1)] * 10)
+    root = np.array([[0, 0, 0, 0], [root_x, 1, 1, 0]], np.float32)
+    nodes, _, _ = ora.build_sah(boxes, np.arange(20, dtype=np.uint32), root)
+    if leaf:
+        assert [_sah_node(nd) for nd in nodes] == [("L", 0, 20)]
+    else:
+        assert [_sah_node(nd) for nd in nodes] == [("N", 0, 1), ("L", 0, 10), ("L", 10, 10)]
+        np.testing.assert_array_equal(nodes[1]["bmax"], [1, 1, 1])
+        np.testing.assert_array_equal(nodes[2]["bmin"], [9, 0, 0])
+
+
+@pytest.mark.parametrize("n,leaf", [(223, True), (224, False)])
+def test_sah_max_leaf_size(n, leaf):
+    """n boxes of side 200 whose centres step by 1/1024 along x: every child box
+    is almost the parent box, so every cut costs about 2 + 0.125 * n and the
+    cost test alone would make a leaf. A leaf needs fewer than 224 objects
+    (SAH.cu:479-496), so 223 is a leaf and 224 splits."""
+    import oracle as ora
+    c = np.arange(n, dtype=np.float32) / np.float32(1024)
+    lo = np.stack([c - 100, np.full(n, -100, np.float32), np.full(n, -100, np.float32)], 1)
+    hi = np.stack([c + 100, np.full(n, 100, np.float32), np.full(n, 100, np.float32)], 1)
+    boxes = _boxes(lo, hi)
+    root = np.array([[-100, -100, -100, 0], [100 + c[-1], 100, 100, 0]], np.float32)
+    nodes, _, fl = ora.build_sah(boxes, np.arange(n, dtype=np.uint32), root)
+    assert fl == 0
+    if leaf:
+        assert [_sah_node(nd) for nd in nodes] == [("L", 0, n)]
+    else:
+        assert _sah_node(nodes[0])[0] == "N" and _sah_node(nodes[0])[1] == 0
+
+
+def test_sah_degenerate_centroids():
+    """All centroids equal: the scale is 31/0 = inf, (c - min) * inf is NaN and
+    converts to bin 0; every cut has an empty side whose area term is
+    inf * 0 = NaN, which never wins, so m_min stays -1. Below 224 objects the
+    cost test (T_min = FLT_MAX) makes a leaf; at 224 or more the reference would
+    push the same item forever, and the oracle makes a leaf and sets bit 0x10."""
+    import oracle as ora
+    for n, flag in ((40, 0), (300, 0x10)):
+        boxes = _boxes([(0, 0, 0)] * n, [(1, 1, 1)] * n)
+        root = np.array([[0, 0, 0, 0], [1, 1, 1, 0]], np.float32)
+        nodes, _, fl = ora.build_sah(boxes, np.arange(n, dtype=np.uint32), root)
+        assert [_sah_node(nd) for nd in nodes] == [("L", 0, n)]
+        assert fl == flag
+
+
+def test_hybrid_refines_only_large_leaves():
+    """Hybrid (CUDAMeshTree.cpp:98-103, SAH.cu:711-735, kernel_initStack
+    SAH.cu:647-666): every leaf of an existing tree is pushed with its own box as
+    the item box, and new node ids continue from the existing count. A hand-built
+    LBVH: root 0 over leaves 1 (objects 0..15, two separable clusters) and 2
+    (objects 16..47: clusters at x = 0 and x = 10, interleaved). Leaf 1 has 16
+    objects and stays a leaf; leaf 2 becomes an internal node with its original
+    (deliberately loose) box and children 3 and 4 of 16 objects each."""
+    import oracle as ora
+    cent = [(0, 0, 0)] * 8 + [(5, 5, 5)] * 8 + [(0, 0, 0), (10, 0, 0)] * 16
+    boxes = _points(cent)
+    nodes = np.zeros(3, ora.NODE_DTYPE)
+    nodes[0]["w0"], nodes[0]["w1"] = 0 | (2 << 1), 1
+    nodes[0]["bmin"], nodes[0]["bmax"] = (-1, -2, -3), (11, 12, 13)
+    nodes[1]["w0"], nodes[1]["w1"] = 1 | (16 << 1), 0
+    nodes[1]["bmin"], nodes[1]["bmax"] = (0, 0, 0), (5, 5, 5)
+    nodes[2]["w0"], nodes[2]["w1"] = 1 | (32 << 1), 16
+    nodes[2]["bmin"], nodes[2]["bmax"] = (-100, -100, -100), (100, 100, 100)
+    root = np.array([[-1, -2, -3, 0], [11, 12, 13, 0]], np.float32)
+    out, idx, fl = ora.build_sah(boxes, np.arange(48, dtype=np.uint32), root, nodes=nodes, n_existing=3)
+    assert fl == 0
+    assert [_sah_node(nd) for nd in out] == [("N", 2, 1), ("L", 0, 16), ("N", 0, 3), ("L", 16, 16), ("L", 32, 16)]
+    for k in (0, 1, 2):  # boxes of the existing nodes are kept
+        np.testing.assert_array_equal(out[k]["bmin"], nodes[k]["bmin"])
+        np.testing.assert_array_equal(out[k]["bmax"], nodes[k]["bmax"])
+    np.testing.assert_array_equal(out[3]["bmax"], [0, 0, 0])
+    np.testing.assert_array_equal(out[4]["bmin"], [10, 0, 0])
+    assert idx[:16].tolist() == list(range(16))
+    assert idx[16:32].tolist() == list(range(16, 48, 2))  # stable partition
+    assert idx[32:].tolist() == list(range(17, 48, 2))
+
+
+def test_hybrid_lbvh_pass_keeps_face_order(ora):
+    """The LBVH pass of Hybrid (tree_complete = false) does not reorder the
+    mesh's faces (CUDA_buildMeshTreeLBVH.cu:669-676 runs only when complete)."""
+    F, V = _mesh_from_codes([0x80000000] * 20 + [0] * 20)
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, False)
+    np.testing.assert_array_equal(r.faces, F)
+    assert r.indices.tolist() == list(range(20, 40)) + list(range(20))
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, True)
+    np.testing.assert_array_equal(r.faces, F[r.indices])
+
+
+def test_refit_flat_left_leaf_empties_ancestors(ora):
+    # the left leaf is flat in z: the root keeps the empty box (LBVH.cu:282)
+    codes = [0] * 16 + [0x80000000] * 16
+    F, V = _mesh_from_codes(codes)
+    V[1:3 * 16:3, 2] = V[0:3 * 16:3, 2]  # left leaf: remove the z offset
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, True)
+    assert r.nodes[1]["bmax"][2] - r.nodes[1]["bmin"][2] == 0.0
+    assert r.nodes[2]["bmax"][2] > r.nodes[2]["bmin"][2]
+    np.testing.assert_array_equal(r.nodes[0]["bmin"], [F32MAX] * 3)
+    np.testing.assert_array_equal(r.nodes[0]["bmax"], [-F32MAX] * 3)
+    # both leaves with z extent: the root is their union
+    F, V = _mesh_from_codes(codes)
+    r = ora.build_lbvh_ref(F, V, UNIT, 12, True)
+    np.testing.assert_array_equal(r.nodes[0]["bmin"], np.minimum(r.nodes[1]["bmin"], r.nodes[2]["bmin"]))
+    np.testing.assert_array_equal(r.nodes[0]["bmax"], np.maximum(r.nodes[1]["bmax"], r.nodes[2]["bmax"]))
+
+
+@pytest.mark.parametrize("o,hit,tuv", [
+    ((1.0, 0.0, 1.0), True, (1.0, 1.0, 0.0)),      # u == 1 at vertex v1: accepted ('u > 1' rejects)
+    ((0.5, 0.5, 1.0), True, (1.0, 0.5, 0.5)),      # u + v == 1 on edge v1-v2: accepted
+    ((0.0, 0.5, 1.0), True, (1.0, 0.0, 0.5)),      # u == 0: accepted ('u < 0' rejects)
+    ((0.5, 0.0, 1.0), True, (1.0, 0.5, 0.0)),      # v == 0: accepted
+    ((0.25, 0.25, 0.0), True, (0.0, 0.25, 0.25)),  # t == 0: accepted ('t >= 0')
+    ((0.25, 0.25, -0.5), False, None),             # t < 0
+    ((1.0078125, 0.0, 1.0), False, None),          # u = 1 + 2^-7
+    ((0.5078125, 0.5, 1.0), False, None),          # u + v = 1 + 2^-7
+    ((-0.0078125, 0.5, 1.0), False, None),         # u = -2^-7
+    ((0.5, -0.0078125, 1.0), False, None),         # v = -2^-7
+])
+def test_moller_trumbore_boundaries(ora, o, hit, tuv):
+    """Triangle (0,0,0), (1,0,0), (0,1,0), ray straight down: det = 1, u = o.x,
+    v = o.y, t = o.z, all exact (RT.cu:57-89). The bounds are inclusive."""
+    h, got = ora.tri_ray([*o, 0], [0, 0, -1, 0], [0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0])
+    assert h == hit
+    if hit:
+        np.testing.assert_array_equal(got, tuv)
