@@ -33,15 +33,23 @@ rk_aabox toRk(const CUDABox& b) {
 }
 
 // .cdm chunks (CUDAMesh.cpp:134-400, CUDAMeshTree.cpp:114-289): int magic,
-// int length (of the whole chunk, header included), uint count, raw array.
+// int length (of the whole chunk, header included), then the payload. An array
+// chunk's payload is uint count and the raw array; the hitbox chunk's payload is
+// the aabox alone (CUDAMesh.cpp:294-296). Readers skip unknown chunks by their
+// length and read known ones in sequence, as the reference does.
 void writeChunk(FILE* f, int magic, const void* data, uint count, size_t unit) {
-  long start = std::ftell(f);
   int length = (int)(sizeof(int) * 2 + sizeof(uint) + (size_t)count * unit);
   std::fwrite(&magic, sizeof(int), 1, f);
   std::fwrite(&length, sizeof(int), 1, f);
   std::fwrite(&count, sizeof(uint), 1, f);
   if (count) std::fwrite(data, unit, count, f);
-  (void)start;
+}
+
+void writeHitboxChunk(FILE* f, int magic, const CUDABox& box) {
+  int length = (int)(sizeof(int) * 2 + sizeof(box));
+  std::fwrite(&magic, sizeof(int), 1, f);
+  std::fwrite(&length, sizeof(int), 1, f);
+  std::fwrite(&box, sizeof(box), 1, f);
 }
 
 const int HITBOX_HEADER = 0x3027AD79;
@@ -113,7 +121,7 @@ void CUDAMesh::buildFromArrays(const float4* hv, uint nv, const uint4* hf, uint 
 void CUDAMesh::quickSave(const char* path, bool overwrite) {
   FILE* f = std::fopen(path, overwrite ? "wb" : "ab");
   _assert(f != nullptr, __FILE__, __LINE__, "CUDAMesh::quickSave() : Invalid file.");
-  writeChunk(f, HITBOX_HEADER, &hitbox, 1, sizeof(hitbox));
+  writeHitboxChunk(f, HITBOX_HEADER, hitbox);
   saveArray(f, VERTICES_HEADER, vertices);
   saveArray(f, NORMALS_HEADER, normals);
   saveArray(f, COLORS_HEADER, colors);
@@ -138,11 +146,12 @@ void CUDAMesh::quickLoad(const char* path) {
     bool known = header == HITBOX_HEADER || header == VERTICES_HEADER || header == NORMALS_HEADER ||
                  header == COLORS_HEADER || header == TEXCOORDS_HEADER || header == FACES_HEADER;
     if (!known) { std::fseek(f, off + length, SEEK_SET); continue; }
-    _assert(std::fread(&count, sizeof(uint), 1, f) == 1, __FILE__, __LINE__, "CUDAMesh::quickLoad() : Truncated file.");
     if (header == HITBOX_HEADER) {
-      _assert(count == 1 && std::fread(&hitbox, sizeof(hitbox), 1, f) == 1, __FILE__, __LINE__,
-              "CUDAMesh::quickLoad() : Invalid hitbox.");
-    } else if (header == VERTICES_HEADER) { delete vertices; vertices = loadArray<float4>(f, count); }
+      _assert(std::fread(&hitbox, sizeof(hitbox), 1, f) == 1, __FILE__, __LINE__, "CUDAMesh::quickLoad() : Truncated file.");
+      continue;
+    }
+    _assert(std::fread(&count, sizeof(uint), 1, f) == 1, __FILE__, __LINE__, "CUDAMesh::quickLoad() : Truncated file.");
+    if (header == VERTICES_HEADER) { delete vertices; vertices = loadArray<float4>(f, count); }
     else if (header == NORMALS_HEADER) { delete normals; normals = loadArray<float4>(f, count); }
     else if (header == COLORS_HEADER) { delete colors; colors = loadArray<float4>(f, count); }
     else if (header == TEXCOORDS_HEADER) { delete texcoords; texcoords = loadArray<float2>(f, count); }

@@ -1252,8 +1252,9 @@ extern "C" rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshe
 extern "C" size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count) {
   (void)mesh_count;
   const size_t n = ray_count ? ray_count : 1;
-  // queue counters, then (ray_order 1) the ray keys, the ray order and the sort's own space
-  return rk_align_up(kQueues * kQueueStride * 4) + rk_align_up(n * 8) + rk_align_up(n * 4) +
+  // the flags word, the queue counters, then (ray_order 1) the ray keys, the ray
+  // order and the sort's own space
+  return rk_align_up(4) + rk_align_up(kQueues * kQueueStride * 4) + rk_align_up(n * 8) + rk_align_up(n * 4) +
          rk_sort_workspace_size((uint32_t)n, 8) + 1024;
 }
 
@@ -1286,8 +1287,13 @@ extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshe
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   hipStream_t st = rk_hip_stream(stream);
   RkArena ar(ws, ws_bytes);
+  uint32_t* flags = ar.take<uint32_t>(1);  // word 0 of ws: RK_FLAG_* bits of this call
   uint32_t* queues = ar.take<uint32_t>(kQueues * kQueueStride);
-  if (!queues) return RK_ERROR_WORKSPACE;
+  if (!flags || !queues) return RK_ERROR_WORKSPACE;
+  {
+    rk_status s = rk_fill_u32(stream, flags, 1, 0u);
+    if (s != RK_SUCCESS) return s;
+  }
   uint32_t grid_persistent = 0;
   if (config->trace_mode == 1) {
     rk_status s = rk_fill_u32(stream, queues, kQueues * kQueueStride, 0u);
@@ -1317,7 +1323,7 @@ extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshe
                        nodes, o, ray_count, keys, order);
     RK_CHECK_LAUNCH();
     rk_status s = rk_radix_sort_impl<uint64_t>(st, keys, order, ray_count, 0, (int)key_passes_for(mesh_count),
-                                               sort_ws, sort_bytes, nullptr);
+                                               sort_ws, sort_bytes, flags);
     if (s != RK_SUCCESS) return s;
   }
 #define RK_BATCH_TRACE(WG, SD)                                                                                     \
@@ -1665,10 +1671,10 @@ extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* b
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   hipStream_t st = rk_hip_stream(stream);
   RkArena ar(ws, ws_bytes);
+  const size_t sort_bytes = rk_sort_workspace_size(n_rays, 4);
+  void* sort_ws = ar.take<char>(sort_bytes);  // first: its word 0 is the sort's error word
   uint32_t* keys = ar.take<uint32_t>(n_rays);
   uint32_t* order = ar.take<uint32_t>(n_rays);
-  const size_t sort_bytes = rk_sort_workspace_size(n_rays, 4);
-  void* sort_ws = ar.take<char>(sort_bytes);
   if (!keys || !order || !sort_ws) return RK_ERROR_WORKSPACE;
   const FNode* nodes = static_cast<const FNode*>(bvh->nodes);
   const float4* o = reinterpret_cast<const float4*>(origins);

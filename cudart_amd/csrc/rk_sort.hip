@@ -339,12 +339,56 @@ __global__ __launch_bounds__(128) void k_bitonic_chunks(uint32_t* __restrict__ k
   }
 }
 
+// Stable sort of at most 512 keys (and values) in one workgroup: each thread
+// ranks its key by counting the smaller keys and the equal keys before it. This
+// is the order of the reference's odd-even transposition sort, which swaps only
+// on '>' (CUDA_bubbleSort512.cu:42-96) and is therefore stable. 'init' takes
+// the values to be 0..n-1 (kernel_bubbleSortInit, CUDA_bubbleSort512.cu:98-104).
+// 'clear' (may be NULL) is set to 0. Every global read happens before the
+// barrier and every write after it, so the sort is in place.
+constexpr uint32_t kSmallSort = 512;
+__global__ __launch_bounds__(512) void k_small_sort(uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    uint32_t n, int init, uint32_t* __restrict__ clear) {
+  __shared__ uint32_t sk[kSmallSort];
+  const uint32_t t = threadIdx.x;
+  if (clear && t == 0) *clear = 0u;
+  uint32_t k = 0, v = 0;
+  if (t < n) {
+    k = keys[t];
+    sk[t] = k;
+    if (vals) v = init ? t : vals[t];
+  }
+  __syncthreads();
+  if (t < n) {
+    uint32_t r = 0;
+    for (uint32_t u = 0; u < n; ++u) {
+      const uint32_t x = sk[u];
+      r += (x < k || (x == k && u < t)) ? 1u : 0u;
+    }
+    keys[r] = k;
+    if (vals) vals[r] = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ v, uint32_t n) {
   size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
 }
 
 inline uint32_t tiles_of(uint32_t n) { return (n + kTile - 1) / kTile; }
+// The error word of a sort called with this workspace: its first 4 bytes.
+inline uint32_t* ws_flags_word(void* ws, size_t ws_bytes) {
+  return (ws && ws_bytes >= 4) ? static_cast<uint32_t*>(ws) : nullptr;
+}
+inline rk_status clear_ws_flags(hipStream_t st, void* ws, size_t ws_bytes) {
+  uint32_t* w = ws_flags_word(ws, ws_bytes);
+  return w ? rk_fill_u32(reinterpret_cast<rk_stream>(st), w, 1, 0u) : RK_SUCCESS;
+}
+rk_status launch_small_sort(hipStream_t st, uint32_t* keys, uint32_t* vals, uint32_t n, int init, uint32_t* clear) {
+  hipLaunchKernelGGL(k_small_sort, dim3(1), dim3(kSmallSort), 0, st, keys, vals, n, init, clear);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
 inline unsigned grid_for(size_t n, size_t per_block, unsigned cap) {
   size_t g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -364,22 +408,25 @@ extern "C" size_t rk_sort_workspace_size(uint32_t length, uint32_t key_bytes) {
 }
 
 // Sort keys (and values) stably on digits first_pass .. first_pass+passes-1.
-// 'err' (device, may be NULL) receives RK_FLAG_SPIN_LIMIT on a look-back timeout.
+// 'err' (device, may be NULL) receives RK_FLAG_SPIN_LIMIT on a look-back
+// timeout; when it is NULL the first word of 'ws' does, cleared by this call.
+// The look-back status words count keys in 30 bits, so n < 2^30.
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
                              void* ws, size_t ws_bytes, uint32_t* err) {
-  if (n <= 1 || passes <= 0) return RK_SUCCESS;
+  if (n >= kAgg) return RK_ERROR_UNSUPPORTED;
+  if (n <= 1 || passes <= 0) return err ? RK_SUCCESS : clear_ws_flags(st, ws, ws_bytes);
   if (!keys || passes > kMaxPasses) return RK_ERROR_INVALID_VALUE;
   const uint32_t T = tiles_of(n);
   RkArena ar(ws, ws_bytes);
-  const size_t ctl_words = (size_t)passes * 256 + 1 + (size_t)passes * T * 256;
-  uint32_t* ctl = ar.take<uint32_t>(ctl_words);
+  const size_t ctl_words = 1 + (size_t)passes * 256 + (size_t)passes * T * 256;
+  uint32_t* ctl = ar.take<uint32_t>(ctl_words);  // first in ws: ctl[0] is the error word
   K* kalt = ar.take<K>(n);
   uint32_t* valt = vals ? ar.take<uint32_t>(n) : nullptr;
   if (!ctl || !kalt || (vals && !valt)) return RK_ERROR_WORKSPACE;
-  uint32_t* hist = ctl;
-  uint32_t* werr = hist + (size_t)passes * 256;
-  uint32_t* status = werr + 1;
+  uint32_t* werr = ctl;
+  uint32_t* hist = ctl + 1;
+  uint32_t* status = hist + (size_t)passes * 256;
   if (!err) err = werr;
 
   rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, ctl_words, 0u);
@@ -432,15 +479,16 @@ template rk_status rk_radix_sort_impl<uint64_t>(hipStream_t, uint64_t*, uint32_t
 rk_status rk_sort_general_impl(hipStream_t st, uint32_t* keys, uint32_t* indices, uint32_t length,
                                uint32_t bit_depth, int init_indices, void* ws, size_t ws_bytes, uint32_t* err) {
   if (bit_depth == 0 || bit_depth > 32) return RK_ERROR_INVALID_VALUE;
-  if (length == 0) return RK_SUCCESS;
+  if (length >= kAgg) return RK_ERROR_UNSUPPORTED;
+  if (length == 0) return err ? RK_SUCCESS : clear_ws_flags(st, ws, ws_bytes);
   if (!keys || !indices) return RK_ERROR_INVALID_VALUE;
   const uint32_t byte_depth = 1 + (bit_depth - 1) / 8;
+  if (length <= kSmallSort)
+    return launch_small_sort(st, keys, indices, length, init_indices, err ? nullptr : ws_flags_word(ws, ws_bytes));
   if (init_indices) {
     hipLaunchKernelGGL(k_iota, dim3(grid_for(length, 256 * 8, 4096)), dim3(256), 0, st, indices, length);
     RK_CHECK_LAUNCH();
   }
-  if (length <= 1) return RK_SUCCESS;
-  if (length <= 512) return rk_radix_sort_impl<uint32_t>(st, keys, indices, length, 0, 4, ws, ws_bytes, err);
   if (length <= byte_depth * 8192u) {
     hipLaunchKernelGGL(k_bitonic_chunks, dim3((length + 127) / 128), dim3(128), 0, st, keys, indices, length);
     RK_CHECK_LAUNCH();
@@ -461,7 +509,10 @@ extern "C" rk_status rk_radix_sort_general(rk_stream stream, uint32_t* keys, uin
 extern "C" rk_status rk_radix_sort(rk_stream stream, uint32_t* values, uint32_t length, uint32_t bit_depth, void* ws,
                                    size_t ws_bytes) {
   if (bit_depth == 0 || bit_depth > 32) return RK_ERROR_INVALID_VALUE;
-  if (length <= 1) return RK_SUCCESS;
+  if (length >= kAgg) return RK_ERROR_UNSUPPORTED;
+  if (length > 1 && !values) return RK_ERROR_INVALID_VALUE;
+  if (length <= kSmallSort)
+    return launch_small_sort(rk_hip_stream(stream), values, nullptr, length, 0, ws_flags_word(ws, ws_bytes));
   const uint32_t byte_depth = 1 + (bit_depth - 1) / 8;
   int passes = (length <= byte_depth * 8192u) ? 4 : (int)byte_depth;
   return rk_radix_sort_impl<uint32_t>(rk_hip_stream(stream), values, nullptr, length, 0, passes, ws, ws_bytes,
@@ -471,7 +522,62 @@ extern "C" rk_status rk_radix_sort(rk_stream stream, uint32_t* values, uint32_t 
 extern "C" rk_status rk_radix_sort_pairs_u64(rk_stream stream, uint64_t* keys, uint32_t* values, uint32_t length,
                                              uint32_t bit_depth, void* ws, size_t ws_bytes) {
   if (bit_depth == 0 || bit_depth > 64) return RK_ERROR_INVALID_VALUE;
-  if (length <= 1) return RK_SUCCESS;
   int passes = (int)(1 + (bit_depth - 1) / 8);
   return rk_radix_sort_impl<uint64_t>(rk_hip_stream(stream), keys, values, length, 0, passes, ws, ws_bytes, nullptr);
+}
+
+// template <typename T> void CUDA_bitonicSort(T* values, uint length),
+// samples/09_how_to_use_a_sorter.cpp:39-40 / CUDA_bitonicSort.cu:402-518
+// (instantiated for unsigned int, :523-537). The values are their own keys, so
+// any correct sort gives the reference's result: at most 512 values in one
+// workgroup, more by the radix passes over all 32 bits.
+extern "C" rk_status rk_bitonic_sort(rk_stream stream, uint32_t* values, uint32_t length, void* ws, size_t ws_bytes) {
+  if (length >= kAgg) return RK_ERROR_UNSUPPORTED;
+  if (length > 1 && !values) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  if (length <= kSmallSort) return launch_small_sort(st, values, nullptr, length, 0, ws_flags_word(ws, ws_bytes));
+  return rk_radix_sort_impl<uint32_t>(st, values, nullptr, length, 0, 4, ws, ws_bytes, nullptr);
+}
+
+// template <typename K, typename V> void CUDA_bitonicSortGeneral(K* keys, V* values, uint length),
+// samples/09_how_to_use_a_sorter.cpp:42-43 / CUDA_bitonicSortGeneral.cu:504-690
+// (instantiated for <uint, uint>, :694-697), at any length. The reference sorts
+// each full 128-element chunk with a bitonic network, the partial last chunk
+// by odd-even transposition, then merges chunk pairs and buckets of doubling
+// size with stable '<=' merges (:172-481) until one bucket remains, carrying a
+// local index array that finally permutes the values. The same order is the
+// chunk networks followed by a stable sort on the whole key.
+extern "C" rk_status rk_bitonic_sort_general(rk_stream stream, uint32_t* keys, uint32_t* values, uint32_t length,
+                                             void* ws, size_t ws_bytes) {
+  hipStream_t st = rk_hip_stream(stream);
+  if (length >= kAgg) return RK_ERROR_UNSUPPORTED;
+  if (length <= 1) return clear_ws_flags(st, ws, ws_bytes);
+  if (!keys || !values) return RK_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(k_bitonic_chunks, dim3((length + 127) / 128), dim3(128), 0, st, keys, values, length);
+  RK_CHECK_LAUNCH();
+  if (length <= 128) return clear_ws_flags(st, ws, ws_bytes);
+  return rk_radix_sort_impl<uint32_t>(st, keys, values, length, 0, 4, ws, ws_bytes, nullptr);
+}
+
+// void CUDA_bubbleSort512(uint* keys, uint* indices, uint length, bool init_indices),
+// CUDA_radixSortGeneral.cu:27 / CUDA_bubbleSort512.cu:114-167: a stable sort of
+// at most 512 (key, index) pairs; init_indices first sets indices[i] = i. The
+// reference asserts on length > 512 (:116); here that is RK_ERROR_INVALID_VALUE.
+extern "C" rk_status rk_bubble_sort512(rk_stream stream, uint32_t* keys, uint32_t* indices, uint32_t length,
+                                       int init_indices) {
+  if (length > kSmallSort) return RK_ERROR_INVALID_VALUE;
+  if (length == 0) return RK_SUCCESS;
+  if (!keys || !indices) return RK_ERROR_INVALID_VALUE;
+  return launch_small_sort(rk_hip_stream(stream), keys, indices, length, init_indices, nullptr);
+}
+
+// The RK_FLAG_* bits a sort left in the first word of its workspace (waits for 'stream').
+extern "C" rk_status rk_workspace_flags(rk_stream stream, const void* ws, uint32_t* flags) {
+  if (!ws || !flags) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  uint32_t h = 0;
+  RK_HIP(hipMemcpyAsync(&h, ws, 4, hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  *flags = h;
+  return RK_SUCCESS;
 }

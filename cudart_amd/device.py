@@ -6,6 +6,7 @@ these functions raise.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 from dataclasses import dataclass
 
@@ -25,9 +26,17 @@ def _dev(a, dtype=None) -> torch.Tensor:
 
 
 def _stream_ptr(stream) -> int | None:
-    if stream is None:
-        return None
-    return int(stream.cuda_stream) or None
+    """The HIP stream of 'stream', or of torch's current stream when None."""
+    s = torch.cuda.current_stream() if stream is None else stream
+    return int(s.cuda_stream) or None
+
+
+def _on(stream):
+    """Allocations made inside belong to 'stream' (torch's caching allocator
+    reuses a freed block only in order on the stream it was allocated on), so a
+    temporary freed on return is not handed out while the kernels that use it
+    still run on 'stream'."""
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
 
 
 def u32_view(t: torch.Tensor) -> np.ndarray:
@@ -43,22 +52,24 @@ def _check(s, what):
 
 
 def calc_mesh_boxes(F: torch.Tensor, V: torch.Tensor, stream=None):
-    n = F.shape[0]
-    boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
-    idx = torch.empty(n, dtype=_U32, device="cuda")
-    _check(N.lib().rk_calc_mesh_boxes(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, boxes.data_ptr(),
-                                      idx.data_ptr()), "calc_mesh_boxes")
-    return boxes, idx
+    with _on(stream):
+        n = F.shape[0]
+        boxes = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+        idx = torch.empty(n, dtype=_U32, device="cuda")
+        _check(N.lib().rk_calc_mesh_boxes(_stream_ptr(stream), F.data_ptr(), V.data_ptr(), n, boxes.data_ptr(),
+                                          idx.data_ptr()), "calc_mesh_boxes")
+        return boxes, idx
 
 
 def morton(boxes: torch.Tensor, root: np.ndarray, mode: int, stream=None) -> torch.Tensor:
-    n = boxes.shape[0]
-    dt = torch.int64 if mode == N.RK_MORTON_63 else torch.int32
-    out = torch.empty(n, dtype=dt, device="cuda")
-    rb = N.aabox(root)
-    _check(N.lib().rk_calc_morton_codes(_stream_ptr(stream), boxes.data_ptr(), n, C.byref(rb), mode, out.data_ptr()),
-           "morton")
-    return out
+    with _on(stream):
+        n = boxes.shape[0]
+        dt = torch.int64 if mode == N.RK_MORTON_63 else torch.int32
+        out = torch.empty(n, dtype=dt, device="cuda")
+        rb = N.aabox(root)
+        _check(N.lib().rk_calc_morton_codes(_stream_ptr(stream), boxes.data_ptr(), n, C.byref(rb), mode, out.data_ptr()),
+               "morton")
+        return out
 
 
 def workspace(nbytes: int) -> torch.Tensor:
@@ -66,27 +77,68 @@ def workspace(nbytes: int) -> torch.Tensor:
 
 
 def sort_general(keys: torch.Tensor, vals: torch.Tensor, bits: int = 32, init: bool = False, stream=None):
-    n = keys.shape[0]
-    ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
-    _check(N.lib().rk_radix_sort_general(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits, int(init),
-                                         ws.data_ptr(), ws.numel()), "radix_sort_general")
-    return keys, vals
+    with _on(stream):
+        n = keys.shape[0]
+        ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
+        _check(N.lib().rk_radix_sort_general(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits, int(init),
+                                             ws.data_ptr(), ws.numel()), "radix_sort_general")
+        return keys, vals
 
 
 def sort_values(values: torch.Tensor, bits: int = 32, stream=None):
-    n = values.shape[0]
-    ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
-    _check(N.lib().rk_radix_sort(_stream_ptr(stream), values.data_ptr(), n, bits, ws.data_ptr(), ws.numel()),
-           "radix_sort")
-    return values
+    with _on(stream):
+        n = values.shape[0]
+        ws = workspace(N.lib().rk_sort_workspace_size(n, 4))
+        _check(N.lib().rk_radix_sort(_stream_ptr(stream), values.data_ptr(), n, bits, ws.data_ptr(), ws.numel()),
+               "radix_sort")
+        return values
 
 
 def sort_pairs_u64(keys: torch.Tensor, vals: torch.Tensor, bits: int = 64, stream=None):
-    n = keys.shape[0]
-    ws = workspace(N.lib().rk_sort_workspace_size(n, 8))
-    _check(N.lib().rk_radix_sort_pairs_u64(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits,
-                                           ws.data_ptr(), ws.numel()), "radix_sort_pairs_u64")
-    return keys, vals
+    with _on(stream):
+        n = keys.shape[0]
+        ws = workspace(N.lib().rk_sort_workspace_size(n, 8))
+        _check(N.lib().rk_radix_sort_pairs_u64(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n, bits,
+                                               ws.data_ptr(), ws.numel()), "radix_sort_pairs_u64")
+        return keys, vals
+
+
+def bitonic_sort(values: torch.Tensor, stream=None, ws: torch.Tensor | None = None):
+    """rk_bitonic_sort (CUDA_bitonicSort<uint>): values ascending, in place."""
+    with _on(stream):
+        n = values.shape[0]
+        need = N.lib().rk_sort_workspace_size(n, 4)
+        if ws is None or ws.numel() < need:
+            ws = workspace(need)
+        _check(N.lib().rk_bitonic_sort(_stream_ptr(stream), values.data_ptr(), n, ws.data_ptr(), ws.numel()),
+               "bitonic_sort")
+        return values, ws
+
+
+def bitonic_sort_general(keys: torch.Tensor, vals: torch.Tensor, stream=None, ws: torch.Tensor | None = None):
+    """rk_bitonic_sort_general (CUDA_bitonicSortGeneral<uint, uint>), in place."""
+    with _on(stream):
+        n = keys.shape[0]
+        need = N.lib().rk_sort_workspace_size(n, 4)
+        if ws is None or ws.numel() < need:
+            ws = workspace(need)
+        _check(N.lib().rk_bitonic_sort_general(_stream_ptr(stream), keys.data_ptr(), vals.data_ptr(), n,
+                                               ws.data_ptr(), ws.numel()), "bitonic_sort_general")
+        return keys, vals, ws
+
+
+def bubble_sort512(keys: torch.Tensor, indices: torch.Tensor, init: bool = False, stream=None):
+    """rk_bubble_sort512 (CUDA_bubbleSort512), in place."""
+    _check(N.lib().rk_bubble_sort512(_stream_ptr(stream), keys.data_ptr(), indices.data_ptr(), keys.shape[0],
+                                     int(init)), "bubble_sort512")
+    return keys, indices
+
+
+def workspace_flags(ws: torch.Tensor, stream=None) -> int:
+    """The RK_FLAG_* word a sort left at the start of its workspace."""
+    f = N._U32()
+    _check(N.lib().rk_workspace_flags(_stream_ptr(stream), ws.data_ptr(), C.byref(f)), "workspace_flags")
+    return f.value
 
 
 @dataclass
@@ -143,13 +195,14 @@ def new_outputs(n: int, tmax: float | None = None):
 
 def trace_ref(tree: RefTree, V: torch.Tensor, O: torch.Tensor, D: torch.Tensor, width: int, height: int,
               coherency: bool = True, out=None, stream=None):
-    n = width * height
-    fid, uv, dep = out if out is not None else new_outputs(n)
-    node_count = tree.node_count if tree.node_count > 0 else tree.nodes.shape[0]
-    _check(N.lib().rk_raytrace_mesh_tree(_stream_ptr(stream), tree.nodes.data_ptr(), node_count, tree.faces.data_ptr(),
-                                         tree.faces.shape[0], V.data_ptr(), O.data_ptr(), D.data_ptr(), width, height, fid.data_ptr(),
-                                         uv.data_ptr(), dep.data_ptr(), int(coherency)), "raytrace_mesh_tree")
-    return fid, uv, dep
+    with _on(stream):
+        n = width * height
+        fid, uv, dep = out if out is not None else new_outputs(n)
+        node_count = tree.node_count if tree.node_count > 0 else tree.nodes.shape[0]
+        _check(N.lib().rk_raytrace_mesh_tree(_stream_ptr(stream), tree.nodes.data_ptr(), node_count, tree.faces.data_ptr(),
+                                             tree.faces.shape[0], V.data_ptr(), O.data_ptr(), D.data_ptr(), width, height, fid.data_ptr(),
+                                             uv.data_ptr(), dep.data_ptr(), int(coherency)), "raytrace_mesh_tree")
+        return fid, uv, dep
 
 
 @dataclass
@@ -195,22 +248,24 @@ def _opt(det_epsilon):
 
 def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
     """det_epsilon None = the reference threshold (1e-6); 0.0 = only parallel rays rejected."""
-    n = O.shape[0]
-    fid, uv, dep = out if out is not None else new_outputs(n)
-    _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                 fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)), "fast_trace")
-    return fid, uv, dep
+    with _on(stream):
+        n = O.shape[0]
+        fid, uv, dep = out if out is not None else new_outputs(n)
+        _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                     fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)), "fast_trace")
+        return fid, uv, dep
 
 
 def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
     """fast_trace plus the total child-box and triangle tests performed."""
-    n = O.shape[0]
-    fid, uv, dep = out if out is not None else new_outputs(n)
-    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
-    _check(N.lib().rk_fast_trace_counted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                         fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
-                                         cnt.data_ptr()), "fast_trace_counted")
-    return fid, uv, dep, cnt
+    with _on(stream):
+        n = O.shape[0]
+        fid, uv, dep = out if out is not None else new_outputs(n)
+        cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+        _check(N.lib().rk_fast_trace_counted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                             fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                             cnt.data_ptr()), "fast_trace_counted")
+        return fid, uv, dep, cnt
 
 
 class SAHBuilder:
@@ -360,21 +415,23 @@ def camera_rays(cam: np.ndarray) -> N.CameraRays:
 
 def fast_trace_camera(tree: FastTree, cam: np.ndarray, width: int, height: int, out=None, det_epsilon=None,
                       stream=None):
-    n = width * height
-    fid, uv, dep = out if out is not None else new_outputs(n)
-    c = camera_rays(cam)
-    _check(N.lib().rk_fast_trace_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
-                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)),
-           "fast_trace_camera")
-    return fid, uv, dep
+    with _on(stream):
+        n = width * height
+        fid, uv, dep = out if out is not None else new_outputs(n)
+        c = camera_rays(cam)
+        _check(N.lib().rk_fast_trace_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
+                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)),
+               "fast_trace_camera")
+        return fid, uv, dep
 
 
 def deform(base: torch.Tensor, amplitude: float, omega: float, time: float, seed: int, along_z: bool,
            out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
-    out = torch.empty_like(base) if out is None else out
-    _check(N.lib().rk_deform_vertices(_stream_ptr(stream), base.data_ptr(), out.data_ptr(), base.shape[0], amplitude,
-                                      omega, time, seed, int(along_z)), "deform")
-    return out
+    with _on(stream):
+        out = torch.empty_like(base) if out is None else out
+        _check(N.lib().rk_deform_vertices(_stream_ptr(stream), base.data_ptr(), out.data_ptr(), base.shape[0], amplitude,
+                                          omega, time, seed, int(along_z)), "deform")
+        return out
 
 
 class Server:
@@ -437,30 +494,32 @@ def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height
                 V: torch.Tensor, normals=None, colors=None, texcoords=None, outs=None, stream=None):
     """rk_interpolate_data; outs = (points, normals, colors, texcoords, materials)
     tensors (any may be None), allocated zeroed when not given."""
-    n = width * height
-    if outs is None:
-        outs = (torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
-                torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
-                torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
-                torch.zeros((n, 2), dtype=torch.float32, device="cuda"),
-                torch.zeros(n, dtype=_U32, device="cuda"))
-    ptr = lambda t: None if t is None else t.data_ptr()
-    _check(N.lib().rk_interpolate_data(_stream_ptr(stream), faces_id.data_ptr(), coords.data_ptr(), width, height,
-                                       F.data_ptr(), F.shape[0], V.data_ptr(), ptr(normals), ptr(colors),
-                                       ptr(texcoords), *(ptr(t) for t in outs)), "interpolate_data")
-    return outs
+    with _on(stream):
+        n = width * height
+        if outs is None:
+            outs = (torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                    torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                    torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+                    torch.zeros((n, 2), dtype=torch.float32, device="cuda"),
+                    torch.zeros(n, dtype=_U32, device="cuda"))
+        ptr = lambda t: None if t is None else t.data_ptr()
+        _check(N.lib().rk_interpolate_data(_stream_ptr(stream), faces_id.data_ptr(), coords.data_ptr(), width, height,
+                                           F.data_ptr(), F.shape[0], V.data_ptr(), ptr(normals), ptr(colors),
+                                           ptr(texcoords), *(ptr(t) for t in outs)), "interpolate_data")
+        return outs
 
 
 def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None,
                       ws: torch.Tensor | None = None):
     """rk_fast_trace_sorted: the same results as fast_trace, rays traced in
     origin-Morton order (for incoherent rays)."""
-    n = O.shape[0]
-    fid, uv, dep = out if out is not None else new_outputs(n)
-    need = N.lib().rk_fast_trace_sorted_workspace_size(n)
-    if ws is None or ws.numel() < need:
-        ws = workspace(need)
-    _check(N.lib().rk_fast_trace_sorted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
-                                        ws.data_ptr(), ws.numel()), "fast_trace_sorted")
-    return fid, uv, dep
+    with _on(stream):
+        n = O.shape[0]
+        fid, uv, dep = out if out is not None else new_outputs(n)
+        need = N.lib().rk_fast_trace_sorted_workspace_size(n)
+        if ws is None or ws.numel() < need:
+            ws = workspace(need)
+        _check(N.lib().rk_fast_trace_sorted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                            ws.data_ptr(), ws.numel()), "fast_trace_sorted")
+        return fid, uv, dep

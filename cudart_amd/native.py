@@ -141,6 +141,10 @@ _SIGS = {
     "rk_radix_sort_general": (_I, [_P, _P, _P, _U32, _U32, _I, _P, _SZ]),
     "rk_radix_sort": (_I, [_P, _P, _U32, _U32, _P, _SZ]),
     "rk_radix_sort_pairs_u64": (_I, [_P, _P, _P, _U32, _U32, _P, _SZ]),
+    "rk_bitonic_sort": (_I, [_P, _P, _U32, _P, _SZ]),
+    "rk_bitonic_sort_general": (_I, [_P, _P, _P, _U32, _P, _SZ]),
+    "rk_bubble_sort512": (_I, [_P, _P, _P, _U32, _I]),
+    "rk_workspace_flags": (_I, [_P, _P, C.POINTER(_U32)]),
     "rk_calc_morton_codes": (_I, [_P, _P, _U32, C.POINTER(AABox), _I, _P]),
     "rk_build_lbvh_workspace_size": (_SZ, [_U32]),
     "rk_build_lbvh": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, _P, _P, _P, _U32, _P, _P, _SZ]),

@@ -29,8 +29,9 @@ class CUDAMesh {
   // Direct construction from host arrays (w of vertices as given; hitbox = the
   // merge of every vertex as float4 with w = 0, as CUDAModel accumulates it).
   void buildFromArrays(const float4* host_vertices, uint vertex_count, const uint4* host_faces, uint face_count);
-  // Chunked .cdm file, CUDAMesh.cpp:134-400 (HITBOX, VERTICES, NORMALS, COLORS,
-  // TEXCOORDS, FACES chunks: int magic, int length, uint count, raw array).
+  // Chunked .cdm file, CUDAMesh.cpp:134-400: HITBOX (int magic, int length, aabox),
+  // then VERTICES, NORMALS, COLORS, TEXCOORDS, FACES (int magic, int length, uint
+  // count, raw array); unknown chunks are skipped by their length.
   void quickLoad(const char* path);
   void quickSave(const char* path, bool overwrite = true);
 

@@ -122,8 +122,16 @@ rk_status rk_reorder_mesh_faces(rk_stream stream, rk_uint4* faces, const uint32_
                                 size_t ws_bytes);
 
 /* ---- sorts ------------------------------------------------------------------ */
-/* Workspace for any sort below of 'length' elements with keys of key_bytes (4 or 8). */
+/* Workspace for any sort below of 'length' elements with keys of key_bytes (4 or 8).
+ * Every sort here accepts length < 2^30 (RK_ERROR_UNSUPPORTED beyond). The first 4
+ * bytes of the workspace receive the call's RK_FLAG_* bits (RK_FLAG_SPIN_LIMIT: a
+ * look-back wait of the radix pass hit its iteration limit and the output is not
+ * sorted); each call clears them first. Read them with rk_workspace_flags. */
 size_t rk_sort_workspace_size(uint32_t length, uint32_t key_bytes);
+/* Copies the RK_FLAG_* word at the start of 'ws' to *flags after waiting for
+ * 'stream' (no reference counterpart: the reference's sorts cannot fail silently
+ * because they cannot fail at all). */
+rk_status rk_workspace_flags(rk_stream stream, const void* ws, uint32_t* flags);
 /* void CUDA_radixSortGeneral(uint* keys, uint* indices, uint length, uint bit_depth, bool init_indices),
  * CUDA_buildMeshTreeLBVH.cu:29 / CUDA_radixSortGeneral.cu:397. Same tie order by size class:
  * length <= 512 stable; <= byte_depth*8192 bitonic order within 128-chunks then stable by key;
@@ -136,6 +144,23 @@ rk_status rk_radix_sort(rk_stream stream, uint32_t* values, uint32_t length, uin
 /* Stable (u64 key, u32 value) sort on the low 'bit_depth' bits (63-bit Morton keys; no reference counterpart). */
 rk_status rk_radix_sort_pairs_u64(rk_stream stream, uint64_t* keys, uint32_t* values, uint32_t length,
                                   uint32_t bit_depth, void* ws, size_t ws_bytes);
+/* template <typename T> void CUDA_bitonicSort(T* values, uint length),
+ * samples/09_how_to_use_a_sorter.cpp:39-40 / kernels/CUDA_bitonicSort.cu:402-518, instantiated
+ * for unsigned int only (:523-537). Sorts values ascending in place, any length. */
+rk_status rk_bitonic_sort(rk_stream stream, uint32_t* values, uint32_t length, void* ws, size_t ws_bytes);
+/* template <typename K, typename V> void CUDA_bitonicSortGeneral(K* keys, V* values, uint length),
+ * samples/09_how_to_use_a_sorter.cpp:42-43 / kernels/CUDA_bitonicSortGeneral.cu:504-690,
+ * instantiated for <uint, uint> only (:694-697). Sorts (key, value) pairs by key, any length,
+ * in the reference's tie order: a bitonic network inside each full 128-element chunk (on keys
+ * only), the partial last chunk stably, then stable merges. Where the reference's Merge2x
+ * writes past 'length' (BSG.cu:254) this writes nothing outside the arrays. */
+rk_status rk_bitonic_sort_general(rk_stream stream, uint32_t* keys, uint32_t* values, uint32_t length, void* ws,
+                                  size_t ws_bytes);
+/* void CUDA_bubbleSort512(uint* keys, uint* indices, uint length, bool init_indices = false),
+ * kernels/CUDA_radixSortGeneral.cu:27 / kernels/CUDA_bubbleSort512.cu:114-167. A stable sort of
+ * at most 512 (key, index) pairs in one workgroup; init_indices first sets indices[i] = i. Needs
+ * no workspace. length > 512: RK_ERROR_INVALID_VALUE (the reference asserts, :116). */
+rk_status rk_bubble_sort512(rk_stream stream, uint32_t* keys, uint32_t* indices, uint32_t length, int init_indices);
 
 /* ---- Morton codes (kernel_calcMortonCodes, CUDA_buildMeshTreeLBVH.cu:115-173) ---- */
 #define RK_MORTON_REF32 0 /* the reference 11/11/10-bit code */
@@ -225,7 +250,8 @@ rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float
  * then traced in that order, so rays of a wave start near each other and share
  * node and triangle fetches. Results are written to each ray's own slot and are
  * the same as rk_fast_trace's. Pays off when rays start scattered through the
- * scene (secondary or random rays); camera rays are coherent already. */
+ * scene (secondary or random rays); camera rays are coherent already. The first
+ * 4 bytes of ws receive the sort's RK_FLAG_* bits, as for the sorts above. */
 size_t rk_fast_trace_sorted_workspace_size(uint32_t n_rays);
 rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
@@ -326,7 +352,8 @@ rk_status rk_batch_build(rk_stream stream, const rk_batch_mesh* meshes, uint32_t
                          size_t ws_bytes);
 /* Closest hit of every ray of every mesh in that mesh's tree. faces_id receives
  * the sorted position (original batch face = bvh->indices[position]); depths is
- * read as tmax; a miss leaves the outputs unchanged. */
+ * read as tmax; a miss leaves the outputs unchanged. The first 4 bytes of ws
+ * receive the call's RK_FLAG_* bits (the ray sort of ray_order 1), as for the sorts. */
 size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ray_count);
 rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshes, uint32_t mesh_count, const rk_fast_bvh* bvh,
                          const rk_float4* origins, const rk_float4* directions, uint32_t ray_count, uint32_t* faces_id,

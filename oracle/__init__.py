@@ -28,6 +28,9 @@ _SIGS = {
     "ora_morton_wide": (None, [_P, _U, _P, _P, _I, _P]),
     "ora_sort_pairs_ref": (_I, [_P, _P, _U, _U, _I]),
     "ora_sort_pairs_u64": (None, [_P, _P, _U]),
+    "ora_bitonic_sort_general": (_I, [_P, _P, _U]),
+    "ora_bitonic_sort": (_I, [_P, _U]),
+    "ora_bubble_sort512": (_I, [_P, _P, _U, _I]),
     "ora_find_splits": (_U, [_P, _U, _P, _P]),
     "ora_build_lbvh_ref": (_I, [_P, _P, _U, _P, _U, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ora_refit_recursive": (None, [_P]),
@@ -99,6 +102,29 @@ def sort_pairs_u64(keys: np.ndarray, vals: np.ndarray):
     k = np.ascontiguousarray(keys, np.uint64).copy()
     v = np.ascontiguousarray(vals, np.uint32).copy()
     lib().ora_sort_pairs_u64(_p(k), _p(v), k.shape[0])
+    return k, v
+
+
+def bitonic_sort_general(keys: np.ndarray, vals: np.ndarray):
+    """CUDA_bitonicSortGeneral<uint, uint> at any length; returns keys, values
+    and whether the reference's Merge2x would write past the arrays."""
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    oob = lib().ora_bitonic_sort_general(_p(k), _p(v), k.shape[0])
+    return k, v, bool(oob)
+
+
+def bitonic_sort(values: np.ndarray):
+    v = np.ascontiguousarray(values, np.uint32).copy()
+    lib().ora_bitonic_sort(_p(v), v.shape[0])
+    return v
+
+
+def bubble_sort512(keys: np.ndarray, indices: np.ndarray, init: bool = False):
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(indices, np.uint32).copy()
+    if lib().ora_bubble_sort512(_p(k), _p(v), k.shape[0], int(init)) != 0:
+        raise ValueError("CUDA_bubbleSort512 takes at most 512 elements")
     return k, v
 
 

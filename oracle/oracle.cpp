@@ -278,6 +278,25 @@ ORA_API int ora_sort_pairs_ref(uint32_t* keys, uint32_t* vals, uint32_t n, uint3
   return 0;
 }
 
+// CUDA_bitonicSortGeneral<uint, uint> (BSG.cu:504-690) called directly, at any
+// length. Returns 1 when kernel_bitonicMerge2x would write past 'n'.
+ORA_API int ora_bitonic_sort_general(uint32_t* keys, uint32_t* vals, uint32_t n) {
+  return bitonic_general(keys, vals, n);
+}
+
+// CUDA_bitonicSort<uint> (BS.cu:402-518): the same networks and merges on the
+// values alone (BS.cu:35-136 is BSG.cu's chunk kernel without the indices).
+ORA_API int ora_bitonic_sort(uint32_t* values, uint32_t n) { return bitonic_general(values, nullptr, n); }
+
+// CUDA_bubbleSort512 (CUDA_bubbleSort512.cu:114-167): n <= 512, indices set to
+// 0..n-1 first when init_indices. Returns -1 for n > 512 (the reference asserts).
+ORA_API int ora_bubble_sort512(uint32_t* keys, uint32_t* indices, uint32_t n, int init_indices) {
+  if (n > 512) return -1;
+  if (init_indices) for (uint32_t i = 0; i < n; ++i) indices[i] = i;
+  if (n > 1) odd_even_sort(keys, indices, n);
+  return 0;
+}
+
 // Stable 64-bit key sort (the 63-bit Morton mode; no reference counterpart).
 ORA_API void ora_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint32_t n) {
   std::vector<uint32_t> perm(n);

@@ -44,6 +44,26 @@ def test_status_strings_and_errors(rk):
     assert L.rk_build_lbvh(None, None, None, 0, None, 1, 12, None, None, None, 0, None, None, 0) == 1
     assert L.rk_radix_sort_general(None, None, None, 10, 0, 0, None, 0) == 1
     assert L.rk_fast_build_workspace_size(1000, 1) > 0
+    # lengths of 2^30 or more are refused before anything is launched
+    big, fake = 1 << 30, 4096
+    assert L.rk_radix_sort(None, fake, big, 32, fake, 16) == 6
+    assert L.rk_radix_sort_general(None, fake, fake, big, 32, 1, fake, 16) == 6
+    assert L.rk_radix_sort_pairs_u64(None, fake, fake, big, 64, fake, 16) == 6
+    assert L.rk_bitonic_sort(None, fake, big, fake, 16) == 6
+    assert L.rk_bitonic_sort_general(None, fake, fake, big, fake, 16) == 6
+    assert L.rk_bubble_sort512(None, fake, fake, 513, 0) == 1
+
+
+def test_renderkit_exports_the_reference_sort_prototypes(rk):
+    """The C++ free functions as the reference's callers declare them
+    (samples/09_how_to_use_a_sorter.cpp:36-44, CUDA_radixSortGeneral.cu:27-30),
+    and CUDA_bitonicSortGeneral also in the form of its definition."""
+    out = subprocess.run(["nm", "-D", "--defined-only", str(rk.LIB_DIR / "librenderkit.so")], capture_output=True,
+                         text=True, check=True).stdout
+    for sym in ("_Z14CUDA_radixSortPjjj", "_Z21CUDA_radixSortGeneralPjS_jjb", "_Z16CUDA_bitonicSortIjEvPT_j",
+                "_Z23CUDA_bitonicSortGeneralIjjEvPT0_S1_j", "_Z23CUDA_bitonicSortGeneralIjjEvPT_PT0_j",
+                "_Z18CUDA_bubbleSort512PjS_jb"):
+        assert sym in out, sym
 
 
 def test_scene_generators_are_deterministic(rk):

@@ -707,3 +707,64 @@ def test_moller_trumbore_boundaries(ora, o, hit, tuv):
     assert h == hit
     if hit:
         np.testing.assert_array_equal(got, tuv)
+
+
+# ---- the standalone sorts (samples/09_how_to_use_a_sorter.cpp:36-44) ---------------
+
+def _batcher_rows(k, v):
+    """The Batcher bitonic network of _batcher_bitonic applied to every row of
+    (rows, 128) arrays at once."""
+    k, v = k.copy(), v.copy()
+    n = np.arange(128)
+    i = 2
+    while i <= 128:
+        j = i // 2
+        while j >= 1:
+            lo = n[(n ^ j) > n]
+            hi = lo ^ j
+            up = (lo & i) == 0
+            a, b = k[:, lo], k[:, hi]
+            sw = np.where(up, a > b, a < b)
+            kl, kh, vl, vh = a.copy(), b.copy(), v[:, lo].copy(), v[:, hi].copy()
+            k[:, lo], k[:, hi] = np.where(sw, kh, kl), np.where(sw, kl, kh)
+            v[:, lo], v[:, hi] = np.where(sw, vh, vl), np.where(sw, vl, vh)
+            j //= 2
+        i *= 2
+    return k, v
+
+
+@pytest.mark.parametrize("n", [40000, 300001, 1397121])
+def test_bitonic_general_any_length(ora, n):
+    """CUDA_bitonicSortGeneral called directly has no 32768 limit: the bucket
+    merges (CUDA_bitonicSortGeneral.cu:584-640) run until one bucket is left.
+    Independent formulation: network per full 128-chunk, stable partial chunk,
+    then a stable sort by key; the values follow the keys."""
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 11, n, dtype=np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, _ = ora.bitonic_sort_general(keys, vals)
+    full = n // 128 * 128
+    k2, v2 = keys.copy(), vals.copy()
+    rk_, rv_ = _batcher_rows(k2[:full].reshape(-1, 128), v2[:full].reshape(-1, 128))
+    k2[:full], v2[:full] = rk_.reshape(-1), rv_.reshape(-1)
+    tail = np.argsort(k2[full:], kind="stable")
+    k2[full:], v2[full:] = k2[full:][tail], v2[full:][tail]
+    order = np.argsort(k2, kind="stable")
+    np.testing.assert_array_equal(sk, k2[order])
+    np.testing.assert_array_equal(sv, v2[order])
+    np.testing.assert_array_equal(ora.bitonic_sort(keys), np.sort(keys))
+
+
+def test_bubble_sort512_stable_and_bounded(ora):
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 2, 77, 511, 512):
+        keys = rng.integers(0, 4, n, dtype=np.uint32)
+        idx = rng.integers(0, 1 << 31, n, dtype=np.uint32)
+        k, i = ora.bubble_sort512(keys, idx, False)
+        order = np.argsort(keys, kind="stable")
+        np.testing.assert_array_equal(k, keys[order])
+        np.testing.assert_array_equal(i, idx[order])
+        k, i = ora.bubble_sort512(keys, idx, True)
+        np.testing.assert_array_equal(i, order.astype(np.uint32))
+    with pytest.raises(ValueError):
+        ora.bubble_sort512(np.zeros(513, np.uint32), np.zeros(513, np.uint32))
