@@ -45,6 +45,10 @@ _SIGS = {
     "ora_reorder_faces": (None, [_P, _P, _U]),
     "ora_num_threads": (_I, []),
     "ora_interpolate": (None, [_P, _P, _U, _P, _U, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ora_camera_look_at": (None, [_P, _P, _P, _P]),
+    "ora_camera_set_angle": (None, [_P, _P]),
+    "ora_primary_ray_constants": (None, [_P, _P, C.c_float, C.c_float, _I, _I, _P]),
+    "ora_primary_rays": (None, [_P, _I, _I, _P, _P]),
 }
 
 NODE_DTYPE = np.dtype([("w0", "<u4"), ("w1", "<u4"), ("bmin", "<f4", 3), ("bmax", "<f4", 3)])
@@ -277,6 +281,51 @@ def interpolate(face_id, coords, F, V, normals=None, colors=None, texcoords=None
                           None if keep[1] is None else _p(keep[1]), None if keep[2] is None else _p(keep[2]),
                           _p(p), _p(nn), _p(c), _p(t), _p(m))
     return p, nn, c, t, m
+
+
+class Camera:
+    """The reference camera's state (CUDACamera.cpp:29-50 defaults) driven
+    through the oracle's restatement of setAngle, lookAt and the host part of
+    CUDA_calcPrimaryRays. Values are float32, as the reference keeps them."""
+
+    def __init__(self):
+        self.center = np.zeros(3, np.float32)
+        self.angle = np.array([np.float32(3.14159265359) * np.float32(0.5), 0, 0], np.float32)
+        self.dir = np.array([0, 0, 1], np.float32)
+        self.fov = np.float32(np.float32(3.14159265359) * np.float32(0.5))
+        self.ratio = np.float32(np.float32(4.0) / np.float32(3.0))
+
+    def set_center(self, x, y, z):
+        self.center = np.array([x, y, z], np.float32)
+
+    def set_angle(self, x, y, z):
+        self.angle = np.array([x, y, z], np.float32)
+        self.dir = np.zeros(3, np.float32)
+        lib().ora_camera_set_angle(_p(self.angle), _p(self.dir))
+
+    def look_at(self, x, y, z):
+        pt = np.array([x, y, z], np.float32)
+        self.dir = np.zeros(3, np.float32)
+        lib().ora_camera_look_at(_p(self.center), _p(pt), _p(self.angle), _p(self.dir))
+
+    def set_fov(self, fov):
+        self.fov = np.float32(fov)
+
+    def set_ratio(self, width, height=None):
+        self.ratio = np.float32(width) if height is None else np.float32(np.float32(width) / np.float32(height))
+
+    def ray_constants(self, width: int, height: int) -> np.ndarray:
+        out = np.zeros((4, 4), np.float32)
+        lib().ora_primary_ray_constants(_p(self.center), _p(self.angle), float(self.fov), float(self.ratio), width,
+                                        height, _p(out))
+        return out
+
+    def primary_rays(self, width: int, height: int):
+        cst = self.ray_constants(width, height)
+        O = np.zeros((width * height, 4), np.float32)
+        D = np.zeros((width * height, 4), np.float32)
+        lib().ora_primary_rays(_p(cst), width, height, _p(O), _p(D))
+        return O, D
 
 
 def num_threads() -> int:

@@ -34,6 +34,20 @@ MUT = [
     ("tri v<0 -> <=", "if (v < 0.0f || u + v > 1.0f) return false;", "if (v <= 0.0f || u + v > 1.0f) return false;"),
     ("tri u+v>1 -> >=", "if (v < 0.0f || u + v > 1.0f) return false;", "if (v < 0.0f || u + v >= 1.0f) return false;"),
     ("tri t>=0 -> >", "return t >= 0.0f;\n}\n\n// intersectBoxRay", "return t > 0.0f;\n}\n\n// intersectBoxRay"),
+    ("cam ray x step sign", "sub4(sub4(cst[1], muls(cst[2], (float)x))", "add4(sub4(cst[1], muls(cst[2], (float)x))"),
+    ("cam ray y step sign", "muls(cst[3], (float)y));", "muls(cst[3], -(float)y));"),
+    ("cam ratio", "float x = y * ratio;", "float x = y / ratio;"),
+    ("cam 2x step", "muls(muls(right, 2.0f * x)", "muls(muls(right, x)"),
+    ("cam pitch/yaw swap", "F4 camDir = {cosf(pitch) * cosf(yaw), sinf(yaw), sinf(pitch) * cosf(yaw), 0.0f};",
+     "F4 camDir = {cosf(yaw) * cosf(pitch), sinf(pitch), sinf(yaw) * cosf(pitch), 0.0f};"),
+    ("cam camUp roll", "F4 camUp = {cosf(pitch) * sinf(roll), cosf(roll), 0.0f, 0.0f};",
+     "F4 camUp = {0.0f, 1.0f, 0.0f, 0.0f};"),
+    ("cam up order", "F4 up = normalize4(cross4(right, camDir));", "F4 up = normalize4(cross4(camDir, right));"),
+    ("cam branch3 +PI", "else angle.x = acosf(-dir.x / cosf(-angle.y)) + CAM_PI;",
+     "else angle.x = acosf(-dir.x / cosf(-angle.y));"),
+    ("cam yaw clamp", "if (angle.y > CAM_PI * 0.5f - 0.001f) angle.y = CAM_PI * 0.5f - 0.001f;\n  if (dir.x",
+     "if (angle.y > CAM_PI * 0.5f - 0.01f) angle.y = CAM_PI * 0.5f - 0.01f;\n  if (dir.x"),
+    ("cam dirFromAngle z", "dir.z = sinf(angle.x) * cosf(angle.y);", "dir.z = cosf(angle.x) * sinf(angle.y);"),
 ]
 only = sys.argv[1:]
 base = Path(tempfile.mkdtemp(prefix="oracle_mut_"))

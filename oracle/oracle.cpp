@@ -22,7 +22,8 @@
 //     not take part in the block-wide hit vote;
 //   - SAH node numbering (SAH.cu:625): breadth-first, children allocated in
 //     queue order;
-//   - normalize() uses 1.0f/sqrtf, not the device rsqrtf (types/_cuda_float4.h:181-186).
+//   - normalize() uses 1.0f/sqrtf, not the device rsqrtf (types/_cuda_float4.h:181-186);
+//   - the camera's trigonometry is the host C library's sinf/cosf/tanf/asinf/acosf.
 
 #include <algorithm>
 #include <cmath>
@@ -1048,3 +1049,98 @@ ORA_API void ora_reorder_faces(U4* faces, const uint32_t* indices, uint32_t n) {
 }
 
 ORA_API int ora_num_threads(void) { return nthreads(); }
+
+// ---- camera and primary rays --------------------------------------------------
+// CUDACamera state is (center, angle, dir, fov, ratio); angle.x = pitch (the
+// azimuth in the xz plane), angle.y = yaw (the elevation towards +y), angle.z =
+// roll (CUDACamera.h:235-247). PI is the reference's 3.14159265359f
+// (CUDACommon.h:108). CUDACamera.cpp is host code, so its normalize() is
+// v * (1.0f / sqrtf(dot)) (types/_cuda_float.h:21-24, types/_cuda_float3.h:203-208)
+// with dot = x*x + y*y + z*z in order.
+// DEFINED: sinf/cosf/tanf/asinf/acosf are the host C library's (the reference
+// calls its compiler's host math library; no result of the reference pins them).
+namespace {
+const float CAM_PI = 3.14159265359f;
+
+inline F3 normalize3(F3 v) {
+  float inv_len = 1.0f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  return {v.x * inv_len, v.y * inv_len, v.z * inv_len};
+}
+
+// dirFromAngle, CUDACamera.cpp:56-68 (clamps angle in place, then dir).
+void cam_dir_from_angle(F3& angle, F3& dir) {
+  if (angle.x < 0.0f) angle.x += CAM_PI * 2.0f;
+  if (angle.x > CAM_PI * 2.0f) angle.x -= CAM_PI * 2.0f;
+  if (angle.y < -CAM_PI * 0.5f + 0.001f) angle.y = -CAM_PI * 0.5f + 0.001f;
+  if (angle.y > CAM_PI * 0.5f - 0.001f) angle.y = CAM_PI * 0.5f - 0.001f;
+  dir.x = cosf(angle.x) * cosf(angle.y);
+  dir.y = sinf(angle.y);
+  dir.z = sinf(angle.x) * cosf(angle.y);
+  dir = normalize3(dir);
+}
+
+// angleFromDir, CUDACamera.cpp:70-88.
+void cam_angle_from_dir(F3& angle, F3& dir) {
+  dir = normalize3(dir);
+  angle.y = asinf(dir.y);
+  if (angle.y < -CAM_PI * 0.5f + 0.001f) angle.y = -CAM_PI * 0.5f + 0.001f;
+  if (angle.y > CAM_PI * 0.5f - 0.001f) angle.y = CAM_PI * 0.5f - 0.001f;
+  if (dir.x > 0) angle.x = asinf(dir.z / cosf(angle.y));
+  else if (dir.z > 0) angle.x = acosf(dir.x / cosf(angle.y));
+  else angle.x = acosf(-dir.x / cosf(-angle.y)) + CAM_PI;
+  if (angle.x < 0.0f) angle.x += CAM_PI * 2.0f;
+  if (angle.x > CAM_PI * 2.0f) angle.x -= CAM_PI * 2.0f;
+}
+}  // namespace
+
+// CUDACamera::lookAt, CUDACamera.cpp:119-135: dir = point - center, then
+// angleFromDir. io_angle holds the camera's angle before the call (angle.z,
+// the roll, is kept). Writes the new angle and dir.
+ORA_API void ora_camera_look_at(const float* center, const float* point, float* io_angle, float* dir_out) {
+  F3 angle = {io_angle[0], io_angle[1], io_angle[2]};
+  F3 dir = {point[0] - center[0], point[1] - center[1], point[2] - center[2]};
+  cam_angle_from_dir(angle, dir);
+  io_angle[0] = angle.x; io_angle[1] = angle.y; io_angle[2] = angle.z;
+  dir_out[0] = dir.x; dir_out[1] = dir.y; dir_out[2] = dir.z;
+}
+
+// CUDACamera::setAngle, CUDACamera.cpp:101-117: angle, then dirFromAngle.
+ORA_API void ora_camera_set_angle(float* io_angle, float* dir_out) {
+  F3 angle = {io_angle[0], io_angle[1], io_angle[2]};
+  F3 dir = {0.0f, 0.0f, 0.0f};
+  cam_dir_from_angle(angle, dir);
+  io_angle[0] = angle.x; io_angle[1] = angle.y; io_angle[2] = angle.z;
+  dir_out[0] = dir.x; dir_out[1] = dir.y; dir_out[2] = dir.z;
+}
+
+// CUDA_calcPrimaryRays host part, CUDA_calcPrimaryRays.cu:43-55, on float4
+// (w = 0 throughout: make_float4(float3) and cross() set w = 0). The float4
+// normalize is v * (1.0f / sqrtf(dot)) (DEFINED, as above); float4 / float is
+// a * (1.0f / s) (types/_cuda_float4.h:123-128); scalar * float4 multiplies each
+// component by the scalar (:109-112). out = {rayOrig, rayDir, rayX, rayY}.
+ORA_API void ora_primary_ray_constants(const float* center, const float* angle, float fov, float ratio, int width,
+                                       int height, F4* out) {
+  float pitch = angle[0], yaw = angle[1], roll = angle[2];
+  float y = tanf(fov * 0.5f);
+  float x = y * ratio;
+  F4 camDir = {cosf(pitch) * cosf(yaw), sinf(yaw), sinf(pitch) * cosf(yaw), 0.0f};
+  F4 camUp = {cosf(pitch) * sinf(roll), cosf(roll), 0.0f, 0.0f};
+  F4 right = normalize4(cross4(camDir, camUp));
+  F4 up = normalize4(cross4(right, camDir));
+  out[0] = {center[0], center[1], center[2], 0.0f};
+  out[1] = add4(add4(muls(up, y), muls(right, x)), camDir);
+  out[2] = muls(muls(right, 2.0f * x), 1.0f / (float)width);
+  out[3] = muls(muls(up, 2.0f * y), 1.0f / (float)height);
+}
+
+// kernel_calcPrimaryRays, CUDA_calcPrimaryRays.cu:20-32: pixel (x, y) at
+// n = y * width + x gets origin rayOrig and direction
+// rayDir - (float)x * rayX - (float)y * rayY, evaluated left to right.
+ORA_API void ora_primary_rays(const F4* cst, int width, int height, F4* origins, F4* directions) {
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      size_t n = (size_t)y * width + x;
+      origins[n] = cst[0];
+      directions[n] = sub4(sub4(cst[1], muls(cst[2], (float)x)), muls(cst[3], (float)y));
+    }
+}

@@ -5,6 +5,7 @@ import subprocess
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -92,3 +93,22 @@ def test_camera_matches_reference_convention(rk):
     c = D[128 * 256 + 128, :3]
     assert c[2] < -0.99 and abs(c[0]) < 1e-2 and abs(c[1]) < 1e-2
     assert D[128 * 256, 1] == 0.0  # row 128: y * rayY cancels the up term exactly
+
+
+@pytest.mark.parametrize("eye,tgt,w,h", [((0, 0, 3), (0, 0, 0), 256, 256), ((1.2, 0.7, -2.5), (0.1, -0.3, 0.4), 100, 70),
+                                         ((-4, 1, 0.5), (0, 0, 0), 33, 17)])
+def test_host_camera_equals_oracle(rk, ora, eye, tgt, w, h):
+    """The library's look-at camera and host ray generator give the oracle's
+    restatement of CUDACamera::lookAt + CUDA_calcPrimaryRays bit for bit."""
+    fov = np.float32(60.0) * np.float32(0.01745329252)
+    cam = rk.camera_lookat(eye, tgt, float(fov), w, h)
+    c = ora.Camera()
+    c.set_center(*eye)
+    c.look_at(*tgt)
+    c.set_fov(fov)
+    c.set_ratio(w, h)
+    np.testing.assert_array_equal(cam, c.ray_constants(w, h))
+    O, D = rk.primary_rays_host(cam, w, h)
+    Oo, Do = c.primary_rays(w, h)
+    np.testing.assert_array_equal(O, Oo)
+    np.testing.assert_array_equal(D, Do)

@@ -49,11 +49,16 @@ def test_sample02_dropin_matches_oracle(rk, ora, tmp_path, sub, w, h):
     pf, puv, pd = (np.frombuffer(ch[i], t) for i, t in ((3, np.uint32), (4, np.float32), (5, np.float32)))
     O = np.frombuffer(ch[6], np.float32).reshape(-1, 4).copy()
     D = np.frombuffer(ch[7], np.float32).reshape(-1, 4).copy()
-    # the drop-in camera gives the reference frame (camera at z = 3 looking at the origin)
-    cam = rk.camera_lookat([0, 0, 3], [0, 0, 0], 60 * 0.01745329252, w, h)
-    Oh, Dh = rk.primary_rays_host(cam, w, h)
-    np.testing.assert_array_equal(O, Oh)
-    np.testing.assert_allclose(D, Dh, rtol=0, atol=2e-6)  # device vs host evaluation of (d - x*rx) - y*ry
+    # the drop-in camera and CUDA_calcPrimaryRays give the oracle's rays bit for bit
+    # (setCenter(0, 0, 3), lookAt(0, 0, 0), setFOV(60 * DEG2RAD), setRatio(w, h))
+    cam = ora.Camera()
+    cam.set_center(0, 0, 3)
+    cam.look_at(0, 0, 0)
+    cam.set_fov(np.float32(60.0) * np.float32(0.01745329252))
+    cam.set_ratio(w, h)
+    Oo, Do = cam.primary_rays(w, h)
+    np.testing.assert_array_equal(O, Oo)
+    np.testing.assert_array_equal(D, Do)
     of, ouv, od, _ = ora.trace_ref_packet(ref.nodes, ref.faces, V, O, D, w, h)
     np.testing.assert_array_equal(pf, of)
     np.testing.assert_array_equal(puv.reshape(-1, 2), ouv)

@@ -768,3 +768,126 @@ def test_bubble_sort512_stable_and_bounded(ora):
         np.testing.assert_array_equal(i, order.astype(np.uint32))
     with pytest.raises(ValueError):
         ora.bubble_sort512(np.zeros(513, np.uint32), np.zeros(513, np.uint32))
+
+
+# ---- camera and primary rays (CUDACamera.cpp:29-142, CUDA_calcPrimaryRays.cu:20-55) ----
+
+PI32 = np.float32(3.14159265359)
+
+
+def test_camera_axis_frame_exact(ora):
+    """angle (0, 0, 0): camDir = (1, 0, 0), camUp = (0, 1, 0), so right =
+    (0, 0, 1) and up = (0, 1, 0) exactly. fov = PI/2 gives tan(fov/2) = 1.0f, so
+    with ratio 2 on a 4 x 2 surface: rayDir = up + 2 right + camDir = (1, 1, 2),
+    rayX = 2*2*right/4 = (0, 0, 1), rayY = 2*1*up/2 = (0, 1, 0); pixel (x, y)
+    looks along (1, 1 - y, 2 - x)."""
+    c = ora.Camera()
+    c.set_center(0.5, -2.0, 7.0)
+    c.set_angle(0, 0, 0)
+    np.testing.assert_array_equal(c.dir, [1, 0, 0])
+    c.set_fov(PI32 * np.float32(0.5))
+    c.set_ratio(2.0)
+    cst = c.ray_constants(4, 2)
+    np.testing.assert_array_equal(cst, [[0.5, -2, 7, 0], [1, 1, 2, 0], [0, 0, 1, 0], [0, 1, 0, 0]])
+    O, D = c.primary_rays(4, 2)
+    want = np.array([[1, 1 - y, 2 - x, 0] for y in range(2) for x in range(4)], np.float32)
+    np.testing.assert_array_equal(D, want)
+    np.testing.assert_array_equal(O, np.tile([0.5, -2, 7, 0], (8, 1)).astype(np.float32))
+
+
+def test_camera_roll_pi_flips_the_frame(ora):
+    """roll = PI: camUp = (cos 0 * sin PI, cos PI, 0) = (-8.7e-8, -1, 0), so
+    right = (0, 0, -1) and up = (0, -1, 0) exactly: rayDir = (1, -1, -2),
+    rayX = (0, 0, -1), rayY = (0, -1, 0) on the same 4 x 2 surface."""
+    c = ora.Camera()
+    c.set_angle(0, 0, PI32)
+    c.set_fov(PI32 * np.float32(0.5))
+    c.set_ratio(4, 2)
+    np.testing.assert_array_equal(c.ray_constants(4, 2), [[0, 0, 0, 0], [1, -1, -2, 0], [0, 0, -1, 0],
+                                                          [0, -1, 0, 0]])
+
+
+def test_camera_defaults(ora):
+    """CUDACamera.cpp:29-50: fov PI/2, ratio 4/3, center 0, angle (PI/2, 0, 0),
+    dir (0, 0, 1)."""
+    c = ora.Camera()
+    assert c.fov == PI32 * np.float32(0.5) and c.ratio == np.float32(4) / np.float32(3)
+    np.testing.assert_array_equal(c.angle, [PI32 * np.float32(0.5), 0, 0])
+    np.testing.assert_array_equal(c.dir, [0, 0, 1])
+
+
+@pytest.mark.parametrize("point,angle_x,dir_", [
+    ((2, 0, 0), np.float32(0), (1, 0, 0)),                              # dir.x > 0: asin(z / cos y)
+    ((0, 0, 5), np.float32(np.arccos(np.float32(0))), (0, 0, 1)),        # dir.z > 0: acos(x / cos y)
+    ((-3, 0, 0), PI32, (-1, 0, 0)),                                      # else: acos(-x / cos -y) + PI
+    ((0, 0, -1), np.float32(np.float32(np.arccos(np.float32(0))) + PI32), (0, 0, -1)),
+])
+def test_camera_look_at_branches(ora, point, angle_x, dir_):
+    """angleFromDir (CUDACamera.cpp:70-88): the three branches of the azimuth;
+    the roll (angle.z) is kept."""
+    c = ora.Camera()
+    c.set_angle(0, 0, 0.25)
+    c.look_at(*point)
+    assert c.angle[0] == angle_x and c.angle[1] == 0 and c.angle[2] == np.float32(0.25)
+    np.testing.assert_array_equal(c.dir, dir_)
+
+
+def test_camera_look_at_pole_clamps_yaw(ora):
+    """Looking straight up: yaw = asin(1) is clamped to PI/2 - 0.001 (in float),
+    and dir.x = dir.z = 0 takes the third branch: acos(-0) + PI."""
+    c = ora.Camera()
+    c.look_at(0, 4, 0)
+    assert c.angle[1] == PI32 * np.float32(0.5) - np.float32(0.001)
+    assert c.angle[0] == np.float32(np.float32(np.arccos(np.float32(0))) + PI32)
+    c.look_at(0, -4, 0)
+    assert c.angle[1] == -PI32 * np.float32(0.5) + np.float32(0.001)
+
+
+def test_camera_frame_follows_look_at(ora):
+    """For any look direction off the poles, the camera's forward vector camDir
+    (rays of a 2 x 2 surface: rayDir - rayX - rayY) and setAngle(angle) both point
+    along normalize(point - center) within 5e-5, and right and up are
+    orthogonal to it. A wrong branch, a swapped pitch/yaw or a sign error turns
+    a direction around and fails this."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        ctr = rng.uniform(-5, 5, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if abs(d[1]) > 0.99:
+            continue
+        pt = (ctr + rng.uniform(0.5, 4) * d).astype(np.float32)
+        c = ora.Camera()
+        c.set_center(*ctr)
+        c.look_at(*pt)
+        want = (pt.astype(np.float64) - ctr) / np.linalg.norm(pt.astype(np.float64) - ctr)
+        np.testing.assert_allclose(c.dir, want, atol=2e-6)
+        cst = c.ray_constants(2, 2).astype(np.float64)
+        fwd = cst[1, :3] - cst[2, :3] - cst[3, :3]
+        np.testing.assert_allclose(fwd, want, atol=5e-5)  # float asin/acos of the azimuth amplify rounding
+        assert abs(cst[2, :3] @ want) < 1e-4 and abs(cst[3, :3] @ want) < 1e-4
+        # pixel (0, 0) is up and to the right of centre: +up, +right (y * up + x * right + camDir)
+        assert cst[3, 1] > 0
+        a = c.angle.copy()
+        c2 = ora.Camera()
+        c2.set_angle(*a)
+        np.testing.assert_allclose(c2.dir, want, atol=5e-5)
+
+
+def test_camera_c1_frame(ora):
+    """The icosphere camera (centre (0, 0, 3), looking at the origin, 60 degrees,
+    256 x 256): azimuth 3 PI / 2 in float, right = (1, 0, ~0), up = (0, 1, 0),
+    and the steps are 2 tan(30 deg) / 256 apart."""
+    c = ora.Camera()
+    c.set_center(0, 0, 3)
+    c.look_at(0, 0, 0)
+    c.set_fov(np.float32(60) * np.float32(0.01745329252))
+    c.set_ratio(256, 256)
+    assert c.angle[0] == np.float32(np.float32(np.arccos(np.float32(0))) + PI32)
+    cst = c.ray_constants(256, 256)
+    t = np.tan(np.float32(np.float32(60) * np.float32(0.01745329252)) * np.float32(0.5))
+    np.testing.assert_allclose(cst[1], [t, t, -1, 0], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(cst[2], [2 * t / 256, 0, 0, 0], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(cst[3], [0, 2 * t / 256, 0, 0], rtol=1e-6, atol=1e-9)
+    O, D = c.primary_rays(256, 256)
+    assert D[128 * 256 + 5, 1] == 0  # row 128 has dir.y == 0 exactly
