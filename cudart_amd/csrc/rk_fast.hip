@@ -23,6 +23,8 @@
 // positions [l, r] is record r when it is its parent's left child and record
 // l when it is the right child, so the children of the split between p and
 // p + 1 are records p and p + 1. n records for n faces.
+#include <mutex>
+
 #include "rk_device.h"
 
 template <typename K>
@@ -973,6 +975,48 @@ extern "C" rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces
   if (!sorted_keys) return RK_ERROR_INVALID_VALUE;
   return fast_build_common(stream, faces, vertices, face_count, hitbox, morton_mode, leaf_max, bvh, info,
                            sorted_keys, ws, ws_bytes);
+}
+
+// ---- rk_fast_build without a caller workspace ------------------------------------
+// The workspace of rk_fast_build (keys, hierarchy slots and counters, sort
+// buffers and look-back status) lives in one file-scope device array. Every
+// device the library runs on holds its own copy until the process ends. Host
+// code never passes the array's name to a kernel: the device address comes
+// from hipGetSymbolAddress, once per device, on that device's first call.
+namespace {
+constexpr size_t kNoAllocScratchBytes = RK_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+rk_status scratch_for_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return RK_ERROR_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return RK_ERROR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RK_ERROR_DEVICE;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RK_SUCCESS;
+}
+}  // namespace
+
+extern "C" rk_status rk_fast_build_noalloc(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                           uint32_t face_count, const rk_aabox* hitbox, int morton_mode,
+                                           uint32_t leaf_max, const rk_fast_bvh* bvh, uint32_t* info) {
+  if (morton_mode != RK_MORTON_30 && morton_mode != RK_MORTON_63) return RK_ERROR_INVALID_VALUE;
+  if (face_count >= (1u << 27) || rk_fast_build_workspace_size(face_count, morton_mode) > kNoAllocScratchBytes)
+    return RK_ERROR_WORKSPACE;
+  void* ws = nullptr;
+  rk_status s = scratch_for_current_device(&ws);
+  if (s != RK_SUCCESS) return s;
+  return fast_build_common(stream, faces, vertices, face_count, hitbox, morton_mode, leaf_max, bvh, info, nullptr, ws,
+                           kNoAllocScratchBytes);
 }
 
 static rk_status fast_refit(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,

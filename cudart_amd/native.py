@@ -153,6 +153,7 @@ _SIGS = {
     "rk_raytrace_mesh_tree": (_I, [_P, _P, _U32, _P, _U32, _P, _P, _P, _I, _I, _P, _P, _P, _I]),
     "rk_fast_build_workspace_size": (_SZ, [_U32, _I]),
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
+    "rk_fast_build_noalloc": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P]),
     "rk_fast_trace": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P]),
     "rk_fast_trace_counted": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _P]),
     "rk_fast_trace_sorted_workspace_size": (_SZ, [_U32]),

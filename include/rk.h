@@ -14,8 +14,9 @@
  *    returns without waiting for the GPU, except where noted.
  *  - Temporaries come from a caller-provided workspace ('ws', 'ws_bytes'), sized
  *    by the matching *_workspace_size function. The library keeps no global
- *    device state, so calls on different streams with different workspaces are
- *    independent and may run concurrently.
+ *    device state (one exception: rk_fast_build_noalloc's own array), so calls
+ *    on different streams with different workspaces are independent and may run
+ *    concurrently.
  *  - Errors are returned as rk_status; nothing exits the process.
  */
 #ifndef RK_H
@@ -267,6 +268,22 @@ rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const 
 rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t face_count,
                               const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max, const rk_fast_bvh* bvh,
                               uint32_t* info, void* sorted_keys, void* ws, size_t ws_bytes);
+/* rk_fast_build with no workspace argument, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc). No counterpart in the reference. It calls no allocator: the
+ * workspace is a file-scope device array of RK_NOALLOC_SCRATCH_BYTES (128 MiB),
+ * one per device, held until the process ends. face_count is limited to what
+ * fits it, rk_fast_build_workspace_size(face_count, morton_mode) <=
+ * RK_NOALLOC_SCRATCH_BYTES: 2,556,064 faces with RK_MORTON_30 and 2,199,856
+ * with RK_MORTON_63; larger counts return RK_ERROR_WORKSPACE. Everything the
+ * build reads from the array is cleared by the build itself. All calls on one
+ * device share that array, so calls that overlap on one device (concurrent
+ * streams, threads) race: order them on one stream. Same results as
+ * rk_fast_build. */
+#define RK_NOALLOC_SCRATCH_BYTES (128ull << 20)
+rk_status rk_fast_build_noalloc(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices,
+                                uint32_t face_count, const rk_aabox* hitbox, int morton_mode, uint32_t leaf_max,
+                                const rk_fast_bvh* bvh, uint32_t* info);
 /* Refit after the vertices moved: keeps the topology of the rk_fast_build_keyed
  * that produced sorted_keys and bvh->indices (same faces, same morton_mode and
  * leaf_max) and recomputes every node box and triangle record from 'vertices'.
