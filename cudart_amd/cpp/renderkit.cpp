@@ -409,6 +409,134 @@ void CUDACamera::rotate(float x, float y, float z) {
   dirty = true;
 }
 
+void CUDACamera::moveForward(float k) {  // CUDACamera.cpp:163-170
+  center.x += k * dir.x;
+  center.y += k * dir.y;
+  center.z += k * dir.z;
+  dirty = true;
+}
+void CUDACamera::moveBackward(float k) {  // :172-179
+  center.x -= k * dir.x;
+  center.y -= k * dir.y;
+  center.z -= k * dir.z;
+  dirty = true;
+}
+void CUDACamera::moveLeft(float k) {  // :181-187
+  center.x -= k * dir.z;
+  center.z += k * dir.x;
+  dirty = true;
+}
+void CUDACamera::moveRight(float k) {  // :189-195
+  center.x += k * dir.z;
+  center.z -= k * dir.x;
+  dirty = true;
+}
+
+namespace {
+float4x4 identity4() {
+  float4x4 m = {};
+  for (int i = 0; i < 4; ++i) m.m[i][i] = 1.0f;
+  return m;
+}
+// Rotation by 'angle' about 'axis' for row vectors (p' = p * M); identity for
+// a zero angle or a zero axis.
+float4x4 rotation_axis(float3 axis, float angle) {
+  const float len2 = dot(axis, axis);
+  if (angle == 0.0f || !(len2 > 0.0f)) return identity4();
+  const float3 a = normalize(axis);
+  const float c = cosf(angle), s = sinf(angle), t = 1.0f - c;
+  float4x4 m = identity4();
+  m.m[0][0] = t * a.x * a.x + c;       m.m[0][1] = t * a.x * a.y + s * a.z; m.m[0][2] = t * a.x * a.z - s * a.y;
+  m.m[1][0] = t * a.x * a.y - s * a.z; m.m[1][1] = t * a.y * a.y + c;       m.m[1][2] = t * a.y * a.z + s * a.x;
+  m.m[2][0] = t * a.x * a.z + s * a.y; m.m[2][1] = t * a.y * a.z - s * a.x; m.m[2][2] = t * a.z * a.z + c;
+  return m;
+}
+float3 transform_point(const float4x4& m, float3 p) {  // (p, 1) * M, w dropped
+  return make_float3(p.x * m.m[0][0] + p.y * m.m[1][0] + p.z * m.m[2][0] + m.m[3][0],
+                     p.x * m.m[0][1] + p.y * m.m[1][1] + p.z * m.m[2][1] + m.m[3][1],
+                     p.x * m.m[0][2] + p.y * m.m[1][2] + p.z * m.m[2][2] + m.m[3][2]);
+}
+// Left-handed look-at: the eye goes to the origin, at - eye to +z.
+float4x4 look_at_lh(float3 eye, float3 at, float3 up) {
+  const float3 z = normalize(at - eye);
+  const float3 x = normalize(cross(up, z));
+  const float3 y = cross(z, x);
+  float4x4 m = {};
+  m.m[0][0] = x.x; m.m[0][1] = y.x; m.m[0][2] = z.x;
+  m.m[1][0] = x.y; m.m[1][1] = y.y; m.m[1][2] = z.y;
+  m.m[2][0] = x.z; m.m[2][1] = y.z; m.m[2][2] = z.z;
+  m.m[3][0] = -dot(x, eye); m.m[3][1] = -dot(y, eye); m.m[3][2] = -dot(z, eye);
+  m.m[3][3] = 1.0f;
+  return m;
+}
+// Left-handed perspective: vertical fov, aspect = width / height; view z in
+// [zn, zf] maps to depth [0, 1] after the divide by w = z.
+float4x4 perspective_fov_lh(float fovy, float aspect, float zn, float zf) {
+  const float ys = 1.0f / tanf(fovy * 0.5f), xs = ys / aspect;
+  float4x4 m = {};
+  m.m[0][0] = xs;
+  m.m[1][1] = ys;
+  m.m[2][2] = zf / (zf - zn);
+  m.m[2][3] = 1.0f;
+  m.m[3][2] = -zn * zf / (zf - zn);
+  return m;
+}
+// Gauss-Jordan inverse with partial pivoting, in double; a singular matrix
+// gives all-NaN.
+float4x4 inverse4(const float4x4& m) {
+  double a[4][8];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      a[i][j] = m.m[i][j];
+      a[i][j + 4] = i == j ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+    if (a[p][c] == 0.0) {
+      float4x4 bad;
+      for (auto& row : bad.m)
+        for (float& v : row) v = NAN;
+      return bad;
+    }
+    if (p != c)
+      for (int j = 0; j < 8; ++j) std::swap(a[p][j], a[c][j]);
+    const double inv = 1.0 / a[c][c];
+    for (int j = 0; j < 8; ++j) a[c][j] *= inv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      if (f != 0.0)
+        for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  float4x4 out;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out.m[i][j] = (float)a[i][j + 4];
+  return out;
+}
+}  // namespace
+
+// CUDACamera.cpp:244-263: the view looks from the centre along dir (at a
+// point 4999.999 away), with the up vector (0, 1, 0) turned by the roll about
+// the axis (dir.x, 0, dir.y), as the reference writes it.
+void CUDACamera::update() {
+  const float3 target = center + dir * 4999.999f;
+  const float3 axis = make_float3(dir.x, 0.0f, dir.y);
+  const float4x4 up_rot = rotation_axis(axis, angle.z);
+  const float3 up = transform_point(up_rot, make_float3(0.0f, 1.0f, 0.0f));
+  matView = look_at_lh(center, target, up);
+  matInvView = inverse4(matView);
+  matProj = perspective_fov_lh(fov, ratio, zNearPlane, zFarPlane);
+  matInvProj = inverse4(matProj);
+  dirty = false;
+}
+float4x4 CUDACamera::getViewMatrix() { if (dirty) update(); return matView; }
+float4x4 CUDACamera::getProjMatrix() { if (dirty) update(); return matProj; }
+float4x4 CUDACamera::getInvViewMatrix() { if (dirty) update(); return matInvView; }
+float4x4 CUDACamera::getInvProjMatrix() { if (dirty) update(); return matInvProj; }
+
 void CUDACamera::primaryRayConstants(int width, int height, rk_float4 out[4]) const {
   // CUDA_calcPrimaryRays.cu:43-55
   float y = tanf(fov * 0.5f);
@@ -479,11 +607,63 @@ void CUDAHeadlessRenderer::finalize() {
   storage = nullptr;
 }
 
-void CUDAEngine::update() {
-  double now = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  if (last >= 0) frame_time = now - last;
-  last = now;
+CUDAEngine::CUDAEngine() {  // CUDAEngine.cpp:30-54
+  const double t = getTime();
+  frame_time = cycle_time = total_time = t;
+}
+
+double CUDAEngine::getTime() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void CUDAEngine::update() {  // CUDAEngine.cpp:113-132
+  ++cycle_count;
   ++frames;
+  const double t = getTime();
+  frame_delta = t - frame_time;
+  frame_time = t;
+  if (frame_time - cycle_time >= 1.0) {
+    total_count += cycle_count;
+    average_framerate = (double)total_count / (frame_time - total_time);
+    current_framerate = (double)cycle_count / (frame_time - cycle_time);
+    cycle_time = frame_time;
+    cycle_count = 0;
+  }
+}
+
+// ---- CUDAWindow ----------------------------------------------------------------------
+
+CUDAWindow::CUDAWindow(CUDAEngine* _engine) : engine(_engine) { if (engine) engine->window = this; }
+CUDAWindow::~CUDAWindow() {
+  if (engine && engine->window == this) engine->window = nullptr;
+}
+void CUDAWindow::close() {
+  if (closed) return;
+  closed = true;
+  if (on_close) on_close();
+}
+bool CUDAHeadlessWindow::isActive() {
+  if (!closed && engine && engine->getFrameCount() >= frame_limit) close();
+  return !closed;
+}
+
+// ---- CUDAShader ------------------------------------------------------------------------
+
+CUDAShader::CUDAShader() { backColor = make_float4(0.227451f, 0.431373f, 0.647059f, 1.0f); }
+
+void CUDAShader::run(CUDARenderSurface<uint>* output, CUDABuffer* input, CUDACamera* camera) {
+  _assert(output && input && camera, __FILE__, __LINE__, "CUDAShader::run() : Invalid parameter.");
+  _assert(input->getOutputNormalSurface() != nullptr, __FILE__, __LINE__,
+          "CUDAShader::run() : The buffer has no normal surface (interpolateOutputSurfaces with normals).");
+  _assert(output->getWidth() == input->getWidth() && output->getHeight() == input->getHeight(), __FILE__, __LINE__,
+          "CUDAShader::run() : Output and buffer sizes differ.");
+  const float3 d = camera->getDir();
+  const rk_float4 back = {backColor.x, backColor.y, backColor.z, backColor.w};
+  RK_CALL(rk_shade(nullptr, input->getOutputDepthSurface()->getPointer(),
+                   reinterpret_cast<const rk_float4*>(input->getOutputNormalSurface()->getPointer()),
+                   input->getWidth(), input->getHeight(), back, rk_float4{d.x, d.y, d.z, 0.0f},
+                   output->getPointer(), output->getPitch() / (int)sizeof(uint)));
+  RK_CALL(rk_stream_synchronize(nullptr));
 }
 
 // ---- CUDARaytracer ------------------------------------------------------------------
@@ -499,6 +679,25 @@ CUDARaytracer::~CUDARaytracer() {
   delete internal_camera;
 }
 
+void CUDARaytracer::updateRenderSurface() {  // CUDARaytracer.cpp:110-121
+  CUDARenderer* renderer = engine->getRenderer();
+  renderer->lockSurface();
+  current_shader->run(renderer->getSurface(), current_buffer, current_camera);
+  renderer->unlockSurface();
+}
+
+void CUDARaytracer::saveRenderSurface(const char* path) {  // CUDARaytracer.cpp:94-108
+  (void)path;
+  _assert(false, __FILE__, __LINE__,
+          "CUDARaytracer::saveRenderSurface() : Function not available, invalid renderer type, CUDAD3DRenderer "
+          "needed.");
+}
+
+void CUDARaytracer::rasterizeMeshTree(CUDAMesh* mesh, CUDAMeshTree* tree) {
+  if (mesh->getVertexCount() == 0) return;  // CUDARaytracer.cpp:139
+  raytraceMeshTree(mesh, tree, true);
+}
+
 void CUDARaytracer::initialize() {  // CUDARaytracer.cpp:53-83
   int count = 0;
   rk_device_count(&count);
@@ -510,12 +709,17 @@ void CUDARaytracer::initialize() {  // CUDARaytracer.cpp:53-83
   delete internal_buffer;
   internal_buffer = new CUDABuffer(width, height);
   current_buffer = internal_buffer;
+  if (!internal_shader) internal_shader = new CUDAShader();
+  current_shader = internal_shader;
 }
 
 void CUDARaytracer::finalize() {
   if (current_buffer == internal_buffer) current_buffer = nullptr;
   delete internal_buffer;
   internal_buffer = nullptr;
+  if (current_shader == internal_shader) current_shader = nullptr;
+  delete internal_shader;
+  internal_shader = nullptr;
 }
 
 void CUDARaytracer::calcPrimaryRays() {

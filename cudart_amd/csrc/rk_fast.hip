@@ -1535,8 +1535,11 @@ __global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ f
 // mapped for the device), sets the counter back to zero for the next launch,
 // and then writes done_value to out[2]: the host sees the frame finished
 // without a runtime query.
+// out[0] = hash sum, out[1] = hits; with build_flags, out[3] = *build_flags;
+// out[2] = done_value last.
 __device__ __forceinline__ void finish_checksum(uint64_t bh, uint64_t bn, uint64_t* __restrict__ partials,
-                                                uint32_t* __restrict__ counter, uint64_t* out, uint64_t done_value) {
+                                                uint32_t* __restrict__ counter, uint64_t* out, uint64_t done_value,
+                                                const uint32_t* build_flags = nullptr) {
   __shared__ uint32_t s_last;
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = bh;
@@ -1563,6 +1566,7 @@ __device__ __forceinline__ void finish_checksum(uint64_t bh, uint64_t bn, uint64
   if (threadIdx.x == 0) {
     out[0] = s_h[0] + s_h[1] + s_h[2] + s_h[3];
     out[1] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+    if (build_flags) out[3] = __hip_atomic_load(build_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *counter = 0u;
     __threadfence_system();  // the total lands before the done word
     __hip_atomic_store(out + 2, done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1576,10 +1580,10 @@ __global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __rest
                                                           const uint32_t* __restrict__ indices, uint32_t n_faces,
                                                           uint64_t* __restrict__ partials,
                                                           uint32_t* __restrict__ counter, uint64_t* out,
-                                                          uint64_t done_value) {
+                                                          uint64_t done_value, const uint32_t* build_flags) {
   uint64_t bh, bn;
   block_checksum(faces_id, coords, depths, count, indices, n_faces, bh, bn);
-  finish_checksum(bh, bn, partials, counter, out, done_value);
+  finish_checksum(bh, bn, partials, counter, out, done_value, build_flags);
 }
 
 // Primary ray of pixel (x, y) generated in the trace kernel itself (the
@@ -1692,11 +1696,11 @@ extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_i
 rk_status rk_frame_checksum_to_host(hipStream_t st, const uint32_t* faces_id, const rk_float2* coords,
                                     const float* depths, uint32_t count, const uint32_t* indices,
                                     uint32_t face_count, uint64_t* partials, uint32_t* counter, uint64_t* out_host,
-                                    uint64_t done_value) {
+                                    uint64_t done_value, const uint32_t* build_flags) {
   if (!faces_id || !coords || !depths || !partials || !counter || !out_host) return RK_ERROR_INVALID_VALUE;
   hipLaunchKernelGGL(k_checksum_to_host, dim3(std::max(1u, std::min(blocks_for(count, 256), kSumBlocks))), dim3(256),
                      0, st, faces_id, reinterpret_cast<const float2*>(coords), depths, count, indices, face_count,
-                     partials, counter, out_host, done_value);
+                     partials, counter, out_host, done_value, build_flags);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

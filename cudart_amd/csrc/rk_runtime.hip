@@ -369,3 +369,51 @@ extern "C" rk_status rk_interpolate_data(rk_stream stream, const uint32_t* faces
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
+
+// ---- shading (kernel_shade, CUDAShaderKernel.cu:14-43) ---------------------------
+namespace {
+
+// make_pixel (kernels/CUDAShaderCommon.cuh): scale by 255, clamp to [0, 255]
+// with fminf/fmaxf, truncate, pack as A << 24 | R << 16 | G << 8 | B.
+__device__ __forceinline__ uint32_t make_pixel(float4 v) {
+  v = rk::muls4(v, 255.0f);
+  v = make_float4(fmaxf(fminf(v.x, 255.0f), 0.0f), fmaxf(fminf(v.y, 255.0f), 0.0f), fmaxf(fminf(v.z, 255.0f), 0.0f),
+                  fmaxf(fminf(v.w, 255.0f), 0.0f));
+  return (((uint32_t)v.x & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.z & 0xFFu) |
+         (((uint32_t)v.w & 0xFFu) << 24);
+}
+
+// One thread per pixel, in row order over the input surfaces (k = y * width +
+// x); the output row of pixel k starts at y * pitch.
+__global__ __launch_bounds__(256) void k_shade(const float* __restrict__ depths, const float4* __restrict__ normals,
+                                               int width, uint32_t count, int pitch, float4 back, float4 cam,
+                                               uint32_t* __restrict__ pixels) {
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t y = k / (uint32_t)width, x = k - y * (uint32_t)width;
+  uint32_t px;
+  if (depths[k] < RK_HUGE) {
+    float4 n = normals[k];
+    n.w = 0.0f;
+    const float c = -rk::dot4(cam, n);
+    px = make_pixel(make_float4(c, c, c, c));
+  } else {
+    px = make_pixel(back);
+  }
+  pixels[(size_t)y * (uint32_t)pitch + x] = px;
+}
+
+}  // namespace
+
+extern "C" rk_status rk_shade(rk_stream stream, const float* depths, const rk_float4* normals, int width, int height,
+                              rk_float4 back_color, rk_float4 cam_dir, uint32_t* pixels, int pitch) {
+  if (width <= 0 || height <= 0 || pitch < width || !depths || !normals || !pixels) return RK_ERROR_INVALID_VALUE;
+  const uint64_t count = (uint64_t)width * (uint64_t)height;
+  if (count > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_shade, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, rk_hip_stream(stream), depths,
+                     reinterpret_cast<const float4*>(normals), width, (uint32_t)count, pitch,
+                     make_float4(back_color.x, back_color.y, back_color.z, back_color.w),
+                     make_float4(cam_dir.x, cam_dir.y, cam_dir.z, 0.0f), pixels);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}

@@ -34,7 +34,7 @@
 rk_status rk_frame_checksum_to_host(hipStream_t st, const uint32_t* faces_id, const rk_float2* coords,
                                     const float* depths, uint32_t count, const uint32_t* indices,
                                     uint32_t face_count, uint64_t* partials, uint32_t* counter, uint64_t* out_host,
-                                    uint64_t done_value);
+                                    uint64_t done_value, const uint32_t* build_flags);
 rk_status rk_deform_and_clear(hipStream_t st, const rk_float4* base, rk_float4* out, uint32_t count, float amplitude,
                               float omega, float time, uint64_t seed, int along_z, uint32_t* clear,
                               uint32_t clear_n);
@@ -80,7 +80,7 @@ struct Slot {
   rk_float2* uv = nullptr;
   float* dep = nullptr;
   uint64_t* sum_parts = nullptr;   // per-workgroup shares of the checksum, then a zeroed counter
-  uint64_t* sum_host = nullptr;    // pinned, coherent: checksum, hits, then ticket + 1 once the frame is done
+  uint64_t* sum_host = nullptr;    // pinned, coherent: checksum, hits, ticket + 1 once the frame is done, build flags
   uint64_t* sum_host_dev = nullptr;  // the device's address of sum_host
   rk_float4 *rays_o = nullptr, *rays_d = nullptr;  // plain path: primary rays
   uint64_t* sum_dev = nullptr;                      // plain path: checksum in device memory
@@ -116,7 +116,8 @@ struct rk_server_st {
   std::vector<uint8_t> finished;
   uint64_t tick = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-  uint64_t rebuilds = 0, refits = 0, uploads = 0, evictions = 0;
+  uint64_t rebuilds = 0, refits = 0, uploads = 0, evictions = 0, failed_requests = 0;
+  rk_status failed = RK_SUCCESS;  // the first launch failure; the server admits nothing after it
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -208,10 +209,10 @@ struct rk_server_st {
 
   // The plain per-request path (rk_server_config.plain): what a caller of the
   // plain entry points does per frame, in the same order of results.
-  rk_status launch_plain(Slot& sl, Scene& s, const rk_request& r, uint64_t ticket, rk_result& res,
-                         rk_fast_bvh& bvh) {
+  rk_status launch_plain(Slot& sl, Scene& s, const rk_request& r, uint64_t ticket, rk_result& res) {
     rk_stream st = reinterpret_cast<rk_stream>(sl.st);
     rk_status e;
+    rk_fast_bvh bvh = {s.nodes, s.tris, s.indices, nullptr, s.nf, 0};
     if ((e = rk_deform_vertices(st, s.base, s.verts, s.nv, s.amplitude, cfg.omega, r.time, 0x5EED0000ull + r.scene,
                                 s.along_z)) != RK_SUCCESS)
       return e;
@@ -236,18 +237,34 @@ struct rk_server_st {
     if ((e = rk_fast_trace(st, &bvh, sl.rays_o, sl.rays_d, n, fid, uv, dep, &opt)) != RK_SUCCESS) return e;
     if ((e = rk_frame_checksum(st, fid, uv, dep, n, s.indices, s.nf, sl.sum_dev)) != RK_SUCCESS) return e;
     RK_HIP(hipMemcpyAsync(sl.sum_host, sl.sum_dev, 16, hipMemcpyDeviceToHost, sl.st));
+    uint32_t flags = 0;
+    RK_HIP(hipMemcpyAsync(&flags, s.info + RK_INFO_FLAGS, 4, hipMemcpyDeviceToHost, sl.st));
     RK_HIP(hipStreamSynchronize(sl.st));  // one request at a time, finished before the next
+    sl.sum_host[3] = flags;
     sl.sum_host[2] = ticket + 1;
     return RK_SUCCESS;
   }
 
   rk_status launch(Slot& sl, int slot_index, const rk_request& r, uint64_t ticket, uint32_t evicted) {
     Scene& s = scenes[r.scene];
+    rk_status e = enqueue(sl, slot_index, s, r, ticket, evicted);
+    if (e != RK_SUCCESS) {
+      // work for this scene may be partly enqueued: its device copy is no
+      // longer trusted, so a later request uploads and rebuilds it
+      s.uploaded = false;
+      s.built = false;
+      failed = e;
+      return e;
+    }
+    mark_running(sl, s, slot_index, ticket);
+    return RK_SUCCESS;
+  }
+
+  rk_status enqueue(Slot& sl, int slot_index, Scene& s, const rk_request& r, uint64_t ticket, uint32_t evicted) {
     rk_result& res = results[ticket];
     res.start = now();
     res.evictions = evicted;
     res.slot = (uint32_t)slot_index;
-    rk_stream st = reinterpret_cast<rk_stream>(sl.st);
     if (s.in_use > 0 && s.last_slot >= 0 && s.last_slot != slot_index) {
       // the scene's latest request still runs on another slot: mark its end
       // there (that slot holds nothing else) and wait for it
@@ -266,13 +283,15 @@ struct rk_server_st {
       res.uploaded = 1;
       ++uploads;
     }
+    return cfg.plain ? launch_plain(sl, s, r, ticket, res) : launch_loop(sl, s, r, ticket, res);
+  }
+
+  // The request loop's work for one request: deform, refit or rebuild, trace
+  // the camera frame, checksum (with the build's flags) to host memory.
+  rk_status launch_loop(Slot& sl, Scene& s, const rk_request& r, uint64_t ticket, rk_result& res) {
+    rk_stream st = reinterpret_cast<rk_stream>(sl.st);
     rk_status e;
     rk_fast_bvh bvh = {s.nodes, s.tris, s.indices, nullptr, s.nf, 0};
-    if (cfg.plain) {
-      if ((e = launch_plain(sl, s, r, ticket, res, bvh)) != RK_SUCCESS) return e;
-      mark_running(sl, s, slot_index, ticket);
-      return RK_SUCCESS;
-    }
     const bool force_rebuild = (r.flags & RK_REQUEST_FORCE_REBUILD) != 0;
     const bool force_refit = (r.flags & RK_REQUEST_FORCE_REFIT) != 0;
     const bool rebuild = !s.built || force_rebuild ||
@@ -285,9 +304,11 @@ struct rk_server_st {
                                  0x5EED0000ull + r.scene, s.along_z, counters, rebuild ? 0u : s.nf)) != RK_SUCCESS)
       return e;
     if (rebuild) {
+      s.built = false;
       e = rk_fast_build_keyed(st, s.faces, s.verts, s.nf, &s.hitbox, RK_MORTON_30, cfg.leaf_max, &bvh, s.info, s.keys,
                               sl.ws, sl.ws_bytes);
-      s.built = true;
+      if (e != RK_SUCCESS) return e;
+      s.built = true;  // until the frame's checksum reports the build's flags
       s.refits = 0;
       ++rebuilds;
     } else {
@@ -307,9 +328,8 @@ struct rk_server_st {
       return e;
     if ((e = rk_frame_checksum_to_host(sl.st, fid, uv, dep, r.width * r.height, s.indices, s.nf, sl.sum_parts,
                                        reinterpret_cast<uint32_t*>(sl.sum_parts + kSumParts), sl.sum_host_dev,
-                                       ticket + 1)) != RK_SUCCESS)
+                                       ticket + 1, s.info + RK_INFO_FLAGS)) != RK_SUCCESS)
       return e;
-    mark_running(sl, s, slot_index, ticket);
     return RK_SUCCESS;
   }
 
@@ -329,14 +349,23 @@ struct rk_server_st {
       res.done = now();
       res.checksum = sl.sum_host[0];
       res.hits = sl.sum_host[1];
+      res.build_flags = (uint32_t)sl.sum_host[3];
       finished[(size_t)sl.ticket] = 1;
       Scene& s = scenes[res.scene];
+      if (res.build_flags & RK_FLAG_SPIN_LIMIT) {
+        // a look-back wait of the build's sort gave up: the tree and this
+        // frame are invalid; the scene's next request rebuilds
+        res.status = RK_ERROR_DEVICE;
+        s.built = false;
+        ++failed_requests;
+      }
       s.in_use--;
       sl.ticket = -1;
       ++done_now;
     }
-    // admit in arrival order while a slot is free and the head fits
-    while (!queue.empty()) {
+    // admit in arrival order while a slot is free and the head fits; nothing
+    // more after a launch failed
+    while (!queue.empty() && failed == RK_SUCCESS) {
       int free_slot = -1;
       for (size_t i = 0; i < slots.size(); ++i)
         if (slots[i].ticket < 0) { free_slot = (int)i; break; }
@@ -349,7 +378,7 @@ struct rk_server_st {
       queue.pop_front();
     }
     if (completed) *completed = done_now;
-    return RK_SUCCESS;
+    return failed;
   }
 
   bool idle() const {
@@ -469,7 +498,11 @@ extern "C" rk_status rk_server_add_scene(rk_server s, const rk_float4* vertices,
   std::memcpy(sc.h_faces, faces, 16ull * face_count);
   rk_aabox hb;
   rk_status e = rk_scene_hitbox(vertices, vertex_count, &hb);
-  if (e != RK_SUCCESS) return e;
+  if (e != RK_SUCCESS) {
+    (void)hipHostFree(sc.h_verts);
+    (void)hipHostFree(sc.h_faces);
+    return e;
+  }
   const float ext = std::max(hb.max.x - hb.min.x, std::max(hb.max.y - hb.min.y, hb.max.z - hb.min.z));
   sc.amplitude = s->cfg.amplitude_frac * ext;
   const float a = sc.amplitude;
@@ -544,6 +577,7 @@ extern "C" rk_status rk_server_stats_get(rk_server s, uint64_t first_ticket, uin
     o.refits += r.rebuilt ? 0 : 1;
     o.uploads += r.uploaded;
     o.evictions += r.evictions;
+    o.failed += r.status != RK_SUCCESS ? 1 : 0;
   }
   o.completed = lat.size();
   if (!lat.empty()) {

@@ -92,7 +92,8 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("checksum", C.c_uint64), ("hits", C.c_uint64), ("arrival", C.c_double), ("start", C.c_double),
                 ("done", C.c_double), ("scene", C.c_uint32), ("pixels", C.c_uint32), ("rebuilt", C.c_uint32),
-                ("uploaded", C.c_uint32), ("evictions", C.c_uint32), ("slot", C.c_uint32)]
+                ("uploaded", C.c_uint32), ("evictions", C.c_uint32), ("slot", C.c_uint32),
+                ("build_flags", C.c_uint32), ("status", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -100,7 +101,7 @@ class ServerStats(C.Structure):
                 ("uploads", C.c_uint64), ("evictions", C.c_uint64), ("rays", C.c_uint64),
                 ("first_arrival", C.c_double), ("last_done", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("mrays_per_s", C.c_double), ("peak_resident_bytes", C.c_uint64)]
+                ("mrays_per_s", C.c_double), ("peak_resident_bytes", C.c_uint64), ("failed", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +135,7 @@ _SIGS = {
     "rk_fill_f2": (_I, [_P, _P, _SZ, Float2]),
     "rk_fill_f4": (_I, [_P, _P, _SZ, Float4]),
     "rk_calc_primary_rays": (_I, [_P, _I, _I, Float4, Float4, Float4, Float4, _P, _P]),
+    "rk_shade": (_I, [_P, _P, _P, _I, _I, Float4, Float4, _P, _I]),
     "rk_calc_mesh_boxes": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rk_reorder_workspace_size": (_SZ, [_U32]),
     "rk_reorder_mesh_faces": (_I, [_P, _P, _P, _U32, _P, _SZ]),

@@ -34,6 +34,8 @@ struct float3 { float x, y, z; };
 struct alignas(16) float4 { float x, y, z, w; };
 struct alignas(16) uint4 { uint x, y, z, w; };
 struct uint2 { uint x, y; };
+// A 4 x 4 float matrix, row-major (m[row][column]).
+struct float4x4 { float m[4][4]; };
 
 inline float2 make_float2(float x, float y) { return {x, y}; }
 inline float2 make_float2(float s) { return {s, s}; }

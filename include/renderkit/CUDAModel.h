@@ -24,6 +24,29 @@ class CUDAModel {
     hasVertices = vertex_count > 0;
   }
 
+  // CUDAModel.cpp:74-101: resize, keeping the first entries.
+  void setVertexCount(uint count) {
+    vertices.resize(count, make_float4(0.0f));
+    if (hasNormals) normals.resize(count, make_float4(0.0f));
+    if (hasColors) colors.resize(count, make_float4(0.0f));
+    if (hasTexcoords) texcoords.resize(count, make_float2(0.0f));
+  }
+  void setFaceCount(uint count) { faces.resize(count, make_uint4(0, 0, 0, 0)); }
+  // CUDAModel.cpp:318-349: an axis-aligned box centred at the origin, 8
+  // vertices and 12 faces. Its vertices are set through setVertex(uint, float,
+  // float, float), which does not merge into the hitbox: the hitbox stays empty,
+  // as in the reference.
+  void createCube(float width, float height, float depth) {
+    hasVertices = true;
+    setVertexCount(8);
+    const float hx = 0.5f * width, hy = 0.5f * height, hz = 0.5f * depth;
+    for (uint i = 0; i < 8; ++i) setVertex(i, (i & 4) ? hx : -hx, (i & 2) ? hy : -hy, (i & 1) ? hz : -hz);
+    static const uint tri[12][3] = {{0, 1, 2}, {1, 2, 3}, {4, 5, 6}, {5, 6, 7}, {0, 1, 4}, {1, 4, 5},
+                                    {2, 3, 6}, {3, 6, 7}, {0, 2, 4}, {2, 4, 6}, {1, 3, 5}, {3, 5, 7}};
+    setFaceCount(12);
+    for (uint f = 0; f < 12; ++f) setFace(f, tri[f][0], tri[f][1], tri[f][2], 0);
+  }
+
   uint getVertexCount() const { return (uint)vertices.size(); }
   uint getFaceCount() const { return (uint)faces.size(); }
   CUDABox getHitbox() const { return box; }

@@ -10,6 +10,8 @@
 #include "CUDACamera.h"
 #include "CUDAMeshTree.h"
 #include "CUDARenderer.h"
+#include "CUDAShader.h"
+#include "CUDAWindow.h"
 
 namespace renderkit {
 
@@ -23,9 +25,21 @@ class CUDARaytracer {
   void setCamera(CUDACamera* c) { current_camera = c; }
   CUDABuffer* getBuffer() { return current_buffer; }
   void setBuffer(CUDABuffer* b) { current_buffer = b; }
+  CUDAShader* getShader() { return current_shader; }
+  void setShader(CUDAShader* s) { current_shader = s; }
 
-  virtual void initialize();  // CUDARaytracer.cpp:53-83
-  virtual void finalize();    // :85-92
+  // CUDARaytracer.cpp:53-83: a buffer the size of the renderer's surface and a
+  // base shader.
+  virtual void initialize();
+  virtual void finalize();  // :85-92
+
+  // CUDARaytracer.cpp:110-121: runs the current shader on the current buffer
+  // into the renderer's surface.
+  void updateRenderSurface();
+  // CUDARaytracer.cpp:94-108: the reference saves the shaded surface as a JPEG
+  // through its Direct3D renderer and asserts for any other renderer; there is
+  // no Direct3D renderer here, so this asserts as it does for those.
+  void saveRenderSurface(const char* path);
 
   // CUDA_calcPrimaryRays (CUDARaytracer.cpp:125-128).
   void calcPrimaryRays();
@@ -34,6 +48,12 @@ class CUDARaytracer {
   void raytraceMeshTree(CUDAMesh* mesh, CUDAMeshTree* tree, bool coherency = true);
   // Stream-ordered variant for callers that manage streams (rk_stream).
   void raytraceMeshTreeAsync(CUDAMesh* mesh, CUDAMeshTree* tree, bool coherency, rk_stream stream);
+  // CUDARaytracer.cpp:137-142. The reference rasterises the faces of the mesh
+  // (CUDA_rasterizeMeshTree, a Direct3D-era experimental path) into the same
+  // output surfaces as a primary-ray trace; this library has no rasteriser and
+  // fills them with the coherent traversal of the buffer's rays instead: the
+  // closest face, (u, v) and distance of each pixel's ray.
+  void rasterizeMeshTree(CUDAMesh* mesh, CUDAMeshTree* tree);
 
   // CUDARaytracer.cpp:144-161: allocates the requested output surfaces of the
   // current buffer when missing, then interpolates the mesh's attributes at
@@ -54,6 +74,8 @@ class CUDARaytracer {
   CUDACamera* current_camera = nullptr;
   CUDABuffer* internal_buffer = nullptr;
   CUDABuffer* current_buffer = nullptr;
+  CUDAShader* internal_shader = nullptr;
+  CUDAShader* current_shader = nullptr;
 };
 
 }  // namespace renderkit

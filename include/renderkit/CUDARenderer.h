@@ -85,25 +85,37 @@ class CUDAHeadlessRenderer : public CUDARenderer {
   unsigned long long frames = 0;
 };
 
-// CUDAEngine (CUDAEngine.h) reduced to what the raytracer needs: the renderer
-// and raytracer it links, and a frame counter with the frame time.
+// CUDAEngine (CUDAEngine.h:19-130): links the window, renderer and raytracer
+// of an application and keeps its frame clock. update() ends a frame:
+// getDelta() is the time since the previous update(); the current and average
+// frame rates are refreshed once a second (CUDAEngine.cpp:113-132).
 class CUDARaytracer;
+class CUDAWindow;
 class CUDAEngine {
  public:
-  CUDAEngine() {}
+  CUDAEngine();
+  CUDAWindow* getWindow() { return window; }
   CUDARenderer* getRenderer() { return renderer; }
   void setRenderer(CUDARenderer* r) { renderer = r; }
   CUDARaytracer* getRaytracer() { return raytracer; }
+  // Seconds on a monotonic clock.
+  double getTime();
+  double getDelta() { return frame_delta; }
+  double getCurrentFramerate() { return current_framerate; }
+  double getAverageFramerate() { return average_framerate; }
   void update();
-  double getFrameTime() const { return frame_time; }
+  // Frames ended by update() so far (no counterpart in the reference).
   unsigned long long getFrameCount() const { return frames; }
+  double getFrameTime() const { return frame_delta; }
 
+  CUDAWindow* window = nullptr;
   CUDARenderer* renderer = nullptr;
   CUDARaytracer* raytracer = nullptr;
 
  private:
-  double last = -1.0, frame_time = 0.0;
-  unsigned long long frames = 0;
+  double frame_time = 0.0, cycle_time = 0.0, total_time = 0.0, frame_delta = 0.0;
+  double current_framerate = 0.0, average_framerate = 0.0;
+  unsigned long long cycle_count = 0, total_count = 0, frames = 0;
 };
 
 }  // namespace renderkit

@@ -113,6 +113,18 @@ rk_status rk_interpolate_data(rk_stream stream, const uint32_t* faces_id, const 
                               rk_float4* out_points, rk_float4* out_normals, rk_float4* out_colors,
                               rk_float2* out_texcoords, uint32_t* out_materials);
 
+/* ---- shading ------------------------------------------------------------------ */
+/* void runCUDAShader(CUDARenderSurface<uint>*, CUDABuffer*, CUDACamera*, CUDAShader*),
+ * CUDAShader.cpp:8-15 / CUDAShaderKernel.cu:14-76. Per pixel k = y*width + x of a
+ * traced and interpolated frame: when depths[k] < FLT_MAX, c = -dot(cam_dir,
+ * normals[k]) with both w taken as 0, and the pixel is make_pixel(c, c, c, c);
+ * otherwise make_pixel(back_color). make_pixel (kernels/CUDAShaderCommon.cuh)
+ * scales by 255, clamps to [0, 255] with fminf/fmaxf, truncates and packs
+ * A << 24 | R << 16 | G << 8 | B. Pixel (x, y) goes to pixels[y*pitch + x]
+ * (pitch in pixels, >= width). cam_dir is the camera's direction. */
+rk_status rk_shade(rk_stream stream, const float* depths, const rk_float4* normals, int width, int height,
+                   rk_float4 back_color, rk_float4 cam_dir, uint32_t* pixels, int pitch);
+
 /* ---- build pieces ----------------------------------------------------------- */
 /* void CUDA_calcMeshBoxes(aabox*, uint*, uint4*, float4*, uint), CUDA_calcMeshBoxes.cu:65. */
 rk_status rk_calc_mesh_boxes(rk_stream stream, const rk_uint4* faces, const rk_float4* vertices, uint32_t count,
@@ -474,13 +486,20 @@ typedef struct rk_result {
   uint32_t uploaded;        /* the scene was uploaded for this request */
   uint32_t evictions;       /* scenes evicted to admit it */
   uint32_t slot;
+  uint32_t build_flags;     /* RK_FLAG_* bits of the scene's latest build, read after this frame */
+  rk_status status;         /* RK_SUCCESS, or RK_ERROR_DEVICE when build_flags has RK_FLAG_SPIN_LIMIT:
+                               the tree and this frame are invalid (the scene's next request rebuilds) */
 } rk_result;
 double rk_server_now(rk_server server);
 /* Queues a request; *ticket identifies it. Nothing runs until progress. */
 rk_status rk_server_submit(rk_server server, const rk_request* request, uint64_t* ticket);
-/* Non-blocking: records finished requests, admits queued ones, launches their work. */
+/* Non-blocking: records finished requests, admits queued ones, launches their work.
+ * When a launch fails, that error is returned by this call and every later one:
+ * requests already running still finish and are recorded, queued ones are not
+ * admitted. A request whose build reports RK_FLAG_SPIN_LIMIT finishes with
+ * rk_result.status = RK_ERROR_DEVICE; the server goes on. */
 rk_status rk_server_progress(rk_server server, uint32_t* completed);
-/* Progress until nothing is queued or running. */
+/* Progress until nothing is queued or running, or a launch fails (its error). */
 rk_status rk_server_wait_all(rk_server server);
 /* RK_ERROR_UNSUPPORTED while the request has not finished. */
 rk_status rk_server_result(rk_server server, uint64_t ticket, rk_result* result);
@@ -490,6 +509,7 @@ typedef struct rk_server_stats {
   double p50_ms, p99_ms, mean_ms, max_ms;  /* latency = done - arrival */
   double mrays_per_s;                      /* rays / (last_done - first_arrival) */
   uint64_t peak_resident_bytes;
+  uint64_t failed;                         /* finished requests whose rk_result.status is not RK_SUCCESS */
 } rk_server_stats;
 /* Statistics over tickets [first_ticket, first_ticket + count). */
 rk_status rk_server_stats_get(rk_server server, uint64_t first_ticket, uint64_t count, rk_server_stats* stats);

@@ -49,6 +49,7 @@ _SIGS = {
     "ora_camera_set_angle": (None, [_P, _P]),
     "ora_primary_ray_constants": (None, [_P, _P, C.c_float, C.c_float, _I, _I, _P]),
     "ora_primary_rays": (None, [_P, _I, _I, _P, _P]),
+    "ora_shade": (None, [_P, _P, _I, _I, _P, _P, _P, _I]),
 }
 
 NODE_DTYPE = np.dtype([("w0", "<u4"), ("w1", "<u4"), ("bmin", "<f4", 3), ("bmax", "<f4", 3)])
@@ -281,6 +282,19 @@ def interpolate(face_id, coords, F, V, normals=None, colors=None, texcoords=None
                           None if keep[1] is None else _p(keep[1]), None if keep[2] is None else _p(keep[2]),
                           _p(p), _p(nn), _p(c), _p(t), _p(m))
     return p, nn, c, t, m
+
+
+def shade(depths, normals, width, height, back_color, cam_dir, pitch=None):
+    """kernel_shade: the packed pixels (uint32, height x pitch) of a frame."""
+    pitch = width if pitch is None else pitch
+    d = np.ascontiguousarray(depths, np.float32)
+    nrm = np.ascontiguousarray(normals, np.float32)
+    back = np.ascontiguousarray(back_color, np.float32)
+    cam = np.zeros(4, np.float32)
+    cam[:len(cam_dir)] = cam_dir
+    out = np.zeros(height * pitch, np.uint32)
+    lib().ora_shade(_p(d), _p(nrm), width, height, _p(back), _p(cam), _p(out), pitch)
+    return out
 
 
 class Camera:

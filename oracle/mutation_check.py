@@ -48,6 +48,12 @@ MUT = [
     ("cam yaw clamp", "if (angle.y > CAM_PI * 0.5f - 0.001f) angle.y = CAM_PI * 0.5f - 0.001f;\n  if (dir.x",
      "if (angle.y > CAM_PI * 0.5f - 0.01f) angle.y = CAM_PI * 0.5f - 0.01f;\n  if (dir.x"),
     ("cam dirFromAngle z", "dir.z = sinf(angle.x) * cosf(angle.y);", "dir.z = cosf(angle.x) * sinf(angle.y);"),
+    ("shade R/B swap", "(((uint32_t)v.x & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.z & 0xFFu) |",
+     "(((uint32_t)v.z & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.x & 0xFFu) |"),
+    ("shade round", "v = muls(v, 255.0f);", "v = add4(muls(v, 255.0f), F4{0.5f, 0.5f, 0.5f, 0.5f});"),
+    ("shade depth <=", "if (depths[k] < HUGE_F) {", "if (depths[k] <= HUGE_F) {"),
+    ("shade sign", "float c = -dot4(cam, n);", "float c = dot4(cam, n);"),
+    ("shade normal w", "        n.w = 0.0f;\n        float c", "        float c"),
 ]
 only = sys.argv[1:]
 base = Path(tempfile.mkdtemp(prefix="oracle_mut_"))

@@ -1144,3 +1144,40 @@ ORA_API void ora_primary_rays(const F4* cst, int width, int height, F4* origins,
       directions[n] = sub4(sub4(cst[1], muls(cst[2], (float)x)), muls(cst[3], (float)y));
     }
 }
+
+// ---- shading -----------------------------------------------------------------
+// make_pixel, kernels/CUDAShaderCommon.cuh: v * 255, fminf(v, 255), fmaxf(v, 0)
+// (device minNum/maxNum), truncate to uint, pack A << 24 | R << 16 | G << 8 | B.
+namespace {
+inline uint32_t make_pixel(F4 v) {
+  v = muls(v, 255.0f);
+  v = {fmx(fmn(v.x, 255.0f), 0.0f), fmx(fmn(v.y, 255.0f), 0.0f), fmx(fmn(v.z, 255.0f), 0.0f),
+       fmx(fmn(v.w, 255.0f), 0.0f)};
+  return (((uint32_t)v.x & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.z & 0xFFu) |
+         (((uint32_t)v.w & 0xFFu) << 24);
+}
+}  // namespace
+
+// kernel_shade, CUDAShaderKernel.cu:14-43: pixel k = y * width + x reads
+// depths[k]; a hit (depth < CUDART_NORM_HUGE_F) is make_pixel(make_float4(
+// -dot(cam_dir, normal))) with cam_dir.w = normal.w = 0; a miss is
+// make_pixel(back_color). The result goes to pixels[y * pitch + x].
+ORA_API void ora_shade(const float* depths, const F4* normals, int width, int height, const F4* back_color,
+                       const F4* cam_dir, uint32_t* pixels, int pitch) {
+  F4 cam = *cam_dir;
+  cam.w = 0.0f;
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      size_t k = (size_t)y * width + x;
+      uint32_t px;
+      if (depths[k] < HUGE_F) {
+        F4 n = normals[k];
+        n.w = 0.0f;
+        float c = -dot4(cam, n);
+        px = make_pixel(F4{c, c, c, c});
+      } else {
+        px = make_pixel(*back_color);
+      }
+      pixels[(size_t)y * pitch + x] = px;
+    }
+}

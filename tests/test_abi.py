@@ -112,3 +112,11 @@ def test_host_camera_equals_oracle(rk, ora, eye, tgt, w, h):
     Oo, Do = c.primary_rays(w, h)
     np.testing.assert_array_equal(O, Oo)
     np.testing.assert_array_equal(D, Do)
+
+
+def test_camera_api_host_program():
+    """CUDACamera movement, accessors and matrices (tests/cpp/camera_api.cpp, no GPU call)."""
+    exe = ROOT / "tests" / "cpp" / "_build" / "camera_api"
+    assert exe.exists(), "build() compiles tests/cpp first"
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr

@@ -78,3 +78,70 @@ def test_sample02_dropin_matches_oracle(rk, ora, tmp_path, sub, w, h):
     fast_ids = np.frombuffer(ch[17], np.uint32)[np.frombuffer(ch[14], np.uint32)[hit]]
     ref_ids = ref.indices[of[hit]]
     assert (fast_ids == ref_ids).mean() > 0.99
+
+
+LOOP = ROOT / "tests" / "cpp" / "_build" / "sample02_loop"
+
+
+def _loop_chunks(path):
+    data = path.read_bytes()
+    hdr = np.frombuffer(data[:16], np.uint32)
+    off, chunks = 16, []
+    while off < len(data):
+        n = int(np.frombuffer(data[off:off + 4], np.uint32)[0])
+        chunks.append(data[off + 4:off + 4 + n])
+        off += 4 + n
+    return hdr, chunks
+
+
+@pytest.mark.parametrize("scene,frames", [("cube", 3), ("ico", 2)])
+def test_sample02_frame_loop(rk, ora, tmp_path, scene, frames):
+    """Sample 02's loop (clear, primary rays, trace, interpolate points and
+    normals, updateRenderSurface, renderer and engine update) until the headless
+    window closes. The sample's own scene is a createCube model, whose hitbox
+    stays empty, so its SAH root box is empty and every pixel is background, as
+    in the reference; the icosphere scene has a hitbox and shades. Rays, depths,
+    normals and the shaded pixels equal the oracle's."""
+    assert LOOP.exists(), "build() compiles tests/cpp first"
+    out = tmp_path / "loop.bin"
+    r = subprocess.run([str(LOOP), str(out), scene, str(frames)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    hdr, ch = _loop_chunks(out)
+    w, h, rframes, eframes = (int(x) for x in hdr)
+    assert (w, h, rframes, eframes) == (400, 300, frames, frames)
+    V = np.frombuffer(ch[0], np.float32).reshape(-1, 4)
+    F = np.frombuffer(ch[1], np.uint32).reshape(-1, 4)
+    O = np.frombuffer(ch[2], np.float32).reshape(-1, 4)
+    D = np.frombuffer(ch[3], np.float32).reshape(-1, 4)
+    fid = np.frombuffer(ch[4], np.uint32)
+    uv = np.frombuffer(ch[5], np.float32).reshape(-1, 2)
+    dep = np.frombuffer(ch[6], np.float32)
+    pts = np.frombuffer(ch[7], np.float32).reshape(-1, 4)
+    nrm = np.frombuffer(ch[8], np.float32).reshape(-1, 4)
+    px = np.frombuffer(ch[9], np.uint32)
+    extra = np.frombuffer(ch[10], np.float32)
+    cam_dir, back = extra[:3], extra[4:8]
+    np.testing.assert_array_equal(back, [0.227451, 0.431373, 0.647059, 1.0])
+    cam = ora.Camera()
+    cam.set_ratio(400, 300)
+    cam.set_center(-10, 20, 30)
+    cam.look_at(0, 0, 0)
+    np.testing.assert_array_equal(cam_dir, cam.dir)
+    Oo, Do = cam.primary_rays(w, h)
+    np.testing.assert_array_equal(O, Oo)
+    np.testing.assert_array_equal(D, Do)
+    of, ouv, od = ora.trace_bruteforce(F, V, O, D)
+    if scene == "cube":
+        assert np.all(dep == F32MAX) and np.all(fid == 0)  # cleared surfaces, no hit: empty root box
+        assert np.all(px == 0xFF3A6EA5)
+    else:
+        np.testing.assert_array_equal(dep, od)
+        hit = od < F32MAX
+        assert 0.1 < hit.mean() < 0.9
+        assert (fid[hit] == of[hit]).mean() > 0.99  # the packet traversal keeps the first of exact t ties
+        same = hit & (fid == of)
+        np.testing.assert_array_equal(uv[same], ouv[same])
+    p0, n0, _, _, _ = ora.interpolate(fid, uv, F, V)
+    np.testing.assert_array_equal(pts, p0)
+    np.testing.assert_array_equal(nrm, n0)
+    np.testing.assert_array_equal(px, ora.shade(dep, nrm, w, h, back, cam_dir))

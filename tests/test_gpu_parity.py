@@ -172,9 +172,12 @@ def _camera_rays(rk, name, w, h):
     return rk.primary_rays_host(cam, w, h)
 
 
-@pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29)])
+@pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29),
+                                      ("hf708", 1920, 1080)])
 @pytest.mark.parametrize("coherency", [False, True])
 def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
+    """Both reference traversals (per-ray and packet) against the oracle's
+    restatement of each, every ray, up to the full 1920 x 1080 heightfield frame."""
     V, F = _scene(rk, name)
     tree, ref = _compare_tree(dev, rk, ora, V, F, 12, True)
     O, D = _camera_rays(rk, name, w, h)
@@ -194,7 +197,7 @@ def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
     np.testing.assert_array_equal(_h(out[0], np.uint32), of)
     np.testing.assert_array_equal(_h(out[1]), ouv)
     np.testing.assert_array_equal(_h(out[2]), od)
-    assert (od < dep0).sum() > n // 10
+    assert (od < dep0).sum() > n // 20
 
 
 def _check_fast_boxes(tree, F, V, leaf_max):

@@ -265,3 +265,63 @@ def test_server_rejects_what_cannot_fit(dev, rk):
     with pytest.raises(rk.RkError):
         srv.submit(s, 0.0, _camera(rk, V2, 65), 65, 65)
     srv.close()
+
+
+def _server_hitbox(V, amp):
+    """The server's Morton box of a scene: the vertex hitbox grown by the
+    deformation amplitude on every side (rk_server_add_scene)."""
+    hb = np.zeros((2, 4), np.float32)
+    hb[0, :3] = V[:, :3].min(0) - np.float32(amp)
+    hb[1, :3] = V[:, :3].max(0) + np.float32(amp)
+    return hb
+
+
+def test_served_frames_at_full_scale(dev, rk, ora):
+    """The serving workload's largest sizes: a 2,000,000-face terrain and a
+    2,000,000-face soup at 1024 x 1024, under an HBM budget that holds one of
+    them, so the third request (terrain again) evicts the soup, uploads and
+    rebuilds. Every 37th pixel of each frame equals the oracle's closest hit on
+    the deformed scene; each result reports a clean build."""
+    terrain = rk.heightfield(1001, seed=0xC5)
+    soup = rk.soup(2_000_000, seed=0xC5)
+    scenes = [(terrain[0], terrain[1], True), (soup[0], soup[1], False)]
+    sizes = [rk.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in scenes]
+    srv = dev.Server(int(1.2 * max(sizes)), slots=2, det_epsilon=0.0)
+    ids = [srv.add_scene(V, F, z) for V, F, z in scenes]
+    side = 1024
+    jobs = []
+    for k, (s, t) in enumerate([(0, 0.13), (1, 0.29), (0, 0.61)]):
+        V, F, z = scenes[s]
+        cam = _camera(rk, V, side, az=0.4 + k)
+        out = dev.new_outputs(side * side)
+        print(f"full-scale request {k}: scene {s}, {F.shape[0]} faces, {side}x{side}", flush=True)
+        jobs.append((srv.submit(ids[s], t, cam, side, side, out=out), s, t, cam, out))
+    srv.wait_all()
+    res = [srv.result(j[0]) for j in jobs]
+    assert [r.status for r in res] == [0, 0, 0] and [r.build_flags for r in res] == [0, 0, 0]
+    assert res[2].evictions >= 1 and res[2].uploaded == 1 and res[2].rebuilt == 1
+    sel = np.arange(0, side * side, 37)
+    for (ticket, s, t, cam, out), r in zip(jobs, res):
+        V, F, z = scenes[s]
+        hb = rk.hitbox(V)
+        amp = float(np.float32(srv.cfg.amplitude_frac) * np.float32((hb[1, :3] - hb[0, :3]).max()))
+        Vd = _h(dev.deform(_t(V), amp, float(srv.cfg.omega), t, 0x5EED0000 + s, z))
+        O, D = rk.primary_rays_host(cam, side, side)
+        Os, Ds = np.ascontiguousarray(O[sel]), np.ascontiguousarray(D[sel])
+        otree = ora.build_lbvh_ref(F, Vd, rk.hitbox(Vd), 31, True)
+        of, ouv, od = ora.trace_closest(otree, Vd, Os, Ds, det_epsilon=0.0)
+        fid, uv, dep = _h(out[0], np.uint32), _h(out[1]), _h(out[2])
+        np.testing.assert_array_equal(dep[sel], od)
+        hit = od < F32MAX
+        assert hit.sum() > 1000
+        np.testing.assert_array_equal(uv[sel][hit], ouv[hit])
+        # every request here rebuilds, so the frame's sorted positions are those
+        # of a fresh build over the deformed scene in the server's Morton box
+        tree = dev.FastBuilder(F.shape[0], rk.RK_MORTON_30).build(_t(F), _t(Vd), _server_hitbox(V, amp), 1)
+        order = _h(tree.indices, np.uint32)
+        orig_sel = order[fid[sel][hit]]
+        np.testing.assert_array_equal(orig_sel, of[hit])
+        orig = np.where(dep < F32MAX, order[np.minimum(fid, F.shape[0] - 1)], 0xFFFFFFFF).astype(np.uint32)
+        assert (r.checksum, r.hits) == _checksum(orig, np.where((dep < F32MAX)[:, None], uv, 0).astype(np.float32),
+                                                 dep)
+    srv.close()

@@ -891,3 +891,48 @@ def test_camera_c1_frame(ora):
     np.testing.assert_allclose(cst[3], [0, 2 * t / 256, 0, 0], rtol=1e-6, atol=1e-9)
     O, D = c.primary_rays(256, 256)
     assert D[128 * 256 + 5, 1] == 0  # row 128 has dir.y == 0 exactly
+
+
+# ---- shading (CUDAShaderKernel.cu:14-43, make_pixel in kernels/CUDAShaderCommon.cuh) ----
+
+BACK = np.array([0.227451, 0.431373, 0.647059, 1.0], np.float32)  # CUDAShader.cpp:28-33
+
+
+def test_shade_known_pixels(ora):
+    """Hand-computed pixels: the default back colour is (58, 110, 165, 255)
+    (0.227451 * 255 = 58.000005 and so on), a normal facing the camera is
+    white, c = 0.5 gives 127 in every channel (truncation of 127.5), c = 0.999
+    gives 254 (not rounded), a normal facing away clamps to 0. w of the normal
+    (even infinite) and of the camera direction are ignored."""
+    H = np.finfo(np.float32).max
+    depths = np.array([H, 1.0, 1.0, 1.0, 1.0, np.nextafter(H, np.float32(0))], np.float32)
+    normals = np.array([[0, 0, 1, 0], [0, 0, 1, 7], [0, 0, 0.5, np.inf], [0, 0, 0.999, 0], [0, 0, -1, 0],
+                        [0, 0, 1, 0]], np.float32)
+    px = ora.shade(depths, normals, 6, 1, BACK, [0, 0, -1, 3])
+    assert [hex(p) for p in px] == ["0xff3a6ea5", "0xffffffff", "0x7f7f7f7f", "0xfefefefe", "0x0", "0xffffffff"]
+
+
+def test_shade_channel_order_and_pitch(ora):
+    """A << 24 | R << 16 | G << 8 | B; rows start at y * pitch and the pixels
+    past the width of each row are left alone."""
+    H = np.finfo(np.float32).max
+    d = np.full(4, H, np.float32)
+    n = np.zeros((4, 4), np.float32)
+    for back, want in (([1, 0, 0, 0], 0x00FF0000), ([0, 1, 0, 0], 0x0000FF00), ([0, 0, 1, 0], 0x000000FF),
+                       ([0, 0, 0, 1], 0xFF000000), ([2, -1, 0.5, 0], 0x00FF007F)):
+        px = ora.shade(d, n, 2, 2, np.array(back, np.float32), [0, 0, 1], pitch=3).reshape(2, 3)
+        assert np.all(px[:, :2] == want) and np.all(px[:, 2] == 0)
+
+
+def test_shade_obliquity(ora):
+    """c = -dot(cam, n) over random unit normals equals the clamp of the cosine
+    between the view direction and the normal, to a step of 1/255."""
+    rng = np.random.default_rng(9)
+    n = rng.normal(size=(500, 4)).astype(np.float32)
+    n[:, :3] /= np.linalg.norm(n[:, :3], axis=1, keepdims=True)
+    cam = np.array([0.6, 0.0, -0.8], np.float32)
+    px = ora.shade(np.ones(500, np.float32), n, 500, 1, BACK, cam)
+    c = np.clip(-(n[:, :3].astype(np.float64) @ cam), 0, 1)
+    got = px & 0xFF
+    assert np.all(np.abs(got - np.floor(c * 255)) <= 1)
+    assert np.all((px >> 24) == got) and np.all(((px >> 16) & 0xFF) == got)
