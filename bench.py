@@ -6,18 +6,21 @@ azimuth, distance 1.5 from (0.5, 0.5, 0.1), 60 degree fov.
 
 A step is one frame with the tree resident in HBM: a fill kernel resets tmax
 and rk_fast_trace finds the closest hit of all 2,073,600 rays. `value` is
-Mrays/s over the timed frames. The triangle test's parallel-ray threshold is 0
-for the headline (det_epsilon_headline). The reference's absolute 1e-6
-rejects the grazing hits on this mesh, whose edges are ~1.4e-3 so that
-|det| ~ 1e-6; that frame is reported beside it (mrays_s_reference_epsilon,
-hit_fraction_reference_epsilon). The same frame with the rays formed inside
+Mrays/s over the timed frames. The triangle test uses the reference's
+absolute parallel-ray threshold 1e-6, so the hits are the reference's hits.
+That threshold rejects the grazing hits on this mesh, whose edges are ~1.4e-3
+so that |det| ~ 1e-6; the frame with threshold 0 (only exactly parallel rays
+rejected) is reported beside it (mrays_s_det_epsilon_0,
+hit_fraction_det_epsilon_0). The same frame with the rays formed inside
 the trace kernel is camera_fused_mrays_s. lbvh_mtri_s times rk_fast_build
 (30-bit Morton, one face per leaf). The reference tree (rk_build_lbvh, depth
 12) and both reference traversals are timed once on the same frame, and so
 are the reference's 32-bin SAH build (rk_build_sah), its per-ray traversal of
 the SAH tree, and the Hybrid build (depth-12 LBVH, each leaf refined by SAH).
 
-Further legs, each skippable:
+Further legs, each skippable except the first:
+- icosphere: the reference's CPU-runnable case (20,480 faces, 256x256 rays):
+  reference build and both reference traversals, the fast tree, the oracle.
 - soup: 10M random faces, 63-bit keys, 2^24 incoherent rays; build and
   frames with the rays as given and sorted by origin (rk_fast_trace_sorted).
 - batch: 4096 soups built and traced together (autotuned, one captured graph
@@ -28,7 +31,7 @@ Further legs, each skippable:
   trace, checksum, one request at a time).
 - cpu_baseline: the oracle, i.e. the reference algorithm (its LBVH build and
   per-ray traversal, reference threshold) on the host's cores, on a bounded
-  sample of the frame.
+  sample of the frame; each leg carries its own cpu_baseline of the same kind.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-ref]
                 [--no-soup] [--no-batch] [--no-serve] [--profile-only | --profile-soup]
@@ -57,7 +60,7 @@ METRIC = "Mrays/s closest-hit + LBVH Mtri/s, batched vs plain on 1 GPU; p99 late
 FP32_PEAK_TFLOPS = 157.3
 OPS_PER_BOX_TEST = 24   # algorithmic FP32 ops of one slab test
 OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
-HEADLINE_DET_EPSILON = 0.0
+HEADLINE_DET_EPSILON = 1e-6  # the reference's absolute parallel-ray threshold (CUDA_raytraceMeshTree.cu:36)
 # HBM bytes per launch of the headline k_fast_trace, from two rocprofv3 --pmc
 # runs of `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
 # FETCH_SIZE 117,868 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,512 KiB
@@ -99,6 +102,22 @@ class Timer:
         return s.elapsed_time(e) / reps  # ms
 
 
+def cpu_trace_sampled(oracle, ref, V, O, D, budget_s):
+    """Per-ray reference traversal (the oracle) of every stride-th ray, the
+    stride chosen so the timed pass takes about budget_s / 2 seconds."""
+    n = O.shape[0]
+    stride = max(1, n // 1024)
+    t0 = time.perf_counter()
+    oracle.trace_ref_ray(ref.nodes, ref.faces, V, np.ascontiguousarray(O[::stride]), np.ascontiguousarray(D[::stride]))
+    dt = time.perf_counter() - t0
+    if dt < budget_s / 4 and stride > 1:
+        stride = max(1, int(stride * dt / (budget_s / 2)))
+    O2, D2 = np.ascontiguousarray(O[::stride]), np.ascontiguousarray(D[::stride])
+    t0 = time.perf_counter()
+    oracle.trace_ref_ray(ref.nodes, ref.faces, V, O2, D2)
+    return O2.shape[0], time.perf_counter() - t0, stride
+
+
 def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
     """The reference algorithm on the host (the oracle): LBVH build (depth 12)
     and per-ray traversal with the reference threshold, on a sample of rays."""
@@ -134,7 +153,94 @@ def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
     }
 
 
-def soup_leg(torch, dev, native, timer, steps, warmup):
+def icosphere_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
+    """The reference's own CPU-runnable case: icosphere (5 subdivisions,
+    20,480 faces, unit radius), the reference LBVH (depth 12, 32-bit Morton),
+    256 x 256 primary rays from (0, 0, 3) at the origin with a 60 degree fov,
+    closest hit at the reference threshold. A step is one frame: clear the
+    outputs, trace. Timed: the reference build and both reference traversals,
+    the fast build and trace, and the oracle on the host."""
+    V, F = native.icosphere(5)
+    hb = native.hitbox(V)
+    W = H = 256
+    cam = native.camera_lookat([0, 0, 3], [0, 0, 0], 60 * 0.01745329252, W, H)
+    O, D = native.primary_rays_host(cam, W, H)
+    n, nr = F.shape[0], O.shape[0]
+    Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
+    Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
+    f32max = float(np.finfo(np.float32).max)
+    rb = dev.LBVHBuilder(n)
+    Fr = Ft.clone()
+    rtree = rb.build(Fr, Vt, hb, 12, True)
+    torch.cuda.synchronize()
+    rtree.node_count = int(rtree.host_info()[native.RK_INFO_NODE_COUNT])
+    ref_build = lambda: rb.build(Fr, Vt, hb, 12, True)
+    out = dev.new_outputs(nr)
+
+    def ref_frame(coherency):
+        native.check(native.lib().rk_fill_f32(None, out[2].data_ptr(), nr, f32max), "fill")
+        dev.trace_ref(rtree, Vt, Ot, Dt, W, H, coherency, out=out)
+
+    for _ in range(warmup):
+        ref_build()
+    ref_build_ms = timer.time(ref_build, steps)
+    rtree = rb.build(Ft.clone(), Vt, hb, 12, True)  # a build of the faces in mesh order, as traced below
+    torch.cuda.synchronize()
+    rtree.node_count = int(rtree.host_info()[native.RK_INFO_NODE_COUNT])
+    for _ in range(warmup):
+        ref_frame(True)
+    pkt_ms = timer.time(lambda: ref_frame(True), steps)
+    ref_dep = out[2].clone()
+    ray_ms = timer.time(lambda: ref_frame(False), steps)
+    builder = dev.FastBuilder(n, native.RK_MORTON_30)
+    fbuild = lambda: builder.build(Ft, Vt, hb, 1)
+    tree = fbuild()
+    fout = dev.new_outputs(nr)
+
+    def fast_frame():
+        native.check(native.lib().rk_fill_f32(None, fout[2].data_ptr(), nr, f32max), "fill")
+        dev.fast_trace(tree, Ot, Dt, out=fout, det_epsilon=HEADLINE_DET_EPSILON)
+
+    for _ in range(warmup):
+        fbuild()
+    fbuild_ms = timer.time(fbuild, steps)
+    tree = fbuild()
+    for _ in range(warmup):
+        fast_frame()
+    fast_ms = timer.time(fast_frame, steps)
+    res = {
+        "workload": "icosphere 5 subdivisions (20,480 faces), 256x256 primary rays, closest hit, threshold 1e-6",
+        "faces": n, "rays": nr,
+        "reference_lbvh_mtri_s": round(n / (ref_build_ms * 1e-3) / 1e6, 2),
+        "reference_packet_mrays_s": round(nr / (pkt_ms * 1e-3) / 1e6, 2),
+        "reference_per_ray_mrays_s": round(nr / (ray_ms * 1e-3) / 1e6, 2),
+        "fast_lbvh_mtri_s": round(n / (fbuild_ms * 1e-3) / 1e6, 2),
+        "fast_mrays_s": round(nr / (fast_ms * 1e-3) / 1e6, 2),
+        "fast_frame_ms": round(fast_ms, 4),
+        "same_depths": bool(torch.equal(fout[2], ref_dep)),
+        "hit_fraction": round(int((ref_dep < f32max).sum().item()) / nr, 4),
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        import oracle
+        bt, tt = [], []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ref = oracle.build_lbvh_ref(F, V, hb, 12, True)
+            bt.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            oracle.trace_ref_ray(ref.nodes, ref.faces, V, O, D)
+            tt.append(time.perf_counter() - t0)
+        b, t = float(np.median(bt)), float(np.median(tt))
+        res["cpu_baseline"] = {
+            "value": round(nr / t / 1e6, 4), "unit": "Mrays/s", "cores": int(oracle.num_threads()), "kind": "port",
+            "lbvh_mtri_s": round(n / b / 1e6, 3),
+            "sample": "the whole workload, median of 5: reference LBVH build (depth 12) and per-ray traversal of "
+                      "all 65,536 rays"}
+    return res
+
+
+def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     """Random soup of 10M faces (centroids uniform in the unit cube, lognormal
     edges around 0.002), 63-bit Morton keys sorted as uint64 pairs, and 2^24
     incoherent rays (origins uniform in the cube, directions uniform on the
@@ -145,6 +251,18 @@ def soup_leg(torch, dev, native, timer, steps, warmup):
     O, D = native.random_rays(nr, seed=0xC3)
     Ft, Vt = dev.to_u32_tensor(F), torch.from_numpy(V).cuda()
     Ot, Dt = torch.from_numpy(O).cuda(), torch.from_numpy(D).cuda()
+    cpu = None
+    if with_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.build_lbvh_ref(F, V, hb, 12, True)
+        cpu_build_s = time.perf_counter() - t0
+        rays, dt, stride = cpu_trace_sampled(oracle, ref, V, O, D, 10.0)
+        cpu = {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": int(oracle.num_threads()),
+               "kind": "port", "lbvh_mtri_s": round(n / cpu_build_s / 1e6, 3),
+               "sample": f"reference LBVH (depth 12, 32-bit Morton) of all {n} faces and per-ray traversal of "
+                         f"every {stride}th ray ({rays} rays), reference threshold"}
+        del ref
     del V, F, O, D
     builder = dev.FastBuilder(n, native.RK_MORTON_63)
     build = lambda: builder.build(Ft, Vt, hb, 1)
@@ -186,10 +304,11 @@ def soup_leg(torch, dev, native, timer, steps, warmup):
         "sorted_rays_mrays_s": round(nr / (sorted_ms * 1e-3) / 1e6, 2),
         "sorted_rays_frame_ms": round(sorted_ms, 3),
         "sorted_rays_same_results": same,
+        "cpu_baseline": cpu,
     }
 
 
-def batch_leg(torch, dev, native, timer, steps, warmup):
+def batch_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     """4096 soups (log-uniform 64..65536 faces, 256..16384 rays each): one
     step builds every mesh and traces every ray. Batched = the autotuned
     rk_batch_build + rk_batch_trace captured in one graph; plain = one
@@ -250,6 +369,29 @@ def batch_leg(torch, dev, native, timer, steps, warmup):
     plain()
     plain_ms = timer.time(plain, 2)
     same = bool(torch.equal(pout[2], out[2]))
+    cpu = None
+    if with_cpu:
+        # the reference per mesh on the host: LBVH build and per-ray traversal of
+        # every ray of every 64th mesh
+        import oracle
+        rays = faces = 0
+        t0 = time.perf_counter()
+        for k in range(0, m, 64):
+            f0, fc = int(table["face_offset"][k]), int(table["face_count"][k])
+            v0, vc = int(table["vertex_offset"][k]), int(table["vertex_count"][k])
+            r0, rc = int(table["ray_offset"][k]), int(table["ray_count"][k])
+            Fm = np.ascontiguousarray(F[f0:f0 + fc])
+            Vm = np.ascontiguousarray(V[v0:v0 + vc])
+            ref = oracle.build_lbvh_ref(Fm, Vm, hb[k], 12, True)
+            oracle.trace_ref_ray(ref.nodes, ref.faces, Vm, np.ascontiguousarray(O[r0:r0 + rc]),
+                                 np.ascontiguousarray(D[r0:r0 + rc]))
+            rays += rc
+            faces += fc
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": int(oracle.num_threads()),
+               "kind": "port", "mtri_s": round(faces / dt / 1e6, 3),
+               "sample": f"every 64th mesh ({len(range(0, m, 64))} meshes, {faces} faces, {rays} rays): reference "
+                         f"LBVH build (depth 12) and per-ray traversal, build and trace together"}
     return {
         "workload": "4096 soups, log-uniform 64..65536 faces and 256..16384 rays per mesh; build all + trace all",
         "meshes": m, "faces": nf, "rays": nr,
@@ -263,10 +405,11 @@ def batch_leg(torch, dev, native, timer, steps, warmup):
         "autotuned": config.as_dict(),
         "autotune_best_ms": round(tuned_ms, 3),
         "graph": "fill + rk_batch_build + rk_batch_trace captured once, replayed per step",
+        "cpu_baseline": cpu,
     }
 
 
-def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000):
+def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000, with_cpu=False):
     """Serving: 32 deforming scenes (log-uniform 1K..2M faces, alternately a
     terrain and a soup) behind rk_server; Poisson arrivals at 'rate' per
     second, square frames of log-uniform side 64..1024, refit or rebuild per
@@ -312,6 +455,33 @@ def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000):
     srvp.run_poisson(0.0, 64, 64, 1024, 1)
     plain, _ = srvp.run_poisson(0.0, 1000, 64, 1024, 2)
     srvp.close()
+    cpu = None
+    if with_cpu:
+        # the reference per request on the host: rebuild the scene's LBVH
+        # (depth 12) and trace its camera frame per ray, for 12 requests drawn
+        # like the stream's (scene uniform, side log-uniform in 64..1024)
+        import oracle
+        r = np.random.default_rng(0xC5)
+        rays = 0
+        t0 = time.perf_counter()
+        for _ in range(12):
+            V, F, z = pool[int(r.integers(0, len(pool)))]
+            side = int(round(math.exp(r.uniform(math.log(64), math.log(1024)))))
+            hbs = native.hitbox(V)
+            ctr = 0.5 * (hbs[0, :3] + hbs[1, :3])
+            diag = float(np.linalg.norm(hbs[1, :3] - hbs[0, :3]))
+            el, az = math.radians(30.0), r.uniform(0, 2 * math.pi)
+            eye = ctr + 1.2 * diag * np.array([math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)])
+            Oc, Dc = native.primary_rays_host(native.camera_lookat(eye, ctr, math.radians(60.0), side, side,
+                                                                   up=[0, 0, 1]), side, side)
+            ref = oracle.build_lbvh_ref(F, V, hbs, 12, True)
+            oracle.trace_ref_ray(ref.nodes, ref.faces, V, Oc, Dc)
+            rays += side * side
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": int(oracle.num_threads()),
+               "kind": "port", "ms_per_request": round(dt / 12 * 1e3, 2),
+               "sample": f"12 requests drawn like the stream's ({rays} rays): per request, the reference LBVH "
+                         f"rebuild (depth 12) and per-ray traversal of the frame, one request at a time"}
     return {
         "workload": "32 deforming scenes (1K..2M faces, terrain/soup), Poisson arrivals, frames 64^2..1024^2",
         "faces_total": int(sum(F.shape[0] for _, F, _ in pool)),
@@ -328,6 +498,8 @@ def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000):
         "served_vs_one_at_a_time": round(sat.mrays_per_s / single.mrays_per_s, 2) if single.mrays_per_s else None,
         "plain_per_request_mrays_s": round(plain.mrays_per_s, 2),
         "served_vs_plain": round(sat.mrays_per_s / plain.mrays_per_s, 2) if plain.mrays_per_s else None,
+        "failed_requests": int(st.failed + sat.failed),
+        "cpu_baseline": cpu,
     }
 
 
@@ -374,18 +546,24 @@ def main():
     build = lambda: builder.build(Ft, Vt, hb, args.leaf)
     tree = build()
     fid, uv, dep = dev.new_outputs(nr)
+    tuned = {"config": 0}
 
-    def frame(eps=HEADLINE_DET_EPSILON):
+    def frame(eps=HEADLINE_DET_EPSILON, config=None):
         native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=eps)
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=eps,
+                       config=tuned["config"] if config is None else config)
 
     torch.cuda.synchronize()
     if args.profile_only:
         for _ in range(3):
             build()
+        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        tuned["config"] = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON,
+                                       reps=1).config
+        log(f"tuned variant {dev.trace_variant(tuned['config']).as_dict()}")
         for _ in range(3):
             frame()
-        frame(native.DET_EPSILON_REFERENCE)
+        frame(0.0)
         rb = dev.LBVHBuilder(n)
         rtree = rb.build(Ft.clone(), Vt, hb, 12, True)
         torch.cuda.synchronize()
@@ -404,19 +582,27 @@ def main():
     build_ms = timer.time(build, args.steps)
     flags = int(dev.u32_view(tree.info)[native.RK_INFO_FLAGS])
 
+    log("autotuning the traversal on the frame")
+    native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=5)
+    tuned["config"] = tuning.config
+
     log("timing frames")
     for _ in range(args.warmup):
         frame()
     frame_ms = timer.time(frame, args.steps)
+    default_ms = timer.time(lambda: frame(config=0), args.steps)
     hits = int((dep < f32max).sum().item())
     fill_ms = timer.time(lambda: native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max)),
                          args.steps)
     trace_ms = max(frame_ms - fill_ms, 1e-6)
-    ref_eps_frame_ms = timer.time(lambda: frame(native.DET_EPSILON_REFERENCE), args.steps)
-    ref_eps_hits = int((dep < f32max).sum().item())
-    # the same frame with the rays formed inside the trace kernel (8-pixel runs from 8 bands per wave)
+    eps0_frame_ms = timer.time(lambda: frame(0.0), args.steps)
+    eps0_hits = int((dep < f32max).sum().item())
+    # the same frame with the rays formed inside the trace kernel (its own tuned variant)
     cout = dev.new_outputs(nr)
-    cam_frame = lambda: dev.fast_trace_camera(tree, cam, W, H, out=cout, det_epsilon=HEADLINE_DET_EPSILON)
+    cam_tuning = dev.autotune_camera(tree, cam, W, H, cout, det_epsilon=HEADLINE_DET_EPSILON, reps=5)
+    cam_frame = lambda: dev.fast_trace_camera(tree, cam, W, H, out=cout, det_epsilon=HEADLINE_DET_EPSILON,
+                                              config=cam_tuning.config)
     for _ in range(args.warmup):
         cam_frame()
     cam_ms = timer.time(cam_frame, args.steps)
@@ -446,14 +632,17 @@ def main():
             "workload": "heightfield 708x708 fBm (999,698 FP32 triangles), 1920x1080 primary rays, closest hit",
             "faces": n, "rays": nr, "tree": "LBVH 30-bit Morton (Karras), " + ("one face per leaf" if args.leaf == 1 else f"leaf <= {args.leaf} faces"),
             "step": "one frame: reset tmax + closest-hit trace of every ray, tree resident in HBM",
-            "det_epsilon_headline": HEADLINE_DET_EPSILON,
+            "det_epsilon": HEADLINE_DET_EPSILON,
         },
         "hit_fraction": round(hits / nr, 4),
         "trace_kernel_ms": round(trace_ms, 4),
-        "mrays_s_reference_epsilon": round(nr / (ref_eps_frame_ms * 1e-3) / 1e6, 2),
+        "trace_autotuned": tuning.as_dict(),
+        "mrays_s_default_variant": round(nr / (default_ms * 1e-3) / 1e6, 2),
+        "camera_autotuned": cam_tuning.as_dict(),
+        "mrays_s_det_epsilon_0": round(nr / (eps0_frame_ms * 1e-3) / 1e6, 2),
+        "hit_fraction_det_epsilon_0": round(eps0_hits / nr, 4),
         "camera_fused_mrays_s": round(nr / (cam_ms * 1e-3) / 1e6, 2),
         "camera_fused_same_depths": cam_same,
-        "hit_fraction_reference_epsilon": round(ref_eps_hits / nr, 4),
         "lbvh_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
         "lbvh_build_ms": round(build_ms, 4),
         "box_tests_per_ray": round(n_box / nr, 2),
@@ -468,7 +657,8 @@ def main():
             "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
             "traffic": TRACE_TRAFFIC_BYTES,
             "algorithmic_bytes": 52 * nr,
-            "note": "algorithmic FP32 ops = 24 per slab test + 53 per triangle test, counted on this frame; "
+            "note": "algorithmic FP32 ops = 24 per slab test + 53 per triangle test, counted on this frame "
+                    "(default variant); achieved over the tuned variant's kernel time; "
                     "52 B/ray of compulsory HBM traffic makes this kernel VALU/latency-bound, not HBM-bound; "
                     "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs (rays, outputs "
                     "and node/triangle records fetched past the L2)",
@@ -524,18 +714,28 @@ def main():
         result["reference_sah_same_depths"] = bool(torch.equal(sdep, rdep))
         result["reference_hybrid_mtri_s"] = round(n / (hybrid_ms * 1e-3) / 1e6, 2)
         result["reference_hybrid_build_ms"] = round(hybrid_ms, 3)
+        if not args.no_cpu:
+            import oracle
+            ob, oi = oracle.calc_boxes(F, V)
+            t0 = time.perf_counter()
+            oracle.build_sah(ob, oi, hb)
+            result["reference_sah_cpu_mtri_s"] = round(n / (time.perf_counter() - t0) / 1e6, 3)
+
+    log("icosphere 20,480 faces, 256x256 rays")
+    result["icosphere"] = icosphere_leg(torch, dev, native, timer, args.steps, args.warmup, not args.no_cpu)
 
     if not args.no_soup:
         log("soup 10M faces, 63-bit keys, 2^24 incoherent rays")
-        result["soup"] = soup_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
+        result["soup"] = soup_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2), not args.no_cpu)
 
     if not args.no_batch:
         log("batch of 4096 meshes: autotune, graph, batched vs plain")
-        result["batch"] = batch_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2))
+        result["batch"] = batch_leg(torch, dev, native, timer, min(args.steps, 10), min(args.warmup, 2),
+                                    not args.no_cpu)
 
     if not args.no_serve:
         log("serving: Poisson 2000 req/s over 32 deforming scenes")
-        result["serve"] = serve_leg(dev, native)
+        result["serve"] = serve_leg(dev, native, with_cpu=not args.no_cpu)
 
     if not args.no_cpu:
         log("cpu baseline")

@@ -749,7 +749,7 @@ void CUDARaytracer::raytraceMeshTreeAsync(CUDAMesh* mesh, CUDAMeshTree* tree, bo
     rk_fast_bvh desc = {tree->getFastNodes(), tree->getFastTriangles(), tree->getIndicesArray()->getPointer(),
                         reinterpret_cast<rk_aabox*>(tree->getBoxesArray()->getPointer()),
                         (uint32_t)mesh->getFaceCount(), 0};
-    rk_trace_options opt = {det_epsilon, 0u, {0u, 0u}};
+    rk_trace_options opt = {det_epsilon, 0u, 0u, 0u};
     RK_CALL(rk_fast_trace(stream, &desc, o, d, (uint32_t)w * (uint32_t)h, fid, uv, dep, &opt));
     return;
   }

@@ -16,7 +16,7 @@
 
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err);
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false);
 rk_status rk_sort_general_impl(hipStream_t st, uint32_t* keys, uint32_t* indices, uint32_t length,
                                uint32_t bit_depth, int init_indices, void* ws, size_t ws_bytes, uint32_t* err);
 

@@ -29,7 +29,7 @@
 
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err);
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false);
 
 namespace {
 
@@ -59,15 +59,18 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v) {
   return v;
 }
 
-// Also clears the build's info words and the hierarchy's split counters
-// (flags, n words), which saves two fill launches per build.
+// Also clears the build's info words, the hierarchy's split counters
+// (flags, n words) and the sort's error word and digit counts (sort_head,
+// head_words), which saves three fill launches per build.
 template <typename K>
 __global__ __launch_bounds__(256) void k_fast_prep(const uint4* __restrict__ faces, const float4* __restrict__ verts,
                                                    uint32_t n, float4 rmin, float4 rmax, Box* __restrict__ boxes,
                                                    K* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                   uint32_t* __restrict__ info, uint32_t* __restrict__ flags) {
+                                                   uint32_t* __restrict__ info, uint32_t* __restrict__ flags,
+                                                   uint32_t* __restrict__ sort_head, uint32_t head_words) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k < RK_INFO_WORDS) info[k] = 0u;
+  if (k < head_words) sort_head[k] = 0u;
   if (k >= n) return;
   flags[k] = 0u;
   uint4 f = faces[k];
@@ -822,22 +825,25 @@ __device__ __forceinline__ uint32_t spread_ray(uint32_t t, uint32_t n_rays) {
 constexpr int kTraceWG = RK_TRACE_WG;
 static_assert(kTraceWG % 64 == 0 && kTraceWG <= 1024, "whole waves");
 
-template <bool kCount>
-__global__ __launch_bounds__(kTraceWG) void k_fast_trace(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
-                                                    uint32_t n_tris, const float4* __restrict__ origins,
-                                                    const float4* __restrict__ dirs, uint32_t n_rays,
-                                                    uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                    float* __restrict__ depths, float eps,
-                                                    unsigned long long* __restrict__ counters,
-                                                    const uint32_t* __restrict__ order) {
-  const uint32_t t = blockIdx.x * kTraceWG + threadIdx.x;
-  const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : spread_ray(t, n_rays);
+// kWG threads per workgroup, runs of kRun rays per band (spread_ray; 0: 64
+// consecutive rays per wave), kLds stack entries per lane in LDS. The
+// defaults above are one variant; rk_fast_autotune chooses among several.
+template <bool kCount, int kWG, uint32_t kRun, int kLds>
+__global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                                      uint32_t n_tris, const float4* __restrict__ origins,
+                                                      const float4* __restrict__ dirs, uint32_t n_rays,
+                                                      uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                      float* __restrict__ depths, float eps,
+                                                      unsigned long long* __restrict__ counters,
+                                                      const uint32_t* __restrict__ order) {
+  const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+  const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : rk::band_ray<kRun>(t, n_rays);
   const bool has_ray = l < n_rays;
-  __shared__ uint2 s_stack[kLdsStack * kTraceWG];
-  __shared__ uint32_t s_xchg[kTraceWG / 64][kXchgWords];
+  __shared__ uint2 s_stack[kLds * kWG];
+  __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const RayHit h = trace_ray_shared<kCount, kStack, kLdsStack, kTraceWG>(
+  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
       has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
@@ -920,11 +926,15 @@ static rk_status fast_build_keys(hipStream_t st, const uint4* f, const float4* v
                                  float4 rmax, const rk_fast_bvh* bvh, uint32_t leaf_max, K* keys, const HierWs& hw,
                                  void* sort_ws, size_t sort_bytes, uint32_t* info) {
   uint32_t* order = bvh->indices;
-  hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(n < RK_INFO_WORDS ? RK_INFO_WORDS : n, 256)), dim3(256), 0, st, f,
-                     v, n, rmin, rmax, reinterpret_cast<Box*>(bvh->boxes), keys, order, info, hw.flags);
+  // the sort's workspace starts with its error word and digit counts
+  const uint32_t head_words = 1u + (uint32_t)sizeof(K) * 256u;
+  const uint32_t grid_n = std::max(n, std::max((uint32_t)RK_INFO_WORDS, head_words));
+  hipLaunchKernelGGL(k_fast_prep<K>, dim3(blocks_for(grid_n, 256)), dim3(256), 0, st, f, v, n, rmin, rmax,
+                     reinterpret_cast<Box*>(bvh->boxes), keys, order, info, hw.flags, static_cast<uint32_t*>(sort_ws),
+                     head_words);
   RK_CHECK_LAUNCH();
   rk_status s = rk_radix_sort_impl<K>(st, keys, order, n, 0, (int)sizeof(K), sort_ws, sort_bytes,
-                                      info + RK_INFO_FLAGS);
+                                      info + RK_INFO_FLAGS, true);
   if (s != RK_SUCCESS) return s;
   return launch_hierarchy<K, false>(st, keys, order, f, v, n, leaf_max, hw, static_cast<FNode*>(bvh->nodes),
                                     static_cast<float4*>(bvh->tris), (const rk_batch_mesh*)nullptr, true);
@@ -1388,20 +1398,6 @@ static inline float det_eps_of(const rk_trace_options* o) {
   return o ? o->det_epsilon : RK_DET_EPSILON_REFERENCE;
 }
 
-extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
-                                   const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
-                                   float* depths, const rk_trace_options* options) {
-  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
-  if (n_rays == 0) return RK_SUCCESS;
-  if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
-                     static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
-                     reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
-                     faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
-                     (unsigned long long*)nullptr, (const uint32_t*)nullptr);
-  RK_CHECK_LAUNCH();
-  return RK_SUCCESS;
-}
 
 extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                            const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id,
@@ -1410,7 +1406,7 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !counters) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL(k_fast_trace<true>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL((k_fast_trace_v<true, kTraceWG, kSpreadRun, kLdsStack>), dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
@@ -1586,32 +1582,57 @@ __global__ __launch_bounds__(256) void k_checksum_to_host(const uint32_t* __rest
   finish_checksum(bh, bn, partials, counter, out, done_value, build_flags);
 }
 
+// Which pixel a camera-kernel lane traces. Wave g takes a row segment of
+// 64 / kBands pixels in each of kBands horizontal bands of the image, at the
+// same place in every band, for the reason given at spread_ray.
+//
+// With kXcd, workgroups are dealt to the eight XCDs round-robin (block b runs
+// on XCD b mod 8), and XCD x takes the x-th of eight vertical strips of the
+// image: its L2 then holds the nodes of one eighth of the view, where each L2
+// would otherwise fetch the nodes of the whole view. The waves of a strip are
+// taken row segment by row segment; the horizon, which crosses every strip,
+// spreads over all eight. Blocks past a strip's waves have no pixels.
+template <int kWG, int kBands, bool kXcd>
+__device__ __forceinline__ bool camera_pixel(int width, int height, int& x, int& y) {
+  constexpr int kRun = 64 / kBands;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int segs = (width + kRun - 1) / kRun, band_h = (height + kBands - 1) / kBands;
+  int seg, row_in_band;
+  if (kXcd) {
+    const int sw = (segs + 7) / 8;  // segments per strip
+    const int xcd = blockIdx.x & 7;
+    const int w = (int)(blockIdx.x >> 3) * (kWG / 64) + wave;
+    seg = xcd * sw + w % sw;
+    row_in_band = seg < segs ? w / sw : band_h;  // the last strip can be narrower
+  } else {
+    const int g = (int)blockIdx.x * (kWG / 64) + wave;
+    seg = g % segs;
+    row_in_band = g / segs;
+  }
+  x = seg * kRun + lane % kRun;
+  y = (lane / kRun) * band_h + row_in_band;
+  return row_in_band < band_h && x < width && y < height;
+}
+
 // Primary ray of pixel (x, y) generated in the trace kernel itself (the
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
 // a miss gives face ~0, coords (0, 0), depth FLT_MAX.
-__global__ __launch_bounds__(kTraceWG) void k_fast_trace_camera(const FNode* __restrict__ nodes,
-                                                           const float4* __restrict__ tris, uint32_t n_tris,
-                                                           float4 ro, float4 rd, float4 rx, float4 ry, int width,
-                                                           int height, uint32_t* __restrict__ faces_id,
-                                                           float2* __restrict__ coords, float* __restrict__ depths,
-                                                           float eps) {
-  // Wave g takes a row segment of kCamRun pixels in each of kCamBands
-  // horizontal bands of the image, at the same place in every band, for the
-  // reason given at spread_ray.
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = blockIdx.x * (kTraceWG / 64) + wave;
-  const int segs = (width + kCamRun - 1) / kCamRun, band_h = (height + kCamBands - 1) / kCamBands;
-  const int row_in_band = g / segs;
-  const int x = (g % segs) * kCamRun + lane % kCamRun;
-  const int y = (lane / kCamRun) * band_h + row_in_band;
-  const bool has_ray = row_in_band < band_h && x < width && y < height;
+template <int kWG, int kBands, bool kXcd>
+__global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __restrict__ nodes,
+                                                             const float4* __restrict__ tris, uint32_t n_tris,
+                                                             float4 ro, float4 rd, float4 rx, float4 ry, int width,
+                                                             int height, uint32_t* __restrict__ faces_id,
+                                                             float2* __restrict__ coords, float* __restrict__ depths,
+                                                             float eps) {
+  int x, y;
+  const bool has_ray = camera_pixel<kWG, kBands, kXcd>(width, height, x, y);
   const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
   uint32_t nb = 0, nt = 0;
-  __shared__ uint2 s_stack[kLdsStack * kTraceWG];
-  __shared__ uint32_t s_xchg[kTraceWG / 64][kXchgWords];
-  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kTraceWG>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
-                                                                   eps, nb, nt, s_stack, s_xchg[wave]);
+  __shared__ uint2 s_stack[kLdsStack * kWG];
+  __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
+                                                                   eps, nb, nt, s_stack, s_xchg[threadIdx.x >> 6]);
   if (!has_ray) return;
   const size_t p = (size_t)y * (size_t)width + (size_t)x;
   const bool hit = h.pos != 0xFFFFFFFFu;
@@ -1620,9 +1641,229 @@ __global__ __launch_bounds__(kTraceWG) void k_fast_trace_camera(const FNode* __r
   depths[p] = hit ? h.t : RK_HUGE;
 }
 
+// Workgroups of a camera-kernel variant for a width x height frame.
+uint32_t camera_blocks(int width, int height, int wg, int bands, bool xcd) {
+  const int run = 64 / bands;
+  const uint64_t segs = (uint64_t)((width + run - 1) / run), band_h = (uint64_t)((height + bands - 1) / bands);
+  const uint64_t wpb = (uint64_t)(wg / 64);
+  if (xcd) {
+    const uint64_t sw = (segs + 7) / 8;
+    return (uint32_t)(8 * ((sw * band_h + wpb - 1) / wpb));
+  }
+  return (uint32_t)((segs * band_h + wpb - 1) / wpb);
+}
+
 }  // namespace
 
-uint32_t rk_camera_blocks(int width, int height);
+// ---- traversal variants and the autotuner ------------------------------------------
+namespace {
+
+typedef void (*TraceKernel)(const FNode*, const float4*, uint32_t, const float4*, const float4*, uint32_t, uint32_t*,
+                            float2*, float*, float, unsigned long long*, const uint32_t*);
+typedef void (*CameraKernel)(const FNode*, const float4*, uint32_t, float4, float4, float4, float4, int, int,
+                             uint32_t*, float2*, float*, float);
+struct TraceVariant {
+  uint32_t wg, run, lds;
+  TraceKernel kernel;
+};
+struct CameraVariant {
+  uint32_t wg, bands, xcd;
+  CameraKernel kernel;
+};
+
+// rk_trace_options.config k selects entry k - 1 (0: the first, the default).
+#define RK_TRACE_VARIANT(wg, run, lds) {wg, run, lds, k_fast_trace_v<false, wg, run, lds>}
+const TraceVariant kTraceVariants[] = {
+    RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack),
+    RK_TRACE_VARIANT(64, 4, 8),  RK_TRACE_VARIANT(64, 16, 8),  RK_TRACE_VARIANT(64, 0, 8),
+    RK_TRACE_VARIANT(64, 8, 4),  RK_TRACE_VARIANT(64, 8, 12),  RK_TRACE_VARIANT(128, 8, 8),
+    RK_TRACE_VARIANT(128, 16, 8), RK_TRACE_VARIANT(256, 8, 8),
+};
+#undef RK_TRACE_VARIANT
+#define RK_CAMERA_VARIANT(wg, bands, xcd) {wg, bands, xcd, k_fast_trace_camera_v<wg, bands, xcd != 0>}
+const CameraVariant kCameraVariants[] = {
+    RK_CAMERA_VARIANT(kTraceWG, kCamBands, 0),
+    RK_CAMERA_VARIANT(64, 8, 1),  RK_CAMERA_VARIANT(64, 4, 0),   RK_CAMERA_VARIANT(64, 4, 1),
+    RK_CAMERA_VARIANT(64, 16, 0), RK_CAMERA_VARIANT(64, 16, 1),  RK_CAMERA_VARIANT(128, 8, 0),
+    RK_CAMERA_VARIANT(128, 8, 1), RK_CAMERA_VARIANT(128, 16, 0), RK_CAMERA_VARIANT(128, 16, 1),
+};
+#undef RK_CAMERA_VARIANT
+constexpr uint32_t kNumTraceVariants = sizeof(kTraceVariants) / sizeof(kTraceVariants[0]);
+constexpr uint32_t kNumCameraVariants = sizeof(kCameraVariants) / sizeof(kCameraVariants[0]);
+
+const TraceVariant* trace_variant(uint32_t config) {
+  return config == 0 ? &kTraceVariants[0] : (config <= kNumTraceVariants ? &kTraceVariants[config - 1] : nullptr);
+}
+const CameraVariant* camera_variant(uint32_t config) {
+  return config == 0 ? &kCameraVariants[0] : (config <= kNumCameraVariants ? &kCameraVariants[config - 1] : nullptr);
+}
+
+rk_status launch_trace_variant(hipStream_t st, const TraceVariant& tv, const rk_fast_bvh* bvh,
+                               const rk_float4* origins, const rk_float4* directions, uint32_t n_rays,
+                               uint32_t* faces_id, rk_float2* coords, float* depths, float eps) {
+  hipLaunchKernelGGL(tv.kernel, dim3(blocks_for(n_rays, tv.wg)), dim3(tv.wg), 0, st,
+                     static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
+                     reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
+                     faces_id, reinterpret_cast<float2*>(coords), depths, eps, (unsigned long long*)nullptr,
+                     (const uint32_t*)nullptr);
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+void describe_trace(uint32_t config, rk_trace_tuning* out) {
+  const TraceVariant& v = *trace_variant(config);
+  *out = rk_trace_tuning{};
+  out->config = config;
+  out->workgroup = v.wg;
+  out->ray_run = v.run;
+  out->lds_stack = v.lds;
+}
+void describe_camera(uint32_t config, rk_trace_tuning* out) {
+  const CameraVariant& v = *camera_variant(config);
+  *out = rk_trace_tuning{};
+  out->config = config;
+  out->workgroup = v.wg;
+  out->ray_run = 64 / v.bands;
+  out->bands = v.bands;
+  out->lds_stack = kLdsStack;
+  out->xcd_strips = v.xcd;
+}
+
+// Times fn() reps times after one untimed call, each call preceded by
+// prep() outside the timed region; returns the mean in ms.
+template <typename Prep, typename Fn>
+rk_status time_calls(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, Prep prep, Fn fn, float* ms) {
+  float total = 0.0f;
+  for (int r = 0; r <= reps; ++r) {
+    rk_status s = prep();
+    if (s != RK_SUCCESS) return s;
+    RK_HIP(hipEventRecord(e0, st));
+    s = fn();
+    if (s != RK_SUCCESS) return s;
+    RK_HIP(hipEventRecord(e1, st));
+    RK_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    RK_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (r > 0) total += t;
+  }
+  *ms = total / (float)reps;
+  return RK_SUCCESS;
+}
+
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+}  // namespace
+
+extern "C" rk_status rk_fast_trace_variant(uint32_t config, int camera, rk_trace_tuning* out) {
+  if (!out) return RK_ERROR_INVALID_VALUE;
+  if (camera ? !camera_variant(config) : !trace_variant(config)) return RK_ERROR_INVALID_VALUE;
+  if (camera)
+    describe_camera(config, out);
+  else
+    describe_trace(config, out);
+  return RK_SUCCESS;
+}
+
+extern "C" size_t rk_fast_autotune_workspace_size(uint32_t n_rays) {
+  return rk_align_up((size_t)(n_rays ? n_rays : 1) * 4) + 256;
+}
+
+extern "C" rk_status rk_fast_autotune(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                                      const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id,
+                                      rk_float2* coords, float* depths, const rk_trace_options* options, int reps,
+                                      void* ws, size_t ws_bytes, rk_trace_tuning* best) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !best) return RK_ERROR_INVALID_VALUE;
+  if (n_rays == 0 || !origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  if (reps < 1 || reps > 100) return RK_ERROR_INVALID_VALUE;
+  if (!ws || ws_bytes < rk_fast_autotune_workspace_size(n_rays)) return RK_ERROR_WORKSPACE;
+  hipStream_t st = rk_hip_stream(stream);
+  const float eps = det_eps_of(options);
+  float* tmax = static_cast<float*>(ws);
+  RK_HIP(hipMemcpyAsync(tmax, depths, 4ull * n_rays, hipMemcpyDeviceToDevice, st));
+  EventPair ev;
+  RK_HIP(hipEventCreate(&ev.e0));
+  RK_HIP(hipEventCreate(&ev.e1));
+  auto restore = [&]() -> rk_status {
+    RK_HIP(hipMemcpyAsync(depths, tmax, 4ull * n_rays, hipMemcpyDeviceToDevice, st));
+    return RK_SUCCESS;
+  };
+  uint32_t best_cfg = 0;
+  float best_ms = 0.0f;
+  for (uint32_t k = 0; k < kNumTraceVariants; ++k) {
+    float ms = 0.0f;
+    const TraceVariant& tv = kTraceVariants[k];
+    rk_status s = time_calls(st, ev.e0, ev.e1, reps, restore, [&]() {
+      return launch_trace_variant(st, tv, bvh, origins, directions, n_rays, faces_id, coords, depths, eps);
+    }, &ms);
+    if (s != RK_SUCCESS) return s;
+    if (k == 0 || ms < best_ms) {
+      best_ms = ms;
+      best_cfg = k + 1;
+    }
+  }
+  // leave the caller's frame traced by the chosen variant
+  rk_status s = restore();
+  if (s != RK_SUCCESS) return s;
+  s = launch_trace_variant(st, *trace_variant(best_cfg), bvh, origins, directions, n_rays, faces_id, coords, depths,
+                           eps);
+  if (s != RK_SUCCESS) return s;
+  describe_trace(best_cfg, best);
+  best->ms = best_ms;
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_fast_autotune_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
+                                             int width, int height, uint32_t* faces_id, rk_float2* coords,
+                                             float* depths, const rk_trace_options* options, int reps,
+                                             rk_trace_tuning* best) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !camera || !best) return RK_ERROR_INVALID_VALUE;
+  if (width <= 0 || height <= 0 || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
+  if (reps < 1 || reps > 100) return RK_ERROR_INVALID_VALUE;
+  hipStream_t st = rk_hip_stream(stream);
+  EventPair ev;
+  RK_HIP(hipEventCreate(&ev.e0));
+  RK_HIP(hipEventCreate(&ev.e1));
+  rk_trace_options opt = options ? *options : rk_trace_options{RK_DET_EPSILON_REFERENCE, 0u, 0u, 0u};
+  uint32_t best_cfg = 0;
+  float best_ms = 0.0f;
+  for (uint32_t k = 0; k < kNumCameraVariants; ++k) {
+    float ms = 0.0f;
+    opt.config = k + 1;
+    rk_status s = time_calls(st, ev.e0, ev.e1, reps, []() { return (rk_status)RK_SUCCESS; }, [&]() {
+      return rk_fast_trace_camera(stream, bvh, camera, width, height, faces_id, coords, depths, &opt);
+    }, &ms);
+    if (s != RK_SUCCESS) return s;
+    if (k == 0 || ms < best_ms) {
+      best_ms = ms;
+      best_cfg = k + 1;
+    }
+  }
+  opt.config = best_cfg;
+  rk_status s = rk_fast_trace_camera(stream, bvh, camera, width, height, faces_id, coords, depths, &opt);
+  if (s != RK_SUCCESS) return s;
+  describe_camera(best_cfg, best);
+  best->ms = best_ms;
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                                   const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
+                                   float* depths, const rk_trace_options* options) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0) return RK_ERROR_INVALID_VALUE;
+  if (n_rays == 0) return RK_SUCCESS;
+  if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
+  const TraceVariant* tv = trace_variant(options ? options->config : 0u);
+  if (!tv) return RK_ERROR_INVALID_VALUE;
+  return launch_trace_variant(rk_hip_stream(stream), *tv, bvh, origins, directions, n_rays, faces_id, coords, depths,
+                              det_eps_of(options));
+}
 
 extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
                                           int width, int height, uint32_t* faces_id, rk_float2* coords, float* depths,
@@ -1631,12 +1872,13 @@ extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* b
   if (width <= 0 || height <= 0 || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
   if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull) return RK_ERROR_UNSUPPORTED;
   auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
-  // one wave per row segment of each band
-  dim3 grid(rk_camera_blocks(width, height));
-  hipLaunchKernelGGL(k_fast_trace_camera, grid, dim3(kTraceWG), 0, rk_hip_stream(stream),
-                     static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
-                     f4(camera->origin), f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
-                     faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
+  const CameraVariant* cv = camera_variant(options ? options->config : 0u);
+  if (!cv) return RK_ERROR_INVALID_VALUE;
+  dim3 grid(camera_blocks(width, height, (int)cv->wg, (int)cv->bands, cv->xcd != 0));
+  hipLaunchKernelGGL(cv->kernel, grid, dim3(cv->wg), 0, rk_hip_stream(stream), static_cast<const FNode*>(bvh->nodes),
+                     static_cast<const float4*>(bvh->tris), bvh->face_count, f4(camera->origin),
+                     f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height, faces_id,
+                     reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -1666,12 +1908,6 @@ rk_status rk_deform_and_clear(hipStream_t st, const rk_float4* base, rk_float4* 
   return RK_SUCCESS;
 }
 
-// Workgroups of k_fast_trace_camera for a width x height frame.
-uint32_t rk_camera_blocks(int width, int height) {
-  const uint64_t waves =
-      (uint64_t)((height + kCamBands - 1) / kCamBands) * (uint64_t)((width + kCamRun - 1) / kCamRun);
-  return (uint32_t)((waves + kTraceWG / 64 - 1) / (kTraceWG / 64));
-}
 
 extern "C" rk_status rk_frame_checksum(rk_stream stream, const uint32_t* faces_id, const rk_float2* coords,
                                        const float* depths, uint32_t count, const uint32_t* indices,
@@ -1730,7 +1966,7 @@ extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* b
   RK_CHECK_LAUNCH();
   rk_status s = rk_radix_sort_impl<uint32_t>(st, keys, order, n_rays, 0, 4, sort_ws, sort_bytes, nullptr);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL(k_fast_trace<false>, dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, st, nodes,
+  hipLaunchKernelGGL((k_fast_trace_v<false, kTraceWG, kSpreadRun, kLdsStack>), dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, st, nodes,
                      static_cast<const float4*>(bvh->tris), bvh->face_count, o,
                      reinterpret_cast<const float4*>(directions), n_rays, faces_id, reinterpret_cast<float2*>(coords),
                      depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order);

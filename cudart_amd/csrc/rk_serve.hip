@@ -233,7 +233,7 @@ struct rk_server_st {
     if ((e = rk_calc_primary_rays(st, (int)r.width, (int)r.height, r.camera.origin, r.camera.direction,
                                   r.camera.step_x, r.camera.step_y, sl.rays_o, sl.rays_d)) != RK_SUCCESS)
       return e;
-    rk_trace_options opt = {cfg.det_epsilon, 0, {0, 0}};
+    rk_trace_options opt = {cfg.det_epsilon, 0u, 0u, 0u};
     if ((e = rk_fast_trace(st, &bvh, sl.rays_o, sl.rays_d, n, fid, uv, dep, &opt)) != RK_SUCCESS) return e;
     if ((e = rk_frame_checksum(st, fid, uv, dep, n, s.indices, s.nf, sl.sum_dev)) != RK_SUCCESS) return e;
     RK_HIP(hipMemcpyAsync(sl.sum_host, sl.sum_dev, 16, hipMemcpyDeviceToHost, sl.st));
@@ -322,7 +322,7 @@ struct rk_server_st {
     uint32_t* fid = r.faces_id ? r.faces_id : sl.fid;
     rk_float2* uv = r.coords ? r.coords : sl.uv;
     float* dep = r.depths ? r.depths : sl.dep;
-    rk_trace_options opt = {cfg.det_epsilon, 0, {0, 0}};
+    rk_trace_options opt = {cfg.det_epsilon, 0u, cfg.trace_config, 0u};
     if ((e = rk_fast_trace_camera(st, &bvh, &r.camera, (int)r.width, (int)r.height, fid, uv, dep, &opt)) !=
         RK_SUCCESS)
       return e;

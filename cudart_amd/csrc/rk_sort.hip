@@ -71,10 +71,14 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_t
 constexpr int kHistUnroll = 8;
 constexpr uint32_t kHistMaxBlocks = 1024;
 
+// Also zeroes the look-back status words of the passes that follow (zero,
+// zero_words), which saves a fill launch per sort.
 template <typename K>
 __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, uint32_t n, int passes, int first_pass,
-                                                 uint32_t chunk, uint32_t* __restrict__ hist) {
+                                                 uint32_t chunk, uint32_t* __restrict__ hist,
+                                                 uint32_t* __restrict__ zero, size_t zero_words) {
   __shared__ uint32_t s[kMaxPasses * 256];
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * 256) zero[i] = 0u;
   for (int i = threadIdx.x; i < passes * 256; i += 256) s[i] = 0;
   __syncthreads();
   const size_t b0 = (size_t)blockIdx.x * chunk;
@@ -411,9 +415,12 @@ extern "C" size_t rk_sort_workspace_size(uint32_t length, uint32_t key_bytes) {
 // 'err' (device, may be NULL) receives RK_FLAG_SPIN_LIMIT on a look-back
 // timeout; when it is NULL the first word of 'ws' does, cleared by this call.
 // The look-back status words count keys in 30 bits, so n < 2^30.
+// head_cleared: the caller has zeroed the first rk_sort_head_words(passes)
+// words of ws (error word and digit counts) in an earlier kernel on 'st', so
+// no fill runs here.
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err) {
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false) {
   if (n >= kAgg) return RK_ERROR_UNSUPPORTED;
   if (n <= 1 || passes <= 0) return err ? RK_SUCCESS : clear_ws_flags(st, ws, ws_bytes);
   if (!keys || passes > kMaxPasses) return RK_ERROR_INVALID_VALUE;
@@ -429,11 +436,14 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
   uint32_t* status = hist + (size_t)passes * 256;
   if (!err) err = werr;
 
-  rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, ctl_words, 0u);
-  if (s != RK_SUCCESS) return s;
+  if (!head_cleared) {
+    rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, 1 + (size_t)passes * 256, 0u);
+    if (s != RK_SUCCESS) return s;
+  }
   const unsigned hist_blocks = grid_for(n, 256 * kHistUnroll * 2, kHistMaxBlocks);
   const uint32_t chunk = (uint32_t)(((size_t)n + hist_blocks - 1) / hist_blocks);
-  hipLaunchKernelGGL(k_rs_hist<K>, dim3(hist_blocks), dim3(256), 0, st, keys, n, passes, first_pass, chunk, hist);
+  hipLaunchKernelGGL(k_rs_hist<K>, dim3(hist_blocks), dim3(256), 0, st, keys, n, passes, first_pass, chunk, hist,
+                     status, (size_t)passes * T * 256);
   RK_CHECK_LAUNCH();
   K* ks = keys;
   K* kd = kalt;
@@ -467,9 +477,9 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
 }
 
 template rk_status rk_radix_sort_impl<uint32_t>(hipStream_t, uint32_t*, uint32_t*, uint32_t, int, int, void*,
-                                                size_t, uint32_t*);
+                                                size_t, uint32_t*, bool);
 template rk_status rk_radix_sort_impl<uint64_t>(hipStream_t, uint64_t*, uint32_t*, uint32_t, int, int, void*,
-                                                size_t, uint32_t*);
+                                                size_t, uint32_t*, bool);
 
 // The reference tie order of CUDA_radixSortGeneral (CUDA_radixSortGeneral.cu:397-482):
 //  length <= 512 (bubble, CUDA_bubbleSort512.cu): stable by key;

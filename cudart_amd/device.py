@@ -241,19 +241,55 @@ class FastBuilder:
         return FastTree(self.nodes, self.tris, self.indices[:n], self.boxes[:n], self.info, desc)
 
 
-def _opt(det_epsilon):
-    o = N.trace_options(det_epsilon)
+def _opt(det_epsilon, config=0):
+    o = N.trace_options(det_epsilon, config)
     return None if o is None else C.byref(o)
 
 
-def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
-    """det_epsilon None = the reference threshold (1e-6); 0.0 = only parallel rays rejected."""
+def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None, config=0):
+    """det_epsilon None = the reference threshold (1e-6); 0.0 = only parallel rays rejected.
+    config: a traversal variant from autotune() (0 = the default)."""
     with _on(stream):
         n = O.shape[0]
         fid, uv, dep = out if out is not None else new_outputs(n)
         _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                     fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)), "fast_trace")
+                                     fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon, config)),
+               "fast_trace")
         return fid, uv, dep
+
+
+def autotune(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out, det_epsilon=None, reps=3, stream=None):
+    """rk_fast_autotune: times every traversal variant on these rays (out[2]
+    holds tmax and is restored before each call) and returns the fastest
+    (N.TraceTuning); out then holds the frame it traced."""
+    with _on(stream):
+        n = O.shape[0]
+        fid, uv, dep = out
+        ws = workspace(N.lib().rk_fast_autotune_workspace_size(n))
+        best = N.TraceTuning()
+        _check(N.lib().rk_fast_autotune(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
+                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon), reps,
+                                        ws.data_ptr(), ws.numel(), C.byref(best)), "fast_autotune")
+        return best
+
+
+def autotune_camera(tree: FastTree, cam: np.ndarray, width: int, height: int, out, det_epsilon=None, reps=3,
+                    stream=None):
+    """rk_fast_autotune_camera: the fastest camera-kernel variant for this frame."""
+    with _on(stream):
+        fid, uv, dep = out
+        best = N.TraceTuning()
+        c = camera_rays(cam)
+        _check(N.lib().rk_fast_autotune_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
+                                               fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                               reps, C.byref(best)), "fast_autotune_camera")
+        return best
+
+
+def trace_variant(config: int, camera: bool = False) -> N.TraceTuning:
+    t = N.TraceTuning()
+    _check(N.lib().rk_fast_trace_variant(config, int(camera), C.byref(t)), "fast_trace_variant")
+    return t
 
 
 def fast_trace_counted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None):
@@ -414,13 +450,13 @@ def camera_rays(cam: np.ndarray) -> N.CameraRays:
 
 
 def fast_trace_camera(tree: FastTree, cam: np.ndarray, width: int, height: int, out=None, det_epsilon=None,
-                      stream=None):
+                      stream=None, config=0):
     with _on(stream):
         n = width * height
         fid, uv, dep = out if out is not None else new_outputs(n)
         c = camera_rays(cam)
         _check(N.lib().rk_fast_trace_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
-                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon)),
+                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon, config)),
                "fast_trace_camera")
         return fid, uv, dep
 

@@ -39,17 +39,27 @@ class AABox(C.Structure):
 
 
 class TraceOptions(C.Structure):
-    _fields_ = [("det_epsilon", C.c_float), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("det_epsilon", C.c_float), ("flags", C.c_uint32), ("config", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class TraceTuning(C.Structure):
+    _fields_ = [("config", C.c_uint32), ("workgroup", C.c_uint32), ("ray_run", C.c_uint32), ("bands", C.c_uint32),
+                ("lds_stack", C.c_uint32), ("xcd_strips", C.c_uint32), ("ms", C.c_float), ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 DET_EPSILON_REFERENCE = 0.000001
 
 
-def trace_options(det_epsilon: float | None) -> TraceOptions | None:
-    """None = the reference threshold (1e-6); 0.0 = reject only parallel rays."""
-    if det_epsilon is None:
+def trace_options(det_epsilon: float | None, config: int = 0) -> TraceOptions | None:
+    """None = the reference threshold (1e-6); 0.0 = reject only parallel rays.
+    config: a traversal variant (rk_fast_autotune); 0 = the default."""
+    if det_epsilon is None and config == 0:
         return None
-    return TraceOptions(float(det_epsilon), 0, (C.c_uint32 * 2)(0, 0))
+    eps = DET_EPSILON_REFERENCE if det_epsilon is None else float(det_epsilon)
+    return TraceOptions(eps, 0, int(config), 0)
 
 
 class BatchMesh(C.Structure):
@@ -80,7 +90,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("leaf_max", C.c_uint32), ("rebuild_every", C.c_uint32),
                 ("max_faces", C.c_uint32), ("max_pixels", C.c_uint32), ("det_epsilon", C.c_float),
                 ("amplitude_frac", C.c_float), ("omega", C.c_float), ("hbm_budget", C.c_uint64),
-                ("plain", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("plain", C.c_uint32), ("trace_config", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class Request(C.Structure):
@@ -163,6 +173,12 @@ _SIGS = {
     "rk_fast_build_keyed": (_I, [_P, _P, _P, _U32, _P, _I, _U32, C.POINTER(FastBVH), _P, _P, _P, _SZ]),
     "rk_fast_refit": (_I, [_P, _P, _P, _U32, _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, _P]),
+    "rk_fast_trace_variant": (_I, [_U32, _I, C.POINTER(TraceTuning)]),
+    "rk_fast_autotune_workspace_size": (_SZ, [_U32]),
+    "rk_fast_autotune": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _I, _P, _SZ,
+                              C.POINTER(TraceTuning)]),
+    "rk_fast_autotune_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, _P, _I,
+                                     C.POINTER(TraceTuning)]),
     "rk_deform_vertices": (_I, [_P, _P, _P, _U32, C.c_float, C.c_float, C.c_float, C.c_uint64, _I]),
     "rk_frame_checksum": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P]),
     # serving

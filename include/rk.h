@@ -251,7 +251,8 @@ rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4
 typedef struct rk_trace_options {
   float det_epsilon;
   uint32_t flags;
-  uint32_t reserved[2];
+  uint32_t config;   /* traversal variant (rk_fast_autotune, rk_fast_trace_variant); 0: the default */
+  uint32_t reserved;
 } rk_trace_options;
 /* Closest hit for n_rays rays. faces_id receives the sorted position (index into
  * bvh->tris; original face id = bvh->indices[position]); depths is read as tmax. */
@@ -318,6 +319,40 @@ typedef struct rk_camera_rays {
 rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera, int width,
                                int height, uint32_t* faces_id, rk_float2* coords, float* depths,
                                const rk_trace_options* options);
+
+/* Traversal variants and their autotuner (no counterpart in the reference, whose
+ * launch shapes are fixed: 16x12 packets, 96-entry stacks, RT.cu:434-480). The
+ * results of every variant are identical; they differ in speed. A variant of
+ * rk_fast_trace sets threads per workgroup (workgroup), how many consecutive rays
+ * a wave takes from each band of the ray array (ray_run; 0: 64 consecutive rays)
+ * and how many stack entries per lane live in LDS (lds_stack); a variant of
+ * rk_fast_trace_camera sets the workgroup, the bands of rows a wave draws its
+ * pixels from (bands, of ray_run = 64 / bands pixels each) and whether each of
+ * the eight XCDs takes one vertical strip of the image (xcd_strips). The variant
+ * is rk_trace_options.config; rk_fast_trace_variant describes one (camera != 0:
+ * a camera variant). */
+typedef struct rk_trace_tuning {
+  uint32_t config;
+  uint32_t workgroup, ray_run, bands, lds_stack, xcd_strips;
+  float ms;  /* mean time per call of the chosen variant in the tuning run */
+  uint32_t reserved;
+} rk_trace_tuning;
+rk_status rk_fast_trace_variant(uint32_t config, int camera, rk_trace_tuning* out);
+/* Times every rk_fast_trace variant on the caller's rays (reps timed calls each
+ * after one untimed call, tmax restored from depths' contents at entry before
+ * each call), writes the fastest to *best, and leaves the frame traced by it in
+ * faces_id/coords/depths. ws: rk_fast_autotune_workspace_size(n_rays) bytes
+ * (a copy of the entry tmax). Synchronises 'stream'. */
+size_t rk_fast_autotune_workspace_size(uint32_t n_rays);
+rk_status rk_fast_autotune(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
+                           const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
+                           float* depths, const rk_trace_options* options, int reps, void* ws, size_t ws_bytes,
+                           rk_trace_tuning* best);
+/* The same for rk_fast_trace_camera variants on the caller's camera frame
+ * (options->config is ignored). Synchronises 'stream'. */
+rk_status rk_fast_autotune_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera, int width,
+                                  int height, uint32_t* faces_id, rk_float2* coords, float* depths,
+                                  const rk_trace_options* options, int reps, rk_trace_tuning* best);
 
 /* Deforming meshes: out[v] = base[v] + amplitude * sin(omega * time + phase_v) * axis_v,
  * with phase_v and axis_v (uniform on the sphere, or +z when along_z != 0) hashed
@@ -447,7 +482,8 @@ typedef struct rk_server_config {
   float omega;             /* deformation angular rate, radians per unit of request time */
   uint64_t hbm_budget;     /* cap on arena_bytes */
   uint32_t plain;          /* != 0: the plain per-request path, a baseline (below) */
-  uint32_t reserved[3];
+  uint32_t trace_config;   /* rk_fast_trace_camera variant (rk_fast_autotune_camera); 0: the default */
+  uint32_t reserved[2];
 } rk_server_config;
 /* 16 slots, leaf 1, rebuild every 8 refits, 2^21 faces, 1024^2 pixels, threshold 0,
  * amplitude 1% of extent, omega 2 pi, budget 200 GiB, plain 0. Slots are streams: they

@@ -387,7 +387,9 @@ def test_fast_trace_exact_ties_in_shared_waves(dev, rk, ora, leaf_max):
     assert hit > 0.9
 
 
-@pytest.mark.parametrize("dist", [1e2, 1e3, 1e4])
+@pytest.mark.parametrize("dist", [1e2, 1e3, 1e4, pytest.param(1e6, marks=pytest.mark.xfail(
+    strict=True, reason="documented limit: at 1e6 times the scene's size the float32 origin carries ~0.06 units of "
+    "rounding, more than the boxes' padding, and hits at box boundaries are lost"))])
 def test_fast_trace_far_origins(dev, rk, ora, dist):
     """Rays from far away (dist times the scene's size) aimed at points of a
     small soup: the padded box test stays conservative while the rounding of
