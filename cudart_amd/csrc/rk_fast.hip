@@ -554,7 +554,13 @@ struct RayHit {
 // distance, owner lane and donor lane of each of up to 64 donations.
 constexpr int kXchgWords = 4 * 64;
 
-template <bool kCount, int kStackN, int kLds, int kWG>
+// kOldest: a donor hands out its oldest pending entry (the bottom of its
+// stack: the subtree nearest the root, the largest) instead of its newest, as
+// a work-stealing deque does, so a helper gets more work and soon has pending
+// entries of its own to hand on. The stack is then the range [sb, sp); it is
+// reset to empty at [0, 0) whenever it drains, and an index still stands for
+// one level of the current path, so sp stays within the depth bound.
+template <bool kCount, int kStackN, int kLds, int kWG, bool kOldest = false>
 __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                    uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,
                                                    float4 dir, float depth0, float eps, uint32_t& n_box,
@@ -581,7 +587,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   // wait on every memory counter) per pop.
   typedef __attribute__((address_space(3))) unsigned long long lds_u64;
   lds_u64* my_lds = (lds_u64*)(lds + threadIdx.x);
-  int sp = 0;
+  int sp = 0, sb = 0;
   auto st_get = [&](int i, uint32_t& r, float& t) {
     if (i < kLds) {
       const unsigned long long e = my_lds[i * kWG];
@@ -605,18 +611,23 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
     const uint64_t idle = __ballot(!working);
     if (idle == ~0ull) break;
     // hand the newest pending subtree of up to popc(idle) lanes to idle lanes
-    const uint64_t donors = __ballot(working && sp > 0);
+    const uint64_t donors = __ballot(working && sp > sb);
     if (idle != 0ull && donors != 0ull) {
       const uint32_t nd = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
       const uint64_t lt = rk::lanemask_lt();
       const uint32_t rd = (uint32_t)__popcll(donors & lt), ri = (uint32_t)__popcll(idle & lt);
-      if (working && sp > 0 && rd < nd) {
+      if (working && sp > sb && rd < nd) {
         // the newest entry: taking it is a pop, so the stack stays one
-        // entry per level of the current path at most
+        // entry per level of the current path at most (kOldest: the oldest)
         uint32_t r0;
         float t0;
-        --sp;
-        st_get(sp, r0, t0);
+        if (kOldest) {
+          st_get(sb, r0, t0);
+          if (++sb == sp) sb = sp = 0;
+        } else {
+          --sp;
+          st_get(sp, r0, t0);
+        }
         __hip_atomic_store(xchg + rd, r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         __hip_atomic_store(xchg + 64 + rd, __float_as_uint(t0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         __hip_atomic_store(xchg + 128 + rd, (uint32_t)owner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -655,6 +666,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
         oz = noz;
         cur = g_ref;
         sp = 0;
+        sb = 0;
         visits = 0;
         j_pos = 0xFFFFFFFFu;
         j_oid = 0xFFFFFFFFu;
@@ -739,7 +751,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
         // the nearest pending entry that can still hold a closer hit
         bound = own ? o_t : (j_pos != 0xFFFFFFFFu ? fminf(dmax, j_t) : dmax);
         bool found = false;
-        while (sp > 0) {
+        while (sp > sb) {
           --sp;
           uint32_t r;
           float t;
@@ -828,7 +840,7 @@ static_assert(kTraceWG % 64 == 0 && kTraceWG <= 1024, "whole waves");
 // kWG threads per workgroup, runs of kRun rays per band (spread_ray; 0: 64
 // consecutive rays per wave), kLds stack entries per lane in LDS. The
 // defaults above are one variant; rk_fast_autotune chooses among several.
-template <bool kCount, int kWG, uint32_t kRun, int kLds>
+template <bool kCount, int kWG, uint32_t kRun, int kLds, bool kOldest = false>
 __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                       uint32_t n_tris, const float4* __restrict__ origins,
                                                       const float4* __restrict__ dirs, uint32_t n_rays,
@@ -843,7 +855,7 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG>(
+  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
       has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
@@ -1618,7 +1630,7 @@ __device__ __forceinline__ bool camera_pixel(int width, int height, int& x, int&
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
 // a miss gives face ~0, coords (0, 0), depth FLT_MAX.
-template <int kWG, int kBands, bool kXcd>
+template <int kWG, int kBands, bool kXcd, bool kOldest = false>
 __global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __restrict__ nodes,
                                                              const float4* __restrict__ tris, uint32_t n_tris,
                                                              float4 ro, float4 rd, float4 rx, float4 ry, int width,
@@ -1631,8 +1643,9 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __rest
   uint32_t nb = 0, nt = 0;
   __shared__ uint2 s_stack[kLdsStack * kWG];
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
-  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG>(nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE,
-                                                                   eps, nb, nt, s_stack, s_xchg[threadIdx.x >> 6]);
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG, kOldest>(nodes, tris, n_tris, 0u, has_ray, ro, d,
+                                                                            RK_HUGE, eps, nb, nt, s_stack,
+                                                                            s_xchg[threadIdx.x >> 6]);
   if (!has_ray) return;
   const size_t p = (size_t)y * (size_t)width + (size_t)x;
   const bool hit = h.pos != 0xFFFFFFFFu;
@@ -1663,29 +1676,33 @@ typedef void (*TraceKernel)(const FNode*, const float4*, uint32_t, const float4*
 typedef void (*CameraKernel)(const FNode*, const float4*, uint32_t, float4, float4, float4, float4, int, int,
                              uint32_t*, float2*, float*, float);
 struct TraceVariant {
-  uint32_t wg, run, lds;
+  uint32_t wg, run, lds, oldest;
   TraceKernel kernel;
 };
 struct CameraVariant {
-  uint32_t wg, bands, xcd;
+  uint32_t wg, bands, xcd, oldest;
   CameraKernel kernel;
 };
 
 // rk_trace_options.config k selects entry k - 1 (0: the first, the default).
-#define RK_TRACE_VARIANT(wg, run, lds) {wg, run, lds, k_fast_trace_v<false, wg, run, lds>}
+#define RK_TRACE_VARIANT(wg, run, lds, old) {wg, run, lds, old, k_fast_trace_v<false, wg, run, lds, old != 0>}
 const TraceVariant kTraceVariants[] = {
-    RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack),
-    RK_TRACE_VARIANT(64, 4, 8),  RK_TRACE_VARIANT(64, 16, 8),  RK_TRACE_VARIANT(64, 0, 8),
-    RK_TRACE_VARIANT(64, 8, 4),  RK_TRACE_VARIANT(64, 8, 12),  RK_TRACE_VARIANT(128, 8, 8),
-    RK_TRACE_VARIANT(128, 16, 8), RK_TRACE_VARIANT(256, 8, 8),
+    RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack, 0),
+    RK_TRACE_VARIANT(64, 4, 8, 0),   RK_TRACE_VARIANT(64, 16, 8, 0), RK_TRACE_VARIANT(64, 0, 8, 0),
+    RK_TRACE_VARIANT(64, 8, 4, 0),   RK_TRACE_VARIANT(64, 8, 12, 0), RK_TRACE_VARIANT(128, 8, 8, 0),
+    RK_TRACE_VARIANT(128, 16, 8, 0), RK_TRACE_VARIANT(256, 8, 8, 0), RK_TRACE_VARIANT(64, 8, 8, 1),
+    RK_TRACE_VARIANT(64, 4, 8, 1),   RK_TRACE_VARIANT(64, 0, 8, 1),  RK_TRACE_VARIANT(128, 8, 8, 1),
 };
 #undef RK_TRACE_VARIANT
-#define RK_CAMERA_VARIANT(wg, bands, xcd) {wg, bands, xcd, k_fast_trace_camera_v<wg, bands, xcd != 0>}
+#define RK_CAMERA_VARIANT(wg, bands, xcd, old) \
+  {wg, bands, xcd, old, k_fast_trace_camera_v<wg, bands, xcd != 0, old != 0>}
 const CameraVariant kCameraVariants[] = {
-    RK_CAMERA_VARIANT(kTraceWG, kCamBands, 0),
-    RK_CAMERA_VARIANT(64, 8, 1),  RK_CAMERA_VARIANT(64, 4, 0),   RK_CAMERA_VARIANT(64, 4, 1),
-    RK_CAMERA_VARIANT(64, 16, 0), RK_CAMERA_VARIANT(64, 16, 1),  RK_CAMERA_VARIANT(128, 8, 0),
-    RK_CAMERA_VARIANT(128, 8, 1), RK_CAMERA_VARIANT(128, 16, 0), RK_CAMERA_VARIANT(128, 16, 1),
+    RK_CAMERA_VARIANT(kTraceWG, kCamBands, 0, 0),
+    RK_CAMERA_VARIANT(64, 8, 1, 0),   RK_CAMERA_VARIANT(64, 4, 0, 0),   RK_CAMERA_VARIANT(64, 4, 1, 0),
+    RK_CAMERA_VARIANT(64, 16, 0, 0),  RK_CAMERA_VARIANT(64, 16, 1, 0),  RK_CAMERA_VARIANT(128, 8, 0, 0),
+    RK_CAMERA_VARIANT(128, 8, 1, 0),  RK_CAMERA_VARIANT(128, 16, 0, 0), RK_CAMERA_VARIANT(128, 16, 1, 0),
+    RK_CAMERA_VARIANT(64, 8, 0, 1),   RK_CAMERA_VARIANT(64, 4, 0, 1),   RK_CAMERA_VARIANT(128, 8, 0, 1),
+    RK_CAMERA_VARIANT(64, 8, 1, 1),
 };
 #undef RK_CAMERA_VARIANT
 constexpr uint32_t kNumTraceVariants = sizeof(kTraceVariants) / sizeof(kTraceVariants[0]);
@@ -1717,6 +1734,7 @@ void describe_trace(uint32_t config, rk_trace_tuning* out) {
   out->workgroup = v.wg;
   out->ray_run = v.run;
   out->lds_stack = v.lds;
+  out->steal_oldest = v.oldest;
 }
 void describe_camera(uint32_t config, rk_trace_tuning* out) {
   const CameraVariant& v = *camera_variant(config);
@@ -1727,6 +1745,7 @@ void describe_camera(uint32_t config, rk_trace_tuning* out) {
   out->bands = v.bands;
   out->lds_stack = kLdsStack;
   out->xcd_strips = v.xcd;
+  out->steal_oldest = v.oldest;
 }
 
 // Times fn() reps times after one untimed call, each call preceded by

@@ -44,10 +44,11 @@ class TraceOptions(C.Structure):
 
 class TraceTuning(C.Structure):
     _fields_ = [("config", C.c_uint32), ("workgroup", C.c_uint32), ("ray_run", C.c_uint32), ("bands", C.c_uint32),
-                ("lds_stack", C.c_uint32), ("xcd_strips", C.c_uint32), ("ms", C.c_float), ("reserved", C.c_uint32)]
+                ("lds_stack", C.c_uint32), ("xcd_strips", C.c_uint32), ("ms", C.c_float),
+                ("steal_oldest", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 DET_EPSILON_REFERENCE = 0.000001

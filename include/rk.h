@@ -328,14 +328,16 @@ rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const r
  * and how many stack entries per lane live in LDS (lds_stack); a variant of
  * rk_fast_trace_camera sets the workgroup, the bands of rows a wave draws its
  * pixels from (bands, of ray_run = 64 / bands pixels each) and whether each of
- * the eight XCDs takes one vertical strip of the image (xcd_strips). The variant
+ * the eight XCDs takes one vertical strip of the image (xcd_strips); in either
+ * kind, lanes that are done may take the oldest pending subtree of a lane that
+ * is not, rather than its newest (steal_oldest). The variant
  * is rk_trace_options.config; rk_fast_trace_variant describes one (camera != 0:
  * a camera variant). */
 typedef struct rk_trace_tuning {
   uint32_t config;
   uint32_t workgroup, ray_run, bands, lds_stack, xcd_strips;
-  float ms;  /* mean time per call of the chosen variant in the tuning run */
-  uint32_t reserved;
+  float ms;              /* mean time per call of the chosen variant in the tuning run */
+  uint32_t steal_oldest; /* idle lanes take a busy lane's oldest pending subtree, not its newest */
 } rk_trace_tuning;
 rk_status rk_fast_trace_variant(uint32_t config, int camera, rk_trace_tuning* out);
 /* Times every rk_fast_trace variant on the caller's rays (reps timed calls each

@@ -7,8 +7,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F32MAX = np.finfo(np.float32).max
-N_TRACE_VARIANTS = 9
-N_CAMERA_VARIANTS = 10
+N_TRACE_VARIANTS = 13
+N_CAMERA_VARIANTS = 14
 
 
 @pytest.fixture(scope="module")
@@ -41,13 +41,14 @@ def _scene(rk, w, h):
 
 
 def test_variant_table(dev, rk):
-    shapes = {(dev.trace_variant(k).workgroup, dev.trace_variant(k).ray_run, dev.trace_variant(k).lds_stack)
-              for k in range(1, N_TRACE_VARIANTS + 1)}
+    shapes = {(dev.trace_variant(k).workgroup, dev.trace_variant(k).ray_run, dev.trace_variant(k).lds_stack,
+               dev.trace_variant(k).steal_oldest) for k in range(1, N_TRACE_VARIANTS + 1)}
     assert len(shapes) == N_TRACE_VARIANTS
     d = dev.trace_variant(0)
     assert (d.workgroup, d.ray_run, d.lds_stack) == (64, 8, 8)
     cams = {(dev.trace_variant(k, True).workgroup, dev.trace_variant(k, True).bands,
-             dev.trace_variant(k, True).xcd_strips) for k in range(1, N_CAMERA_VARIANTS + 1)}
+             dev.trace_variant(k, True).xcd_strips, dev.trace_variant(k, True).steal_oldest)
+            for k in range(1, N_CAMERA_VARIANTS + 1)}
     assert len(cams) == N_CAMERA_VARIANTS
     with pytest.raises(rk.RkError):
         dev.trace_variant(N_TRACE_VARIANTS + 1)
