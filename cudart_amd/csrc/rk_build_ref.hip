@@ -16,7 +16,8 @@
 
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false);
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false,
+                             const uint32_t** digit_counts = nullptr);
 rk_status rk_sort_general_impl(hipStream_t st, uint32_t* keys, uint32_t* indices, uint32_t length,
                                uint32_t bit_depth, int init_indices, void* ws, size_t ws_bytes, uint32_t* err);
 
@@ -125,22 +126,20 @@ __global__ __launch_bounds__(256) void k_morton(const Box* __restrict__ boxes, u
 // codes with code[s]^code[s+1] != 0, level = clz (0 = bit 31). Here every
 // position gets a key (32 = no split) so one stable 8-bit pass groups the
 // splits by level, ascending within a level.
+// The per-level counts come from the digit counts of that pass (its first
+// 33 digits), except for a single split, which no sort runs on: then this
+// kernel counts it (level_count, may be NULL otherwise).
 __global__ __launch_bounds__(256) void k_split_levels(const uint32_t* __restrict__ codes, uint32_t n,
                                                       uint32_t* __restrict__ lev, uint32_t* __restrict__ pos,
                                                       uint32_t* __restrict__ level_count) {
-  __shared__ uint32_t s_cnt[33];
-  if (threadIdx.x < 33) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s + 1 < n) {
     uint32_t diff = codes[s] ^ codes[s + 1];
     uint32_t level = diff ? (uint32_t)__clz((int)diff) : 32u;
     lev[s] = level;
     pos[s] = s + 1;
-    atomicAdd(&s_cnt[level], 1u);
+    if (level_count) atomicAdd(&level_count[level], 1u);
   }
-  __syncthreads();
-  if (threadIdx.x < 33 && s_cnt[threadIdx.x]) atomicAdd(&level_count[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 constexpr uint32_t kTopoThreads = 1024;
@@ -162,6 +161,15 @@ struct TopoWs {
 };
 
 __device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ A, uint32_t lo, uint32_t hi, uint32_t a) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (A[mid] <= a) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t upper_bound_lds(const uint32_t* A, uint32_t n, uint32_t a) {
+  uint32_t lo = 0, hi = n;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
     if (A[mid] <= a) lo = mid + 1; else hi = mid;
@@ -215,10 +223,16 @@ __device__ __forceinline__ uint32_t block_excl_scan1024(uint32_t v, uint32_t* s_
 //    chain_walk(a).
 // Preorder ranks follow from per-subtree internal-node counts; an internal node
 // of rank r has children 2r+1, 2r+2 (:325-326, :394-395).
+// A level's split list, when it has at most kTopoLdsA entries, is copied to
+// LDS for the searches of that level: a search then costs LDS round trips
+// instead of dependent global loads, which bound this one-workgroup kernel.
+constexpr uint32_t kTopoLdsA = 8192;
+
 __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ A,
                                                    const uint32_t* __restrict__ level_count, uint32_t n, uint32_t d,
                                                    int complete, TopoWs w, Node* __restrict__ nodes,
                                                    uint32_t* __restrict__ info) {
+  __shared__ uint32_t s_A[kTopoLdsA];
   __shared__ uint32_t s_idx[34];
   __shared__ uint32_t s_lev[kMaxLevels + 2];
   __shared__ uint32_t s_tmp[16];
@@ -247,6 +261,10 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
     const uint32_t cntL = L < 32 ? level_count[L] : 0u;
     const uint32_t EL = L < 32 ? s_idx[L + 1] : S;
     const uint32_t lastA = cntL ? A[EL - 1] : 0u;
+    const uint32_t BL = L < 32 ? s_idx[L] : S;  // level L's list is A[BL, EL)
+    const bool cached = cntL != 0 && cntL <= kTopoLdsA && L < d;
+    if (cached)
+      for (uint32_t k = tid; k < cntL; k += kTopoThreads) s_A[k] = A[BL + k];
     if (tid == 0) s_j0 = 0xFFFFFFFFu;
     __syncthreads();
     for (uint32_t r = cur + tid; r < end; r += kTopoThreads) {
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
             uint32_t q = S;
             bool entry_leaf = false;
             if (cntL == 0) entry_leaf = true;
-            else if (a < lastA) q = upper_bound(A, s_idx[L], EL, a);
+            else if (a < lastA) q = cached ? BL + upper_bound_lds(s_A, cntL, a) : upper_bound(A, BL, EL, a);
             else if (r == j0) q = q0;
             else if (q0 == EL) entry_leaf = true;
             else q = chain_walk(A, s_idx, L, a, S);
@@ -287,7 +305,7 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
                 atomicOr(&s_flags, (uint32_t)RK_FLAG_WALK_PAST_END);
               } else {
                 if (q >= past_depth) atomicOr(&s_flags, (uint32_t)RK_FLAG_WALK_PAST_DEPTH);
-                uint32_t sp = A[q];
+                uint32_t sp = (cached && q >= BL && q < EL) ? s_A[q - BL] : A[q];
                 if (sp < b) { leafish = false; internal = 1; split = sp; ckind = 0; }
               }
             }
@@ -573,14 +591,20 @@ extern "C" rk_status rk_build_lbvh(rk_stream stream, rk_uint4* faces, const rk_f
   // CUDA_radixSortGeneral(codes, indices, N, 32, true), CUDA_buildMeshTreeLBVH.cu:522-528.
   s = rk_sort_general_impl(st, codes, indices, n, 32, 0, sort_ws, sort_bytes, info + RK_INFO_FLAGS);
   if (s != RK_SUCCESS) return s;
+  const uint32_t* counts = level_count;  // zero when there is no split
   if (n > 1) {
+    // one split is not sorted, so its level is counted here; more are counted
+    // by the sort that groups them by level
     hipLaunchKernelGGL(k_split_levels, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, codes, n, lev, pos,
-                       level_count);
+                       n == 2 ? level_count : (uint32_t*)nullptr);
     RK_CHECK_LAUNCH();
-    s = rk_radix_sort_impl<uint32_t>(st, lev, pos, n - 1, 0, 1, sort_ws, sort_bytes, info + RK_INFO_FLAGS);
+    const uint32_t* digit_counts = nullptr;
+    s = rk_radix_sort_impl<uint32_t>(st, lev, pos, n - 1, 0, 1, sort_ws, sort_bytes, info + RK_INFO_FLAGS, false,
+                                     &digit_counts);
     if (s != RK_SUCCESS) return s;
+    if (digit_counts) counts = digit_counts;
   }
-  hipLaunchKernelGGL(k_topo_ref, dim3(1), dim3(kTopoThreads), 0, st, pos, level_count, n, tree_depth, tree_complete,
+  hipLaunchKernelGGL(k_topo_ref, dim3(1), dim3(kTopoThreads), 0, st, pos, counts, n, tree_depth, tree_complete,
                      tw, reinterpret_cast<Node*>(nodes), info);
   RK_CHECK_LAUNCH();
   unsigned lb = blocks_for(n, 64 * 4);

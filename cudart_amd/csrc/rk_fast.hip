@@ -29,7 +29,8 @@
 
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false);
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false,
+                             const uint32_t** digit_counts = nullptr);
 
 namespace {
 

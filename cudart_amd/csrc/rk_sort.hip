@@ -418,9 +418,13 @@ extern "C" size_t rk_sort_workspace_size(uint32_t length, uint32_t key_bytes) {
 // head_cleared: the caller has zeroed the first rk_sort_head_words(passes)
 // words of ws (error word and digit counts) in an earlier kernel on 'st', so
 // no fill runs here.
+// digit_counts (may be NULL): receives the device address of the first pass's
+// 256 digit counts, valid once the sort has run on 'st' (NULL when n <= 1).
 template <typename K>
 rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n, int first_pass, int passes,
-                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false) {
+                             void* ws, size_t ws_bytes, uint32_t* err, bool head_cleared = false,
+                             const uint32_t** digit_counts = nullptr) {
+  if (digit_counts) *digit_counts = nullptr;
   if (n >= kAgg) return RK_ERROR_UNSUPPORTED;
   if (n <= 1 || passes <= 0) return err ? RK_SUCCESS : clear_ws_flags(st, ws, ws_bytes);
   if (!keys || passes > kMaxPasses) return RK_ERROR_INVALID_VALUE;
@@ -435,6 +439,7 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
   uint32_t* hist = ctl + 1;
   uint32_t* status = hist + (size_t)passes * 256;
   if (!err) err = werr;
+  if (digit_counts) *digit_counts = hist;
 
   if (!head_cleared) {
     rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, 1 + (size_t)passes * 256, 0u);
@@ -477,9 +482,9 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
 }
 
 template rk_status rk_radix_sort_impl<uint32_t>(hipStream_t, uint32_t*, uint32_t*, uint32_t, int, int, void*,
-                                                size_t, uint32_t*, bool);
+                                                size_t, uint32_t*, bool, const uint32_t**);
 template rk_status rk_radix_sort_impl<uint64_t>(hipStream_t, uint64_t*, uint32_t*, uint32_t, int, int, void*,
-                                                size_t, uint32_t*, bool);
+                                                size_t, uint32_t*, bool, const uint32_t**);
 
 // The reference tie order of CUDA_radixSortGeneral (CUDA_radixSortGeneral.cu:397-482):
 //  length <= 512 (bubble, CUDA_bubbleSort512.cu): stable by key;
