@@ -61,11 +61,67 @@ FP32_PEAK_TFLOPS = 157.3
 OPS_PER_BOX_TEST = 24   # algorithmic FP32 ops of one slab test
 OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 1e-6  # the reference's absolute parallel-ray threshold (CUDA_raytraceMeshTree.cu:36)
-# HBM bytes per launch of the headline k_fast_trace, from two rocprofv3 --pmc
-# runs of `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
-# FETCH_SIZE 117,868 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 3,512 KiB
-# (means of the three det_epsilon 0 frames).
-TRACE_TRAFFIC_BYTES = (117_868 * 2 + 3_512) * 1024
+# HBM bytes per launch of the headline trace kernel (the default traversal
+# variant), from two rocprofv3 --pmc runs of `bench.py --profile-only`
+# (profiles/fast_build_trace_heightfield708.md): FETCH_SIZE 123,571 KiB, scaled
+# x2 by a 256 MiB copy, plus WRITE_SIZE 3,329 KiB (means of the two warm frames
+# at the 1e-6 threshold).
+TRACE_TRAFFIC_BYTES = (123_571 * 2 + 3_329) * 1024
+
+
+# The fast build's kernels, per launch, measured with rocprofv3 on the headline
+# heightfield (`bench.py --profile-only`) and on the 10M soup (`--profile-soup`):
+# average duration (us) from --kernel-trace, HBM bytes read and written (KiB)
+# from --pmc FETCH_SIZE (x2, the factor a 256 MiB copy gives) and WRITE_SIZE
+# (profiles/fast_build_trace_heightfield708.md, profiles/soup10m_build_trace.md).
+BUILD_KERNELS = {  # kernel: (us, HBM read KiB, HBM write KiB, launches per build)
+    "heightfield": {
+        "k_fast_prep": (15.0, 32_286, 42_961, 1),
+        "k_rs_hist": (12.5, 3_991, 857, 1),
+        "k_rs_scatter": (18.85, 9_454, 9_218, 4),
+        "k_fast_hierarchy": (77.3, 43_035, 118_626, 1),
+        "k_fast_hierarchy_top": (54.3, 3_694, 2_066, 1),
+    },
+    "soup": {
+        "k_fast_prep": (209.5, 625_383, 468_771, 1),
+        "k_rs_hist": (65.1, 78_282, 18_714, 1),
+        "k_rs_scatter": (59.2, 132_229, 134_771, 8),
+        "k_fast_hierarchy": (994.0, 2_921_394, 1_163_942, 1),
+        "k_fast_hierarchy_top": (95.5, 34_002, 18_151, 1),
+    },
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def build_algorithmic_bytes(n_faces, n_verts, key_bytes):
+    """Compulsory HBM bytes of one rk_fast_build with one face per leaf: the
+    key kernel reads faces (16 B) and vertices and writes boxes (32), keys and
+    indices (4); the sort reads the keys once for its counts and each of its
+    passes reads and writes key and index; the hierarchy reads keys, indices,
+    faces and vertices and writes a 64-B record per split and a 48-B triangle
+    record per face. Vertices are counted once per kernel that reads them."""
+    passes = key_bytes
+    per_face = (16 + 32 + key_bytes + 4) + key_bytes + passes * 2 * (key_bytes + 4) + (key_bytes + 4 + 16 + 64 + 48)
+    return per_face * n_faces + 32 * n_verts
+
+
+def build_roofline(workload, n_faces, n_verts, key_bytes, build_ms):
+    alg = build_algorithmic_bytes(n_faces, n_verts, key_bytes)
+    achieved = alg / (build_ms * 1e-3) / 1e9
+    kernels = BUILD_KERNELS.get(workload, {})
+    per = {}
+    traffic = 0
+    for k, (us, rd_kib, wr_kib, launches) in kernels.items():
+        b = (rd_kib + wr_kib) * 1024
+        traffic += b * launches
+        per[k] = {"us": us, "launches": launches, "hbm_mb": round(b / 1e6, 1),
+                  "hbm_frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3)}
+    return {"bound": "hbm", "kernel": "rk_fast_build (all its kernels)", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic or None, "algorithmic_bytes": alg, "per_kernel": per,
+            "note": "achieved = compulsory bytes of the build (build_algorithmic_bytes) / its time here; "
+                    "per_kernel = measured HBM bytes per launch / average launch time / peak, from the "
+                    "rocprofv3 runs cited at BUILD_KERNELS"}
 
 
 def log(*a):
@@ -298,6 +354,7 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
         "workload": "soup 10M faces, 63-bit Morton, one face per leaf, 2^24 incoherent rays, closest hit",
         "lbvh63_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
         "build_ms": round(build_ms, 3),
+        "build_roofline": build_roofline("soup", n, 3 * n, 8, build_ms),
         "mrays_s": round(nr / (frame_ms * 1e-3) / 1e6, 2),
         "frame_ms": round(frame_ms, 3),
         "hit_fraction": round(hits / nr, 4),
@@ -557,11 +614,7 @@ def main():
     if args.profile_only:
         for _ in range(3):
             build()
-        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-        tuned["config"] = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON,
-                                       reps=1).config
-        log(f"tuned variant {dev.trace_variant(tuned['config']).as_dict()}")
-        for _ in range(3):
+        for _ in range(3):  # the default traversal variant, which the tuner keeps on this frame
             frame()
         frame(0.0)
         rb = dev.LBVHBuilder(n)
@@ -645,6 +698,7 @@ def main():
         "camera_fused_same_depths": cam_same,
         "lbvh_mtri_s": round(n / (build_ms * 1e-3) / 1e6, 2),
         "lbvh_build_ms": round(build_ms, 4),
+        "build_roofline": build_roofline("heightfield", n, V.shape[0], 4, build_ms),
         "box_tests_per_ray": round(n_box / nr, 2),
         "tri_tests_per_ray": round(n_tri / nr, 2),
         "build_flags": flags,

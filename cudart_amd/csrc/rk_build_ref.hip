@@ -168,15 +168,6 @@ __device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ A, 
   return lo;
 }
 
-__device__ __forceinline__ uint32_t upper_bound_lds(const uint32_t* A, uint32_t n, uint32_t a) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (A[mid] <= a) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // Where the reference's level-L split pointer stops when every level-L split
 // is <= a: it walks on through the (ascending) lists of levels L+1, L+2, ...
 // and stops at the first entry > a (CUDA_buildMeshTreeLBVH.cu:381-392).
@@ -223,16 +214,10 @@ __device__ __forceinline__ uint32_t block_excl_scan1024(uint32_t v, uint32_t* s_
 //    chain_walk(a).
 // Preorder ranks follow from per-subtree internal-node counts; an internal node
 // of rank r has children 2r+1, 2r+2 (:325-326, :394-395).
-// A level's split list, when it has at most kTopoLdsA entries, is copied to
-// LDS for the searches of that level: a search then costs LDS round trips
-// instead of dependent global loads, which bound this one-workgroup kernel.
-constexpr uint32_t kTopoLdsA = 8192;
-
 __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ A,
                                                    const uint32_t* __restrict__ level_count, uint32_t n, uint32_t d,
                                                    int complete, TopoWs w, Node* __restrict__ nodes,
                                                    uint32_t* __restrict__ info) {
-  __shared__ uint32_t s_A[kTopoLdsA];
   __shared__ uint32_t s_idx[34];
   __shared__ uint32_t s_lev[kMaxLevels + 2];
   __shared__ uint32_t s_tmp[16];
@@ -261,10 +246,6 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
     const uint32_t cntL = L < 32 ? level_count[L] : 0u;
     const uint32_t EL = L < 32 ? s_idx[L + 1] : S;
     const uint32_t lastA = cntL ? A[EL - 1] : 0u;
-    const uint32_t BL = L < 32 ? s_idx[L] : S;  // level L's list is A[BL, EL)
-    const bool cached = cntL != 0 && cntL <= kTopoLdsA && L < d;
-    if (cached)
-      for (uint32_t k = tid; k < cntL; k += kTopoThreads) s_A[k] = A[BL + k];
     if (tid == 0) s_j0 = 0xFFFFFFFFu;
     __syncthreads();
     for (uint32_t r = cur + tid; r < end; r += kTopoThreads) {
@@ -296,7 +277,7 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
             uint32_t q = S;
             bool entry_leaf = false;
             if (cntL == 0) entry_leaf = true;
-            else if (a < lastA) q = cached ? BL + upper_bound_lds(s_A, cntL, a) : upper_bound(A, BL, EL, a);
+            else if (a < lastA) q = upper_bound(A, s_idx[L], EL, a);
             else if (r == j0) q = q0;
             else if (q0 == EL) entry_leaf = true;
             else q = chain_walk(A, s_idx, L, a, S);
@@ -305,7 +286,7 @@ __global__ __launch_bounds__(1024) void k_topo_ref(const uint32_t* __restrict__ 
                 atomicOr(&s_flags, (uint32_t)RK_FLAG_WALK_PAST_END);
               } else {
                 if (q >= past_depth) atomicOr(&s_flags, (uint32_t)RK_FLAG_WALK_PAST_DEPTH);
-                uint32_t sp = (cached && q >= BL && q < EL) ? s_A[q - BL] : A[q];
+                uint32_t sp = A[q];
                 if (sp < b) { leafish = false; internal = 1; split = sp; ckind = 0; }
               }
             }
