@@ -58,7 +58,14 @@ MUT = [
 only = sys.argv[1:]
 base = Path(tempfile.mkdtemp(prefix="oracle_mut_"))
 shutil.rmtree(base)
-shutil.copytree(REPO, base, ignore=shutil.ignore_patterns(".git", "_work", "gpurun_out", "__pycache__", "_build"))
+def _skip(d, names):  # the oracle's own build is redone per mutation; the rest is copied as built
+    skip = {".git", "_work", "gpurun_out", "__pycache__", ".pytest_cache"}
+    if Path(d).resolve() == (REPO / "oracle").resolve():
+        skip.add("_build")
+    return [n for n in names if n in skip]
+
+
+shutil.copytree(REPO, base, ignore=_skip)
 src = (REPO / "oracle/oracle.cpp").read_text()
 res = []
 for name, a, b in MUT:
