@@ -326,13 +326,21 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     fid, uv, dep = dev.new_outputs(nr)
     f32max = float(np.finfo(np.float32).max)
 
+    tuned = {"config": 0}
+
     def frame():
         native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON)
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, config=tuned["config"])
 
     for _ in range(warmup):
         build()
     build_ms = timer.time(build, steps)
+    for _ in range(warmup):
+        frame()
+    default_ms = timer.time(frame, steps)
+    native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=1)
+    tuned["config"] = int(tuning.config)
     for _ in range(warmup):
         frame()
     frame_ms = timer.time(frame, steps)
@@ -344,7 +352,8 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
 
     def frame_sorted():
         native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-        dev.fast_trace_sorted(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, ws=sws)
+        dev.fast_trace_sorted(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, ws=sws,
+                              config=tuned["config"])
 
     for _ in range(warmup):
         frame_sorted()
@@ -357,6 +366,8 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
         "build_roofline": build_roofline("soup", n, 3 * n, 8, build_ms),
         "mrays_s": round(nr / (frame_ms * 1e-3) / 1e6, 2),
         "frame_ms": round(frame_ms, 3),
+        "trace_autotuned": tuning.as_dict(),
+        "mrays_s_default_variant": round(nr / (default_ms * 1e-3) / 1e6, 2),
         "hit_fraction": round(hits / nr, 4),
         "sorted_rays_mrays_s": round(nr / (sorted_ms * 1e-3) / 1e6, 2),
         "sorted_rays_frame_ms": round(sorted_ms, 3),

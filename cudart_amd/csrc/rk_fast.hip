@@ -10,9 +10,11 @@
 //   k_fast_hierarchy  topology, node boxes and leaf-ordered triangle records
 //                     (v0, v1-v0, v2-v0; 48 B) bottom-up (agglomerative
 //                     LBVH, Apetrei 2014) for the splits inside each
-//                     1024-position chunk; subtrees of <= leaf_max faces
-//                     become leaves
-//   k_fast_hierarchy_top  the same climbs through the splits that cross chunks
+//                     512-position chunk; subtrees of <= leaf_max faces
+//                     become leaves; up to 2^21 faces it also climbs the
+//                     splits that cross chunks
+//   k_fast_hierarchy_top  above that, the climbs through the splits that cross
+//                     chunks
 //
 // Node record (64 B): float4 (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y),
 //                     float4 (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y),
@@ -146,9 +148,12 @@ __device__ __forceinline__ void write_node(FNode* __restrict__ nodes, uint32_t i
   p->d = make_uint4(c0.ref, c1.ref, n0, n1);
 }
 
-// Max-scan over one workgroup of kLocalThreads threads (waves of 64).
+// Max-scan over one workgroup of kLocalThreads threads (waves of 64). The
+// hierarchy pass's chunks are one position per thread: 512 measured best on
+// the 1M-face heightfield and the 10M-face soup (builds of 224 and 1718 us,
+// against 236 and 1816 with 1024 and 222 and 1810 with 256; the same trees).
 #ifndef RK_HIER_THREADS
-#define RK_HIER_THREADS 1024
+#define RK_HIER_THREADS 512
 #endif
 constexpr uint32_t kLocalThreads = RK_HIER_THREADS;
 constexpr uint32_t kLocalWaves = kLocalThreads / 64;
@@ -523,6 +528,28 @@ __device__ __forceinline__ float safe_inv(float d) {
   return 1.0f / a;
 }
 
+// The slab test of a node record's two child boxes against a ray given by its
+// inverse direction (ix, iy, iz) and origin terms (ox = o.x * ix, ...), over
+// [0, bound]: entry distances and whether each box is hit.
+struct Slab2 {
+  float tmin0, tmin1;
+  bool h0, h1;
+};
+__device__ __forceinline__ Slab2 slab2(const FNode& nd, float ix, float iy, float iz, float ox, float oy, float oz,
+                                       float bound) {
+  const float a0 = __builtin_fmaf(nd.a.x, ix, -ox), a1 = __builtin_fmaf(nd.a.y, ix, -ox);
+  const float a2 = __builtin_fmaf(nd.a.z, iy, -oy), a3 = __builtin_fmaf(nd.a.w, iy, -oy);
+  const float a4 = __builtin_fmaf(nd.c.x, iz, -oz), a5 = __builtin_fmaf(nd.c.y, iz, -oz);
+  const float b0 = __builtin_fmaf(nd.b.x, ix, -ox), b1 = __builtin_fmaf(nd.b.y, ix, -ox);
+  const float b2 = __builtin_fmaf(nd.b.z, iy, -oy), b3 = __builtin_fmaf(nd.b.w, iy, -oy);
+  const float b4 = __builtin_fmaf(nd.c.z, iz, -oz), b5 = __builtin_fmaf(nd.c.w, iz, -oz);
+  const float tmin0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
+  const float tmax0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), bound));
+  const float tmin1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
+  const float tmax1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), bound));
+  return Slab2{tmin0, tmin1, tmin0 <= tmax0, tmin1 <= tmax1};
+}
+
 // A traversal's result: the sorted position of the closest hit (~0: none),
 // its t (the tmax it started with when there is no hit), u and v.
 struct RayHit {
@@ -561,7 +588,49 @@ constexpr int kXchgWords = 4 * 64;
 // entries of its own to hand on. The stack is then the range [sb, sp); it is
 // reset to empty at [0, 0) whenever it drains, and an index still stands for
 // one level of the current path, so sp stays within the depth bound.
-template <bool kCount, int kStackN, int kLds, int kWG, bool kOldest = false>
+//
+// kLeafBatch > 0 postpones leaves: a lane that reaches a leaf keeps it as its
+// one pending leaf and goes on to its next pending subtree, and the wave
+// tests the pending leaves together once kLeafBatch lanes hold one (or a lane
+// cannot go on without its leaf: it reached a second leaf, or its subtree is
+// done). With one face per leaf some lane of a busy wave is at a leaf on
+// almost every step, so testing leaves at once issues the triangle test on
+// almost every step. A postponed leaf leaves the culling bound looser for a
+// few steps, so a lane may visit a node it would have culled; the closest hit
+// is the same. kDonateMin: the exchange runs only on steps where at least this
+// many lanes are idle.
+// The faces of leaf 'ref' against the job's ray (jo, jd). own: the ray is
+// this lane's, and a hit goes to its best (o_*); otherwise to this helper's
+// best (j_*), kept only if it can beat the owner's best as of this step (dmax,
+// d_hit). Ties on t go to the smaller original id. A macro, not a function: the
+// best-hit variables stay in registers only if no reference to them is taken.
+#define RK_LEAF_HITS(ref)                                                                              \
+  do {                                                                                                 \
+    const uint32_t first_ = ((ref) & ~kLeafBit) >> 4;                                                 \
+    const uint32_t cnt_ = first_ < n_tris ? min(((ref) & 15u) + 1, n_tris - first_) : 0u;             \
+    for (uint32_t k = 0; k < cnt_; ++k) {                                                              \
+      const float4 r0 = tris[3 * (size_t)(first_ + k) + 0];                                            \
+      float4 v0 = r0;                                                                                  \
+      v0.w = 0.0f;                                                                                     \
+      const float4 e1 = tris[3 * (size_t)(first_ + k) + 1], e2 = tris[3 * (size_t)(first_ + k) + 2];   \
+      float t, u, v;                                                                                   \
+      if (kCount) ++n_tri;                                                                             \
+      if (tri_ray(jo, jd, v0, e1, e2, eps, t, u, v)) {                                                 \
+        const uint32_t oid = __float_as_uint(r0.w);                                                    \
+        if (own) {                                                                                     \
+          if (t < o_t || (o_pos != 0xFFFFFFFFu && t == o_t && oid < o_oid)) {                          \
+            o_t = t; o_pos = first_ + k; o_oid = oid; o_u = u; o_v = v;                                \
+          }                                                                                            \
+        } else if ((t < dmax || (d_hit && t == dmax)) &&                                               \
+                   (j_pos == 0xFFFFFFFFu || t < j_t || (t == j_t && oid < j_oid))) {                   \
+          j_t = t; j_pos = first_ + k; j_oid = oid; j_u = u; j_v = v;                                  \
+        }                                                                                              \
+      }                                                                                                \
+    }                                                                                                  \
+  } while (0)
+
+template <bool kCount, int kStackN, int kLds, int kWG, bool kOldest = false, int kLeafBatch = 0,
+          int kDonateMin = 1>
 __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                    uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,
                                                    float4 dir, float depth0, float eps, uint32_t& n_box,
@@ -579,6 +648,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   uint32_t j_pos = 0xFFFFFFFFu, j_oid = 0xFFFFFFFFu;
   float j_t = 0.0f, j_u = 0.0f, j_v = 0.0f;
   uint32_t cur = root, visits = 0;
+  uint32_t pend = 0u;  // a postponed leaf (a leaf reference always has kLeafBit set; 0: none)
   static_assert(kStackN > kLds, "scratch part of the stack");
   uint32_t stack_ref[kStackN - kLds];
   float stack_t[kStackN - kLds];
@@ -613,7 +683,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
     if (idle == ~0ull) break;
     // hand the newest pending subtree of up to popc(idle) lanes to idle lanes
     const uint64_t donors = __ballot(working && sp > sb);
-    if (idle != 0ull && donors != 0ull) {
+    if (idle != 0ull && donors != 0ull && (kDonateMin <= 1 || __popcll(idle) >= kDonateMin)) {
       const uint32_t nd = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
       const uint64_t lt = rk::lanemask_lt();
       const uint32_t rd = (uint32_t)__popcll(donors & lt), ri = (uint32_t)__popcll(idle & lt);
@@ -686,28 +756,21 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
         d_hit = op != 0xFFFFFFFFu;
       }
     }
-    bool job_end = false;
+    bool job_end = false, flush_me = false, end_after = false;
     if (owner >= 0) {
       float bound = own ? o_t : (j_pos != 0xFFFFFFFFu ? fminf(dmax, j_t) : dmax);
       bool need_pop = false;
       if (++visits > max_visits) {
         job_end = true;
+        pend = 0u;
       } else if (!(cur & kLeafBit)) {
         if (cur >= n_tris) {
           job_end = true;  // never taken for a tree built here
         } else {
           const FNode nd = nodes[cur];
-          const float a0 = __builtin_fmaf(nd.a.x, ix, -ox), a1 = __builtin_fmaf(nd.a.y, ix, -ox);
-          const float a2 = __builtin_fmaf(nd.a.z, iy, -oy), a3 = __builtin_fmaf(nd.a.w, iy, -oy);
-          const float a4 = __builtin_fmaf(nd.c.x, iz, -oz), a5 = __builtin_fmaf(nd.c.y, iz, -oz);
-          const float b0 = __builtin_fmaf(nd.b.x, ix, -ox), b1 = __builtin_fmaf(nd.b.y, ix, -ox);
-          const float b2 = __builtin_fmaf(nd.b.z, iy, -oy), b3 = __builtin_fmaf(nd.b.w, iy, -oy);
-          const float b4 = __builtin_fmaf(nd.c.z, iz, -oz), b5 = __builtin_fmaf(nd.c.w, iz, -oz);
-          const float tmin0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
-          const float tmax0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), bound));
-          const float tmin1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
-          const float tmax1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), bound));
-          const bool h0 = tmin0 <= tmax0, h1 = tmin1 <= tmax1;
+          const Slab2 sl = slab2(nd, ix, iy, iz, ox, oy, oz, bound);
+          const float tmin0 = sl.tmin0, tmin1 = sl.tmin1;
+          const bool h0 = sl.h0, h1 = sl.h1;
           if (kCount) n_box += 2;
           if (h0 && h1) {
             const bool first0 = tmin0 <= tmin1;
@@ -724,29 +787,14 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
             need_pop = true;
           }
         }
-      } else {
-        const uint32_t first = (cur & ~kLeafBit) >> 4;
-        const uint32_t cnt = first < n_tris ? min((cur & 15u) + 1, n_tris - first) : 0u;
-        for (uint32_t k = 0; k < cnt; ++k) {
-          const float4 r0 = tris[3 * (size_t)(first + k) + 0];
-          float4 v0 = r0;
-          v0.w = 0.0f;
-          const float4 e1 = tris[3 * (size_t)(first + k) + 1], e2 = tris[3 * (size_t)(first + k) + 2];
-          float t, u, v;
-          if (kCount) ++n_tri;
-          if (tri_ray(jo, jd, v0, e1, e2, eps, t, u, v)) {
-            const uint32_t oid = __float_as_uint(r0.w);
-            if (own) {
-              if (t < o_t || (o_pos != 0xFFFFFFFFu && t == o_t && oid < o_oid)) {
-                o_t = t; o_pos = first + k; o_oid = oid; o_u = u; o_v = v;
-              }
-            } else if ((t < dmax || (d_hit && t == dmax)) &&
-                       (j_pos == 0xFFFFFFFFu || t < j_t || (t == j_t && oid < j_oid))) {
-              j_t = t; j_pos = first + k; j_oid = oid; j_u = u; j_v = v;
-            }
-          }
-        }
+      } else if (kLeafBatch == 0) {
+        RK_LEAF_HITS(cur);
         need_pop = true;
+      } else if (pend == 0u) {
+        pend = cur;  // postponed: go on to the next pending subtree
+        need_pop = true;
+      } else {
+        flush_me = true;  // a second leaf: test the pending one first, keep cur
       }
       if (need_pop) {
         // the nearest pending entry that can still hold a closer hit
@@ -763,7 +811,24 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
             break;
           }
         }
-        if (!found) job_end = true;
+        if (!found) {
+          if (kLeafBatch > 0 && pend != 0u) {
+            flush_me = true;  // the subtree is done but for its pending leaf
+            end_after = true;
+          } else {
+            job_end = true;
+          }
+        }
+      }
+    }
+    if (kLeafBatch > 0) {
+      const uint64_t pm = __ballot(pend != 0u);
+      if (pm != 0ull && (__ballot(flush_me) != 0ull || __popcll(pm) >= kLeafBatch)) {
+        if (pend != 0u) {
+          RK_LEAF_HITS(pend);
+          pend = 0u;
+        }
+        if (end_after) job_end = true;
       }
     }
     // merge finished helpers' hits into their owners, one at a time
@@ -783,6 +848,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   }
   return RayHit{o_pos, o_t, o_u, o_v};
 }
+#undef RK_LEAF_HITS
 
 // Sort key of a ray: the 30-bit Morton code of its origin in the tree's root
 // box (the union of the root record's two child boxes), clamped to the box.
@@ -836,12 +902,23 @@ __device__ __forceinline__ uint32_t spread_ray(uint32_t t, uint32_t n_rays) {
 #define RK_TRACE_WG 64
 #endif
 constexpr int kTraceWG = RK_TRACE_WG;
+// Pending subtrees are handed to idle lanes only on steps where at least this
+// many lanes are idle (trace_ray_shared's kDonateMin): each exchange costs
+// every lane of the wave a dozen shuffles and two LDS round trips, and a lane
+// that waits a few steps for work costs less. 16 measured best on the
+// 1920x1080 heightfield frame (0.183 ms against 0.189 with 1 and 0.215 with
+// 32) and better than 1 on the 10M soup's incoherent rays (45.9 against 48.4
+// ms; 42.5 with 32, which the autotuner picks there).
+#ifndef RK_TRACE_DONATE_MIN
+#define RK_TRACE_DONATE_MIN 16
+#endif
+constexpr int kTraceDonateMin = RK_TRACE_DONATE_MIN;
 static_assert(kTraceWG % 64 == 0 && kTraceWG <= 1024, "whole waves");
 
 // kWG threads per workgroup, runs of kRun rays per band (spread_ray; 0: 64
 // consecutive rays per wave), kLds stack entries per lane in LDS. The
 // defaults above are one variant; rk_fast_autotune chooses among several.
-template <bool kCount, int kWG, uint32_t kRun, int kLds, bool kOldest = false>
+template <bool kCount, int kWG, uint32_t kRun, int kLds, bool kOldest = false, int kLeafBatch = 0, int kDonateMin = 1>
 __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                       uint32_t n_tris, const float4* __restrict__ origins,
                                                       const float4* __restrict__ dirs, uint32_t n_rays,
@@ -856,7 +933,7 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest>(
+  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest, kLeafBatch, kDonateMin>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
       has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
@@ -873,12 +950,13 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
 inline unsigned blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
 // Up to this many faces the chunk pass also climbs the splits that cross
-// chunks, in the thread that reaches them: the chain above a few hundred
-// chunks is short, and a second launch would cost more than it saves. Above
-// it, a workgroup waiting on that chain would hold its LDS while thousands of
+// chunks, in the thread that reaches them: the chain above a few thousand
+// chunks is short, and a second launch would cost more than it saves (the 1M
+// heightfield: 224 us against 229 us with k_fast_hierarchy_top). Above it, a
+// workgroup waiting on that chain would hold its LDS while thousands of
 // chunks queue, so the climbs go to k_fast_hierarchy_top.
 #ifndef RK_INLINE_TOP_FACES
-#define RK_INLINE_TOP_FACES (1u << 18)
+#define RK_INLINE_TOP_FACES (1u << 21)
 #endif
 constexpr uint32_t kInlineTopFaces = RK_INLINE_TOP_FACES;
 
@@ -1183,7 +1261,7 @@ __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict_
   const uint32_t root = has_ray ? meshes[mesh_of_ray(meshes, mesh_count, r)].face_offset : 0u;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t nb = 0, nt = 0;
-  const RayHit h = trace_ray_shared<false, kStackN, kLdsStack, kWG>(
+  const RayHit h = trace_ray_shared<false, kStackN, kLdsStack, kWG, false, 0, kTraceDonateMin>(
       nodes, tris, n_tris, root, has_ray, has_ray ? origins[r] : zero4, has_ray ? dirs[r] : zero4,
       has_ray ? depths[r] : 0.0f, eps, nb, nt, lds_stack, xchg);
   if (has_ray && h.pos != 0xFFFFFFFFu) {
@@ -1419,7 +1497,7 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
   if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !counters) return RK_ERROR_INVALID_VALUE;
   if (n_rays == 0) return RK_SUCCESS;
   if (!origins || !directions || !faces_id || !coords || !depths) return RK_ERROR_INVALID_VALUE;
-  hipLaunchKernelGGL((k_fast_trace_v<true, kTraceWG, kSpreadRun, kLdsStack>), dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
+  hipLaunchKernelGGL((k_fast_trace_v<true, kTraceWG, kSpreadRun, kLdsStack, false, 0, kTraceDonateMin>), dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, rk_hip_stream(stream),
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
@@ -1631,7 +1709,7 @@ __device__ __forceinline__ bool camera_pixel(int width, int height, int& x, int&
 // reference's CUDA_calcPrimaryRays.cu:43-55 formula: origin, and direction
 // (dir - x * step_x) - y * step_y), so a frame reads no ray arrays. Every pixel is written:
 // a miss gives face ~0, coords (0, 0), depth FLT_MAX.
-template <int kWG, int kBands, bool kXcd, bool kOldest = false>
+template <int kWG, int kBands, bool kXcd, bool kOldest = false, int kLeafBatch = 0, int kDonateMin = 1>
 __global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __restrict__ nodes,
                                                              const float4* __restrict__ tris, uint32_t n_tris,
                                                              float4 ro, float4 rd, float4 rx, float4 ry, int width,
@@ -1644,7 +1722,8 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __rest
   uint32_t nb = 0, nt = 0;
   __shared__ uint2 s_stack[kLdsStack * kWG];
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
-  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG, kOldest>(nodes, tris, n_tris, 0u, has_ray, ro, d,
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG, kOldest, kLeafBatch, kDonateMin>(
+      nodes, tris, n_tris, 0u, has_ray, ro, d,
                                                                             RK_HUGE, eps, nb, nt, s_stack,
                                                                             s_xchg[threadIdx.x >> 6]);
   if (!has_ray) return;
@@ -1677,33 +1756,41 @@ typedef void (*TraceKernel)(const FNode*, const float4*, uint32_t, const float4*
 typedef void (*CameraKernel)(const FNode*, const float4*, uint32_t, float4, float4, float4, float4, int, int,
                              uint32_t*, float2*, float*, float);
 struct TraceVariant {
-  uint32_t wg, run, lds, oldest;
+  uint32_t wg, run, lds, oldest, leaf_batch, donate_min;
   TraceKernel kernel;
 };
 struct CameraVariant {
-  uint32_t wg, bands, xcd, oldest;
+  uint32_t wg, bands, xcd, oldest, leaf_batch, donate_min;
   CameraKernel kernel;
 };
 
 // rk_trace_options.config k selects entry k - 1 (0: the first, the default).
-#define RK_TRACE_VARIANT(wg, run, lds, old) {wg, run, lds, old, k_fast_trace_v<false, wg, run, lds, old != 0>}
+#define RK_TRACE_VARIANT(wg, run, lds, old, lb, dm) \
+  {wg, run, lds, old, lb, dm, k_fast_trace_v<false, wg, run, lds, old != 0, lb, dm>}
+
 const TraceVariant kTraceVariants[] = {
-    RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack, 0),
-    RK_TRACE_VARIANT(64, 4, 8, 0),   RK_TRACE_VARIANT(64, 16, 8, 0), RK_TRACE_VARIANT(64, 0, 8, 0),
-    RK_TRACE_VARIANT(64, 8, 4, 0),   RK_TRACE_VARIANT(64, 8, 12, 0), RK_TRACE_VARIANT(128, 8, 8, 0),
-    RK_TRACE_VARIANT(128, 16, 8, 0), RK_TRACE_VARIANT(256, 8, 8, 0), RK_TRACE_VARIANT(64, 8, 8, 1),
-    RK_TRACE_VARIANT(64, 4, 8, 1),   RK_TRACE_VARIANT(64, 0, 8, 1),  RK_TRACE_VARIANT(128, 8, 8, 1),
+    RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack, 0, 0, kTraceDonateMin),
+    RK_TRACE_VARIANT(64, 4, 8, 0, 0, 1),   RK_TRACE_VARIANT(64, 16, 8, 0, 0, 1), RK_TRACE_VARIANT(64, 0, 8, 0, 0, 1),
+    RK_TRACE_VARIANT(64, 8, 4, 0, 0, 1),   RK_TRACE_VARIANT(64, 8, 12, 0, 0, 1), RK_TRACE_VARIANT(128, 8, 8, 0, 0, 1),
+    RK_TRACE_VARIANT(128, 16, 8, 0, 0, 1), RK_TRACE_VARIANT(256, 8, 8, 0, 0, 1), RK_TRACE_VARIANT(64, 8, 8, 1, 0, 1),
+    RK_TRACE_VARIANT(64, 4, 8, 1, 0, 1),   RK_TRACE_VARIANT(64, 0, 8, 1, 0, 1),  RK_TRACE_VARIANT(128, 8, 8, 1, 0, 1),
+    RK_TRACE_VARIANT(64, 8, 8, 0, 8, 1),   RK_TRACE_VARIANT(64, 8, 8, 0, 16, 1), RK_TRACE_VARIANT(64, 8, 8, 0, 32, 1),
+    RK_TRACE_VARIANT(64, 8, 8, 0, 0, 8),   RK_TRACE_VARIANT(64, 8, 8, 0, 0, 1),  RK_TRACE_VARIANT(64, 8, 8, 0, 16, 8),
+    RK_TRACE_VARIANT(64, 8, 8, 0, 16, 16), RK_TRACE_VARIANT(64, 8, 8, 0, 32, 16), RK_TRACE_VARIANT(64, 8, 8, 0, 0, 32),
+    RK_TRACE_VARIANT(64, 8, 8, 1, 0, 16),
 };
 #undef RK_TRACE_VARIANT
-#define RK_CAMERA_VARIANT(wg, bands, xcd, old) \
-  {wg, bands, xcd, old, k_fast_trace_camera_v<wg, bands, xcd != 0, old != 0>}
+#define RK_CAMERA_VARIANT(wg, bands, xcd, old, lb, dm) \
+  {wg, bands, xcd, old, lb, dm, k_fast_trace_camera_v<wg, bands, xcd != 0, old != 0, lb, dm>}
 const CameraVariant kCameraVariants[] = {
-    RK_CAMERA_VARIANT(kTraceWG, kCamBands, 0, 0),
-    RK_CAMERA_VARIANT(64, 8, 1, 0),   RK_CAMERA_VARIANT(64, 4, 0, 0),   RK_CAMERA_VARIANT(64, 4, 1, 0),
-    RK_CAMERA_VARIANT(64, 16, 0, 0),  RK_CAMERA_VARIANT(64, 16, 1, 0),  RK_CAMERA_VARIANT(128, 8, 0, 0),
-    RK_CAMERA_VARIANT(128, 8, 1, 0),  RK_CAMERA_VARIANT(128, 16, 0, 0), RK_CAMERA_VARIANT(128, 16, 1, 0),
-    RK_CAMERA_VARIANT(64, 8, 0, 1),   RK_CAMERA_VARIANT(64, 4, 0, 1),   RK_CAMERA_VARIANT(128, 8, 0, 1),
-    RK_CAMERA_VARIANT(64, 8, 1, 1),
+    RK_CAMERA_VARIANT(kTraceWG, kCamBands, 0, 0, 0, kTraceDonateMin),
+    RK_CAMERA_VARIANT(64, 8, 1, 0, 0, 1),   RK_CAMERA_VARIANT(64, 4, 0, 0, 0, 1),   RK_CAMERA_VARIANT(64, 4, 1, 0, 0, 1),
+    RK_CAMERA_VARIANT(64, 16, 0, 0, 0, 1),  RK_CAMERA_VARIANT(64, 16, 1, 0, 0, 1),  RK_CAMERA_VARIANT(128, 8, 0, 0, 0, 1),
+    RK_CAMERA_VARIANT(128, 8, 1, 0, 0, 1),  RK_CAMERA_VARIANT(128, 16, 0, 0, 0, 1), RK_CAMERA_VARIANT(128, 16, 1, 0, 0, 1),
+    RK_CAMERA_VARIANT(64, 8, 0, 1, 0, 1),   RK_CAMERA_VARIANT(64, 4, 0, 1, 0, 1),   RK_CAMERA_VARIANT(128, 8, 0, 1, 0, 1),
+    RK_CAMERA_VARIANT(64, 8, 1, 1, 0, 1),   RK_CAMERA_VARIANT(64, 8, 0, 0, 16, 1),  RK_CAMERA_VARIANT(64, 8, 0, 0, 32, 1),
+    RK_CAMERA_VARIANT(64, 8, 0, 0, 16, 16), RK_CAMERA_VARIANT(64, 8, 0, 0, 0, 1),   RK_CAMERA_VARIANT(64, 8, 0, 0, 0, 32),
+    RK_CAMERA_VARIANT(128, 8, 0, 1, 0, 16),
 };
 #undef RK_CAMERA_VARIANT
 constexpr uint32_t kNumTraceVariants = sizeof(kTraceVariants) / sizeof(kTraceVariants[0]);
@@ -1736,6 +1823,8 @@ void describe_trace(uint32_t config, rk_trace_tuning* out) {
   out->ray_run = v.run;
   out->lds_stack = v.lds;
   out->steal_oldest = v.oldest;
+  out->leaf_batch = v.leaf_batch;
+  out->donate_min = v.donate_min;
 }
 void describe_camera(uint32_t config, rk_trace_tuning* out) {
   const CameraVariant& v = *camera_variant(config);
@@ -1747,6 +1836,8 @@ void describe_camera(uint32_t config, rk_trace_tuning* out) {
   out->lds_stack = kLdsStack;
   out->xcd_strips = v.xcd;
   out->steal_oldest = v.oldest;
+  out->leaf_batch = v.leaf_batch;
+  out->donate_min = v.donate_min;
 }
 
 // Times fn() reps times after one untimed call, each call preceded by
@@ -1984,9 +2075,12 @@ extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* b
   const float4* o = reinterpret_cast<const float4*>(origins);
   hipLaunchKernelGGL(k_ray_keys, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, nodes, o, n_rays, keys, order);
   RK_CHECK_LAUNCH();
+  const TraceVariant* tv = trace_variant(options ? options->config : 0u);
+  if (!tv) return RK_ERROR_INVALID_VALUE;
   rk_status s = rk_radix_sort_impl<uint32_t>(st, keys, order, n_rays, 0, 4, sort_ws, sort_bytes, nullptr);
   if (s != RK_SUCCESS) return s;
-  hipLaunchKernelGGL((k_fast_trace_v<false, kTraceWG, kSpreadRun, kLdsStack>), dim3(blocks_for(n_rays, kTraceWG)), dim3(kTraceWG), 0, st, nodes,
+  // the variant's workgroup, stack and exchange settings; the order replaces its ray runs
+  hipLaunchKernelGGL(tv->kernel, dim3(blocks_for(n_rays, tv->wg)), dim3(tv->wg), 0, st, nodes,
                      static_cast<const float4*>(bvh->tris), bvh->face_count, o,
                      reinterpret_cast<const float4*>(directions), n_rays, faces_id, reinterpret_cast<float2*>(coords),
                      depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order);

@@ -546,9 +546,9 @@ def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height
 
 
 def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None,
-                      ws: torch.Tensor | None = None):
+                      ws: torch.Tensor | None = None, config=0):
     """rk_fast_trace_sorted: the same results as fast_trace, rays traced in
-    origin-Morton order (for incoherent rays)."""
+    origin-Morton order (for incoherent rays); config: an rk_fast_trace variant."""
     with _on(stream):
         n = O.shape[0]
         fid, uv, dep = out if out is not None else new_outputs(n)
@@ -556,6 +556,6 @@ def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None
         if ws is None or ws.numel() < need:
             ws = workspace(need)
         _check(N.lib().rk_fast_trace_sorted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon),
+                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon, config),
                                             ws.data_ptr(), ws.numel()), "fast_trace_sorted")
         return fid, uv, dep

@@ -265,7 +265,9 @@ rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, const rk_float
  * node and triangle fetches. Results are written to each ray's own slot and are
  * the same as rk_fast_trace's. Pays off when rays start scattered through the
  * scene (secondary or random rays); camera rays are coherent already. The first
- * 4 bytes of ws receive the sort's RK_FLAG_* bits, as for the sorts above. */
+ * 4 bytes of ws receive the sort's RK_FLAG_* bits, as for the sorts above.
+ * options->config selects an rk_fast_trace variant (its ray runs do not apply:
+ * the order replaces them). */
 size_t rk_fast_trace_sorted_workspace_size(uint32_t n_rays);
 rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
                                const rk_float4* directions, uint32_t n_rays, uint32_t* faces_id, rk_float2* coords,
@@ -287,7 +289,7 @@ rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_
  * workspace is a file-scope device array of RK_NOALLOC_SCRATCH_BYTES (128 MiB),
  * one per device, held until the process ends. face_count is limited to what
  * fits it, rk_fast_build_workspace_size(face_count, morton_mode) <=
- * RK_NOALLOC_SCRATCH_BYTES: 2,556,064 faces with RK_MORTON_30 and 2,199,856
+ * RK_NOALLOC_SCRATCH_BYTES: 2,555,904 faces with RK_MORTON_30 and 2,199,712
  * with RK_MORTON_63; larger counts return RK_ERROR_WORKSPACE. Everything the
  * build reads from the array is cleared by the build itself. All calls on one
  * device share that array, so calls that overlap on one device (concurrent
@@ -338,6 +340,8 @@ typedef struct rk_trace_tuning {
   uint32_t workgroup, ray_run, bands, lds_stack, xcd_strips;
   float ms;              /* mean time per call of the chosen variant in the tuning run */
   uint32_t steal_oldest; /* idle lanes take a busy lane's oldest pending subtree, not its newest */
+  uint32_t leaf_batch;   /* leaves are postponed until this many lanes hold one (0: tested at once) */
+  uint32_t donate_min;   /* pending subtrees are handed out only when this many lanes are idle */
 } rk_trace_tuning;
 rk_status rk_fast_trace_variant(uint32_t config, int camera, rk_trace_tuning* out);
 /* Times every rk_fast_trace variant on the caller's rays (reps timed calls each

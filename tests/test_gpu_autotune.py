@@ -7,8 +7,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F32MAX = np.finfo(np.float32).max
-N_TRACE_VARIANTS = 13
-N_CAMERA_VARIANTS = 14
+N_TRACE_VARIANTS = 23
+N_CAMERA_VARIANTS = 20
 
 
 @pytest.fixture(scope="module")
@@ -41,14 +41,13 @@ def _scene(rk, w, h):
 
 
 def test_variant_table(dev, rk):
-    shapes = {(dev.trace_variant(k).workgroup, dev.trace_variant(k).ray_run, dev.trace_variant(k).lds_stack,
-               dev.trace_variant(k).steal_oldest) for k in range(1, N_TRACE_VARIANTS + 1)}
+    key = lambda v: (v.workgroup, v.ray_run, v.lds_stack, v.steal_oldest, v.leaf_batch, v.donate_min)
+    shapes = {key(dev.trace_variant(k)) for k in range(1, N_TRACE_VARIANTS + 1)}
     assert len(shapes) == N_TRACE_VARIANTS
     d = dev.trace_variant(0)
     assert (d.workgroup, d.ray_run, d.lds_stack) == (64, 8, 8)
-    cams = {(dev.trace_variant(k, True).workgroup, dev.trace_variant(k, True).bands,
-             dev.trace_variant(k, True).xcd_strips, dev.trace_variant(k, True).steal_oldest)
-            for k in range(1, N_CAMERA_VARIANTS + 1)}
+    ckey = lambda v: (v.workgroup, v.bands, v.xcd_strips, v.steal_oldest, v.leaf_batch, v.donate_min)
+    cams = {ckey(dev.trace_variant(k, True)) for k in range(1, N_CAMERA_VARIANTS + 1)}
     assert len(cams) == N_CAMERA_VARIANTS
     with pytest.raises(rk.RkError):
         dev.trace_variant(N_TRACE_VARIANTS + 1)
@@ -56,11 +55,12 @@ def test_variant_table(dev, rk):
         dev.trace_variant(N_CAMERA_VARIANTS + 1, True)
 
 
-# 131 x 77 leaves the last band, the last run of each row and the last XCD strip part empty
-@pytest.mark.parametrize("w,h", [(131, 77), (256, 160)])
-def test_every_variant_same_frame(dev, rk, ora, w, h):
+# 131 x 77 leaves the last band, the last run of each row and the last XCD strip part empty;
+# leaves of up to 4 faces exercise the postponed leaves' face loop
+@pytest.mark.parametrize("w,h,leaf", [(131, 77, 1), (256, 160, 1), (256, 160, 4)])
+def test_every_variant_same_frame(dev, rk, ora, w, h, leaf):
     V, F, cam = _scene(rk, w, h)
-    tree = dev.FastBuilder(F.shape[0]).build(_t(F), _t(V), rk.hitbox(V), 1)
+    tree = dev.FastBuilder(F.shape[0]).build(_t(F), _t(V), rk.hitbox(V), leaf)
     O, D = rk.primary_rays_host(cam, w, h)
     Ot, Dt = _t(O), _t(D)
     of, ouv, od = ora.trace_bruteforce(F, V, O, D, det_epsilon=0.0)
@@ -72,6 +72,16 @@ def test_every_variant_same_frame(dev, rk, ora, w, h):
         np.testing.assert_array_equal(dep, od, err_msg=f"trace variant {k}")
         np.testing.assert_array_equal(order[fid.view(np.uint32)[hit]], of[hit], err_msg=f"trace variant {k}")
         np.testing.assert_array_equal(uv[hit], ouv[hit], err_msg=f"trace variant {k}")
+    # a finite tmax on every third ray (the compacting variants test it at the root)
+    base = dev.new_outputs(w * h)
+    base[2][::3] = 0.9
+    want = [_h(x) for x in dev.fast_trace(tree, Ot, Dt, out=tuple(t.clone() for t in base), det_epsilon=0.0)]
+    assert not np.array_equal(want[2], od)
+    for k in range(1, N_TRACE_VARIANTS + 1):
+        got = [_h(x) for x in dev.fast_trace(tree, Ot, Dt, out=tuple(t.clone() for t in base), det_epsilon=0.0,
+                                             config=k)]
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b, err_msg=f"trace variant {k}, finite tmax")
     for k in range(0, N_CAMERA_VARIANTS + 1):
         fid, uv, dep = (_h(x) for x in dev.fast_trace_camera(tree, cam, w, h, det_epsilon=0.0, config=k))
         np.testing.assert_array_equal(dep, od, err_msg=f"camera variant {k}")
