@@ -63,10 +63,10 @@ OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 1e-6  # the reference's absolute parallel-ray threshold (CUDA_raytraceMeshTree.cu:36)
 # HBM bytes per launch of the headline trace kernel (the default traversal
 # variant), from two rocprofv3 --pmc runs of `bench.py --profile-only`
-# (profiles/fast_build_trace_heightfield708.md): FETCH_SIZE 123,571 KiB, scaled
+# (profiles/fast_build_trace_heightfield708.md): FETCH_SIZE 123,657 KiB, scaled
 # x2 by a 256 MiB copy, plus WRITE_SIZE 3,329 KiB (means of the two warm frames
 # at the 1e-6 threshold).
-TRACE_TRAFFIC_BYTES = (123_571 * 2 + 3_329) * 1024
+TRACE_TRAFFIC_BYTES = (123_657 * 2 + 3_329) * 1024
 
 
 # The fast build's kernels, per launch, measured with rocprofv3 on the headline
@@ -74,20 +74,21 @@ TRACE_TRAFFIC_BYTES = (123_571 * 2 + 3_329) * 1024
 # average duration (us) from --kernel-trace, HBM bytes read and written (KiB)
 # from --pmc FETCH_SIZE (x2, the factor a 256 MiB copy gives) and WRITE_SIZE
 # (profiles/fast_build_trace_heightfield708.md, profiles/soup10m_build_trace.md).
+# The heightfield's climbs through the splits that cross chunks run inside
+# k_fast_hierarchy (up to 2^21 faces); the soup's in k_fast_hierarchy_top.
 BUILD_KERNELS = {  # kernel: (us, HBM read KiB, HBM write KiB, launches per build)
     "heightfield": {
-        "k_fast_prep": (15.0, 32_286, 42_961, 1),
-        "k_rs_hist": (12.5, 3_991, 857, 1),
-        "k_rs_scatter": (18.85, 9_454, 9_218, 4),
-        "k_fast_hierarchy": (77.3, 43_035, 118_626, 1),
-        "k_fast_hierarchy_top": (54.3, 3_694, 2_066, 1),
+        "k_fast_prep": (14.91, 32_282, 42_960, 1),
+        "k_rs_hist": (12.35, 3_991, 857, 1),
+        "k_rs_scatter": (18.84, 9_459, 9_218, 4),
+        "k_fast_hierarchy": (121.1, 52_260, 117_930, 1),
     },
     "soup": {
-        "k_fast_prep": (209.5, 625_383, 468_771, 1),
-        "k_rs_hist": (65.1, 78_282, 18_714, 1),
-        "k_rs_scatter": (59.2, 132_229, 134_771, 8),
-        "k_fast_hierarchy": (994.0, 2_921_394, 1_163_942, 1),
-        "k_fast_hierarchy_top": (95.5, 34_002, 18_151, 1),
+        "k_fast_prep": (208.4, 625_438, 468_776, 1),
+        "k_rs_hist": (67.75, 78_284, 18_713, 1),
+        "k_rs_scatter": (61.57, 132_300, 135_800, 8),
+        "k_fast_hierarchy": (851.0, 2_928_140, 1_186_547, 1),
+        "k_fast_hierarchy_top": (142.8, 60_688, 32_715, 1),
     },
 }
 HBM_PEAK_GBS = 8000.0
@@ -648,7 +649,7 @@ def main():
 
     log("autotuning the traversal on the frame")
     native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=5)
+    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=25)
     tuned["config"] = tuning.config
 
     log("timing frames")
@@ -664,7 +665,7 @@ def main():
     eps0_hits = int((dep < f32max).sum().item())
     # the same frame with the rays formed inside the trace kernel (its own tuned variant)
     cout = dev.new_outputs(nr)
-    cam_tuning = dev.autotune_camera(tree, cam, W, H, cout, det_epsilon=HEADLINE_DET_EPSILON, reps=5)
+    cam_tuning = dev.autotune_camera(tree, cam, W, H, cout, det_epsilon=HEADLINE_DET_EPSILON, reps=25)
     cam_frame = lambda: dev.fast_trace_camera(tree, cam, W, H, out=cout, det_epsilon=HEADLINE_DET_EPSILON,
                                               config=cam_tuning.config)
     for _ in range(args.warmup):
