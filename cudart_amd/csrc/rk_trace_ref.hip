@@ -1,7 +1,8 @@
 // Closest-hit traversal of the reference tree layout (bvnode, 32 B) with the
 // reference arithmetic (CUDA_raytraceMeshTree.cu), for gfx950:
 //   k_trace_ray     per-ray DFS (kernel_raytraceMeshTree, :111-219), one lane
-//                   per ray, 96-entry stack per lane in LDS;
+//                   per ray, a leaf's faces fetched two at a time, stack in
+//                   LDS and scratch;
 //   k_trace_packet  the 16x12 packet traversal (kernel_raytraceMeshTreePacket,
 //                   :222-377), one 192-thread workgroup (3 waves) per tile,
 //                   block-uniform stack in LDS, one barrier vote per node.
@@ -80,44 +81,93 @@ __device__ __forceinline__ void children_order(uint32_t w0, uint32_t w1, float4 
   farc = neg ? w1 : w1 + 1;
 }
 
-constexpr int kStack = 96;        // STACK_SIZE, :45
-constexpr int kRayThreads = 64;   // one wave per workgroup: 64 x 96 x 4 B = 24 KB of LDS
+constexpr int kStack = 96;  // STACK_SIZE, :45
+constexpr int kLdsStack = 32;  // entries per lane in LDS (8 KB per wave); the rest in scratch
 
-__global__ __launch_bounds__(kRayThreads) void k_trace_ray(const rk::Node* __restrict__ nodes,
-                                                           const uint4* __restrict__ faces,
-                                                           const float4* __restrict__ verts,
-                                                           const float4* __restrict__ origins,
-                                                           const float4* __restrict__ dirs, uint32_t n_rays,
-                                                           uint32_t node_count, uint32_t face_count,
-                                                           uint32_t* __restrict__ faces_id,
-                                                           float2* __restrict__ coords, float* __restrict__ depths) {
-  __shared__ uint32_t s_stack[kStack * kRayThreads];
-  const uint32_t l = blockIdx.x * kRayThreads + threadIdx.x;
+// The per-ray traversal (kernel_raytraceMeshTree, :111-219), one lane per ray:
+// pop a node, test its box against the depth so far, push the far child, then
+// the near one; at a leaf, test its faces in order, keeping a hit only when
+// t < depth. The same operations in the same order as the reference, so the
+// results are bit-identical.
+//
+// A leaf's faces are fetched kLeafBatch at a time, all their loads issued
+// before the first test: a face costs two dependent loads (the face, then
+// its vertices), and a leaf holds up to a few hundred faces, so one face at a
+// time leaves each lane waiting on memory. The tests still run in face order.
+// On the 1920x1080 heightfield frame (depth-12 tree): 291 ms per frame with
+// batches of 2, 315 with 4, 294 with 8, against 395 ms one face at a time
+// with the whole stack in LDS (6 waves per CU). Testing each leaf with the
+// whole wave, 64 faces at a time, was slower (458 ms): on terrain rows every
+// lane reaches its leaves together, and the wave then walks them one by one.
+//
+// The stack keeps the reference's 96 entries and its overflow rule; the first
+// kLdsStack entries of each lane live in LDS (entry e of lane x at
+// e * 64 + x), deeper ones in scratch (trees of depth <= 31 need at most 33),
+// so a CU holds 20 waves of this kernel instead of 6.
+#ifndef RK_REF_LEAF_BATCH
+#define RK_REF_LEAF_BATCH 2
+#endif
+constexpr int kLeafBatch = RK_REF_LEAF_BATCH;
+
+__global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ nodes,
+                                                  const uint4* __restrict__ faces,
+                                                  const float4* __restrict__ verts,
+                                                  const float4* __restrict__ origins,
+                                                  const float4* __restrict__ dirs, uint32_t n_rays,
+                                                  uint32_t node_count, uint32_t face_count,
+                                                  uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                  float* __restrict__ depths) {
+  __shared__ uint32_t s_stack[kLdsStack * 64];
+  const uint32_t l = blockIdx.x * 64 + threadIdx.x;
   if (l >= n_rays) return;
   const float4 orig = origins[l], dir = dirs[l];
   const float4 invdir = invdir_of(dir);
   float depth = depths[l];
   uint32_t fid = faces_id[l];
   float2 coord = coords[l];
-  uint32_t* st = s_stack + threadIdx.x;  // stride kRayThreads: conflict-free
+  uint32_t spill[kStack - kLdsStack];
+  typedef __attribute__((address_space(3))) uint32_t lds_u32;
+  lds_u32* my_lds = (lds_u32*)(s_stack + threadIdx.x);
   int sp = 0;
-  st[0] = 0;
+  my_lds[0] = 0u;
   sp = 1;
   uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
   while (sp > 0 && ++visits <= max_visits) {
-    const uint32_t nid = st[(--sp) * kRayThreads];
+    --sp;
+    const uint32_t nid = sp < kLdsStack ? my_lds[sp * 64] : spill[sp - kLdsStack];
     if (nid >= node_count) break;  // never taken for a tree built by this library
     const NodeRegs nd = load_node(nodes, nid);
     if (rk::node_is_leaf(nd.w0)) {
       const uint32_t tri_id = nd.w1;
       const uint32_t cnt = tri_id < face_count ? min(rk::leaf_count(nd.w0), face_count - tri_id) : 0u;
-      for (uint32_t k = 0; k < cnt; ++k) {
-        float4 v0, e1, e2;
-        tri_setup(faces, verts, tri_id + k, v0, e1, e2);
-        float t, u, v;
-        if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
-          if (t < depth) { fid = tri_id + k; coord = make_float2(u, v); depth = t; }
+      for (uint32_t k0 = 0; k0 < cnt; k0 += kLeafBatch) {
+        uint4 f[kLeafBatch];
+#pragma unroll
+        for (int b = 0; b < kLeafBatch; ++b) f[b] = k0 + b < cnt ? faces[tri_id + k0 + b] : make_uint4(0, 0, 0, 0);
+        float4 p0[kLeafBatch], p1[kLeafBatch], p2[kLeafBatch];
+#pragma unroll
+        for (int b = 0; b < kLeafBatch; ++b) {
+          if (k0 + b < cnt) {
+            p0[b] = verts[f[b].x];
+            p1[b] = verts[f[b].y];
+            p2[b] = verts[f[b].z];
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < kLeafBatch; ++b) {
+          if (k0 + b < cnt) {
+            // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
+            float4 v0 = p0[b];
+            float4 e1 = rk::sub4(p1[b], v0), e2 = rk::sub4(p2[b], v0);
+            v0.w = 0.0f;
+            e1.w = 0.0f;
+            e2.w = 0.0f;
+            float t, u, v;
+            if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
+              if (t < depth) { fid = tri_id + k0 + b; coord = make_float2(u, v); depth = t; }
+            }
+          }
         }
       }
     } else {
@@ -127,8 +177,12 @@ __global__ __launch_bounds__(kRayThreads) void k_trace_ray(const rk::Node* __res
           uint32_t nearc, farc;
           children_order(nd.w0, nd.w1, dir, nearc, farc);
           if (sp + 2 > kStack) break;  // depth > 94: cannot occur for trees of depth <= 94
-          st[(sp++) * kRayThreads] = farc;
-          st[(sp++) * kRayThreads] = nearc;
+          if (sp < kLdsStack) my_lds[sp * 64] = farc;
+          else spill[sp - kLdsStack] = farc;
+          ++sp;
+          if (sp < kLdsStack) my_lds[sp * 64] = nearc;
+          else spill[sp - kLdsStack] = nearc;
+          ++sp;
         }
       }
     }
@@ -247,7 +301,7 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
                        height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths);
   } else {
-    hipLaunchKernelGGL(k_trace_ray, dim3((n + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, st,
+    hipLaunchKernelGGL(k_trace_ray, dim3((n + 63) / 64), dim3(64), 0, st,
                        reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                        reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                        reinterpret_cast<const float4*>(directions), n, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords),

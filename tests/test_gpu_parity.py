@@ -200,6 +200,37 @@ def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
     assert (od < dep0).sum() > n // 20
 
 
+@pytest.mark.parametrize("depth", [4, 12])
+@pytest.mark.parametrize("coherency", [False, True])
+def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency):
+    """Every face three times: identical triangles hit at identical t, so the
+    reference's strict t < depth keeps the first copy in leaf order. Depth 4
+    gives leaves of about 960 faces (many rounds of 64 in the per-ray kernel,
+    copies on both sides of a round's boundary), depth 12 small ones."""
+    V, F0 = rk.icosphere(4)
+    F = np.ascontiguousarray(np.repeat(F0, 3, axis=0))
+    tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
+    w, h = 96, 80
+    cam = rk.camera_lookat([0.3, 0.2, 3], [0, 0, 0], 60 * 0.01745329252, w, h)
+    O, D = rk.primary_rays_host(cam, w, h)
+    n = w * h
+    fid0 = np.zeros(n, np.uint32)
+    uv0 = np.zeros((n, 2), np.float32)
+    dep0 = np.full(n, F32MAX, np.float32)
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, coherency, out=out)
+    if coherency:
+        of, ouv, od, ovf = ora.trace_ref_packet(ref.nodes, ref.faces, V, O, D, w, h, fid0, uv0, dep0)
+    else:
+        of, ouv, od, ovf = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0)
+    assert ovf == 0
+    hit = od < F32MAX
+    assert hit.mean() > 0.15
+    np.testing.assert_array_equal(_h(out[0], np.uint32), of)
+    np.testing.assert_array_equal(_h(out[1]), ouv)
+    np.testing.assert_array_equal(_h(out[2]), od)
+
+
 def _check_fast_boxes(tree, F, V, leaf_max):
     """Every child slot of every reachable node holds the padded union of the
     boxes of the faces below it, the right face count, and covers them."""
