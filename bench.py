@@ -62,11 +62,11 @@ OPS_PER_BOX_TEST = 24   # algorithmic FP32 ops of one slab test
 OPS_PER_TRI_TEST = 53   # algorithmic FP32 ops of one Moller-Trumbore test
 HEADLINE_DET_EPSILON = 1e-6  # the reference's absolute parallel-ray threshold (CUDA_raytraceMeshTree.cu:36)
 # HBM bytes per launch of the headline trace kernel (the default traversal
-# variant), from two rocprofv3 --pmc runs of `bench.py --profile-only`
-# (profiles/fast_build_trace_heightfield708.md): FETCH_SIZE 123,657 KiB, scaled
-# x2 by a 256 MiB copy, plus WRITE_SIZE 3,329 KiB (means of the two warm frames
-# at the 1e-6 threshold).
-TRACE_TRAFFIC_BYTES = (123_657 * 2 + 3_329) * 1024
+# variant, tmax reset in the kernel), from two rocprofv3 --pmc runs of
+# `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
+# FETCH_SIZE 107,377 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 10,575 KiB
+# (means of the two warm frames at the 1e-6 threshold).
+TRACE_TRAFFIC_BYTES = (107_377 * 2 + 10_575) * 1024
 
 
 # The fast build's kernels, per launch, measured with rocprofv3 on the headline
@@ -255,8 +255,7 @@ def icosphere_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     fout = dev.new_outputs(nr)
 
     def fast_frame():
-        native.check(native.lib().rk_fill_f32(None, fout[2].data_ptr(), nr, f32max), "fill")
-        dev.fast_trace(tree, Ot, Dt, out=fout, det_epsilon=HEADLINE_DET_EPSILON)
+        dev.fast_trace(tree, Ot, Dt, out=fout, det_epsilon=HEADLINE_DET_EPSILON, reset_depth=True)
 
     for _ in range(warmup):
         fbuild()
@@ -330,8 +329,8 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     tuned = {"config": 0}
 
     def frame():
-        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, config=tuned["config"])
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, config=tuned["config"],
+                       reset_depth=True)
 
     for _ in range(warmup):
         build()
@@ -339,8 +338,7 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     for _ in range(warmup):
         frame()
     default_ms = timer.time(frame, steps)
-    native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=1)
+    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=1, reset_depth=True)
     tuned["config"] = int(tuning.config)
     for _ in range(warmup):
         frame()
@@ -352,9 +350,8 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     sws = dev.workspace(native.lib().rk_fast_trace_sorted_workspace_size(nr))
 
     def frame_sorted():
-        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
         dev.fast_trace_sorted(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, ws=sws,
-                              config=tuned["config"])
+                              config=tuned["config"], reset_depth=True)
 
     for _ in range(warmup):
         frame_sorted()
@@ -618,15 +615,19 @@ def main():
     tuned = {"config": 0}
 
     def frame(eps=HEADLINE_DET_EPSILON, config=None):
-        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        # tmax reset to FLT_MAX and trace in one launch (RK_TRACE_RESET_DEPTH)
         dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=eps,
-                       config=tuned["config"] if config is None else config)
+                       config=tuned["config"] if config is None else config, reset_depth=True)
+
+    def frame_fill_then_trace():
+        native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
+        dev.fast_trace(tree, Ot, Dt, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, config=tuned["config"])
 
     torch.cuda.synchronize()
     if args.profile_only:
         for _ in range(3):
             build()
-        for _ in range(3):  # the default traversal variant, which the tuner keeps on this frame
+        for _ in range(3):  # the default traversal variant (the tuned ones are within 1-2% of it on this frame)
             frame()
         frame(0.0)
         rb = dev.LBVHBuilder(n)
@@ -648,8 +649,8 @@ def main():
     flags = int(dev.u32_view(tree.info)[native.RK_INFO_FLAGS])
 
     log("autotuning the traversal on the frame")
-    native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max), "fill")
-    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=25)
+    tuning = dev.autotune(tree, Ot, Dt, (fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON, reps=25,
+                          reset_depth=True)
     tuned["config"] = tuning.config
 
     log("timing frames")
@@ -658,9 +659,8 @@ def main():
     frame_ms = timer.time(frame, args.steps)
     default_ms = timer.time(lambda: frame(config=0), args.steps)
     hits = int((dep < f32max).sum().item())
-    fill_ms = timer.time(lambda: native.check(native.lib().rk_fill_f32(None, dep.data_ptr(), nr, f32max)),
-                         args.steps)
-    trace_ms = max(frame_ms - fill_ms, 1e-6)
+    trace_ms = frame_ms  # one launch per frame: the trace kernel resets tmax itself
+    fill_then_trace_ms = timer.time(frame_fill_then_trace, args.steps)
     eps0_frame_ms = timer.time(lambda: frame(0.0), args.steps)
     eps0_hits = int((dep < f32max).sum().item())
     # the same frame with the rays formed inside the trace kernel (its own tuned variant)
@@ -696,13 +696,15 @@ def main():
         "config": {
             "workload": "heightfield 708x708 fBm (999,698 FP32 triangles), 1920x1080 primary rays, closest hit",
             "faces": n, "rays": nr, "tree": "LBVH 30-bit Morton (Karras), " + ("one face per leaf" if args.leaf == 1 else f"leaf <= {args.leaf} faces"),
-            "step": "one frame: reset tmax + closest-hit trace of every ray, tree resident in HBM",
+            "step": "one frame: reset tmax + closest-hit trace of every ray, tree resident in HBM (one launch, "
+                    "RK_TRACE_RESET_DEPTH)",
             "det_epsilon": HEADLINE_DET_EPSILON,
         },
         "hit_fraction": round(hits / nr, 4),
         "trace_kernel_ms": round(trace_ms, 4),
         "trace_autotuned": tuning.as_dict(),
         "mrays_s_default_variant": round(nr / (default_ms * 1e-3) / 1e6, 2),
+        "mrays_s_fill_then_trace": round(nr / (fill_then_trace_ms * 1e-3) / 1e6, 2),
         "camera_autotuned": cam_tuning.as_dict(),
         "mrays_s_det_epsilon_0": round(nr / (eps0_frame_ms * 1e-3) / 1e6, 2),
         "hit_fraction_det_epsilon_0": round(eps0_hits / nr, 4),
@@ -722,10 +724,12 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
             "traffic": TRACE_TRAFFIC_BYTES,
-            "algorithmic_bytes": 52 * nr,
+            "algorithmic_bytes": 36 * nr + 12 * hits,
             "note": "algorithmic FP32 ops = 24 per slab test + 53 per triangle test, counted on this frame "
-                    "(default variant); achieved over the tuned variant's kernel time; "
-                    "52 B/ray of compulsory HBM traffic makes this kernel VALU/latency-bound, not HBM-bound; "
+                    "(default variant); achieved over the tuned variant's kernel time (the whole frame: one "
+                    "launch that also resets tmax); 32 B/ray read (origin, direction) + 4 B depth written per "
+                    "ray + 12 B id and coords per hit of compulsory HBM traffic makes this kernel "
+                    "VALU/latency-bound, not HBM-bound; "
                     "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs (rays, outputs "
                     "and node/triangle records fetched past the L2)",
         },

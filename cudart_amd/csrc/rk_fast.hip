@@ -925,8 +925,9 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
                                                       uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                       float* __restrict__ depths, float eps,
                                                       unsigned long long* __restrict__ counters,
-                                                      const uint32_t* __restrict__ order) {
+                                                      const uint32_t* __restrict__ order, uint32_t flags) {
   const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+  const bool reset = (flags & RK_TRACE_RESET_DEPTH) != 0u;
   const uint32_t l = order ? (t < n_rays ? order[t] : n_rays) : rk::band_ray<kRun>(t, n_rays);
   const bool has_ray = l < n_rays;
   __shared__ uint2 s_stack[kLds * kWG];
@@ -935,15 +936,19 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest, kLeafBatch, kDonateMin>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
-      has_ray ? depths[l] : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
+      has_ray ? (reset ? RK_HUGE : depths[l]) : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
     atomicAdd(counters, (unsigned long long)n_box);
     atomicAdd(counters + 1, (unsigned long long)n_tri);
   }
-  if (has_ray && h.pos != 0xFFFFFFFFu) {
-    faces_id[l] = h.pos;
-    coords[l] = make_float2(h.u, h.v);
-    depths[l] = h.t;
+  if (has_ray) {
+    if (h.pos != 0xFFFFFFFFu) {
+      faces_id[l] = h.pos;
+      coords[l] = make_float2(h.u, h.v);
+      depths[l] = h.t;
+    } else if (reset) {
+      depths[l] = RK_HUGE;  // the reset a fill kernel would have done; the miss's id and coords stay
+    }
   }
 }
 
@@ -1488,6 +1493,7 @@ extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshe
 static inline float det_eps_of(const rk_trace_options* o) {
   return o ? o->det_epsilon : RK_DET_EPSILON_REFERENCE;
 }
+static inline uint32_t trace_flags_of(const rk_trace_options* o) { return o ? (o->flags & RK_TRACE_RESET_DEPTH) : 0u; }
 
 
 extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* bvh, const rk_float4* origins,
@@ -1501,7 +1507,7 @@ extern "C" rk_status rk_fast_trace_counted(rk_stream stream, const rk_fast_bvh* 
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, det_eps_of(options),
-                     reinterpret_cast<unsigned long long*>(counters), (const uint32_t*)nullptr);
+                     reinterpret_cast<unsigned long long*>(counters), (const uint32_t*)nullptr, trace_flags_of(options));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -1752,7 +1758,7 @@ uint32_t camera_blocks(int width, int height, int wg, int bands, bool xcd) {
 namespace {
 
 typedef void (*TraceKernel)(const FNode*, const float4*, uint32_t, const float4*, const float4*, uint32_t, uint32_t*,
-                            float2*, float*, float, unsigned long long*, const uint32_t*);
+                            float2*, float*, float, unsigned long long*, const uint32_t*, uint32_t);
 typedef void (*CameraKernel)(const FNode*, const float4*, uint32_t, float4, float4, float4, float4, int, int,
                              uint32_t*, float2*, float*, float);
 struct TraceVariant {
@@ -1805,12 +1811,12 @@ const CameraVariant* camera_variant(uint32_t config) {
 
 rk_status launch_trace_variant(hipStream_t st, const TraceVariant& tv, const rk_fast_bvh* bvh,
                                const rk_float4* origins, const rk_float4* directions, uint32_t n_rays,
-                               uint32_t* faces_id, rk_float2* coords, float* depths, float eps) {
+                               uint32_t* faces_id, rk_float2* coords, float* depths, float eps, uint32_t flags) {
   hipLaunchKernelGGL(tv.kernel, dim3(blocks_for(n_rays, tv.wg)), dim3(tv.wg), 0, st,
                      static_cast<const FNode*>(bvh->nodes), static_cast<const float4*>(bvh->tris), bvh->face_count,
                      reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), n_rays,
                      faces_id, reinterpret_cast<float2*>(coords), depths, eps, (unsigned long long*)nullptr,
-                     (const uint32_t*)nullptr);
+                     (const uint32_t*)nullptr, flags);
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }
@@ -1900,8 +1906,10 @@ extern "C" rk_status rk_fast_autotune(rk_stream stream, const rk_fast_bvh* bvh, 
   EventPair ev;
   RK_HIP(hipEventCreate(&ev.e0));
   RK_HIP(hipEventCreate(&ev.e1));
+  // with RK_TRACE_RESET_DEPTH the calls do not read depths: nothing to restore
+  const bool reset = (trace_flags_of(options) & RK_TRACE_RESET_DEPTH) != 0u;
   auto restore = [&]() -> rk_status {
-    RK_HIP(hipMemcpyAsync(depths, tmax, 4ull * n_rays, hipMemcpyDeviceToDevice, st));
+    if (!reset) RK_HIP(hipMemcpyAsync(depths, tmax, 4ull * n_rays, hipMemcpyDeviceToDevice, st));
     return RK_SUCCESS;
   };
   uint32_t best_cfg = 0;
@@ -1910,7 +1918,8 @@ extern "C" rk_status rk_fast_autotune(rk_stream stream, const rk_fast_bvh* bvh, 
     float ms = 0.0f;
     const TraceVariant& tv = kTraceVariants[k];
     rk_status s = time_calls(st, ev.e0, ev.e1, reps, restore, [&]() {
-      return launch_trace_variant(st, tv, bvh, origins, directions, n_rays, faces_id, coords, depths, eps);
+      return launch_trace_variant(st, tv, bvh, origins, directions, n_rays, faces_id, coords, depths, eps,
+                                  trace_flags_of(options));
     }, &ms);
     if (s != RK_SUCCESS) return s;
     if (k == 0 || ms < best_ms) {
@@ -1922,7 +1931,7 @@ extern "C" rk_status rk_fast_autotune(rk_stream stream, const rk_fast_bvh* bvh, 
   rk_status s = restore();
   if (s != RK_SUCCESS) return s;
   s = launch_trace_variant(st, *trace_variant(best_cfg), bvh, origins, directions, n_rays, faces_id, coords, depths,
-                           eps);
+                           eps, trace_flags_of(options));
   if (s != RK_SUCCESS) return s;
   describe_trace(best_cfg, best);
   best->ms = best_ms;
@@ -1973,7 +1982,7 @@ extern "C" rk_status rk_fast_trace(rk_stream stream, const rk_fast_bvh* bvh, con
   const TraceVariant* tv = trace_variant(options ? options->config : 0u);
   if (!tv) return RK_ERROR_INVALID_VALUE;
   return launch_trace_variant(rk_hip_stream(stream), *tv, bvh, origins, directions, n_rays, faces_id, coords, depths,
-                              det_eps_of(options));
+                              det_eps_of(options), trace_flags_of(options));
 }
 
 extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
@@ -2083,7 +2092,8 @@ extern "C" rk_status rk_fast_trace_sorted(rk_stream stream, const rk_fast_bvh* b
   hipLaunchKernelGGL(tv->kernel, dim3(blocks_for(n_rays, tv->wg)), dim3(tv->wg), 0, st, nodes,
                      static_cast<const float4*>(bvh->tris), bvh->face_count, o,
                      reinterpret_cast<const float4*>(directions), n_rays, faces_id, reinterpret_cast<float2*>(coords),
-                     depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order);
+                     depths, det_eps_of(options), (unsigned long long*)nullptr, (const uint32_t*)order,
+                     trace_flags_of(options));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
 }

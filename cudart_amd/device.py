@@ -241,24 +241,28 @@ class FastBuilder:
         return FastTree(self.nodes, self.tris, self.indices[:n], self.boxes[:n], self.info, desc)
 
 
-def _opt(det_epsilon, config=0):
-    o = N.trace_options(det_epsilon, config)
+def _opt(det_epsilon, config=0, reset_depth=False):
+    o = N.trace_options(det_epsilon, config, reset_depth)
     return None if o is None else C.byref(o)
 
 
-def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None, config=0):
+def fast_trace(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None, config=0,
+               reset_depth=False):
     """det_epsilon None = the reference threshold (1e-6); 0.0 = only parallel rays rejected.
-    config: a traversal variant from autotune() (0 = the default)."""
+    config: a traversal variant from autotune() (0 = the default). reset_depth:
+    every ray starts at tmax FLT_MAX and every depth is written (no fill needed)."""
     with _on(stream):
         n = O.shape[0]
         fid, uv, dep = out if out is not None else new_outputs(n)
         _check(N.lib().rk_fast_trace(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                     fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon, config)),
+                                     fid.data_ptr(), uv.data_ptr(), dep.data_ptr(),
+                                     _opt(det_epsilon, config, reset_depth)),
                "fast_trace")
         return fid, uv, dep
 
 
-def autotune(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out, det_epsilon=None, reps=3, stream=None):
+def autotune(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out, det_epsilon=None, reps=3, stream=None,
+             reset_depth=False):
     """rk_fast_autotune: times every traversal variant on these rays (out[2]
     holds tmax and is restored before each call) and returns the fastest
     (N.TraceTuning); out then holds the frame it traced."""
@@ -268,7 +272,8 @@ def autotune(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out, det_epsilon=
         ws = workspace(N.lib().rk_fast_autotune_workspace_size(n))
         best = N.TraceTuning()
         _check(N.lib().rk_fast_autotune(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon), reps,
+                                        fid.data_ptr(), uv.data_ptr(), dep.data_ptr(),
+                                        _opt(det_epsilon, 0, reset_depth), reps,
                                         ws.data_ptr(), ws.numel(), C.byref(best)), "fast_autotune")
         return best
 
@@ -546,7 +551,7 @@ def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height
 
 
 def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None,
-                      ws: torch.Tensor | None = None, config=0):
+                      ws: torch.Tensor | None = None, config=0, reset_depth=False):
     """rk_fast_trace_sorted: the same results as fast_trace, rays traced in
     origin-Morton order (for incoherent rays); config: an rk_fast_trace variant."""
     with _on(stream):
@@ -556,6 +561,7 @@ def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None
         if ws is None or ws.numel() < need:
             ws = workspace(need)
         _check(N.lib().rk_fast_trace_sorted(_stream_ptr(stream), C.byref(tree.desc), O.data_ptr(), D.data_ptr(), n,
-                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(), _opt(det_epsilon, config),
+                                            fid.data_ptr(), uv.data_ptr(), dep.data_ptr(),
+                                            _opt(det_epsilon, config, reset_depth),
                                             ws.data_ptr(), ws.numel()), "fast_trace_sorted")
         return fid, uv, dep

@@ -20,6 +20,7 @@ RK_INFO_NODE_COUNT, RK_INFO_REAL_DEPTH, RK_INFO_FLAGS, RK_INFO_SPLITS, RK_INFO_L
 RK_INFO_WORDS = 8
 RK_FLAG_WALK_PAST_END, RK_FLAG_WALK_PAST_DEPTH, RK_FLAG_MERGE2X_OOB, RK_FLAG_NODES_GT_N = 1, 2, 4, 8
 RK_FLAG_SAH_NO_CUT, RK_FLAG_SPIN_LIMIT = 16, 64
+RK_TRACE_RESET_DEPTH = 1
 
 
 class RkError(RuntimeError):
@@ -54,13 +55,14 @@ class TraceTuning(C.Structure):
 DET_EPSILON_REFERENCE = 0.000001
 
 
-def trace_options(det_epsilon: float | None, config: int = 0) -> TraceOptions | None:
+def trace_options(det_epsilon: float | None, config: int = 0, reset_depth: bool = False) -> TraceOptions | None:
     """None = the reference threshold (1e-6); 0.0 = reject only parallel rays.
-    config: a traversal variant (rk_fast_autotune); 0 = the default."""
-    if det_epsilon is None and config == 0:
+    config: a traversal variant (rk_fast_autotune); 0 = the default.
+    reset_depth: RK_TRACE_RESET_DEPTH (every ray starts at FLT_MAX, every depth written)."""
+    if det_epsilon is None and config == 0 and not reset_depth:
         return None
     eps = DET_EPSILON_REFERENCE if det_epsilon is None else float(det_epsilon)
-    return TraceOptions(eps, 0, int(config), 0)
+    return TraceOptions(eps, RK_TRACE_RESET_DEPTH if reset_depth else 0, int(config), 0)
 
 
 class BatchMesh(C.Structure):

@@ -248,9 +248,15 @@ rk_status rk_fast_build(rk_stream stream, const rk_uint4* faces, const rk_float4
  * absolute bound that rejects every hit once edges are ~1e-3 long; 0 rejects
  * only exactly parallel rays. NULL options = the reference threshold. */
 #define RK_DET_EPSILON_REFERENCE 0.000001f
+/* rk_trace_options.flags. RK_TRACE_RESET_DEPTH: every ray starts with tmax
+ * FLT_MAX (depths is not read) and every ray's depth is written, FLT_MAX for a
+ * miss, whose faces_id and coords are left as they are: the state that
+ * filling depths with FLT_MAX and tracing leaves, in one launch. For
+ * rk_fast_trace, rk_fast_trace_sorted, rk_fast_trace_counted and rk_fast_autotune. */
+#define RK_TRACE_RESET_DEPTH 1u
 typedef struct rk_trace_options {
   float det_epsilon;
-  uint32_t flags;
+  uint32_t flags;  /* RK_TRACE_* */
   uint32_t config;   /* traversal variant (rk_fast_autotune, rk_fast_trace_variant); 0: the default */
   uint32_t reserved;
 } rk_trace_options;

@@ -109,3 +109,33 @@ def test_autotuners_choose_and_leave_the_frame(dev, rk, ora):
     cbest = dev.autotune_camera(tree, cam, w, h, cout, reps=2)
     assert 1 <= cbest.config <= N_CAMERA_VARIANTS and cbest.ms > 0
     np.testing.assert_array_equal(_h(cout[2]), ref[2])
+
+
+def test_reset_depth_equals_fill_then_trace(dev, rk, ora):
+    """RK_TRACE_RESET_DEPTH leaves what filling depths with FLT_MAX and tracing
+    leaves: every depth written (FLT_MAX on a miss), ids and coords of misses
+    untouched; for every rk_fast_trace variant and for the sorted trace."""
+    w, h = 200, 120
+    V, F, cam = _scene(rk, w, h)
+    tree = dev.FastBuilder(F.shape[0]).build(_t(F), _t(V), rk.hitbox(V), 1)
+    O, D = rk.primary_rays_host(cam, w, h)
+    Ot, Dt = _t(O), _t(D)
+    n = w * h
+    rng = np.random.default_rng(9)
+    fid0 = rng.integers(0, 1 << 20, n, dtype=np.uint32)
+    uv0 = rng.random((n, 2), dtype=np.float32)
+    dep0 = rng.random(n, dtype=np.float32) * 0.01  # tiny depths: a trace reading them would hit nothing
+    filled = (_t(fid0), _t(uv0), _t(np.full(n, F32MAX, np.float32)))
+    want = [_h(x) for x in dev.fast_trace(tree, Ot, Dt, out=filled)]
+    hit = want[2] < F32MAX
+    assert 0.05 < hit.mean() < 0.95
+    np.testing.assert_array_equal(want[0][~hit], fid0[~hit])
+    for k in range(0, N_TRACE_VARIANTS + 1):
+        out = (_t(fid0), _t(uv0), _t(dep0))
+        got = [_h(x) for x in dev.fast_trace(tree, Ot, Dt, out=out, config=k, reset_depth=True)]
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b, err_msg=f"trace variant {k}")
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    got = [_h(x) for x in dev.fast_trace_sorted(tree, Ot, Dt, out=out, reset_depth=True)]
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b, err_msg="sorted trace")
