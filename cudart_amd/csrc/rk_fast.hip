@@ -1254,7 +1254,7 @@ __global__ __launch_bounds__(256) void k_batch_ray_keys(const rk_batch_mesh* __r
 
 // Closest hit of ray r (r >= n_rays: none, the lane only helps the others of
 // its wave), in its mesh's tree. Every lane of the wave calls it.
-template <int kStackN, int kWG>
+template <int kStackN, int kWG, int kDM>
 __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict__ meshes, uint32_t mesh_count,
                                                 const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                 uint32_t n_tris, const float4* __restrict__ origins,
@@ -1266,7 +1266,7 @@ __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict_
   const uint32_t root = has_ray ? meshes[mesh_of_ray(meshes, mesh_count, r)].face_offset : 0u;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t nb = 0, nt = 0;
-  const RayHit h = trace_ray_shared<false, kStackN, kLdsStack, kWG, false, 0, kTraceDonateMin>(
+  const RayHit h = trace_ray_shared<false, kStackN, kLdsStack, kWG, false, 0, kDM>(
       nodes, tris, n_tris, root, has_ray, has_ray ? origins[r] : zero4, has_ray ? dirs[r] : zero4,
       has_ray ? depths[r] : 0.0f, eps, nb, nt, lds_stack, xchg);
   if (has_ray && h.pos != 0xFFFFFFFFu) {
@@ -1277,7 +1277,7 @@ __device__ __forceinline__ void batch_trace_one(const rk_batch_mesh* __restrict_
 }
 
 // One ray per thread, kWG rays per workgroup.
-template <int kWG, int kStackN>
+template <int kWG, int kStackN, int kDM>
 __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* __restrict__ meshes,
                                                           uint32_t mesh_count, const FNode* __restrict__ nodes,
                                                           const float4* __restrict__ tris, uint32_t n_tris,
@@ -1290,7 +1290,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
   const uint32_t r = t < n_rays ? (order ? order[t] : t) : n_rays;
   __shared__ uint2 s_stack[kLdsStack * kWG];
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
-  batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
+  batch_trace_one<kStackN, kWG, kDM>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
                                 depths, eps, s_stack, s_xchg[threadIdx.x >> 6]);
 }
 
@@ -1300,7 +1300,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_grid(const rk_batch_mesh* _
 // stays dry, so after eight dry queues in a row the wave has seen all work.
 constexpr uint32_t kQueues = 8, kQueueStride = 32;  // counters 128 B apart
 
-template <int kWG, int kStackN>
+template <int kWG, int kStackN, int kDM>
 __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* __restrict__ meshes,
                                                            uint32_t mesh_count, const FNode* __restrict__ nodes,
                                                            const float4* __restrict__ tris, uint32_t n_tris,
@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(kWG) void k_batch_trace_queue(const rk_batch_mesh* 
     }
     const uint32_t p = tile * 64 + lane;
     const uint32_t r = p < n_rays ? (order ? order[p] : p) : n_rays;
-    batch_trace_one<kStackN, kWG>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
+    batch_trace_one<kStackN, kWG, kDM>(meshes, mesh_count, nodes, tris, n_tris, origins, dirs, r, n_rays, faces_id, coords,
                                   depths, eps, s_stack, s_xchg[threadIdx.x >> 6]);
   }
 }
@@ -1342,7 +1342,8 @@ uint32_t key_passes_for(uint32_t mesh_count) {
 bool config_ok(const rk_batch_config* c) {
   return c && c->leaf_max >= 1 && c->leaf_max <= kMaxLeaf && c->trace_mode <= 1 &&
          (c->trace_wg == 64 || c->trace_wg == 128 || c->trace_wg == 256) &&
-         (c->stack_depth == 64 || c->stack_depth == 96) && c->ray_order <= 1;
+         (c->stack_depth == 64 || c->stack_depth == 96) && c->ray_order <= 1 &&
+         (c->donate_min == 0 || c->donate_min == (uint32_t)kTraceDonateMin || c->donate_min == 32);
 }
 
 }  // namespace
@@ -1408,16 +1409,16 @@ extern "C" size_t rk_batch_trace_workspace_size(uint32_t mesh_count, uint32_t ra
          rk_sort_workspace_size((uint32_t)n, 8) + 1024;
 }
 
-template <int kWG, int kStackN>
+template <int kWG, int kStackN, int kDM>
 static void launch_batch_trace(hipStream_t st, uint32_t mode, uint32_t grid_persistent, const rk_batch_mesh* meshes,
                                uint32_t mesh_count, const FNode* nodes, const float4* tris, uint32_t n_tris,
                                const float4* o, const float4* d, uint32_t n_rays, uint32_t* fid, float2* uv,
                                float* dep, float eps, uint32_t* queues, const uint32_t* order) {
   if (mode == 0)
-    hipLaunchKernelGGL((k_batch_trace_grid<kWG, kStackN>), dim3(blocks_for(n_rays, kWG)), dim3(kWG), 0, st, meshes,
+    hipLaunchKernelGGL((k_batch_trace_grid<kWG, kStackN, kDM>), dim3(blocks_for(n_rays, kWG)), dim3(kWG), 0, st, meshes,
                        mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, order);
   else
-    hipLaunchKernelGGL((k_batch_trace_queue<kWG, kStackN>), dim3(grid_persistent), dim3(kWG), 0, st, meshes,
+    hipLaunchKernelGGL((k_batch_trace_queue<kWG, kStackN, kDM>), dim3(grid_persistent), dim3(kWG), 0, st, meshes,
                        mesh_count, nodes, tris, n_tris, o, d, n_rays, fid, uv, dep, eps, queues, order);
 }
 
@@ -1476,15 +1477,21 @@ extern "C" rk_status rk_batch_trace(rk_stream stream, const rk_batch_mesh* meshe
                                                sort_ws, sort_bytes, flags);
     if (s != RK_SUCCESS) return s;
   }
-#define RK_BATCH_TRACE(WG, SD)                                                                                     \
-  launch_batch_trace<WG, SD>(st, mode, grid_persistent, meshes, mesh_count, nodes, tris, nt, o, d, ray_count,      \
-                             faces_id, uv, depths, eps, queues, order)
-  const bool deep = config->stack_depth == 96;
+#define RK_BATCH_TRACE(WG, SD, DM)                                                                                 \
+  launch_batch_trace<WG, SD, DM>(st, mode, grid_persistent, meshes, mesh_count, nodes, tris, nt, o, d, ray_count,  \
+                                 faces_id, uv, depths, eps, queues, order)
+#define RK_BATCH_TRACE_DM(WG, SD)                        \
+  do {                                                   \
+    if (dm32) RK_BATCH_TRACE(WG, SD, 32);                \
+    else RK_BATCH_TRACE(WG, SD, kTraceDonateMin);        \
+  } while (0)
+  const bool deep = config->stack_depth == 96, dm32 = config->donate_min == 32;
   switch (config->trace_wg) {
-    case 64: if (deep) RK_BATCH_TRACE(64, 96); else RK_BATCH_TRACE(64, 64); break;
-    case 128: if (deep) RK_BATCH_TRACE(128, 96); else RK_BATCH_TRACE(128, 64); break;
-    default: if (deep) RK_BATCH_TRACE(256, 96); else RK_BATCH_TRACE(256, 64); break;
+    case 64: if (deep) RK_BATCH_TRACE_DM(64, 96); else RK_BATCH_TRACE_DM(64, 64); break;
+    case 128: if (deep) RK_BATCH_TRACE_DM(128, 96); else RK_BATCH_TRACE_DM(128, 64); break;
+    default: if (deep) RK_BATCH_TRACE_DM(256, 96); else RK_BATCH_TRACE_DM(256, 64); break;
   }
+#undef RK_BATCH_TRACE_DM
 #undef RK_BATCH_TRACE
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;

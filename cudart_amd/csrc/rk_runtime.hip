@@ -195,8 +195,9 @@ extern "C" rk_status rk_graph_destroy(rk_graph graph) {
 }
 
 // ---- autotuning -----------------------------------------------------------------
-// Candidates: leaf_max 2/4/8 for the build; for the trace, both modes, three
-// workgroup sizes and both exact stack depths. Each candidate is timed on the
+// Candidates: leaf_max 1/2/4/8 for the build; for the trace, rays as given or
+// sorted, both modes, three workgroup sizes, both exact stack depths and
+// donation at 16 or 32 idle lanes. Each candidate is timed on the
 // caller's batch with HIP events on the caller's stream (median of reps).
 namespace {
 float median_of(float* v, uint32_t n) {
@@ -230,6 +231,7 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
   static const uint32_t leaves[] = {1, 2, 4, 8};
   static const uint32_t wgs[] = {64, 128, 256};
   static const uint32_t stacks[] = {64, 96};
+  static const uint32_t donates[] = {16, 32};
   float samples[64];
   float best_total = 3.4e38f, best_build = 0.0f;
   rk_batch_config pick = rk_batch_config_default();
@@ -255,11 +257,13 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
     for (uint32_t oi = 0; oi < 2 && s == RK_SUCCESS; ++oi)
     for (uint32_t mode = 0; mode < 2 && s == RK_SUCCESS; ++mode)
       for (uint32_t wi = 0; wi < 3 && s == RK_SUCCESS; ++wi)
-        for (uint32_t si = 0; si < 2 && s == RK_SUCCESS; ++si) {
+        for (uint32_t si = 0; si < 2 && s == RK_SUCCESS; ++si)
+        for (uint32_t di = 0; di < 2 && s == RK_SUCCESS; ++di) {
           c.ray_order = oi;
           c.trace_mode = mode;
           c.trace_wg = wgs[wi];
           c.stack_depth = stacks[si];
+          c.donate_min = donates[di];
           float trace_ms = 0.0f;
           s = timed([&] {
                 rk_status fs = rk_fill_f32(stream, depths, ray_count, 3.402823466e38f);

@@ -355,10 +355,10 @@ def build_sah(F: torch.Tensor, V: torch.Tensor, hitbox: np.ndarray, lbvh: RefTre
 
 # ---- batches -------------------------------------------------------------------
 
-def batch_config(leaf_max=1, trace_mode=1, trace_wg=256, stack_depth=64, ray_order=0) -> N.BatchConfig:
+def batch_config(leaf_max=1, trace_mode=1, trace_wg=256, stack_depth=64, ray_order=0, donate_min=0) -> N.BatchConfig:
     c = N.BatchConfig()
     c.leaf_max, c.trace_mode, c.trace_wg, c.stack_depth = leaf_max, trace_mode, trace_wg, stack_depth
-    c.ray_order = ray_order
+    c.ray_order, c.donate_min = ray_order, donate_min
     return c
 
 

@@ -78,11 +78,12 @@ BATCH_MESH_DTYPE = np.dtype([("face_offset", "<u4"), ("face_count", "<u4"), ("ve
 
 class BatchConfig(C.Structure):
     _fields_ = [("leaf_max", C.c_uint32), ("trace_mode", C.c_uint32), ("trace_wg", C.c_uint32),
-                ("stack_depth", C.c_uint32), ("ray_order", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("stack_depth", C.c_uint32), ("ray_order", C.c_uint32), ("donate_min", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {"leaf_max": self.leaf_max, "trace_mode": self.trace_mode, "trace_wg": self.trace_wg,
-                "stack_depth": self.stack_depth, "ray_order": self.ray_order}
+                "stack_depth": self.stack_depth, "ray_order": self.ray_order, "donate_min": self.donate_min}
 
 
 class CameraRays(C.Structure):

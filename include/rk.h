@@ -403,15 +403,18 @@ typedef struct rk_batch_mesh {
  * work queues (one per XCD, stealing when its own runs dry); trace_wg is 64,
  * 128 or 256; stack_depth is 64 or 96 (the per-ray traversal stack); ray_order
  * 0 traces the rays as given, 1 first sorts them (within their mesh) by the
- * Morton code of their origin in the mesh's root box, for incoherent rays; the
- * results are the same either way. */
+ * Morton code of their origin in the mesh's root box, for incoherent rays;
+ * donate_min: a wave hands its pending subtrees to idle lanes only on steps
+ * where at least this many of its lanes are idle, 16 or 32 (0: 16). The
+ * results are the same whatever the configuration. */
 typedef struct rk_batch_config {
   uint32_t leaf_max;
   uint32_t trace_mode;
   uint32_t trace_wg;
   uint32_t stack_depth;
   uint32_t ray_order;
-  uint32_t reserved[3];
+  uint32_t donate_min;
+  uint32_t reserved[2];
 } rk_batch_config;
 /* leaf_max 1, persistent queues, 256-thread workgroups, stack 64, rays as given. */
 rk_batch_config rk_batch_config_default(void);

@@ -91,11 +91,13 @@ def test_batch_every_trace_config_matches_oracle(dev, rk, ora, leaf_max):
         for mode in (0, 1):
             for wg in (64, 128, 256):
                 for stack in (64, 96):
-                    c = dev.batch_config(leaf_max, mode, wg, stack, ray_order)
-                    b.build(c)
-                    out = dev.new_outputs(O.shape[0])
-                    b.trace(out, c, det_epsilon=0.0)
-                    assert _check(b, out, ref) > 0.15, (ray_order, mode, wg, stack)  # 0.209 of these rays hit
+                    for donate in (0, 32):
+                        c = dev.batch_config(leaf_max, mode, wg, stack, ray_order, donate)
+                        b.build(c)
+                        out = dev.new_outputs(O.shape[0])
+                        b.trace(out, c, det_epsilon=0.0)
+                        # 0.209 of these rays hit
+                        assert _check(b, out, ref) > 0.15, (ray_order, mode, wg, stack, donate)
 
 
 def test_batch_equals_plain_per_mesh(dev, rk):
@@ -173,7 +175,7 @@ def test_batch_autotune(dev, rk):
     out = dev.new_outputs(O.shape[0])
     best, ms = b.autotune(out, reps=2, det_epsilon=0.0)
     assert best.leaf_max in (1, 2, 4, 8) and best.trace_mode in (0, 1) and best.trace_wg in (64, 128, 256)
-    assert best.stack_depth in (64, 96) and best.ray_order in (0, 1) and ms > 0
+    assert best.stack_depth in (64, 96) and best.ray_order in (0, 1) and best.donate_min in (16, 32) and ms > 0
     got = [_h(x) for x in out]
     np.testing.assert_array_equal(got[2], ref[2])  # depths do not depend on the tree
 
