@@ -374,6 +374,97 @@ def soup_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     }
 
 
+def frame_loop_leg(torch, dev, native, timer, steps, warmup, tree, cam, W, H, F, V, Vt, cam_config, rtree, Ot, Dt,
+                   with_cpu=False):
+    """The frame of samples/02_how_to_use_the_raytracer.cpp:67-85 on the
+    heightfield: primary rays, closest hit, the hit face's normal
+    (rk_interpolate_data) and the shaded pixel (rk_shade). Fast tree: the camera
+    trace kernel, then the two kernels, or all of it in rk_fast_render_camera;
+    reference tree: rk_calc_primary_rays, a fill, the packet traversal, the two
+    kernels. Face normals (the heightfield has no vertex normals)."""
+    log("frame loop: rays, trace, normals, shading")
+    nr = W * H
+    order = dev.u32_view(tree.indices)
+    Fs = dev.to_u32_tensor(np.ascontiguousarray(F[order]))  # faces in leaf order, as the builds leave the mesh
+    c = np.asarray(cam, np.float64)
+    look = c[1, :3] - c[2, :3] * (W / 2.0) - c[3, :3] * (H / 2.0)  # the centre pixel's ray
+    look = (look / np.linalg.norm(look)).astype(np.float32)
+    back = (58 / 255.0, 110 / 255.0, 165 / 255.0, 1.0)
+    fid, uv, dep = dev.new_outputs(nr)
+    nrm = torch.zeros((nr, 4), dtype=torch.float32, device="cuda")
+    px = torch.zeros(nr, dtype=torch.int32, device="cuda")
+    px2 = torch.zeros(nr, dtype=torch.int32, device="cuda")
+    outs = (None, nrm, None, None, None)
+    trace = lambda: dev.fast_trace_camera(tree, cam, W, H, out=(fid, uv, dep), det_epsilon=HEADLINE_DET_EPSILON,
+                                          config=cam_config)
+    interp = lambda: dev.interpolate(fid, uv, W, H, Fs, Vt, outs=outs)
+    shade = lambda: dev.shade(dep, nrm, W, H, back, look, pixels=px)
+
+    def sequence():
+        trace()
+        interp()
+        shade()
+
+    fused = lambda: dev.render_camera(tree, cam, W, H, Fs, Vt, back, look, pixels=px2,
+                                      det_epsilon=HEADLINE_DET_EPSILON)
+    for _ in range(warmup):
+        sequence()
+        fused()
+    seq_ms = timer.time(sequence, steps)
+    fused_ms = timer.time(fused, steps)
+    trace_ms = timer.time(trace, steps)
+    interp_ms = timer.time(interp, steps)
+    shade_ms = timer.time(shade, steps)
+    sequence()
+    fused()
+    same = bool(torch.equal(px, px2))
+    # the reference tree's frame
+    O, D = torch.empty_like(Ot), torch.empty_like(Dt)
+    cr = dev.camera_rays(cam)
+    Fr = rtree.faces
+    rfid, ruv, rdep = dev.new_outputs(nr)
+    rnrm = torch.zeros((nr, 4), dtype=torch.float32, device="cuda")
+    rpx = torch.zeros(nr, dtype=torch.int32, device="cuda")
+    f32max = float(np.finfo(np.float32).max)
+
+    def ref_frame():
+        native.check(native.lib().rk_calc_primary_rays(None, W, H, cr.origin, cr.direction, cr.step_x, cr.step_y,
+                                                       O.data_ptr(), D.data_ptr()), "primary rays")
+        native.check(native.lib().rk_fill_f32(None, rdep.data_ptr(), nr, f32max), "fill")
+        dev.trace_ref(rtree, Vt, O, D, W, H, True, out=(rfid, ruv, rdep))
+        dev.interpolate(rfid, ruv, W, H, Fr, Vt, outs=(None, rnrm, None, None, None))
+        dev.shade(rdep, rnrm, W, H, back, look, pixels=rpx)
+
+    ref_frame()
+    ref_ms = timer.time(ref_frame, 2)
+    res = {
+        "workload": "heightfield 708x708, 1920x1080: primary rays + closest hit + hit normal + shaded pixel "
+                    "(samples/02 frame loop), threshold 1e-6",
+        "fast_sequence_ms": round(seq_ms, 4),
+        "fast_sequence_frames_s": round(1e3 / seq_ms, 1),
+        "fused_ms": round(fused_ms, 4),
+        "fused_frames_s": round(1e3 / fused_ms, 1),
+        "fused_same_pixels": same,
+        "stage_ms": {"camera_trace": round(trace_ms, 4), "interpolate_normals": round(interp_ms, 4),
+                     "shade": round(shade_ms, 4)},
+        "reference_packet_frame_ms": round(ref_ms, 3),
+    }
+    if with_cpu:
+        import oracle
+        fh, uh, dh = (x.cpu().numpy() for x in (fid, uv, dep))
+        fh = fh.view(np.uint32)
+        Fsh = np.ascontiguousarray(F[order])
+        t0 = time.perf_counter()
+        on = oracle.interpolate(fh, uh, Fsh, V)[1]
+        opx = oracle.shade(dh, on, W, H, np.array(back, np.float32), look)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(dt * 1e3, 2), "unit": "ms per frame (normals + shading only)",
+                               "cores": 1, "kind": "port",
+                               "sample": "the whole frame's hit normals and pixels from the GPU trace's outputs",
+                               "same_pixels": bool(np.array_equal(opx, px.cpu().numpy().view(np.uint32)))}
+    return res
+
+
 def batch_leg(torch, dev, native, timer, steps, warmup, with_cpu=False):
     """4096 soups (log-uniform 64..65536 faces, 256..16384 rays each): one
     step builds every mesh and traces every ray. Batched = the autotuned
@@ -753,6 +844,8 @@ def main():
         result["reference_per_ray_mrays_s"] = round(nr / (ref_ray_ms * 1e-3) / 1e6, 3)
         result["reference_packet_mrays_s"] = round(nr / (ref_pkt_ms * 1e-3) / 1e6, 3)
         result["reference_node_count"] = rtree.node_count
+        result["frame_loop"] = frame_loop_leg(torch, dev, native, timer, args.steps, args.warmup, tree, cam, W, H, F, V,
+                                              Vt, cam_tuning.config, rtree, Ot, Dt, with_cpu=not args.no_cpu)
 
         log("reference 32-bin SAH and Hybrid builds")
         Fs = Ft.clone()

@@ -93,6 +93,43 @@ __device__ __forceinline__ uint32_t band_ray(uint32_t t, uint32_t n_rays) {
   return (w < waves && r < n_rays) ? r : n_rays;
 }
 
+// Attribute interpolation at barycentrics (u, v) with a = (1 - u) - v:
+// (a * x0 + u * x1) + v * x2 per component, the reference's float4 operator
+// order (types/_cuda_float4.h:80-111).
+__device__ __forceinline__ float4 bary4(float a, float u, float v, float4 x0, float4 x1, float4 x2) {
+  return add4(add4(muls4(x0, a), muls4(x1, u)), muls4(x2, v));
+}
+// v * (1 / sqrtf(dot(v, v))), the reference's normalize.
+__device__ __forceinline__ float4 normalize4(float4 v) {
+  const float inv = 1.0f / sqrtf(dot4(v, v));
+  return muls4(v, inv);
+}
+// The normal of face f at (u, v): interpolated vertex normals, normalised, or
+// without them the face normal normalize(cross(v1 - v0, v2 - v0))
+// (kernel_interpolateData, CUDA_interpolateData.cu:45-141).
+__device__ __forceinline__ float4 face_normal_at(const uint4 f, float u, float v, const float4* __restrict__ verts,
+                                                 const float4* __restrict__ norms) {
+  if (norms) return normalize4(bary4((1.0f - u) - v, u, v, norms[f.x], norms[f.y], norms[f.z]));
+  const float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
+  return normalize4(cross4(sub4(v1, v0), sub4(v2, v0)));
+}
+// make_pixel (kernels/CUDAShaderCommon.cuh): scale by 255, clamp to [0, 255]
+// with fminf/fmaxf, truncate, pack as A << 24 | R << 16 | G << 8 | B.
+__device__ __forceinline__ uint32_t make_pixel(float4 v) {
+  v = muls4(v, 255.0f);
+  v = make_float4(fmaxf(fminf(v.x, 255.0f), 0.0f), fmaxf(fminf(v.y, 255.0f), 0.0f), fmaxf(fminf(v.z, 255.0f), 0.0f),
+                  fmaxf(fminf(v.w, 255.0f), 0.0f));
+  return (((uint32_t)v.x & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.z & 0xFFu) |
+         (((uint32_t)v.w & 0xFFu) << 24);
+}
+// kernel_shade's pixel (CUDAShaderKernel.cu:14-43) for a hit with normal n
+// (w ignored) seen along cam (w = 0): grey -dot(cam, n).
+__device__ __forceinline__ uint32_t shade_hit(float4 cam, float4 n) {
+  n.w = 0.0f;
+  const float c = -dot4(cam, n);
+  return make_pixel(make_float4(c, c, c, c));
+}
+
 }  // namespace rk
 
 // Host-side launch check: every entry point returns RK_ERROR_LAUNCH when a

@@ -1747,6 +1747,39 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_camera_v(const FNode* __rest
   depths[p] = hit ? h.t : RK_HUGE;
 }
 
+// The sample-02 frame in one launch: each pixel's primary ray, its closest hit
+// (as k_fast_trace_camera_v), the hit face's normal at (u, v) (as
+// rk_interpolate_data) and the pixel (as rk_shade), with the same arithmetic in
+// the same order as those three kernels, so the pixels are bit-identical to
+// theirs. faces: the mesh's faces in the tree's leaf order (faces[position]),
+// as the reference's builds leave them. faces_id, coords and depths, when not
+// NULL, receive what k_fast_trace_camera_v writes.
+template <int kWG, int kBands, int kDM>
+__global__ __launch_bounds__(kWG) void k_fast_render_camera(
+    const FNode* __restrict__ nodes, const float4* __restrict__ tris, uint32_t n_tris, float4 ro, float4 rd,
+    float4 rx, float4 ry, int width, int height, const uint4* __restrict__ faces, const float4* __restrict__ verts,
+    const float4* __restrict__ norms, float4 back, float4 cam, uint32_t* __restrict__ pixels, int pitch,
+    uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths, float eps) {
+  int x, y;
+  const bool has_ray = camera_pixel<kWG, kBands, false>(width, height, x, y);
+  const float4 d = rk::sub4(rk::sub4(rd, rk::muls4(rx, (float)x)), rk::muls4(ry, (float)y));
+  uint32_t nb = 0, nt = 0;
+  __shared__ uint2 s_stack[kLdsStack * kWG];
+  __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
+  const RayHit h = trace_ray_shared<false, kStack, kLdsStack, kWG, false, 0, kDM>(
+      nodes, tris, n_tris, 0u, has_ray, ro, d, RK_HUGE, eps, nb, nt, s_stack, s_xchg[threadIdx.x >> 6]);
+  if (!has_ray) return;
+  const bool hit = h.pos != 0xFFFFFFFFu;  // then t < FLT_MAX: the shader's depth test
+  if (faces_id) {
+    const size_t p = (size_t)y * (size_t)width + (size_t)x;
+    faces_id[p] = h.pos;
+    coords[p] = hit ? make_float2(h.u, h.v) : make_float2(0.0f, 0.0f);
+    depths[p] = hit ? h.t : RK_HUGE;
+  }
+  pixels[(size_t)y * (size_t)pitch + (size_t)x] =
+      hit ? rk::shade_hit(cam, rk::face_normal_at(faces[h.pos], h.u, h.v, verts, norms)) : rk::make_pixel(back);
+}
+
 // Workgroups of a camera-kernel variant for a width x height frame.
 uint32_t camera_blocks(int width, int height, int wg, int bands, bool xcd) {
   const int run = 64 / bands;
@@ -2005,6 +2038,31 @@ extern "C" rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* b
   hipLaunchKernelGGL(cv->kernel, grid, dim3(cv->wg), 0, rk_hip_stream(stream), static_cast<const FNode*>(bvh->nodes),
                      static_cast<const float4*>(bvh->tris), bvh->face_count, f4(camera->origin),
                      f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height, faces_id,
+                     reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
+  RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_fast_render_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera,
+                                           int width, int height, const rk_uint4* faces, const rk_float4* vertices,
+                                           const rk_float4* normals, rk_float4 back_color, rk_float4 cam_dir,
+                                           uint32_t* pixels, int pitch, uint32_t* faces_id, rk_float2* coords,
+                                           float* depths, const rk_trace_options* options) {
+  if (!bvh || !bvh->nodes || !bvh->tris || bvh->face_count == 0 || !camera || !faces || !vertices || !pixels)
+    return RK_ERROR_INVALID_VALUE;
+  if (width <= 0 || height <= 0 || pitch < width) return RK_ERROR_INVALID_VALUE;
+  if ((faces_id || coords || depths) && !(faces_id && coords && depths)) return RK_ERROR_INVALID_VALUE;
+  if ((uint64_t)width * (uint64_t)height > 0xFFFFFFFFull || (uint64_t)pitch * (uint64_t)height > 0xFFFFFFFFull)
+    return RK_ERROR_UNSUPPORTED;
+  auto f4 = [](const rk_float4& v) { return make_float4(v.x, v.y, v.z, v.w); };
+  dim3 grid(camera_blocks(width, height, kTraceWG, kCamBands, false));
+  hipLaunchKernelGGL((k_fast_render_camera<kTraceWG, kCamBands, kTraceDonateMin>), grid, dim3(kTraceWG), 0,
+                     rk_hip_stream(stream), static_cast<const FNode*>(bvh->nodes),
+                     static_cast<const float4*>(bvh->tris), bvh->face_count, f4(camera->origin),
+                     f4(camera->direction), f4(camera->step_x), f4(camera->step_y), width, height,
+                     reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
+                     reinterpret_cast<const float4*>(normals), f4(back_color),
+                     make_float4(cam_dir.x, cam_dir.y, cam_dir.z, 0.0f), pixels, pitch, faces_id,
                      reinterpret_cast<float2*>(coords), depths, det_eps_of(options));
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;

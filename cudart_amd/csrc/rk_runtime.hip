@@ -309,14 +309,8 @@ extern "C" rk_status rk_batch_autotune(rk_stream stream, const rk_batch_mesh* me
 // id is not a face of the mesh (>= face_count) is left unchanged.
 namespace {
 
-__device__ __forceinline__ float4 bary4(float a, float u, float v, float4 x0, float4 x1, float4 x2) {
-  return rk::add4(rk::add4(rk::muls4(x0, a), rk::muls4(x1, u)), rk::muls4(x2, v));
-}
-
-__device__ __forceinline__ float4 normalize4(float4 v) {
-  const float inv = 1.0f / sqrtf(rk::dot4(v, v));
-  return rk::muls4(v, inv);
-}
+using rk::bary4;
+using rk::normalize4;
 
 __global__ __launch_bounds__(256) void k_interpolate(const uint32_t* __restrict__ faces_id,
                                                      const float2* __restrict__ coords, int width, int height,
@@ -337,12 +331,7 @@ __global__ __launch_bounds__(256) void k_interpolate(const uint32_t* __restrict_
   const float4 v0 = verts[f.x], v1 = verts[f.y], v2 = verts[f.z];
   const float a = (1.0f - c.x) - c.y;
   if (out_p) out_p[l] = bary4(a, c.x, c.y, v0, v1, v2);
-  if (out_n) {
-    if (norms)
-      out_n[l] = normalize4(bary4(a, c.x, c.y, norms[f.x], norms[f.y], norms[f.z]));
-    else
-      out_n[l] = normalize4(rk::cross4(rk::sub4(v1, v0), rk::sub4(v2, v0)));
-  }
+  if (out_n) out_n[l] = rk::face_normal_at(f, c.x, c.y, verts, norms);
   if (out_c) out_c[l] = cols ? bary4(a, c.x, c.y, cols[f.x], cols[f.y], cols[f.z]) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
   if (out_t) {
     if (texs) {
@@ -377,16 +366,6 @@ extern "C" rk_status rk_interpolate_data(rk_stream stream, const uint32_t* faces
 // ---- shading (kernel_shade, CUDAShaderKernel.cu:14-43) ---------------------------
 namespace {
 
-// make_pixel (kernels/CUDAShaderCommon.cuh): scale by 255, clamp to [0, 255]
-// with fminf/fmaxf, truncate, pack as A << 24 | R << 16 | G << 8 | B.
-__device__ __forceinline__ uint32_t make_pixel(float4 v) {
-  v = rk::muls4(v, 255.0f);
-  v = make_float4(fmaxf(fminf(v.x, 255.0f), 0.0f), fmaxf(fminf(v.y, 255.0f), 0.0f), fmaxf(fminf(v.z, 255.0f), 0.0f),
-                  fmaxf(fminf(v.w, 255.0f), 0.0f));
-  return (((uint32_t)v.x & 0xFFu) << 16) | (((uint32_t)v.y & 0xFFu) << 8) | ((uint32_t)v.z & 0xFFu) |
-         (((uint32_t)v.w & 0xFFu) << 24);
-}
-
 // One thread per pixel, in row order over the input surfaces (k = y * width +
 // x); the output row of pixel k starts at y * pitch.
 __global__ __launch_bounds__(256) void k_shade(const float* __restrict__ depths, const float4* __restrict__ normals,
@@ -395,15 +374,7 @@ __global__ __launch_bounds__(256) void k_shade(const float* __restrict__ depths,
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= count) return;
   const uint32_t y = k / (uint32_t)width, x = k - y * (uint32_t)width;
-  uint32_t px;
-  if (depths[k] < RK_HUGE) {
-    float4 n = normals[k];
-    n.w = 0.0f;
-    const float c = -rk::dot4(cam, n);
-    px = make_pixel(make_float4(c, c, c, c));
-  } else {
-    px = make_pixel(back);
-  }
+  const uint32_t px = depths[k] < RK_HUGE ? rk::shade_hit(cam, normals[k]) : rk::make_pixel(back);
   pixels[(size_t)y * (uint32_t)pitch + x] = px;
 }
 

@@ -550,6 +550,39 @@ def interpolate(faces_id: torch.Tensor, coords: torch.Tensor, width: int, height
         return outs
 
 
+def shade(depths: torch.Tensor, normals: torch.Tensor, width: int, height: int, back_color, cam_dir,
+          pixels: torch.Tensor | None = None, pitch: int | None = None, stream=None):
+    """rk_shade: one packed A,R,G,B word per pixel, rows of 'pitch' words."""
+    with _on(stream):
+        pitch = width if pitch is None else pitch
+        if pixels is None:
+            pixels = torch.zeros(pitch * height, dtype=_U32, device="cuda")
+        _check(N.lib().rk_shade(_stream_ptr(stream), depths.data_ptr(), normals.data_ptr(), width, height,
+                                N.Float4(*map(float, back_color)), N.Float4(*map(float, cam_dir)),
+                                pixels.data_ptr(), pitch), "shade")
+        return pixels
+
+
+def render_camera(tree: FastTree, cam: np.ndarray, width: int, height: int, faces: torch.Tensor, V: torch.Tensor,
+                  back_color, cam_dir, normals: torch.Tensor | None = None, pixels: torch.Tensor | None = None,
+                  pitch: int | None = None, out=None, det_epsilon=None, stream=None):
+    """rk_fast_render_camera: rays, closest hit, hit normal and shaded pixel in
+    one launch. faces: the mesh's faces in the tree's leaf order. out: None or
+    (faces_id, coords, depths) to receive the trace's outputs too."""
+    with _on(stream):
+        pitch = width if pitch is None else pitch
+        if pixels is None:
+            pixels = torch.zeros(pitch * height, dtype=_U32, device="cuda")
+        c = camera_rays(cam)
+        o = (None, None, None) if out is None else tuple(t.data_ptr() for t in out)
+        _check(N.lib().rk_fast_render_camera(_stream_ptr(stream), C.byref(tree.desc), C.byref(c), width, height,
+                                             faces.data_ptr(), V.data_ptr(),
+                                             None if normals is None else normals.data_ptr(),
+                                             N.Float4(*map(float, back_color)), N.Float4(*map(float, cam_dir)),
+                                             pixels.data_ptr(), pitch, *o, _opt(det_epsilon)), "fast_render_camera")
+        return pixels
+
+
 def fast_trace_sorted(tree: FastTree, O: torch.Tensor, D: torch.Tensor, out=None, stream=None, det_epsilon=None,
                       ws: torch.Tensor | None = None, config=0, reset_depth=False):
     """rk_fast_trace_sorted: the same results as fast_trace, rays traced in

@@ -177,6 +177,8 @@ _SIGS = {
     "rk_fast_build_keyed": (_I, [_P, _P, _P, _U32, _P, _I, _U32, C.POINTER(FastBVH), _P, _P, _P, _SZ]),
     "rk_fast_refit": (_I, [_P, _P, _P, _U32, _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_trace_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, _P]),
+    "rk_fast_render_camera": (_I, [_P, C.POINTER(FastBVH), C.POINTER(CameraRays), _I, _I, _P, _P, _P, Float4, Float4,
+                                   _P, _I, _P, _P, _P, _P]),
     "rk_fast_trace_variant": (_I, [_U32, _I, C.POINTER(TraceTuning)]),
     "rk_fast_autotune_workspace_size": (_SZ, [_U32]),
     "rk_fast_autotune": (_I, [_P, C.POINTER(FastBVH), _P, _P, _U32, _P, _P, _P, _P, _I, _P, _SZ,

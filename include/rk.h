@@ -327,6 +327,23 @@ typedef struct rk_camera_rays {
 rk_status rk_fast_trace_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera, int width,
                                int height, uint32_t* faces_id, rk_float2* coords, float* depths,
                                const rk_trace_options* options);
+/* The frame loop of samples/02_how_to_use_the_raytracer.cpp:67-85 in one launch:
+ * CUDA_calcPrimaryRays.cu:20-55, the closest hit, CUDA_interpolateData.cu:45-141
+ * (the hit face's normal only) and CUDAShaderKernel.cu:14-43, i.e. what
+ * rk_fast_trace_camera, then rk_interpolate_data (out_normals) and rk_shade
+ * leave in pixels, bit for bit, without the intermediate surfaces. No single
+ * reference counterpart: it replaces that sequence of four calls for callers of
+ * the fast tree. faces: the mesh's faces in the tree's leaf order (the order the
+ * reference's builds leave the mesh in); normals: per-vertex normals or NULL
+ * (face normals); pixels: height rows of pitch words (pitch >= width);
+ * faces_id, coords, depths: all NULL, or all set to receive what
+ * rk_fast_trace_camera writes. options->config is not used (the default camera
+ * variant). */
+rk_status rk_fast_render_camera(rk_stream stream, const rk_fast_bvh* bvh, const rk_camera_rays* camera, int width,
+                                int height, const rk_uint4* faces, const rk_float4* vertices, const rk_float4* normals,
+                                rk_float4 back_color, rk_float4 cam_dir, uint32_t* pixels, int pitch,
+                                uint32_t* faces_id, rk_float2* coords, float* depths,
+                                const rk_trace_options* options);
 
 /* Traversal variants and their autotuner (no counterpart in the reference, whose
  * launch shapes are fixed: 16x12 packets, 96-entry stacks, RT.cu:434-480). The
