@@ -183,6 +183,9 @@ def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
     t0 = time.perf_counter()
     ref = oracle.build_lbvh_ref(F, V, hb, 12, True)
     build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.build_lbvh_ref(F, V, hb, 31, True)  # the deepest tree the reference builds (tree_depth 31)
+    build31_s = time.perf_counter() - t0
     n = O.shape[0]
     stride = 64
     t0 = time.perf_counter()
@@ -206,6 +209,7 @@ def cpu_baseline(V, F, hb, O, D, budget_s=20.0):
                   f"per-ray traversal (CUDA_raytraceMeshTree.cu:111-219) with the reference threshold; the "
                   f"reference build of all {F.shape[0]} faces took {build_s:.3f} s",
         "lbvh_mtri_s": round(F.shape[0] / build_s / 1e6, 4),
+        "lbvh_d31_mtri_s": round(F.shape[0] / build31_s / 1e6, 4),
         "host_cores_visible": os.cpu_count(),
     }
 
@@ -841,6 +845,13 @@ def main():
         rf, ruv, rdep = dev.new_outputs(nr)
         ref_pkt_ms = timer.time(lambda: dev.trace_ref(rtree, Vt, Ot, Dt, W, H, True, out=(rf, ruv, rdep)), 1)
         result["reference_lbvh_mtri_s"] = round(n / (ref_build_ms * 1e-3) / 1e6, 2)
+        rb31 = dev.LBVHBuilder(n)
+        Fr31 = Ft.clone()
+        rb31.build(Fr31, Vt, hb, 31, True)
+        torch.cuda.synchronize()
+        result["reference_lbvh_d31_mtri_s"] = round(n / (timer.time(lambda: rb31.build(Fr31, Vt, hb, 31, True), 5)
+                                                         * 1e-3) / 1e6, 2)
+        del rb31, Fr31
         result["reference_per_ray_mrays_s"] = round(nr / (ref_ray_ms * 1e-3) / 1e6, 3)
         result["reference_packet_mrays_s"] = round(nr / (ref_pkt_ms * 1e-3) / 1e6, 3)
         result["reference_node_count"] = rtree.node_count
