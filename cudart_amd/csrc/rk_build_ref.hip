@@ -410,10 +410,23 @@ __global__ __launch_bounds__(256) void k_leaf_boxes(const uint32_t* __restrict__
     if (first >= n) cnt = 0;
     else if (cnt > n - first) cnt = n - first;
     float mnx = RK_HUGE, mny = RK_HUGE, mnz = RK_HUGE, mxx = -RK_HUGE, mxy = -RK_HUGE, mxz = -RK_HUGE;
-    for (uint32_t i = lane; i < cnt; i += 64) {
-      Box b = rk::load_box(boxes + indices[first + i]);
-      mnx = rk::fmn(mnx, b.mn.x); mny = rk::fmn(mny, b.mn.y); mnz = rk::fmn(mnz, b.mn.z);
-      mxx = rk::fmx(mxx, b.mx.x); mxy = rk::fmx(mxy, b.mx.y); mxz = rk::fmx(mxz, b.mx.z);
+    // Faces lane, lane + 64, lane + 128, ... in that order, kLeafFetch at a
+    // time with all their loads issued first: a face costs two dependent loads
+    // (its index, then its box), and the leaves of an incomplete tree (the
+    // Hybrid's first pass) hold thousands of faces.
+    constexpr uint32_t kLeafFetch = 8;
+    for (uint32_t i0 = lane; i0 < cnt; i0 += 64 * kLeafFetch) {
+      uint32_t ix[kLeafFetch];
+#pragma unroll
+      for (uint32_t k = 0; k < kLeafFetch; ++k) ix[k] = i0 + 64 * k < cnt ? indices[first + i0 + 64 * k] : 0u;
+#pragma unroll
+      for (uint32_t k = 0; k < kLeafFetch; ++k) {
+        if (i0 + 64 * k < cnt) {
+          const Box b = rk::load_box(boxes + ix[k]);
+          mnx = rk::fmn(mnx, b.mn.x); mny = rk::fmn(mny, b.mn.y); mnz = rk::fmn(mnz, b.mn.z);
+          mxx = rk::fmx(mxx, b.mx.x); mxy = rk::fmx(mxy, b.mx.y); mxz = rk::fmx(mxz, b.mx.z);
+        }
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
