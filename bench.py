@@ -605,6 +605,15 @@ def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000, with_cpu=F
     st, _ = srv.run_poisson(rate, count, 64, 1024, 0xC5)
     sat, _ = srv.run_poisson(0.0, 1000, 64, 1024, 2)
     srv.close()
+    # the same stream under an HBM budget of half the scenes: requests of
+    # evicted scenes upload them again from pinned host memory and rebuild
+    sizes = [native.lib().rk_server_scene_bytes(V.shape[0], F.shape[0]) for V, F, _ in pool]
+    tight = max(max(sizes), need // 2)
+    srvt = dev.Server(tight, slots=16, hbm_budget=tight, det_epsilon=HEADLINE_DET_EPSILON)
+    for V, F, z in pool:
+        srvt.add_scene(V, F, z)
+    tst, _ = srvt.run_poisson(rate, count // 2, 64, 1024, 0xC6)
+    srvt.close()
     one = server(1)
     single, _ = one.run_poisson(0.0, 1000, 64, 1024, 2)
     one.close()
@@ -654,6 +663,12 @@ def serve_leg(dev, native, budget=200 << 30, rate=2000.0, count=4000, with_cpu=F
         "uploads": int(st.uploads), "evictions": int(st.evictions),
         "resident_gb": round(st.peak_resident_bytes / 2 ** 30, 3), "budget_gb": budget / 2 ** 30,
         "slots": 16, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+        "evicting_budget": {"budget_gb": round(tight / 2 ** 30, 3), "requests": int(tst.completed),
+                            "p50_ms": round(tst.p50_ms, 3), "p99_ms": round(tst.p99_ms, 3),
+                            "max_ms": round(tst.max_ms, 3), "mrays_s": round(tst.mrays_per_s, 2),
+                            "uploads": int(tst.uploads), "evictions": int(tst.evictions),
+                            "rebuilds": int(tst.rebuilds), "refits": int(tst.refits),
+                            "resident_gb": round(tst.peak_resident_bytes / 2 ** 30, 3)},
         "saturated_mrays_s": round(sat.mrays_per_s, 2),
         "one_slot_mrays_s": round(single.mrays_per_s, 2),
         "served_vs_one_at_a_time": round(sat.mrays_per_s / single.mrays_per_s, 2) if single.mrays_per_s else None,
