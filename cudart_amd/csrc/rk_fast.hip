@@ -629,8 +629,13 @@ constexpr int kXchgWords = 4 * 64;
     }                                                                                                  \
   } while (0)
 
+//
+// kBoundEvery: helpers read their owner's best so far on every kBoundEvery-th
+// step only (1: every step); in between they cull against the value they last
+// read, which can only be larger, so they may visit more nodes but find the
+// same closest hit.
 template <bool kCount, int kStackN, int kLds, int kWG, bool kOldest = false, int kLeafBatch = 0,
-          int kDonateMin = 1>
+          int kDonateMin = 1, int kBoundEvery = 1>
 __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                    uint32_t n_tris, uint32_t root, bool has_ray, float4 orig,
                                                    float4 dir, float depth0, float eps, uint32_t& n_box,
@@ -649,6 +654,9 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
   float j_t = 0.0f, j_u = 0.0f, j_v = 0.0f;
   uint32_t cur = root, visits = 0;
   uint32_t pend = 0u;  // a postponed leaf (a leaf reference always has kLeafBit set; 0: none)
+  uint32_t step = 0;   // loop steps so far (the same in every lane)
+  float h_t = depth0;  // kBoundEvery > 1: the owner's best as this helper last read it
+  bool h_hit = false;
   static_assert(kStackN > kLds, "scratch part of the stack");
   uint32_t stack_ref[kStackN - kLds];
   float stack_t[kStackN - kLds];
@@ -681,6 +689,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
     const bool working = owner >= 0;
     const uint64_t idle = __ballot(!working);
     if (idle == ~0ull) break;
+    bool fresh = false;  // this lane took a new job this step
     // hand the newest pending subtree of up to popc(idle) lanes to idle lanes
     const uint64_t donors = __ballot(working && sp > sb);
     if (idle != 0ull && donors != 0ull && (kDonateMin <= 1 || __popcll(idle) >= kDonateMin)) {
@@ -726,6 +735,7 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
       const float nox = __shfl(ox, don, 64), noy = __shfl(oy, don, 64), noz = __shfl(oz, don, 64);
       const float od = __shfl(o_t, src, 64);
       if (take && __uint_as_float(g_t) <= od) {
+        fresh = true;
         owner = src;
         jo = no;
         jd = ndr;
@@ -747,15 +757,26 @@ __device__ __forceinline__ RayHit trace_ray_shared(const FNode* __restrict__ nod
     // the owner's best so far bounds a helper's subtree
     float dmax = o_t;
     bool d_hit = o_pos != 0xFFFFFFFFu;
-    if (__ballot(owner >= 0 && !own) != 0ull) {
-      const int src = owner >= 0 ? owner : (int)lane;
-      const float od = __shfl(o_t, src, 64);
-      const uint32_t op = __shfl(o_pos, src, 64);
-      if (!own) {
-        dmax = od;
-        d_hit = op != 0xFFFFFFFFu;
+    // a lane that just took a job reads its new owner's bound at once
+    if (kBoundEvery <= 1 || (step % (uint32_t)kBoundEvery) == 0u || __ballot(fresh) != 0ull) {
+      if (__ballot(owner >= 0 && !own) != 0ull) {
+        const int src = owner >= 0 ? owner : (int)lane;
+        const float od = __shfl(o_t, src, 64);
+        const uint32_t op = __shfl(o_pos, src, 64);
+        if (!own) {
+          dmax = od;
+          d_hit = op != 0xFFFFFFFFu;
+        }
       }
+      if (kBoundEvery > 1) {
+        h_t = dmax;
+        h_hit = d_hit;
+      }
+    } else if (!own) {
+      dmax = h_t;
+      d_hit = h_hit;
     }
+    ++step;
     bool job_end = false, flush_me = false, end_after = false;
     if (owner >= 0) {
       float bound = own ? o_t : (j_pos != 0xFFFFFFFFu ? fminf(dmax, j_t) : dmax);
@@ -918,7 +939,8 @@ static_assert(kTraceWG % 64 == 0 && kTraceWG <= 1024, "whole waves");
 // kWG threads per workgroup, runs of kRun rays per band (spread_ray; 0: 64
 // consecutive rays per wave), kLds stack entries per lane in LDS. The
 // defaults above are one variant; rk_fast_autotune chooses among several.
-template <bool kCount, int kWG, uint32_t kRun, int kLds, bool kOldest = false, int kLeafBatch = 0, int kDonateMin = 1>
+template <bool kCount, int kWG, uint32_t kRun, int kLds, bool kOldest = false, int kLeafBatch = 0, int kDonateMin = 1,
+          int kBoundEvery = 1>
 __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                       uint32_t n_tris, const float4* __restrict__ origins,
                                                       const float4* __restrict__ dirs, uint32_t n_rays,
@@ -934,7 +956,7 @@ __global__ __launch_bounds__(kWG) void k_fast_trace_v(const FNode* __restrict__ 
   __shared__ uint32_t s_xchg[kWG / 64][kXchgWords];
   uint32_t n_box = 0, n_tri = 0;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest, kLeafBatch, kDonateMin>(
+  const RayHit h = trace_ray_shared<kCount, kStack, kLds, kWG, kOldest, kLeafBatch, kDonateMin, kBoundEvery>(
       nodes, tris, n_tris, 0u, has_ray, has_ray ? origins[l] : zero4, has_ray ? dirs[l] : zero4,
       has_ray ? (reset ? RK_HUGE : depths[l]) : 0.0f, eps, n_box, n_tri, s_stack, s_xchg[threadIdx.x >> 6]);
   if (kCount) {
@@ -1802,7 +1824,7 @@ typedef void (*TraceKernel)(const FNode*, const float4*, uint32_t, const float4*
 typedef void (*CameraKernel)(const FNode*, const float4*, uint32_t, float4, float4, float4, float4, int, int,
                              uint32_t*, float2*, float*, float);
 struct TraceVariant {
-  uint32_t wg, run, lds, oldest, leaf_batch, donate_min;
+  uint32_t wg, run, lds, oldest, leaf_batch, donate_min, bound_every;
   TraceKernel kernel;
 };
 struct CameraVariant {
@@ -1812,7 +1834,8 @@ struct CameraVariant {
 
 // rk_trace_options.config k selects entry k - 1 (0: the first, the default).
 #define RK_TRACE_VARIANT(wg, run, lds, old, lb, dm) \
-  {wg, run, lds, old, lb, dm, k_fast_trace_v<false, wg, run, lds, old != 0, lb, dm>}
+  {wg, run, lds, old, lb, dm, 1, k_fast_trace_v<false, wg, run, lds, old != 0, lb, dm>}
+#define RK_TRACE_VARIANT_B(dm, be) {64, 8, 8, 0, 0, dm, be, k_fast_trace_v<false, 64, 8, 8, false, 0, dm, be>}
 
 const TraceVariant kTraceVariants[] = {
     RK_TRACE_VARIANT(kTraceWG, kSpreadRun, kLdsStack, 0, 0, kTraceDonateMin),
@@ -1824,8 +1847,11 @@ const TraceVariant kTraceVariants[] = {
     RK_TRACE_VARIANT(64, 8, 8, 0, 0, 8),   RK_TRACE_VARIANT(64, 8, 8, 0, 0, 1),  RK_TRACE_VARIANT(64, 8, 8, 0, 16, 8),
     RK_TRACE_VARIANT(64, 8, 8, 0, 16, 16), RK_TRACE_VARIANT(64, 8, 8, 0, 32, 16), RK_TRACE_VARIANT(64, 8, 8, 0, 0, 32),
     RK_TRACE_VARIANT(64, 8, 8, 1, 0, 16),
+    RK_TRACE_VARIANT_B(16, 2),             RK_TRACE_VARIANT_B(16, 4),            RK_TRACE_VARIANT_B(32, 2),
+    RK_TRACE_VARIANT_B(32, 4),
 };
 #undef RK_TRACE_VARIANT
+#undef RK_TRACE_VARIANT_B
 #define RK_CAMERA_VARIANT(wg, bands, xcd, old, lb, dm) \
   {wg, bands, xcd, old, lb, dm, k_fast_trace_camera_v<wg, bands, xcd != 0, old != 0, lb, dm>}
 const CameraVariant kCameraVariants[] = {
@@ -1871,6 +1897,7 @@ void describe_trace(uint32_t config, rk_trace_tuning* out) {
   out->steal_oldest = v.oldest;
   out->leaf_batch = v.leaf_batch;
   out->donate_min = v.donate_min;
+  out->bound_every = v.bound_every;
 }
 void describe_camera(uint32_t config, rk_trace_tuning* out) {
   const CameraVariant& v = *camera_variant(config);
@@ -1884,6 +1911,7 @@ void describe_camera(uint32_t config, rk_trace_tuning* out) {
   out->steal_oldest = v.oldest;
   out->leaf_batch = v.leaf_batch;
   out->donate_min = v.donate_min;
+  out->bound_every = 1;
 }
 
 // Times fn() reps times after one untimed call, each call preceded by

@@ -46,7 +46,8 @@ class TraceOptions(C.Structure):
 class TraceTuning(C.Structure):
     _fields_ = [("config", C.c_uint32), ("workgroup", C.c_uint32), ("ray_run", C.c_uint32), ("bands", C.c_uint32),
                 ("lds_stack", C.c_uint32), ("xcd_strips", C.c_uint32), ("ms", C.c_float),
-                ("steal_oldest", C.c_uint32), ("leaf_batch", C.c_uint32), ("donate_min", C.c_uint32)]
+                ("steal_oldest", C.c_uint32), ("leaf_batch", C.c_uint32), ("donate_min", C.c_uint32),
+                ("bound_every", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -365,6 +365,7 @@ typedef struct rk_trace_tuning {
   uint32_t steal_oldest; /* idle lanes take a busy lane's oldest pending subtree, not its newest */
   uint32_t leaf_batch;   /* leaves are postponed until this many lanes hold one (0: tested at once) */
   uint32_t donate_min;   /* pending subtrees are handed out only when this many lanes are idle */
+  uint32_t bound_every;  /* lanes helping another lane's ray read its best hit every this many steps */
 } rk_trace_tuning;
 rk_status rk_fast_trace_variant(uint32_t config, int camera, rk_trace_tuning* out);
 /* Times every rk_fast_trace variant on the caller's rays (reps timed calls each

@@ -7,7 +7,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F32MAX = np.finfo(np.float32).max
-N_TRACE_VARIANTS = 23
+N_TRACE_VARIANTS = 27
 N_CAMERA_VARIANTS = 20
 
 
@@ -41,7 +41,7 @@ def _scene(rk, w, h):
 
 
 def test_variant_table(dev, rk):
-    key = lambda v: (v.workgroup, v.ray_run, v.lds_stack, v.steal_oldest, v.leaf_batch, v.donate_min)
+    key = lambda v: (v.workgroup, v.ray_run, v.lds_stack, v.steal_oldest, v.leaf_batch, v.donate_min, v.bound_every)
     shapes = {key(dev.trace_variant(k)) for k in range(1, N_TRACE_VARIANTS + 1)}
     assert len(shapes) == N_TRACE_VARIANTS
     d = dev.trace_variant(0)
