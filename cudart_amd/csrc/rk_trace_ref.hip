@@ -1,7 +1,8 @@
 // Closest-hit traversal of the reference tree layout (bvnode, 32 B) with the
 // reference arithmetic (CUDA_raytraceMeshTree.cu), for gfx950:
 //   k_trace_ray     per-ray DFS (kernel_raytraceMeshTree, :111-219), one lane
-//                   per ray, a leaf's faces fetched two at a time, stack in
+//                   per ray for the nodes; a leaf's faces tested by its lane,
+//                   or by the whole wave when few lanes hold a leaf; stack in
 //                   LDS and scratch;
 //   k_trace_packet  the 16x12 packet traversal (kernel_raytraceMeshTreePacket,
 //                   :222-377), one 192-thread workgroup (3 waves) per tile,
@@ -84,21 +85,119 @@ __device__ __forceinline__ void children_order(uint32_t w0, uint32_t w1, float4 
 constexpr int kStack = 96;  // STACK_SIZE, :45
 constexpr int kLdsStack = 32;  // entries per lane in LDS (8 KB per wave); the rest in scratch
 
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float readlane_f32(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ float4 readlane_f4(float4 v, int l) {
+  return make_float4(readlane_f32(v.x, l), readlane_f32(v.y, l), readlane_f32(v.z, l), readlane_f32(v.w, l));
+}
+
+// The hand-off list of the per-ray traversal: word 0 counts the rays handed
+// off, the ray indices start at word kHandoffHead.
+constexpr uint32_t kHandoffHead = 64;
+constexpr uint32_t kHandoffCap = 1u << 16;  // rays per call (256 KB)
+constexpr uint32_t kWaveGrid = 2048;  // one-wave workgroups of the second pass
+#ifndef RK_REF_WAVE_PER
+#define RK_REF_WAVE_PER 4
+#endif
+constexpr int kWavePer = RK_REF_WAVE_PER;  // faces per lane per round of the second pass's leaf tests
+
+struct LeafBest { float t, u, v; uint32_t k; bool found; };
+
+// The whole wave tests faces [first, first + cnt) against one ray (o, d, dep:
+// the same in every lane) and returns the first face of smallest t < dep, the
+// face a sequential pass keeping hits with t < depth would end on: its depth
+// after such a pass is the smallest t below dep, and a later face with the
+// same t fails t < depth. A round takes up to 64 * kPer faces, lane x the
+// consecutive faces [x * per, x * per + per), so that lane order is face
+// order: each lane keeps its first face of smallest t below dep, and the
+// lowest lane holding the wave's smallest t has the first such face. The
+// next round compares with that t.
+template <int kPer>
+__device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, const float4* __restrict__ verts,
+                                              float4 o, float4 d, uint32_t first, uint32_t cnt, float dep) {
+  const uint32_t lane = threadIdx.x & 63u;
+  LeafBest r;
+  r.t = dep;
+  r.u = 0.0f;
+  r.v = 0.0f;
+  r.k = 0;
+  r.found = false;
+  const float inf = __int_as_float(0x7f800000);
+  for (uint32_t c0 = 0; c0 < cnt; c0 += 64u * kPer) {
+    const uint32_t m = min(cnt - c0, 64u * kPer);
+    const uint32_t per = (m + 63u) >> 6;
+    const uint32_t k0 = lane * per;
+    uint4 f[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+      f[j] = ((uint32_t)j < per && k0 + j < m) ? faces[first + c0 + k0 + j] : make_uint4(0, 0, 0, 0);
+    float4 p0[kPer], p1[kPer], p2[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if ((uint32_t)j < per && k0 + j < m) {
+        p0[j] = verts[f[j].x];
+        p1[j] = verts[f[j].y];
+        p2[j] = verts[f[j].z];
+      }
+    }
+    float bt = inf, bu = 0.0f, bv = 0.0f;
+    uint32_t bk = 0;
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if ((uint32_t)j < per && k0 + j < m) {
+        // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
+        float4 v0 = p0[j];
+        float4 e1 = rk::sub4(p1[j], v0), e2 = rk::sub4(p2[j], v0);
+        v0.w = 0.0f;
+        e1.w = 0.0f;
+        e2.w = 0.0f;
+        float t, u, v;
+        if (tri_ray(o, d, v0, e1, e2, t, u, v) && t < r.t && t < bt) {
+          bt = t;  // t < dep <= FLT_MAX or +inf: t is finite, below the sentinel
+          bu = u;
+          bv = v;
+          bk = c0 + k0 + (uint32_t)j;
+          any = true;
+        }
+      }
+    }
+    if (__ballot(any) == 0ull) continue;
+    float mn = any ? bt : inf;
+    for (int off = 32; off > 0; off >>= 1) mn = fminf(mn, __shfl_xor(mn, off, 64));
+    const int w = __builtin_ctzll(__ballot(any && bt == mn));
+    r.t = __shfl(bt, w, 64);  // the winner's own bits (bt == mn holds for -0 against +0 too)
+    r.u = __shfl(bu, w, 64);
+    r.v = __shfl(bv, w, 64);
+    r.k = (uint32_t)__shfl((int)bk, w, 64);
+    r.found = true;
+  }
+  return r;
+}
+
 // The per-ray traversal (kernel_raytraceMeshTree, :111-219), one lane per ray:
 // pop a node, test its box against the depth so far, push the far child, then
 // the near one; at a leaf, test its faces in order, keeping a hit only when
-// t < depth. The same operations in the same order as the reference, so the
-// results are bit-identical.
+// t < depth. Each ray's result is bit-identical to the reference's.
 //
-// A leaf's faces are fetched kLeafBatch at a time, all their loads issued
-// before the first test: a face costs two dependent loads (the face, then
-// its vertices), and a leaf holds up to a few hundred faces, so one face at a
-// time leaves each lane waiting on memory. The tests still run in face order.
-// On the 1920x1080 heightfield frame (depth-12 tree): 291 ms per frame with
-// batches of 2, 315 with 4, 294 with 8, against 395 ms one face at a time
-// with the whole stack in LDS (6 waves per CU). Testing each leaf with the
-// whole wave, 64 faces at a time, was slower (458 ms): on terrain rows every
-// lane reaches its leaves together, and the wave then walks them one by one.
+// The lanes of a wave walk their nodes until each has reached a leaf or its
+// end; then the leaves are tested, and the walk goes on. A frame's cost sits
+// in a few rays that graze the scene and cross dozens of leaves of a few
+// hundred faces each: on the 1920x1080 heightfield the waves that hold them
+// ran alone for most of the frame (0.5 waves per CU on average). So when at
+// most kCoopMax lanes hold a leaf, the whole wave tests each of those leaves
+// in turn, 64 faces at a time, one face per lane: the reference keeps the
+// first face of smallest t below the depth the leaf was entered with, so a
+// round keeps the smallest t below the current depth, ties to the smallest
+// face index, and the next round compares with the depth that leaves (the
+// same depth after every leaf as the reference's, and so the same later
+// pops). When more lanes hold a leaf, each lane tests its own, its faces
+// fetched kLeafBatch at a time with all their loads issued before the first
+// test (a face costs two dependent loads: the face, then its vertices).
 //
 // The stack keeps the reference's 96 entries and its overflow rule; the first
 // kLdsStack entries of each lane live in LDS (entry e of lane x at
@@ -109,6 +208,7 @@ constexpr int kLdsStack = 32;  // entries per lane in LDS (8 KB per wave); the r
 #endif
 constexpr int kLeafBatch = RK_REF_LEAF_BATCH;
 
+template <int kCoopMax>
 __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ nodes,
                                                   const uint4* __restrict__ faces,
                                                   const float4* __restrict__ verts,
@@ -116,67 +216,74 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
                                                   const float4* __restrict__ dirs, uint32_t n_rays,
                                                   uint32_t node_count, uint32_t face_count,
                                                   uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                  float* __restrict__ depths) {
+                                                  float* __restrict__ depths, uint32_t* __restrict__ handoff,
+                                                  uint32_t handoff_cap, uint32_t handoff_faces) {
   __shared__ uint32_t s_stack[kLdsStack * 64];
-  const uint32_t l = blockIdx.x * 64 + threadIdx.x;
-  if (l >= n_rays) return;
-  const float4 orig = origins[l], dir = dirs[l];
-  const float4 invdir = invdir_of(dir);
-  float depth = depths[l];
-  uint32_t fid = faces_id[l];
-  float2 coord = coords[l];
+  const int lane = (int)threadIdx.x;
+  const uint32_t l = blockIdx.x * 64 + (uint32_t)lane;
+  const bool has_ray = l < n_rays;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 orig = has_ray ? origins[l] : zero4, dir = has_ray ? dirs[l] : zero4;
+  const float4 invdir = has_ray ? invdir_of(dir) : zero4;
+  float depth = has_ray ? depths[l] : 0.0f;
+  uint32_t fid = has_ray ? faces_id[l] : 0u;
+  float2 coord = has_ray ? coords[l] : make_float2(0.0f, 0.0f);
   uint32_t spill[kStack - kLdsStack];
   typedef __attribute__((address_space(3))) uint32_t lds_u32;
-  lds_u32* my_lds = (lds_u32*)(s_stack + threadIdx.x);
+  lds_u32* my_lds = (lds_u32*)(s_stack + lane);
   int sp = 0;
-  my_lds[0] = 0u;
-  sp = 1;
+  if (has_ray) {
+    my_lds[0] = 0u;
+    sp = 1;
+  }
+  bool active = has_ray;
+  bool may_hand_off = handoff_cap != 0u, handed_off = false;
+  uint32_t tested = 0;  // faces of the leaves reached, while may_hand_off
   uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
-  while (sp > 0 && ++visits <= max_visits) {
-    --sp;
-    const uint32_t nid = sp < kLdsStack ? my_lds[sp * 64] : spill[sp - kLdsStack];
-    if (nid >= node_count) break;  // never taken for a tree built by this library
-    const NodeRegs nd = load_node(nodes, nid);
-    if (rk::node_is_leaf(nd.w0)) {
-      const uint32_t tri_id = nd.w1;
-      const uint32_t cnt = tri_id < face_count ? min(rk::leaf_count(nd.w0), face_count - tri_id) : 0u;
-      for (uint32_t k0 = 0; k0 < cnt; k0 += kLeafBatch) {
-        uint4 f[kLeafBatch];
-#pragma unroll
-        for (int b = 0; b < kLeafBatch; ++b) f[b] = k0 + b < cnt ? faces[tri_id + k0 + b] : make_uint4(0, 0, 0, 0);
-        float4 p0[kLeafBatch], p1[kLeafBatch], p2[kLeafBatch];
-#pragma unroll
-        for (int b = 0; b < kLeafBatch; ++b) {
-          if (k0 + b < cnt) {
-            p0[b] = verts[f[b].x];
-            p1[b] = verts[f[b].y];
-            p2[b] = verts[f[b].z];
-          }
-        }
-#pragma unroll
-        for (int b = 0; b < kLeafBatch; ++b) {
-          if (k0 + b < cnt) {
-            // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
-            float4 v0 = p0[b];
-            float4 e1 = rk::sub4(p1[b], v0), e2 = rk::sub4(p2[b], v0);
-            v0.w = 0.0f;
-            e1.w = 0.0f;
-            e2.w = 0.0f;
-            float t, u, v;
-            if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
-              if (t < depth) { fid = tri_id + k0 + b; coord = make_float2(u, v); depth = t; }
-            }
-          }
-        }
+  for (;;) {
+    // each lane walks on to its next leaf, or to its end
+    bool at_leaf = false;
+    uint32_t leaf_first = 0, leaf_cnt = 0;
+    while (active && !at_leaf) {
+      if (sp == 0 || ++visits > max_visits) {
+        active = false;
+        break;
       }
-    } else {
-      float t;
-      if (box_ray(nd.bmin, nd.bmax, orig, invdir, t)) {
-        if (t < depth) {
+      --sp;
+      const uint32_t nid = sp < kLdsStack ? my_lds[sp * 64] : spill[sp - kLdsStack];
+      if (nid >= node_count) {  // never taken for a tree built by this library
+        active = false;
+        break;
+      }
+      const NodeRegs nd = load_node(nodes, nid);
+      if (rk::node_is_leaf(nd.w0)) {
+        leaf_first = nd.w1;
+        leaf_cnt = leaf_first < face_count ? min(rk::leaf_count(nd.w0), face_count - leaf_first) : 0u;
+        if (may_hand_off) {
+          if (leaf_cnt > handoff_faces - tested) {
+            may_hand_off = false;
+            const uint32_t slot = atomicAdd(handoff, 1u);
+            if (slot < handoff_cap) {
+              handoff[kHandoffHead + slot] = l;
+              handed_off = true;
+              active = false;
+              break;
+            }
+          } else {
+            tested += leaf_cnt;
+          }
+        }
+        at_leaf = true;
+      } else {
+        float t;
+        if (box_ray(nd.bmin, nd.bmax, orig, invdir, t) && t < depth) {
           uint32_t nearc, farc;
           children_order(nd.w0, nd.w1, dir, nearc, farc);
-          if (sp + 2 > kStack) break;  // depth > 94: cannot occur for trees of depth <= 94
+          if (sp + 2 > kStack) {  // depth > 94: cannot occur for trees of depth <= 94
+            active = false;
+            break;
+          }
           if (sp < kLdsStack) my_lds[sp * 64] = farc;
           else spill[sp - kLdsStack] = farc;
           ++sp;
@@ -186,11 +293,142 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
         }
       }
     }
+    uint64_t pending = __ballot(at_leaf);
+    if (pending == 0ull) break;  // every lane is done
+    if (__popcll(pending) > kCoopMax) {
+      // each lane its own leaf, in face order
+      if (at_leaf) {
+        for (uint32_t k0 = 0; k0 < leaf_cnt; k0 += kLeafBatch) {
+          uint4 f[kLeafBatch];
+#pragma unroll
+          for (int b = 0; b < kLeafBatch; ++b)
+            f[b] = k0 + b < leaf_cnt ? faces[leaf_first + k0 + b] : make_uint4(0, 0, 0, 0);
+          float4 p0[kLeafBatch], p1[kLeafBatch], p2[kLeafBatch];
+#pragma unroll
+          for (int b = 0; b < kLeafBatch; ++b) {
+            if (k0 + b < leaf_cnt) {
+              p0[b] = verts[f[b].x];
+              p1[b] = verts[f[b].y];
+              p2[b] = verts[f[b].z];
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < kLeafBatch; ++b) {
+            if (k0 + b < leaf_cnt) {
+              // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
+              float4 v0 = p0[b];
+              float4 e1 = rk::sub4(p1[b], v0), e2 = rk::sub4(p2[b], v0);
+              v0.w = 0.0f;
+              e1.w = 0.0f;
+              e2.w = 0.0f;
+              float t, u, v;
+              if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
+                if (t < depth) { fid = leaf_first + k0 + b; coord = make_float2(u, v); depth = t; }
+              }
+            }
+          }
+        }
+      }
+    } else {
+      // the whole wave on each pending leaf in turn
+      while (pending != 0ull) {
+        const int L = __builtin_ctzll(pending);
+        pending &= pending - 1;
+        const uint32_t first = readlane_u32(leaf_first, L);
+        const LeafBest b = wave_leaf<1>(faces, verts, readlane_f4(orig, L), readlane_f4(dir, L), first,
+                                     readlane_u32(leaf_cnt, L), readlane_f32(depth, L));
+        if (lane == L && b.found) {
+          depth = b.t;
+          fid = first + b.k;
+          coord = make_float2(b.u, b.v);
+        }
+      }
+    }
   }
-  faces_id[l] = fid;
-  coords[l] = coord;
-  depths[l] = depth;
+  if (has_ray && !handed_off) {
+    faces_id[l] = fid;
+    coords[l] = coord;
+    depths[l] = depth;
+  }
 }
+
+__global__ __launch_bounds__(64) void k_handoff_clear(uint32_t* __restrict__ handoff) {
+  if (threadIdx.x == 0) handoff[0] = 0u;
+}
+
+// The second pass of the per-ray traversal: one wave per ray handed off, the
+// ray's walk from the root again with the same rules (the same pops, box
+// tests, stack overflow and visit limits), each leaf tested by the whole wave
+// (wave_leaf). The stack's entry e is lane e % 64 of st0 (e < 64) or st1.
+__global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restrict__ nodes,
+                                                       const uint4* __restrict__ faces,
+                                                       const float4* __restrict__ verts,
+                                                       const float4* __restrict__ origins,
+                                                       const float4* __restrict__ dirs, uint32_t node_count,
+                                                       uint32_t face_count, const uint32_t* __restrict__ handoff,
+                                                       uint32_t handoff_cap, uint32_t* __restrict__ faces_id,
+                                                       float2* __restrict__ coords, float* __restrict__ depths) {
+  const int lane = (int)threadIdx.x;
+  const uint32_t count = min(handoff[0], handoff_cap);
+  const uint32_t max_visits = 2 * node_count + 2;
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t l = handoff[kHandoffHead + i];
+    const float4 orig = origins[l], dir = dirs[l];
+    const float4 invdir = invdir_of(dir);
+    float depth = depths[l];
+    uint32_t fid = faces_id[l];
+    float2 coord = coords[l];
+    uint32_t st0 = 0u, st1 = 0u;  // entry 0: the root, node 0
+    int sp = 1;
+    uint32_t visits = 0;
+    while (sp > 0 && ++visits <= max_visits) {
+      --sp;
+      const uint32_t nid = readlane_u32(sp < 64 ? st0 : st1, sp & 63);
+      if (nid >= node_count) break;  // never taken for a tree built by this library
+      const NodeRegs nd = load_node(nodes, nid);
+      if (rk::node_is_leaf(nd.w0)) {
+        const uint32_t first = nd.w1;
+        const uint32_t cnt = first < face_count ? min(rk::leaf_count(nd.w0), face_count - first) : 0u;
+        const LeafBest b = wave_leaf<kWavePer>(faces, verts, orig, dir, first, cnt, depth);
+        if (b.found) {
+          depth = b.t;
+          fid = first + b.k;
+          coord = make_float2(b.u, b.v);
+        }
+      } else {
+        float t;
+        if (box_ray(nd.bmin, nd.bmax, orig, invdir, t) && t < depth) {
+          uint32_t nearc, farc;
+          children_order(nd.w0, nd.w1, dir, nearc, farc);
+          if (sp + 2 > kStack) break;  // depth > 94: cannot occur for trees of depth <= 94
+          if (lane == (sp & 63)) {
+            if (sp < 64) st0 = farc;
+            else st1 = farc;
+          }
+          ++sp;
+          if (lane == (sp & 63)) {
+            if (sp < 64) st0 = nearc;
+            else st1 = nearc;
+          }
+          ++sp;
+        }
+      }
+    }
+    if (lane == 0) {
+      faces_id[l] = fid;
+      coords[l] = coord;
+      depths[l] = depth;
+    }
+  }
+}
+
+#ifndef RK_REF_HANDOFF_FACES
+#define RK_REF_HANDOFF_FACES 8192
+#endif
+
+#ifndef RK_REF_COOP_MAX
+#define RK_REF_COOP_MAX 16
+#endif
 
 constexpr int kPW = 16, kPH = 12, kPT = kPW * kPH;  // PACKET_BLOCK_*, :47-49
 constexpr int kPStack = 128;                       // PACKET_STACK_SIZE, :51
@@ -301,11 +539,31 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
                        height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths);
   } else {
-    hipLaunchKernelGGL(k_trace_ray, dim3((n + 63) / 64), dim3(64), 0, st,
+    // A ray whose leaves pass RK_REF_HANDOFF_FACES faces (on the heightfield,
+    // a ray parallel to the ground plane, whose box tests all pass) is handed
+    // to the second pass, one wave per ray, which the list lives for: a
+    // stream-ordered allocation, so calls on other streams have their own.
+    // Without it every ray stays in the first pass (the same results).
+    const uint32_t cap = std::min<uint32_t>(n, kHandoffCap);
+    uint32_t* ho = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&ho), (kHandoffHead + cap) * sizeof(uint32_t), st) != hipSuccess) {
+      ho = nullptr;
+      (void)hipGetLastError();
+    }
+    if (ho) hipLaunchKernelGGL(k_handoff_clear, dim3(1), dim3(64), 0, st, ho);
+    hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st,
                        reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                        reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                        reinterpret_cast<const float4*>(directions), n, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords),
-                       depths);
+                       depths, ho, ho ? cap : 0u, (uint32_t)RK_REF_HANDOFF_FACES);
+    if (ho) {
+      hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kWaveGrid)), dim3(64), 0, st,
+                         reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
+                         reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
+                         reinterpret_cast<const float4*>(directions), node_count, face_count, ho, cap, faces_id,
+                         reinterpret_cast<float2*>(coords), depths);
+      if (hipFreeAsync(ho, st) != hipSuccess) return RK_ERROR_DEVICE;
+    }
   }
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;

@@ -231,6 +231,66 @@ def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency):
     np.testing.assert_array_equal(_h(out[2]), od)
 
 
+@pytest.mark.parametrize("name,w,h,depth", [("hf128", 200, 150, 12), ("ico5", 128, 96, 12), ("hf128", 200, 150, 31)])
+def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth):
+    """Rays with a direction component of exactly 0: the reference's normalised
+    inverse direction is then (0, 0, NaN) up to order, every box test passes
+    at t = 0 and the per-ray traversal tests every face of the mesh. Past 8192
+    faces the library hands such a ray to its one-wave-per-ray second pass;
+    its results stay bit-identical to the oracle's, ties included."""
+    V, F = _scene(rk, name)
+    assert F.shape[0] > 8192
+    tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
+    O, D = _camera_rays(rk, name, w, h)
+    D = D.copy()
+    n = w * h
+    D[::5, 2] = 0.0
+    D[1::11, 0] = 0.0
+    rng = np.random.default_rng(5)
+    fid0 = rng.integers(0, 1000, n, dtype=np.uint32)
+    uv0 = rng.random((n, 2), dtype=np.float32)
+    dep0 = np.full(n, F32MAX, np.float32)
+    dep0[::7] = 2.5
+    dep0[3::13] = 0.0
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, False, out=out)
+    of, ouv, od, ovf, cnt = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0, counters=True)
+    assert ovf == 0
+    assert (cnt[:, 1] > 8192).sum() > n // 10  # the second pass has work
+    assert (od < dep0).sum() > n // 20
+    np.testing.assert_array_equal(_h(out[0], np.uint32), of)
+    np.testing.assert_array_equal(_h(out[1]), ouv)
+    np.testing.assert_array_equal(_h(out[2]), od)
+
+
+def test_reference_trace_handoff_list_full(dev, rk, ora):
+    """More rays crossing every box than the hand-off list holds (65536 per
+    call): the rays past its end finish in the first pass, with the same
+    results."""
+    V, F = rk.heightfield(66)
+    assert F.shape[0] > 8192
+    tree, ref = _compare_tree(dev, rk, ora, V, F, 12, True)
+    w, h = 480, 180
+    O, D = _camera_rays(rk, "hf66", w, h)
+    O, D = O.copy(), D.copy()
+    D[:, 2] = 0.0
+    D[::9, 2] = -0.25
+    O[1::2, 2] = np.linspace(0.04, 0.16, O[1::2].shape[0], dtype=np.float32)  # through the terrain (z in [0, 0.2])
+    n = w * h
+    fid0 = np.zeros(n, np.uint32)
+    uv0 = np.zeros((n, 2), np.float32)
+    dep0 = np.full(n, F32MAX, np.float32)
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, False, out=out)
+    of, ouv, od, ovf, cnt = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0, counters=True)
+    assert ovf == 0
+    assert (cnt[:, 1] > 8192).sum() > 65536
+    assert (od < F32MAX).sum() > n // 20
+    np.testing.assert_array_equal(_h(out[0], np.uint32), of)
+    np.testing.assert_array_equal(_h(out[1]), ouv)
+    np.testing.assert_array_equal(_h(out[2]), od)
+
+
 def _check_fast_boxes(tree, F, V, leaf_max):
     """Every child slot of every reachable node holds the padded union of the
     boxes of the faces below it, the right face count, and covers them."""
