@@ -3,12 +3,21 @@
 //   k_trace_ray     per-ray DFS (kernel_raytraceMeshTree, :111-219), one lane
 //                   per ray for the nodes; a leaf's faces tested by its lane,
 //                   or by the whole wave when few lanes hold a leaf; stack in
-//                   LDS and scratch;
+//                   LDS and scratch; a ray past a face budget is handed off to
+//   k_trace_ray_wave  the same walk, one wave per ray, leaves tested by the
+//                   whole wave;
 //   k_trace_packet  the 16x12 packet traversal (kernel_raytraceMeshTreePacket,
 //                   :222-377), one 192-thread workgroup (3 waves) per tile,
-//                   block-uniform stack in LDS, one barrier vote per node.
+//                   block-uniform stack in LDS, one barrier vote per node; a
+//                   packet past a face budget is handed off to
+//   k_trace_packet_split  the same walk with five threads per ray, each
+//                   testing a fifth of every chunk of faces.
+// The results of both passes are bit-identical to the reference's.
 // Built with -ffp-contract=off, so the triangle and box tests are the same
 // float operations in the same order as the reference source.
+#include <algorithm>
+#include <atomic>
+
 #include "rk_device.h"
 
 namespace {
@@ -434,19 +443,31 @@ constexpr int kPW = 16, kPH = 12, kPT = kPW * kPH;  // PACKET_BLOCK_*, :47-49
 constexpr int kPStack = 128;                       // PACKET_STACK_SIZE, :51
 constexpr int kPCache = 224;                       // TRIANGLES_MAX, :39
 
-__global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict__ nodes,
-                                                      const uint4* __restrict__ faces,
-                                                      const float4* __restrict__ verts,
-                                                      const float4* __restrict__ origins,
-                                                      const float4* __restrict__ dirs, int width, int height,
-                                                      uint32_t node_count, uint32_t face_count,
-                                                      uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
-                                                      float* __restrict__ depths) {
-  __shared__ uint32_t s_stack[kPStack];
-  __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
-  __shared__ float4 s_dir0;
-  const int tx = threadIdx.x % kPW, ty = threadIdx.x / kPW;
-  const int x = blockIdx.x * kPW + tx, y = blockIdx.y * kPH + ty;
+// One packet of the packet traversal (kernel_raytraceMeshTreePacket,
+// :222-377): the block walks the tree once for its 16x12 rays, visiting a
+// node when any of them passes its box test with t < its depth, in the order
+// the first ray's direction gives; at a leaf every ray tests every face
+// against its own depth, in face order. The block holds kS threads per ray:
+// thread x serves ray x % 192, and with kS > 1 tests part x / 192 of each
+// chunk of faces (consecutive faces, parts in face order). Each part keeps its
+// first face of smallest t below the ray's depth at the chunk's start, and the
+// first part with the smallest t wins: the face a sequential pass keeping hits
+// with t < depth ends on, with the same depth after the chunk.
+//
+// A packet whose leaves pass handoff_faces faces (while handoff_cap != 0) is
+// put on the hand-off list and left, outputs untouched: its rays are traced
+// again from the start by the second pass. Returns whether it was.
+template <int kS>
+__device__ __forceinline__ bool trace_packet_job(
+    const rk::Node* __restrict__ nodes, const uint4* __restrict__ faces, const float4* __restrict__ verts,
+    const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, uint32_t node_count,
+    uint32_t face_count, uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
+    int bx, int by, uint32_t* __restrict__ handoff, uint32_t handoff_cap, uint32_t handoff_faces,
+    uint32_t* s_stack, float4* s_v0, float4* s_e1, float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt,
+    float* s_pu, float* s_pv, uint32_t* s_pk) {
+  const int ray = (int)threadIdx.x % kPT, part = (int)threadIdx.x / kPT;
+  const int tx = ray % kPW, ty = ray / kPW;
+  const int x = bx * kPW + tx, y = by * kPH + ty;
   const bool in_image = x < width && y < height;
   const uint32_t l = in_image ? (uint32_t)(y * width + x) : 0u;
   float4 orig = make_float4(0.f, 0.f, 0.f, 0.f), dir = orig, invdir = orig;
@@ -462,12 +483,14 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
     coord = coords[l];
   }
   if (threadIdx.x == 0) {  // thread (0,0) of a block is always inside the image
-    s_dir0 = dir;
+    *s_dir0 = dir;
     s_stack[0] = 0;
   }
   int sp = 1;
   __syncthreads();
-  const float4 dir0 = s_dir0;
+  const float4 dir0 = *s_dir0;
+  bool may_hand_off = handoff_cap != 0u;
+  uint32_t tested = 0;  // block-uniform: faces of the leaves visited, while may_hand_off
   uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
   while (sp > 0 && ++visits <= max_visits) {
@@ -477,20 +500,68 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
     if (rk::node_is_leaf(nd.w0)) {
       const uint32_t tri_id = nd.w1;
       const uint32_t cnt = tri_id < face_count ? min(rk::leaf_count(nd.w0), face_count - tri_id) : 0u;
+      if (may_hand_off) {
+        if (cnt > handoff_faces - tested) {
+          may_hand_off = false;
+          if (threadIdx.x == 0) {
+            const uint32_t slot = atomicAdd(handoff, 1u);
+            if (slot < handoff_cap) handoff[kHandoffHead + slot] = (uint32_t)by * gridDim.x + (uint32_t)bx;
+            *s_slot = slot;
+          }
+          __syncthreads();
+          const uint32_t slot = *s_slot;
+          __syncthreads();
+          if (slot < handoff_cap) return true;
+        } else {
+          tested += cnt;
+        }
+      }
       for (uint32_t c0 = 0; c0 < cnt; c0 += kPCache) {
         const uint32_t cc = (cnt - c0) < (uint32_t)kPCache ? (cnt - c0) : (uint32_t)kPCache;
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < cc; k += kPT) {
+        for (uint32_t k = threadIdx.x; k < cc; k += kPT * kS) {
           float4 v0, e1, e2;
           tri_setup(faces, verts, tri_id + c0 + k, v0, e1, e2);
           s_v0[k] = v0; s_e1[k] = e1; s_e2[k] = e2;
         }
         __syncthreads();
-        if (in_image) {
-          for (uint32_t k = 0; k < cc; ++k) {
-            float t, u, v;
-            if (tri_ray(orig, dir, s_v0[k], s_e1[k], s_e2[k], t, u, v)) {
-              if (t < depth) { fid = tri_id + c0 + k; coord = make_float2(u, v); depth = t; }
+        if (kS == 1) {
+          if (in_image) {
+            for (uint32_t k = 0; k < cc; ++k) {
+              float t, u, v;
+              if (tri_ray(orig, dir, s_v0[k], s_e1[k], s_e2[k], t, u, v)) {
+                if (t < depth) { fid = tri_id + c0 + k; coord = make_float2(u, v); depth = t; }
+              }
+            }
+          }
+        } else {
+          const uint32_t q = (cc + kS - 1) / kS;
+          const uint32_t k0 = (uint32_t)part * q, k1 = min(cc, k0 + q);
+          float pt = depth, pu = 0.0f, pv = 0.0f;
+          uint32_t pk = 0xffffffffu;  // none
+          if (in_image) {
+            for (uint32_t k = k0; k < k1; ++k) {
+              float t, u, v;
+              if (tri_ray(orig, dir, s_v0[k], s_e1[k], s_e2[k], t, u, v)) {
+                if (t < pt) { pk = k; pu = u; pv = v; pt = t; }
+              }
+            }
+          }
+          s_pt[threadIdx.x] = pt;
+          s_pu[threadIdx.x] = pu;
+          s_pv[threadIdx.x] = pv;
+          s_pk[threadIdx.x] = pk;
+          __syncthreads();
+          if (in_image) {
+#pragma unroll
+            for (int p = 0; p < kS; ++p) {
+              const int o = p * kPT + ray;
+              const uint32_t k = s_pk[o];
+              if (k != 0xffffffffu && s_pt[o] < depth) {
+                depth = s_pt[o];
+                fid = tri_id + c0 + k;
+                coord = make_float2(s_pu[o], s_pv[o]);
+              }
             }
           }
         }
@@ -513,14 +584,86 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
       __syncthreads();
     }
   }
-  if (in_image) {
+  if (in_image && part == 0) {
     faces_id[l] = fid;
     coords[l] = coord;
     depths[l] = depth;
   }
+  return false;
+}
+
+// The packet traversal's first pass: one 192-thread block per packet.
+__global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict__ nodes,
+                                                      const uint4* __restrict__ faces,
+                                                      const float4* __restrict__ verts,
+                                                      const float4* __restrict__ origins,
+                                                      const float4* __restrict__ dirs, int width, int height,
+                                                      uint32_t node_count, uint32_t face_count,
+                                                      uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+                                                      float* __restrict__ depths, uint32_t* __restrict__ handoff,
+                                                      uint32_t handoff_cap, uint32_t handoff_faces) {
+  __shared__ uint32_t s_stack[kPStack];
+  __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
+  __shared__ float4 s_dir0;
+  __shared__ uint32_t s_slot;
+  trace_packet_job<1>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count, faces_id, coords,
+                      depths, (int)blockIdx.x, (int)blockIdx.y, handoff, handoff_cap, handoff_faces, s_stack, s_v0,
+                      s_e1, s_e2, &s_dir0, &s_slot, nullptr, nullptr, nullptr, nullptr);
+}
+
+// The second pass: the packets handed off, kPacketSplit threads per ray.
+#ifndef RK_REF_PACKET_SPLIT
+#define RK_REF_PACKET_SPLIT 5
+#endif
+constexpr int kPacketSplit = RK_REF_PACKET_SPLIT;
+
+__global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
+    const rk::Node* __restrict__ nodes, const uint4* __restrict__ faces, const float4* __restrict__ verts,
+    const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, int packets_x,
+    uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
+    uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths) {
+  __shared__ uint32_t s_stack[kPStack];
+  __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
+  __shared__ float4 s_dir0;
+  __shared__ uint32_t s_slot;
+  __shared__ float s_pt[kPT * kPacketSplit], s_pu[kPT * kPacketSplit], s_pv[kPT * kPacketSplit];
+  __shared__ uint32_t s_pk[kPT * kPacketSplit];
+  const uint32_t count = min(handoff[0], handoff_cap);
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t pk = handoff[kHandoffHead + i];
+    trace_packet_job<kPacketSplit>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count,
+                                   faces_id, coords, depths, (int)(pk % (uint32_t)packets_x),
+                                   (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, s_stack, s_v0, s_e1, s_e2,
+                                   &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
+    __syncthreads();  // the next packet rewrites the stack and s_dir0
+  }
 }
 
 }  // namespace
+
+#ifndef RK_REF_PACKET_HANDOFF_FACES
+#define RK_REF_PACKET_HANDOFF_FACES 16384
+#endif
+constexpr uint32_t kPacketGrid = 1024;  // blocks of the packet traversal's second pass
+
+static std::atomic<uint32_t> g_ray_faces{RK_REF_HANDOFF_FACES};
+static std::atomic<uint32_t> g_packet_faces{RK_REF_PACKET_HANDOFF_FACES};
+static std::atomic<uint32_t> g_list_cap{kHandoffCap};
+constexpr uint32_t kNoHandoff = 0xffffffffu;
+
+// The hand-off list of one call, cleared: a stream-ordered allocation, so that
+// calls on other streams have their own. Without it (nullptr) every ray or
+// packet stays in the first pass, with the same results.
+static uint32_t* handoff_alloc(hipStream_t st, uint32_t cap, uint32_t faces) {
+  uint32_t* ho = nullptr;
+  if (faces == kNoHandoff || cap == 0) return nullptr;
+  if (hipMallocAsync(reinterpret_cast<void**>(&ho), (kHandoffHead + cap) * sizeof(uint32_t), st) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  hipLaunchKernelGGL(k_handoff_clear, dim3(1), dim3(64), 0, st, ho);
+  return ho;
+}
 
 extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32_t node_count,
                                            const rk_uint4* faces, uint32_t face_count, const rk_float4* vertices,
@@ -533,29 +676,39 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
   hipStream_t st = rk_hip_stream(stream);
   const uint32_t n = (uint32_t)width * (uint32_t)height;
   if (coherency) {
-    dim3 grid((width + kPW - 1) / kPW, (height + kPH - 1) / kPH);
-    hipLaunchKernelGGL(k_trace_packet, grid, dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
+    // A packet whose leaves pass g_packet_faces faces (on the
+    // heightfield, one holding a ray parallel to the ground plane: the packet
+    // then visits every leaf, and all its rays test every face) is traced by
+    // the second pass, kPacketSplit threads per ray.
+    const uint32_t gx = (uint32_t)(width + kPW - 1) / kPW, gy = (uint32_t)(height + kPH - 1) / kPH;
+    const uint32_t faces_limit = g_packet_faces.load(std::memory_order_relaxed);
+    const uint32_t cap = std::min<uint32_t>(gx * gy, g_list_cap.load(std::memory_order_relaxed));
+    uint32_t* ho = handoff_alloc(st, cap, faces_limit);
+    hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
-                       height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths);
-  } else {
-    // A ray whose leaves pass RK_REF_HANDOFF_FACES faces (on the heightfield,
-    // a ray parallel to the ground plane, whose box tests all pass) is handed
-    // to the second pass, one wave per ray, which the list lives for: a
-    // stream-ordered allocation, so calls on other streams have their own.
-    // Without it every ray stays in the first pass (the same results).
-    const uint32_t cap = std::min<uint32_t>(n, kHandoffCap);
-    uint32_t* ho = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&ho), (kHandoffHead + cap) * sizeof(uint32_t), st) != hipSuccess) {
-      ho = nullptr;
-      (void)hipGetLastError();
+                       height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths, ho,
+                       ho ? cap : 0u, faces_limit);
+    if (ho) {
+      hipLaunchKernelGGL(k_trace_packet_split, dim3(std::min<uint32_t>(cap, kPacketGrid)), dim3(kPT * kPacketSplit),
+                         0, st, reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
+                         reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
+                         reinterpret_cast<const float4*>(directions), width, height, (int)gx, node_count, face_count,
+                         ho, cap, faces_id, reinterpret_cast<float2*>(coords), depths);
+      if (hipFreeAsync(ho, st) != hipSuccess) return RK_ERROR_DEVICE;
     }
-    if (ho) hipLaunchKernelGGL(k_handoff_clear, dim3(1), dim3(64), 0, st, ho);
+  } else {
+    // A ray whose leaves pass g_ray_faces faces (on the heightfield,
+    // a ray parallel to the ground plane, whose box tests all pass) is handed
+    // to the second pass, one wave per ray.
+    const uint32_t faces_limit = g_ray_faces.load(std::memory_order_relaxed);
+    const uint32_t cap = std::min<uint32_t>(n, g_list_cap.load(std::memory_order_relaxed));
+    uint32_t* ho = handoff_alloc(st, cap, faces_limit);
     hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st,
                        reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                        reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                        reinterpret_cast<const float4*>(directions), n, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords),
-                       depths, ho, ho ? cap : 0u, (uint32_t)RK_REF_HANDOFF_FACES);
+                       depths, ho, ho ? cap : 0u, faces_limit);
     if (ho) {
       hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kWaveGrid)), dim3(64), 0, st,
                          reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
@@ -566,5 +719,21 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     }
   }
   RK_CHECK_LAUNCH();
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_raytrace_set_handoff(uint32_t ray_faces, uint32_t packet_faces, uint32_t list_cap) {
+  if (list_cap == 0 || list_cap > kHandoffCap) return RK_ERROR_INVALID_VALUE;
+  g_ray_faces.store(ray_faces, std::memory_order_relaxed);
+  g_packet_faces.store(packet_faces, std::memory_order_relaxed);
+  g_list_cap.store(list_cap, std::memory_order_relaxed);
+  return RK_SUCCESS;
+}
+
+extern "C" rk_status rk_raytrace_get_handoff(uint32_t* ray_faces, uint32_t* packet_faces, uint32_t* list_cap) {
+  if (!ray_faces || !packet_faces || !list_cap) return RK_ERROR_INVALID_VALUE;
+  *ray_faces = g_ray_faces.load(std::memory_order_relaxed);
+  *packet_faces = g_packet_faces.load(std::memory_order_relaxed);
+  *list_cap = g_list_cap.load(std::memory_order_relaxed);
   return RK_SUCCESS;
 }

@@ -205,6 +205,17 @@ def trace_ref(tree: RefTree, V: torch.Tensor, O: torch.Tensor, D: torch.Tensor, 
         return fid, uv, dep
 
 
+def set_ref_handoff(ray_faces: int = 8192, packet_faces: int = 16384, list_cap: int = 65536) -> None:
+    """Hand-off limits of trace_ref's second passes (process-wide; 0xFFFFFFFF turns one off)."""
+    _check(N.lib().rk_raytrace_set_handoff(ray_faces, packet_faces, list_cap), "raytrace_set_handoff")
+
+
+def get_ref_handoff() -> tuple[int, int, int]:
+    a, b, c = N._U32(), N._U32(), N._U32()
+    _check(N.lib().rk_raytrace_get_handoff(C.byref(a), C.byref(b), C.byref(c)), "raytrace_get_handoff")
+    return a.value, b.value, c.value
+
+
 @dataclass
 class FastTree:
     nodes: torch.Tensor

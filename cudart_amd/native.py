@@ -168,6 +168,8 @@ _SIGS = {
     "rk_build_sah_workspace_size": (_SZ, [_U32, _U32]),
     "rk_build_sah": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _P, _P, _P, _U32, _U32, _P, _P, _SZ]),
     "rk_raytrace_mesh_tree": (_I, [_P, _P, _U32, _P, _U32, _P, _P, _P, _I, _I, _P, _P, _P, _I]),
+    "rk_raytrace_set_handoff": (_I, [_U32, _U32, _U32]),
+    "rk_raytrace_get_handoff": (_I, [C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "rk_fast_build_workspace_size": (_SZ, [_U32, _I]),
     "rk_fast_build": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P, _P, _SZ]),
     "rk_fast_build_noalloc": (_I, [_P, _P, _P, _U32, C.POINTER(AABox), _I, _U32, C.POINTER(FastBVH), _P]),

@@ -44,6 +44,8 @@ def _scene(rk, name):
         V, F = rk.icosphere(5)
     elif name == "hf128":
         V, F = rk.heightfield(128)
+    elif name == "hf160":
+        V, F = rk.heightfield(160)
     elif name == "hf708":
         V, F = rk.heightfield(708)
     elif name == "soup50k":
@@ -172,12 +174,38 @@ def _camera_rays(rk, name, w, h):
     return rk.primary_rays_host(cam, w, h)
 
 
+@pytest.fixture(params=["default", "all", "all_cap7"])
+def handoff(request, dev):
+    """The reference traversals' second passes: at their default limits; with
+    every ray or packet that reaches a non-empty leaf handed off; and the same
+    with a hand-off list of 7 entries, so that the rest finish in the first pass."""
+    if request.param == "all":
+        dev.set_ref_handoff(0, 0, 65536)
+    elif request.param == "all_cap7":
+        dev.set_ref_handoff(0, 0, 7)
+    yield request.param
+    dev.set_ref_handoff()
+
+
+def test_reference_handoff_limits(dev):
+    assert dev.get_ref_handoff() == (8192, 16384, 65536)
+    dev.set_ref_handoff(1, 2, 3)
+    assert dev.get_ref_handoff() == (1, 2, 3)
+    with pytest.raises(Exception):
+        dev.set_ref_handoff(1, 2, 0)
+    with pytest.raises(Exception):
+        dev.set_ref_handoff(1, 2, 65537)
+    dev.set_ref_handoff()
+    assert dev.get_ref_handoff() == (8192, 16384, 65536)
+
+
 @pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29),
                                       ("hf708", 1920, 1080)])
 @pytest.mark.parametrize("coherency", [False, True])
-def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
+def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency, handoff):
     """Both reference traversals (per-ray and packet) against the oracle's
-    restatement of each, every ray, up to the full 1920 x 1080 heightfield frame."""
+    restatement of each, every ray, up to the full 1920 x 1080 heightfield frame,
+    with and without their second passes."""
     V, F = _scene(rk, name)
     tree, ref = _compare_tree(dev, rk, ora, V, F, 12, True)
     O, D = _camera_rays(rk, name, w, h)
@@ -202,11 +230,12 @@ def test_reference_trace_bit_exact(dev, rk, ora, name, w, h, coherency):
 
 @pytest.mark.parametrize("depth", [4, 12])
 @pytest.mark.parametrize("coherency", [False, True])
-def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency):
+def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency, handoff):
     """Every face three times: identical triangles hit at identical t, so the
     reference's strict t < depth keeps the first copy in leaf order. Depth 4
     gives leaves of about 960 faces (many rounds of 64 in the per-ray kernel,
-    copies on both sides of a round's boundary), depth 12 small ones."""
+    copies on both sides of a round's boundary and of the parts of a chunk in
+    the packet traversal's second pass), depth 12 small ones."""
     V, F0 = rk.icosphere(4)
     F = np.ascontiguousarray(np.repeat(F0, 3, axis=0))
     tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
@@ -231,15 +260,18 @@ def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency):
     np.testing.assert_array_equal(_h(out[2]), od)
 
 
-@pytest.mark.parametrize("name,w,h,depth", [("hf128", 200, 150, 12), ("ico5", 128, 96, 12), ("hf128", 200, 150, 31)])
-def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth):
+@pytest.mark.parametrize("name,w,h,depth,coherency", [("hf128", 200, 150, 12, False), ("ico5", 128, 96, 12, False),
+                                                      ("hf128", 200, 150, 31, False), ("hf160", 64, 48, 12, True),
+                                                      ("hf160", 64, 48, 31, True)])
+def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth, coherency):
     """Rays with a direction component of exactly 0: the reference's normalised
     inverse direction is then (0, 0, NaN) up to order, every box test passes
-    at t = 0 and the per-ray traversal tests every face of the mesh. Past 8192
-    faces the library hands such a ray to its one-wave-per-ray second pass;
-    its results stay bit-identical to the oracle's, ties included."""
+    at t = 0 and the per-ray traversal tests every face of the mesh (the packet
+    traversal: every ray of a packet holding one). Past 8192 faces for a ray,
+    16384 for a packet, the library hands it to its second pass; the results
+    stay bit-identical to the oracle's."""
     V, F = _scene(rk, name)
-    assert F.shape[0] > 8192
+    assert F.shape[0] > (32768 if coherency else 8192)
     tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
     O, D = _camera_rays(rk, name, w, h)
     D = D.copy()
@@ -253,10 +285,14 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
     dep0[::7] = 2.5
     dep0[3::13] = 0.0
     out = (_t(fid0), _t(uv0), _t(dep0))
-    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, False, out=out)
-    of, ouv, od, ovf, cnt = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0, counters=True)
-    assert ovf == 0
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, coherency, out=out)
+    _, _, _, _, cnt = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0, counters=True)
     assert (cnt[:, 1] > 8192).sum() > n // 10  # the second pass has work
+    if coherency:
+        of, ouv, od, ovf = ora.trace_ref_packet(ref.nodes, ref.faces, V, O, D, w, h, fid0, uv0, dep0)
+    else:
+        of, ouv, od, ovf = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0)
+    assert ovf == 0
     assert (od < dep0).sum() > n // 20
     np.testing.assert_array_equal(_h(out[0], np.uint32), of)
     np.testing.assert_array_equal(_h(out[1]), ouv)
