@@ -60,6 +60,14 @@ __device__ __forceinline__ float4 invdir_of(float4 dir) {
   return rk::muls4(v, inv_len);
 }
 
+// Whether box_ray passes every box for this invdir while t < depth allows:
+// a direction component of 0 makes 1/dir infinite, the normalisation factor
+// 0 and that component NaN; the other components are then +-0, and
+// box_ray's tmin and tmax are both 0.
+__device__ __forceinline__ bool crosses_every_box(float4 invdir) {
+  return __builtin_isnan(invdir.x) || __builtin_isnan(invdir.y) || __builtin_isnan(invdir.z);
+}
+
 __device__ __forceinline__ void tri_setup(const uint4* __restrict__ faces, const float4* __restrict__ verts, uint32_t f,
                                           float4& v0, float4& e1, float4& e2) {
   uint4 face = faces[f];
@@ -248,6 +256,10 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
   bool active = has_ray;
   bool may_hand_off = handoff_cap != 0u, handed_off = false;
   uint32_t tested = 0;  // faces of the leaves reached, while may_hand_off
+  // A direction with a component of 0 leaves a NaN in invdir: every box test
+  // then passes (at t = 0, while depth > 0) and the ray walks the whole tree,
+  // so it goes to the second pass at its first leaf.
+  const uint32_t limit = crosses_every_box(invdir) ? 0u : handoff_faces;
   uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
   for (;;) {
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
         leaf_first = nd.w1;
         leaf_cnt = leaf_first < face_count ? min(rk::leaf_count(nd.w0), face_count - leaf_first) : 0u;
         if (may_hand_off) {
-          if (leaf_cnt > handoff_faces - tested) {
+          if (leaf_cnt > limit - tested) {
             may_hand_off = false;
             const uint32_t slot = atomicAdd(handoff, 1u);
             if (slot < handoff_cap) {
@@ -432,7 +444,7 @@ __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restric
 }
 
 #ifndef RK_REF_HANDOFF_FACES
-#define RK_REF_HANDOFF_FACES 8192
+#define RK_REF_HANDOFF_FACES 4096
 #endif
 
 #ifndef RK_REF_COOP_MAX
@@ -455,16 +467,21 @@ constexpr int kPCache = 224;                       // TRIANGLES_MAX, :39
 // with t < depth ends on, with the same depth after the chunk.
 //
 // A packet whose leaves pass handoff_faces faces (while handoff_cap != 0) is
-// put on the hand-off list and left, outputs untouched: its rays are traced
-// again from the start by the second pass. Returns whether it was.
+// put on the hand-off list and left where it stands: its rays' hits so far in
+// the outputs, and in its record of the state area its stack with the leaf
+// back on top, sp and the visits before that leaf's pop (so that the second
+// pass, given the record as resume, makes the same pops from there). Returns
+// whether it was.
+constexpr uint32_t kPStateWords = kPStack + 2;  // a packet's record: sp, visits, stack
+
 template <int kS>
 __device__ __forceinline__ bool trace_packet_job(
     const rk::Node* __restrict__ nodes, const uint4* __restrict__ faces, const float4* __restrict__ verts,
     const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, uint32_t node_count,
     uint32_t face_count, uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
     int bx, int by, uint32_t* __restrict__ handoff, uint32_t handoff_cap, uint32_t handoff_faces,
-    uint32_t* s_stack, float4* s_v0, float4* s_e1, float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt,
-    float* s_pu, float* s_pv, uint32_t* s_pk) {
+    uint32_t* __restrict__ state, const uint32_t* __restrict__ resume, uint32_t* s_stack, float4* s_v0, float4* s_e1,
+    float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv, uint32_t* s_pk) {
   const int ray = (int)threadIdx.x % kPT, part = (int)threadIdx.x / kPT;
   const int tx = ray % kPW, ty = ray / kPW;
   const int x = bx * kPW + tx, y = by * kPH + ty;
@@ -482,16 +499,25 @@ __device__ __forceinline__ bool trace_packet_job(
     fid = faces_id[l];
     coord = coords[l];
   }
+  int sp = 1;
+  uint32_t visits = 0;
+  if (resume) {
+    sp = (int)resume[0];
+    visits = resume[1];
+    for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) s_stack[k] = resume[2 + k];
+  }
   if (threadIdx.x == 0) {  // thread (0,0) of a block is always inside the image
     *s_dir0 = dir;
-    s_stack[0] = 0;
+    if (!resume) s_stack[0] = 0;
   }
-  int sp = 1;
+  bool may_hand_off = handoff_cap != 0u;
+  uint32_t limit = handoff_faces;
+  if (may_hand_off) {  // a packet holding a ray that passes every box test visits every node
+    if (__syncthreads_or(in_image && crosses_every_box(invdir))) limit = 0u;
+  }
   __syncthreads();
   const float4 dir0 = *s_dir0;
-  bool may_hand_off = handoff_cap != 0u;
   uint32_t tested = 0;  // block-uniform: faces of the leaves visited, while may_hand_off
-  uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
   while (sp > 0 && ++visits <= max_visits) {
     const uint32_t nid = s_stack[--sp];
@@ -501,7 +527,7 @@ __device__ __forceinline__ bool trace_packet_job(
       const uint32_t tri_id = nd.w1;
       const uint32_t cnt = tri_id < face_count ? min(rk::leaf_count(nd.w0), face_count - tri_id) : 0u;
       if (may_hand_off) {
-        if (cnt > handoff_faces - tested) {
+        if (cnt > limit - tested) {
           may_hand_off = false;
           if (threadIdx.x == 0) {
             const uint32_t slot = atomicAdd(handoff, 1u);
@@ -511,7 +537,21 @@ __device__ __forceinline__ bool trace_packet_job(
           __syncthreads();
           const uint32_t slot = *s_slot;
           __syncthreads();
-          if (slot < handoff_cap) return true;
+          if (slot < handoff_cap) {
+            uint32_t* rec = state + (size_t)slot * kPStateWords;
+            for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) rec[2 + k] = s_stack[k];
+            if (threadIdx.x == 0) {
+              rec[0] = (uint32_t)sp + 1u;  // sp < kPStack after the pop: the leaf fits
+              rec[1] = visits - 1u;
+              rec[2 + sp] = nid;
+            }
+            if (in_image && part == 0) {
+              faces_id[l] = fid;
+              coords[l] = coord;
+              depths[l] = depth;
+            }
+            return true;
+          }
         } else {
           tested += cnt;
         }
@@ -607,11 +647,13 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
   __shared__ float4 s_dir0;
   __shared__ uint32_t s_slot;
   trace_packet_job<1>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count, faces_id, coords,
-                      depths, (int)blockIdx.x, (int)blockIdx.y, handoff, handoff_cap, handoff_faces, s_stack, s_v0,
-                      s_e1, s_e2, &s_dir0, &s_slot, nullptr, nullptr, nullptr, nullptr);
+                      depths, (int)blockIdx.x, (int)blockIdx.y, handoff, handoff_cap, handoff_faces,
+                      handoff + kHandoffHead + handoff_cap, nullptr, s_stack, s_v0, s_e1, s_e2, &s_dir0, &s_slot,
+                      nullptr, nullptr, nullptr, nullptr);
 }
 
-// The second pass: the packets handed off, kPacketSplit threads per ray.
+// The second pass: the packets handed off, each resumed from its record,
+// kPacketSplit threads per ray.
 #ifndef RK_REF_PACKET_SPLIT
 #define RK_REF_PACKET_SPLIT 5
 #endif
@@ -631,10 +673,11 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
   const uint32_t count = min(handoff[0], handoff_cap);
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t pk = handoff[kHandoffHead + i];
+    const uint32_t* rec = handoff + kHandoffHead + handoff_cap + (size_t)i * kPStateWords;
     trace_packet_job<kPacketSplit>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count,
                                    faces_id, coords, depths, (int)(pk % (uint32_t)packets_x),
-                                   (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, s_stack, s_v0, s_e1, s_e2,
-                                   &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
+                                   (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, nullptr, rec, s_stack, s_v0,
+                                   s_e1, s_e2, &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
     __syncthreads();  // the next packet rewrites the stack and s_dir0
   }
 }
@@ -642,7 +685,7 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
 }  // namespace
 
 #ifndef RK_REF_PACKET_HANDOFF_FACES
-#define RK_REF_PACKET_HANDOFF_FACES 16384
+#define RK_REF_PACKET_HANDOFF_FACES 4096
 #endif
 constexpr uint32_t kPacketGrid = 1024;  // blocks of the packet traversal's second pass
 
@@ -651,13 +694,15 @@ static std::atomic<uint32_t> g_packet_faces{RK_REF_PACKET_HANDOFF_FACES};
 static std::atomic<uint32_t> g_list_cap{kHandoffCap};
 constexpr uint32_t kNoHandoff = 0xffffffffu;
 
-// The hand-off list of one call, cleared: a stream-ordered allocation, so that
-// calls on other streams have their own. Without it (nullptr) every ray or
+// The hand-off list of one call, cleared, and after it cap records of
+// record_words: a stream-ordered allocation, so that calls on other streams
+// have their own. Without it (nullptr) every ray or
 // packet stays in the first pass, with the same results.
-static uint32_t* handoff_alloc(hipStream_t st, uint32_t cap, uint32_t faces) {
+static uint32_t* handoff_alloc(hipStream_t st, uint32_t cap, uint32_t faces, uint32_t record_words) {
   uint32_t* ho = nullptr;
   if (faces == kNoHandoff || cap == 0) return nullptr;
-  if (hipMallocAsync(reinterpret_cast<void**>(&ho), (kHandoffHead + cap) * sizeof(uint32_t), st) != hipSuccess) {
+  const size_t words = kHandoffHead + (size_t)cap * (1u + record_words);
+  if (hipMallocAsync(reinterpret_cast<void**>(&ho), words * sizeof(uint32_t), st) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
@@ -676,14 +721,14 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
   hipStream_t st = rk_hip_stream(stream);
   const uint32_t n = (uint32_t)width * (uint32_t)height;
   if (coherency) {
-    // A packet whose leaves pass g_packet_faces faces (on the
-    // heightfield, one holding a ray parallel to the ground plane: the packet
-    // then visits every leaf, and all its rays test every face) is traced by
-    // the second pass, kPacketSplit threads per ray.
+    // A packet whose leaves pass g_packet_faces faces, or one holding a ray
+    // that passes every box test (the packet then visits every leaf, and all
+    // its rays test every face; at its first leaf), is traced on by the second
+    // pass, kPacketSplit threads per ray, from where it was left.
     const uint32_t gx = (uint32_t)(width + kPW - 1) / kPW, gy = (uint32_t)(height + kPH - 1) / kPH;
     const uint32_t faces_limit = g_packet_faces.load(std::memory_order_relaxed);
     const uint32_t cap = std::min<uint32_t>(gx * gy, g_list_cap.load(std::memory_order_relaxed));
-    uint32_t* ho = handoff_alloc(st, cap, faces_limit);
+    uint32_t* ho = handoff_alloc(st, cap, faces_limit, kPStateWords);
     hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
@@ -698,12 +743,12 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
       if (hipFreeAsync(ho, st) != hipSuccess) return RK_ERROR_DEVICE;
     }
   } else {
-    // A ray whose leaves pass g_ray_faces faces (on the heightfield,
-    // a ray parallel to the ground plane, whose box tests all pass) is handed
-    // to the second pass, one wave per ray.
+    // A ray whose leaves pass g_ray_faces faces, or a ray that passes every
+    // box test (at its first leaf), is traced again by the second pass, one
+    // wave per ray.
     const uint32_t faces_limit = g_ray_faces.load(std::memory_order_relaxed);
     const uint32_t cap = std::min<uint32_t>(n, g_list_cap.load(std::memory_order_relaxed));
-    uint32_t* ho = handoff_alloc(st, cap, faces_limit);
+    uint32_t* ho = handoff_alloc(st, cap, faces_limit, 0u);
     hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st,
                        reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                        reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),

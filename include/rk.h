@@ -222,12 +222,14 @@ rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32
                                 rk_float2* coords, float* depths, int coherency);
 
 /* Second passes of rk_raytrace_mesh_tree (no reference counterpart). A ray of the per-ray
- * traversal whose leaves pass ray_faces faces, or a packet whose leaves pass packet_faces
- * faces, is traced again from the start by a second pass that spreads it over more threads
- * (a wave per ray; five threads per ray of a packet), with the same results. The rays or
- * packets handed off go to a per-call list of list_cap entries (1 .. 65536, device memory
- * from the stream's pool); past its end they finish in the first pass. UINT32_MAX turns a
- * second pass off. Defaults 8192, 16384, 65536. Process-wide, read by each call. */
+ * traversal whose leaves pass ray_faces faces is traced again from the start by a wave of
+ * its own; a packet whose leaves pass packet_faces faces is resumed where it stands with
+ * five threads per ray. A ray with a direction component of 0 (every box test passes for
+ * it, so it walks the whole tree), or a packet holding one, goes at its first leaf. The
+ * results are the same. What is handed off goes to a per-call list of list_cap entries
+ * (1 .. 65536, device memory from the stream's pool); past its end, rays and packets finish
+ * in the first pass. UINT32_MAX turns a second pass off. Defaults 4096, 4096, 65536.
+ * Process-wide, read by each call. */
 rk_status rk_raytrace_set_handoff(uint32_t ray_faces, uint32_t packet_faces, uint32_t list_cap);
 rk_status rk_raytrace_get_handoff(uint32_t* ray_faces, uint32_t* packet_faces, uint32_t* list_cap);
 

@@ -188,7 +188,7 @@ def handoff(request, dev):
 
 
 def test_reference_handoff_limits(dev):
-    assert dev.get_ref_handoff() == (8192, 16384, 65536)
+    assert dev.get_ref_handoff() == (4096, 4096, 65536)
     dev.set_ref_handoff(1, 2, 3)
     assert dev.get_ref_handoff() == (1, 2, 3)
     with pytest.raises(Exception):
@@ -196,7 +196,7 @@ def test_reference_handoff_limits(dev):
     with pytest.raises(Exception):
         dev.set_ref_handoff(1, 2, 65537)
     dev.set_ref_handoff()
-    assert dev.get_ref_handoff() == (8192, 16384, 65536)
+    assert dev.get_ref_handoff() == (4096, 4096, 65536)
 
 
 @pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29),
@@ -267,11 +267,11 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
     """Rays with a direction component of exactly 0: the reference's normalised
     inverse direction is then (0, 0, NaN) up to order, every box test passes
     at t = 0 and the per-ray traversal tests every face of the mesh (the packet
-    traversal: every ray of a packet holding one). Past 8192 faces for a ray,
-    16384 for a packet, the library hands it to its second pass; the results
-    stay bit-identical to the oracle's."""
+    traversal: every ray of a packet holding one). The library hands such a ray
+    or packet to its second pass at the first leaf, and other rays past their
+    face budget; the results stay bit-identical to the oracle's."""
     V, F = _scene(rk, name)
-    assert F.shape[0] > (32768 if coherency else 8192)
+    assert F.shape[0] > 8192
     tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
     O, D = _camera_rays(rk, name, w, h)
     D = D.copy()
