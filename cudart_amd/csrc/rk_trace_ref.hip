@@ -196,6 +196,45 @@ __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, c
   return r;
 }
 
+// Rays that pass every box test (crosses_every_box) are split by subtree. The
+// walk of such a ray visits every node below one that passes its box test,
+// in an order fixed by the direction, as long as its depth stays above 0
+// (its box tests give t = +-0); at depth 0 no later face can be kept (t < 0
+// never holds). So the subtrees at tree level kUnitLevel (and the leaves
+// above it), each walked on its own from the ray's starting depth, give the
+// walk's result when merged in walk order, first smallest t winning (each
+// subtree walked by a wave: a short chain of pops, leaves tested by the whole
+// wave). A subtree's walk differs from the reference's only after the reference's
+// depth has reached 0, when its result can no longer change. The stack the
+// reference holds when it pops a subtree's root is known from the walk above
+// the cut, so a stack overflow stops both at the same node; the visit limit
+// is checked on the sum (past it, the ray is walked again by one wave).
+#ifndef RK_REF_UNIT_LEVEL
+#define RK_REF_UNIT_LEVEL 6
+#endif
+constexpr int kUnitLevel = RK_REF_UNIT_LEVEL;
+constexpr uint32_t kUnits = 1u << kUnitLevel;  // subtrees per ray at most: the cut has <= 2^6 nodes
+#ifndef RK_REF_CROSS_CAP
+#define RK_REF_CROSS_CAP 4096
+#endif
+constexpr uint32_t kCrossCap = RK_REF_CROSS_CAP;          // rays split per call
+constexpr uint32_t kUnitGrid = 8192;          // one-wave workgroups walking the subtrees
+constexpr uint32_t kNoFace = 0xffffffffu;
+constexpr uint32_t kUnitStop = 1u;            // the walk stopped here (stack overflow, child out of range)
+
+struct UnitResult { float t, u, v; uint32_t face, pops, flags, pad0, pad1; };
+
+// The split rays' state in the hand-off buffer.
+struct UnitArgs {
+  uint32_t* list;        // [cap] the rays, in hand-off order
+  uint32_t* nunits;      // [cap] subtrees of each
+  uint32_t* top_pops;    // [cap] pops of the walk above the cut
+  uint32_t* unit_node;   // [cap * kUnits] each subtree's root, in walk order
+  uint32_t* unit_sp;     // [cap * kUnits] the reference's stack size after popping that root
+  UnitResult* res;       // [cap * kUnits]
+  uint32_t cap;
+};
+
 // The per-ray traversal (kernel_raytraceMeshTree, :111-219), one lane per ray:
 // pop a node, test its box against the depth so far, push the far child, then
 // the near one; at a leaf, test its faces in order, keeping a hit only when
@@ -234,7 +273,7 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
                                                   uint32_t node_count, uint32_t face_count,
                                                   uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                   float* __restrict__ depths, uint32_t* __restrict__ handoff,
-                                                  uint32_t handoff_cap, uint32_t handoff_faces) {
+                                                  uint32_t handoff_cap, uint32_t handoff_faces, UnitArgs ua) {
   __shared__ uint32_t s_stack[kLdsStack * 64];
   const int lane = (int)threadIdx.x;
   const uint32_t l = blockIdx.x * 64 + (uint32_t)lane;
@@ -258,8 +297,9 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
   uint32_t tested = 0;  // faces of the leaves reached, while may_hand_off
   // A direction with a component of 0 leaves a NaN in invdir: every box test
   // then passes (at t = 0, while depth > 0) and the ray walks the whole tree,
-  // so it goes to the second pass at its first leaf.
-  const uint32_t limit = crosses_every_box(invdir) ? 0u : handoff_faces;
+  // so it goes to the split at its first leaf (else to the second pass).
+  const bool crossing = crosses_every_box(invdir);
+  const uint32_t limit = crossing ? 0u : handoff_faces;
   uint32_t visits = 0;
   const uint32_t max_visits = 2 * node_count + 2;
   for (;;) {
@@ -284,6 +324,15 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
         if (may_hand_off) {
           if (leaf_cnt > limit - tested) {
             may_hand_off = false;
+            if (crossing && ua.cap != 0u) {
+              const uint32_t slot = atomicAdd(handoff + 1, 1u);
+              if (slot < ua.cap) {
+                ua.list[slot] = l;
+                handed_off = true;
+                active = false;
+                break;
+              }
+            }
             const uint32_t slot = atomicAdd(handoff, 1u);
             if (slot < handoff_cap) {
               handoff[kHandoffHead + slot] = l;
@@ -374,13 +423,137 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
 }
 
 __global__ __launch_bounds__(64) void k_handoff_clear(uint32_t* __restrict__ handoff) {
-  if (threadIdx.x == 0) handoff[0] = 0u;
+  if (threadIdx.x < 2) handoff[threadIdx.x] = 0u;
 }
 
-// The second pass of the per-ray traversal: one wave per ray handed off, the
-// ray's walk from the root again with the same rules (the same pops, box
-// tests, stack overflow and visit limits), each leaf tested by the whole wave
-// (wave_leaf). The stack's entry e is lane e % 64 of st0 (e < 64) or st1.
+// The walk of a split ray above the cut, one lane per ray: the reference's
+// pops of the nodes above kUnitLevel (box tests at the starting depth), each
+// node at the cut, leaf above it or child out of range recorded as a subtree
+// root with the stack size after its pop. The stack holds at most
+// kUnitLevel + 1 entries here.
+__global__ __launch_bounds__(64) void k_units_split(const rk::Node* __restrict__ nodes,
+                                                    const float4* __restrict__ origins,
+                                                    const float4* __restrict__ dirs,
+                                                    const float* __restrict__ depths, uint32_t node_count,
+                                                    const uint32_t* __restrict__ handoff, UnitArgs ua) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= min(handoff[1], ua.cap)) return;
+  const uint32_t l = ua.list[i];
+  const float4 orig = origins[l], dir = dirs[l];
+  const float4 invdir = invdir_of(dir);
+  const float depth = depths[l];
+  uint32_t st[2 * kUnitLevel + 2], lv[2 * kUnitLevel + 2];
+  int sp = 1;
+  st[0] = 0u;
+  lv[0] = 0u;
+  uint32_t nu = 0, pops = 0;
+  uint32_t* unode = ua.unit_node + (size_t)i * kUnits;
+  uint32_t* usp = ua.unit_sp + (size_t)i * kUnits;
+  while (sp > 0 && nu < kUnits) {
+    --sp;
+    const uint32_t nid = st[sp], level = lv[sp];
+    bool unit = level >= (uint32_t)kUnitLevel || nid >= node_count;
+    NodeRegs nd;
+    if (!unit) {
+      nd = load_node(nodes, nid);
+      unit = rk::node_is_leaf(nd.w0);
+    }
+    if (unit) {  // its walk counts its pop and stops at a child out of range
+      unode[nu] = nid;
+      usp[nu] = (uint32_t)sp;
+      ++nu;
+      if (nid >= node_count) break;
+      continue;
+    }
+    ++pops;
+    float t;
+    if (box_ray(nd.bmin, nd.bmax, orig, invdir, t) && t < depth) {
+      uint32_t nearc, farc;
+      children_order(nd.w0, nd.w1, dir, nearc, farc);
+      if (sp + 2 > 2 * kUnitLevel + 2) break;  // cannot occur: one entry per level, two at the deepest
+      st[sp] = farc;
+      lv[sp] = level + 1u;
+      ++sp;
+      st[sp] = nearc;
+      lv[sp] = level + 1u;
+      ++sp;
+    }
+  }
+  ua.nunits[i] = nu;
+  ua.top_pops[i] = pops;
+}
+
+// The walk of ray l by the whole wave with the reference's rules (the same
+// pops, box tests, stack overflow and visit limits), each leaf tested by the
+// whole wave (wave_leaf), from node root with the reference's stack holding
+// sp_base entries below it (0 and 0: the ray's walk), from depth, fid and
+// coord. The stack's entry e is lane e % 64 of st0 (e < 64) or st1.
+struct WalkResult { float depth; uint32_t fid; float2 coord; uint32_t pops; bool stopped; };
+
+__device__ __forceinline__ WalkResult wave_walk(uint32_t l, uint32_t root, uint32_t sp_base, float depth,
+                                                uint32_t fid, float2 coord, const rk::Node* __restrict__ nodes,
+                                                const uint4* __restrict__ faces, const float4* __restrict__ verts,
+                                                const float4* __restrict__ origins,
+                                                const float4* __restrict__ dirs, uint32_t node_count,
+                                                uint32_t face_count) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t max_visits = 2 * node_count + 2;
+  const float4 orig = origins[l], dir = dirs[l];
+  const float4 invdir = invdir_of(dir);
+  uint32_t st0 = lane == 0 ? root : 0u, st1 = 0u;
+  int sp = 1;
+  uint32_t visits = 0;
+  bool stopped = false;
+  while (sp > 0 && ++visits <= max_visits) {
+    --sp;
+    const uint32_t nid = readlane_u32(sp < 64 ? st0 : st1, sp & 63);
+    if (nid >= node_count) {  // never taken for a tree built by this library
+      stopped = true;
+      break;
+    }
+    const NodeRegs nd = load_node(nodes, nid);
+    if (rk::node_is_leaf(nd.w0)) {
+      const uint32_t first = nd.w1;
+      const uint32_t cnt = first < face_count ? min(rk::leaf_count(nd.w0), face_count - first) : 0u;
+      const LeafBest b = wave_leaf<kWavePer>(faces, verts, orig, dir, first, cnt, depth);
+      if (b.found) {
+        depth = b.t;
+        fid = first + b.k;
+        coord = make_float2(b.u, b.v);
+      }
+    } else {
+      float t;
+      if (box_ray(nd.bmin, nd.bmax, orig, invdir, t) && t < depth) {
+        uint32_t nearc, farc;
+        children_order(nd.w0, nd.w1, dir, nearc, farc);
+        if ((uint32_t)sp + sp_base + 2u > (uint32_t)kStack) {  // cannot occur for trees of depth <= 94
+          stopped = true;
+          break;
+        }
+        if (lane == (sp & 63)) {
+          if (sp < 64) st0 = farc;
+          else st1 = farc;
+        }
+        ++sp;
+        if (lane == (sp & 63)) {
+          if (sp < 64) st0 = nearc;
+          else st1 = nearc;
+        }
+        ++sp;
+      }
+    }
+  }
+  WalkResult r;
+  r.depth = depth;
+  r.fid = fid;
+  r.coord = coord;
+  r.pops = visits;
+  r.stopped = stopped;
+  return r;
+}
+
+// The second pass of the per-ray traversal: one wave per ray handed off, its
+// walk again from the root.
 __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restrict__ nodes,
                                                        const uint4* __restrict__ faces,
                                                        const float4* __restrict__ verts,
@@ -389,53 +562,93 @@ __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restric
                                                        uint32_t face_count, const uint32_t* __restrict__ handoff,
                                                        uint32_t handoff_cap, uint32_t* __restrict__ faces_id,
                                                        float2* __restrict__ coords, float* __restrict__ depths) {
-  const int lane = (int)threadIdx.x;
   const uint32_t count = min(handoff[0], handoff_cap);
-  const uint32_t max_visits = 2 * node_count + 2;
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t l = handoff[kHandoffHead + i];
-    const float4 orig = origins[l], dir = dirs[l];
-    const float4 invdir = invdir_of(dir);
+    const WalkResult r = wave_walk(l, 0u, 0u, depths[l], faces_id[l], coords[l], nodes, faces, verts, origins,
+                                   dirs, node_count, face_count);
+    if (threadIdx.x == 0) {
+      faces_id[l] = r.fid;
+      coords[l] = r.coord;
+      depths[l] = r.depth;
+    }
+  }
+}
+
+// The subtrees of the split rays, one wave per subtree: its walk from its
+// root at the ray's starting depth, the result in its UnitResult.
+__global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ nodes,
+                                                   const uint4* __restrict__ faces,
+                                                   const float4* __restrict__ verts,
+                                                   const float4* __restrict__ origins,
+                                                   const float4* __restrict__ dirs, const float* __restrict__ depths,
+                                                   uint32_t node_count, uint32_t face_count,
+                                                   const uint32_t* __restrict__ handoff, UnitArgs ua) {
+  const uint32_t count = min(handoff[1], ua.cap), jobs = count * kUnits;
+  for (uint32_t jj = blockIdx.x; jj < jobs; jj += gridDim.x) {
+    // subtree k of every ray before subtree k + 1 of any: the waves running
+    // together walk the same part of the tree (the rays handed off here have
+    // mostly the same walk order), and share its nodes and faces in the L2
+    const uint32_t k = jj / count, i = jj % count, j = i * kUnits + k;
+    if (k >= ua.nunits[i]) continue;
+    const uint32_t l = ua.list[i];
+    const WalkResult w = wave_walk(l, ua.unit_node[j], ua.unit_sp[j], depths[l], kNoFace,
+                                   make_float2(0.0f, 0.0f), nodes, faces, verts, origins, dirs, node_count,
+                                   face_count);
+    if (threadIdx.x == 0) {
+      UnitResult r;
+      r.t = w.depth;
+      r.u = w.coord.x;
+      r.v = w.coord.y;
+      r.face = w.fid;
+      r.pops = w.pops;
+      r.flags = w.stopped ? kUnitStop : 0u;
+      r.pad0 = 0u;
+      r.pad1 = 0u;
+      ua.res[j] = r;
+    }
+  }
+}
+
+// The split rays' results: the subtrees' results merged in walk order (the
+// first smallest t below the depth so far wins; a subtree that stopped ends
+// the walk), when the pops add up to no more than the visit limit; else the
+// ray's walk by the whole wave.
+__global__ __launch_bounds__(64) void k_units_merge(const rk::Node* __restrict__ nodes,
+                                                    const uint4* __restrict__ faces,
+                                                    const float4* __restrict__ verts,
+                                                    const float4* __restrict__ origins,
+                                                    const float4* __restrict__ dirs, uint32_t node_count,
+                                                    uint32_t face_count, const uint32_t* __restrict__ handoff,
+                                                    UnitArgs ua, uint32_t* __restrict__ faces_id,
+                                                    float2* __restrict__ coords, float* __restrict__ depths) {
+  const uint32_t count = min(handoff[1], ua.cap);
+  const uint64_t max_visits = 2 * node_count + 2;
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t l = ua.list[i];
     float depth = depths[l];
     uint32_t fid = faces_id[l];
     float2 coord = coords[l];
-    uint32_t st0 = 0u, st1 = 0u;  // entry 0: the root, node 0
-    int sp = 1;
-    uint32_t visits = 0;
-    while (sp > 0 && ++visits <= max_visits) {
-      --sp;
-      const uint32_t nid = readlane_u32(sp < 64 ? st0 : st1, sp & 63);
-      if (nid >= node_count) break;  // never taken for a tree built by this library
-      const NodeRegs nd = load_node(nodes, nid);
-      if (rk::node_is_leaf(nd.w0)) {
-        const uint32_t first = nd.w1;
-        const uint32_t cnt = first < face_count ? min(rk::leaf_count(nd.w0), face_count - first) : 0u;
-        const LeafBest b = wave_leaf<kWavePer>(faces, verts, orig, dir, first, cnt, depth);
-        if (b.found) {
-          depth = b.t;
-          fid = first + b.k;
-          coord = make_float2(b.u, b.v);
-        }
-      } else {
-        float t;
-        if (box_ray(nd.bmin, nd.bmax, orig, invdir, t) && t < depth) {
-          uint32_t nearc, farc;
-          children_order(nd.w0, nd.w1, dir, nearc, farc);
-          if (sp + 2 > kStack) break;  // depth > 94: cannot occur for trees of depth <= 94
-          if (lane == (sp & 63)) {
-            if (sp < 64) st0 = farc;
-            else st1 = farc;
-          }
-          ++sp;
-          if (lane == (sp & 63)) {
-            if (sp < 64) st0 = nearc;
-            else st1 = nearc;
-          }
-          ++sp;
-        }
+    uint64_t pops = ua.top_pops[i];
+    const uint32_t nu = ua.nunits[i];
+    for (uint32_t k = 0; k < nu; ++k) {
+      const UnitResult r = ua.res[(size_t)i * kUnits + k];
+      pops += r.pops;
+      if (r.face != kNoFace && r.t < depth) {
+        depth = r.t;
+        fid = r.face;
+        coord = make_float2(r.u, r.v);
       }
+      if (r.flags & kUnitStop) break;
     }
-    if (lane == 0) {
+    if (pops > max_visits) {
+      const WalkResult w = wave_walk(l, 0u, 0u, depths[l], faces_id[l], coords[l], nodes, faces, verts, origins,
+                                     dirs, node_count, face_count);
+      depth = w.depth;
+      fid = w.fid;
+      coord = w.coord;
+    }
+    if (threadIdx.x == 0) {
       faces_id[l] = fid;
       coords[l] = coord;
       depths[l] = depth;
@@ -694,14 +907,12 @@ static std::atomic<uint32_t> g_packet_faces{RK_REF_PACKET_HANDOFF_FACES};
 static std::atomic<uint32_t> g_list_cap{kHandoffCap};
 constexpr uint32_t kNoHandoff = 0xffffffffu;
 
-// The hand-off list of one call, cleared, and after it cap records of
-// record_words: a stream-ordered allocation, so that calls on other streams
-// have their own. Without it (nullptr) every ray or
+// The hand-off buffer of one call (words 32-bit words), its counters cleared:
+// a stream-ordered allocation, so that calls on other streams have their own. Without it (nullptr) every ray or
 // packet stays in the first pass, with the same results.
-static uint32_t* handoff_alloc(hipStream_t st, uint32_t cap, uint32_t faces, uint32_t record_words) {
+static uint32_t* handoff_alloc(hipStream_t st, size_t words, uint32_t faces) {
   uint32_t* ho = nullptr;
-  if (faces == kNoHandoff || cap == 0) return nullptr;
-  const size_t words = kHandoffHead + (size_t)cap * (1u + record_words);
+  if (faces == kNoHandoff || words <= kHandoffHead) return nullptr;
   if (hipMallocAsync(reinterpret_cast<void**>(&ho), words * sizeof(uint32_t), st) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
@@ -728,7 +939,7 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     const uint32_t gx = (uint32_t)(width + kPW - 1) / kPW, gy = (uint32_t)(height + kPH - 1) / kPH;
     const uint32_t faces_limit = g_packet_faces.load(std::memory_order_relaxed);
     const uint32_t cap = std::min<uint32_t>(gx * gy, g_list_cap.load(std::memory_order_relaxed));
-    uint32_t* ho = handoff_alloc(st, cap, faces_limit, kPStateWords);
+    uint32_t* ho = handoff_alloc(st, kHandoffHead + (size_t)cap * (1u + kPStateWords), faces_limit);
     hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
@@ -748,12 +959,37 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     // wave per ray.
     const uint32_t faces_limit = g_ray_faces.load(std::memory_order_relaxed);
     const uint32_t cap = std::min<uint32_t>(n, g_list_cap.load(std::memory_order_relaxed));
-    uint32_t* ho = handoff_alloc(st, cap, faces_limit, 0u);
-    hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st,
-                       reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
-                       reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
-                       reinterpret_cast<const float4*>(directions), n, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords),
-                       depths, ho, ho ? cap : 0u, faces_limit);
+    const uint32_t ucap = std::min<uint32_t>(cap, kCrossCap);
+    // [head | list | split rays: list, nunits, top_pops | unit_node, unit_sp | results (8 words each)]
+    const size_t res_off = (kHandoffHead + cap + 3ull * ucap + 2ull * ucap * kUnits + 7) & ~size_t(7);
+    uint32_t* ho = handoff_alloc(st, res_off + 8ull * ucap * kUnits, faces_limit);
+    UnitArgs ua{};
+    if (ho) {
+      uint32_t* p = ho + kHandoffHead + cap;
+      ua.list = p;
+      ua.nunits = p + ucap;
+      ua.top_pops = p + 2 * ucap;
+      ua.unit_node = p + 3 * ucap;
+      ua.unit_sp = p + 3 * ucap + (size_t)ucap * kUnits;
+      ua.res = reinterpret_cast<UnitResult*>(ho + res_off);
+      ua.cap = ucap;
+    }
+    const rk::Node* nd = reinterpret_cast<const rk::Node*>(nodes);
+    const uint4* fc = reinterpret_cast<const uint4*>(faces);
+    const float4* vt = reinterpret_cast<const float4*>(vertices);
+    const float4* org = reinterpret_cast<const float4*>(origins);
+    const float4* drs = reinterpret_cast<const float4*>(directions);
+    float2* uv = reinterpret_cast<float2*>(coords);
+    hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st, nd, fc, vt, org,
+                       drs, n, node_count, face_count, faces_id, uv, depths, ho, ho ? cap : 0u, faces_limit, ua);
+    if (ho && ucap != 0u) {
+      hipLaunchKernelGGL(k_units_split, dim3((ucap + 63) / 64), dim3(64), 0, st, nd, org, drs, depths, node_count,
+                         ho, ua);
+      hipLaunchKernelGGL(k_units_walk, dim3(std::min<uint32_t>(ucap * kUnits, kUnitGrid)), dim3(64), 0, st, nd, fc,
+                         vt, org, drs, depths, node_count, face_count, ho, ua);
+      hipLaunchKernelGGL(k_units_merge, dim3(std::min<uint32_t>(ucap, kWaveGrid)), dim3(64), 0, st, nd, fc, vt, org,
+                         drs, node_count, face_count, ho, ua, faces_id, uv, depths);
+    }
     if (ho) {
       hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kWaveGrid)), dim3(64), 0, st,
                          reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),

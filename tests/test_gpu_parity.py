@@ -44,6 +44,8 @@ def _scene(rk, name):
         V, F = rk.icosphere(5)
     elif name == "hf128":
         V, F = rk.heightfield(128)
+    elif name == "hf32":
+        V, F = rk.heightfield(32)
     elif name == "hf160":
         V, F = rk.heightfield(160)
     elif name == "hf708":
@@ -294,6 +296,67 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
         of, ouv, od, ovf = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0)
     assert ovf == 0
     assert (od < dep0).sum() > n // 20
+    np.testing.assert_array_equal(_h(out[0], np.uint32), of)
+    np.testing.assert_array_equal(_h(out[1]), ouv)
+    np.testing.assert_array_equal(_h(out[2]), od)
+
+
+def _chain_tree(V, levels, per_leaf):
+    """A tree of `levels` inner nodes in a chain (split axis x, the mesh's box):
+    each has a leaf of per_leaf faces and the next inner node (a leaf after the
+    last) as its children, the leaf first at even levels and second at odd
+    ones. A ray then takes the chain first at every other level, pushing the
+    leaf, and tests the leaf first at the others: its stack grows by one entry
+    every two levels, and overflows deep enough down."""
+    hb = np.asarray(V[:, :3], np.float32)
+    lo, hi = hb.min(0), hb.max(0)
+    nodes = np.zeros((2 * levels + 1, 8), np.uint32)
+    inner, nxt = 0, 1
+    for d in range(levels):
+        leaf, chain = (nxt, nxt + 1) if d % 2 == 0 else (nxt + 1, nxt)
+        nodes[inner, 0] = 0 << 1  # axis x
+        nodes[inner, 1] = nxt
+        nodes[inner, 2:5] = lo.view(np.uint32)
+        nodes[inner, 5:8] = hi.view(np.uint32)
+        nodes[leaf, 0] = (per_leaf << 1) | 1
+        nodes[leaf, 1] = d * per_leaf
+        inner, nxt = chain, nxt + 2
+    nodes[inner, 0] = (per_leaf << 1) | 1
+    nodes[inner, 1] = levels * per_leaf
+    return nodes
+
+
+@pytest.mark.parametrize("coherency", [False, True])
+def test_reference_trace_stack_overflow(dev, rk, ora, coherency, handoff):
+    """A tree deeper than the reference's stacks (96 entries per ray, 128 per
+    packet): the rays that take the chain first stop where the reference's
+    stack would overflow, in the first pass, the second and the split of the
+    rays that pass every box test (whose subtree walks start with the
+    reference's stack below them)."""
+    V, F = rk.heightfield(32)
+    levels, per_leaf = 270, 6
+    assert (levels + 1) * per_leaf <= F.shape[0]
+    nodes = _chain_tree(V, levels, per_leaf)
+    w, h = 48, 36
+    O, D = _camera_rays(rk, "hf32", w, h)
+    O, D = O.copy(), D.copy()
+    n = w * h
+    D[1::3, 0] = np.abs(D[1::3, 0])
+    D[::5, 2] = 0.0  # every box test passes
+    D[2::7, 1] = 0.0
+    fid0 = np.zeros(n, np.uint32)
+    uv0 = np.zeros((n, 2), np.float32)
+    dep0 = np.full(n, F32MAX, np.float32)
+    from cudart_amd.device import RefTree
+    tree = RefTree(_t(F), None, None, _t(nodes), None, node_count=nodes.shape[0])
+    out = (_t(fid0), _t(uv0), _t(dep0))
+    dev.trace_ref(tree, _t(V), _t(O), _t(D), w, h, coherency, out=out)
+    if coherency:
+        of, ouv, od, ovf = ora.trace_ref_packet(nodes, F, V, O, D, w, h, fid0, uv0, dep0)
+    else:
+        of, ouv, od, ovf = ora.trace_ref_ray(nodes, F, V, O, D, fid0, uv0, dep0)
+    assert ovf > 0
+    assert (od < F32MAX).sum() > 10  # hits in the leaves tested before the stacks overflow
     np.testing.assert_array_equal(_h(out[0], np.uint32), of)
     np.testing.assert_array_equal(_h(out[1]), ouv)
     np.testing.assert_array_equal(_h(out[2]), od)
