@@ -17,6 +17,7 @@
 // float operations in the same order as the reference source.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 #include "rk_device.h"
 
@@ -116,7 +117,8 @@ __device__ __forceinline__ float4 readlane_f4(float4 v, int l) {
 // off, the ray indices start at word kHandoffHead.
 constexpr uint32_t kHandoffHead = 64;
 constexpr uint32_t kHandoffCap = 1u << 16;  // rays per call (256 KB)
-constexpr uint32_t kWaveGrid = 2048;  // one-wave workgroups of the second pass
+constexpr uint32_t kWaveGrid = 2048;     // one-wave workgroups merging the split rays
+constexpr uint32_t kRayWaveGrid = 8192;  // one-wave workgroups walking the other rays handed off
 #ifndef RK_REF_WAVE_PER
 #define RK_REF_WAVE_PER 4
 #endif
@@ -907,13 +909,50 @@ static std::atomic<uint32_t> g_packet_faces{RK_REF_PACKET_HANDOFF_FACES};
 static std::atomic<uint32_t> g_list_cap{kHandoffCap};
 constexpr uint32_t kNoHandoff = 0xffffffffu;
 
+// A memory pool per device for the hand-off buffers, which keeps what calls
+// free (the default pool returns it at every synchronisation, and the next
+// call would allocate again: 0.3-1 ms per call on the 1920x1080 frame).
+static std::mutex g_pool_mutex;
+static hipMemPool_t g_pools[64];
+static bool g_pool_tried[64];
+
+static hipMemPool_t handoff_pool() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  if (!g_pool_tried[dev]) {
+    g_pool_tried[dev] = true;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) {
+        g_pools[dev] = pool;
+      } else {
+        (void)hipMemPoolDestroy(pool);
+      }
+    }
+    (void)hipGetLastError();
+  }
+  return g_pools[dev];
+}
+
 // The hand-off buffer of one call (words 32-bit words), its counters cleared:
-// a stream-ordered allocation, so that calls on other streams have their own. Without it (nullptr) every ray or
-// packet stays in the first pass, with the same results.
+// a stream-ordered allocation, so that calls on other streams have their own.
+// Without it (nullptr) every ray or packet stays in the first pass, with the
+// same results.
 static uint32_t* handoff_alloc(hipStream_t st, size_t words, uint32_t faces) {
   uint32_t* ho = nullptr;
   if (faces == kNoHandoff || words <= kHandoffHead) return nullptr;
-  if (hipMallocAsync(reinterpret_cast<void**>(&ho), words * sizeof(uint32_t), st) != hipSuccess) {
+  hipMemPool_t pool = handoff_pool();
+  const size_t bytes = words * sizeof(uint32_t);
+  const hipError_t e = pool ? hipMallocFromPoolAsync(reinterpret_cast<void**>(&ho), bytes, pool, st)
+                            : hipMallocAsync(reinterpret_cast<void**>(&ho), bytes, st);
+  if (e != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
@@ -991,7 +1030,7 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
                          drs, node_count, face_count, ho, ua, faces_id, uv, depths);
     }
     if (ho) {
-      hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kWaveGrid)), dim3(64), 0, st,
+      hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kRayWaveGrid)), dim3(64), 0, st,
                          reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                          reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                          reinterpret_cast<const float4*>(directions), node_count, face_count, ho, cap, faces_id,
