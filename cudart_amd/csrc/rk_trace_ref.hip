@@ -3,7 +3,11 @@
 //   k_trace_ray     per-ray DFS (kernel_raytraceMeshTree, :111-219), one lane
 //                   per ray for the nodes; a leaf's faces tested by its lane,
 //                   or by the whole wave when few lanes hold a leaf; stack in
-//                   LDS and scratch; a ray past a face budget is handed off to
+//                   LDS and scratch; a ray that passes every box test is
+//                   handed off to
+//   k_units_split, k_units_walk, k_units_merge  its walk cut into subtrees,
+//                   walked for groups of rays of one direction octant, and
+//                   merged in walk order; another ray past a face budget to
 //   k_trace_ray_wave  the same walk, one wave per ray, leaves tested by the
 //                   whole wave;
 //   k_trace_packet  the 16x12 packet traversal (kernel_raytraceMeshTreePacket,
@@ -11,7 +15,9 @@
 //                   block-uniform stack in LDS, one barrier vote per node; a
 //                   packet past a face budget is handed off to
 //   k_trace_packet_split  the same walk with five threads per ray, each
-//                   testing a fifth of every chunk of faces.
+//                   testing a fifth of every chunk of faces, or, for a packet
+//                   holding a ray that passes every box test, to
+//   k_packet_forced  the packet's walk by parts of 16 rays on different CUs.
 // The results of both passes are bit-identical to the reference's.
 // Built with -ffp-contract=off, so the triangle and box tests are the same
 // float operations in the same order as the reference source.
@@ -802,7 +808,11 @@ constexpr int kPCache = 224;                       // TRIANGLES_MAX, :39
 // back on top, sp and the visits before that leaf's pop (so that the second
 // pass, given the record as resume, makes the same pops from there). Returns
 // whether it was.
-constexpr uint32_t kPStateWords = kPStack + 2;  // a packet's record: sp, visits, stack
+// A packet's record: sp, visits, the forcing ray (the first ray of the
+// packet that passes every box test, or kNone), its forced slot (or kNone;
+// kDone once the forced walk's results are in the outputs), the stack.
+constexpr uint32_t kPStateWords = kPStack + 4;
+constexpr uint32_t kNone = 0xffffffffu, kDone = 0xfffffffeu;
 
 template <int kS>
 __device__ __forceinline__ bool trace_packet_job(
@@ -810,8 +820,9 @@ __device__ __forceinline__ bool trace_packet_job(
     const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, uint32_t node_count,
     uint32_t face_count, uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
     int bx, int by, uint32_t* __restrict__ handoff, uint32_t handoff_cap, uint32_t handoff_faces,
-    uint32_t* __restrict__ state, const uint32_t* __restrict__ resume, uint32_t* s_stack, float4* s_v0, float4* s_e1,
-    float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv, uint32_t* s_pk) {
+    uint32_t* __restrict__ state, uint32_t* __restrict__ forced_list, uint32_t forced_cap,
+    const uint32_t* __restrict__ resume, uint32_t* s_stack, float4* s_v0, float4* s_e1, float4* s_e2, float4* s_dir0,
+    uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv, uint32_t* s_pk) {
   const int ray = (int)threadIdx.x % kPT, part = (int)threadIdx.x / kPT;
   const int tx = ray % kPW, ty = ray / kPW;
   const int x = bx * kPW + tx, y = by * kPH + ty;
@@ -834,16 +845,21 @@ __device__ __forceinline__ bool trace_packet_job(
   if (resume) {
     sp = (int)resume[0];
     visits = resume[1];
-    for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) s_stack[k] = resume[2 + k];
+    for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) s_stack[k] = resume[4 + k];
   }
   if (threadIdx.x == 0) {  // thread (0,0) of a block is always inside the image
     *s_dir0 = dir;
     if (!resume) s_stack[0] = 0;
   }
   bool may_hand_off = handoff_cap != 0u;
-  uint32_t limit = handoff_faces;
+  uint32_t limit = handoff_faces, forcer = kNone;
   if (may_hand_off) {  // a packet holding a ray that passes every box test visits every node
-    if (__syncthreads_or(in_image && crosses_every_box(invdir))) limit = 0u;
+    if (threadIdx.x == 0) *s_slot = kNone;
+    __syncthreads();
+    if (in_image && part == 0 && crosses_every_box(invdir)) atomicMin(s_slot, (uint32_t)ray);
+    __syncthreads();
+    forcer = *s_slot;
+    if (forcer != kNone) limit = 0u;
   }
   __syncthreads();
   const float4 dir0 = *s_dir0;
@@ -869,11 +885,19 @@ __device__ __forceinline__ bool trace_packet_job(
           __syncthreads();
           if (slot < handoff_cap) {
             uint32_t* rec = state + (size_t)slot * kPStateWords;
-            for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) rec[2 + k] = s_stack[k];
+            for (int k = (int)threadIdx.x; k < sp; k += kPT * kS) rec[4 + k] = s_stack[k];
             if (threadIdx.x == 0) {
               rec[0] = (uint32_t)sp + 1u;  // sp < kPStack after the pop: the leaf fits
               rec[1] = visits - 1u;
-              rec[2 + sp] = nid;
+              rec[4 + sp] = nid;
+              uint32_t f = kNone;
+              if (forcer != kNone) {
+                f = atomicAdd(handoff + 2, 1u);
+                if (f < forced_cap) forced_list[f] = slot;
+                else f = kNone;
+              }
+              rec[2] = f != kNone ? forcer : kNone;
+              rec[3] = f;
             }
             if (in_image && part == 0) {
               faces_id[l] = fid;
@@ -971,15 +995,16 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
                                                       uint32_t node_count, uint32_t face_count,
                                                       uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                       float* __restrict__ depths, uint32_t* __restrict__ handoff,
-                                                      uint32_t handoff_cap, uint32_t handoff_faces) {
+                                                      uint32_t handoff_cap, uint32_t handoff_faces,
+                                                      uint32_t* __restrict__ forced_list, uint32_t forced_cap) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float4 s_dir0;
   __shared__ uint32_t s_slot;
   trace_packet_job<1>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count, faces_id, coords,
                       depths, (int)blockIdx.x, (int)blockIdx.y, handoff, handoff_cap, handoff_faces,
-                      handoff + kHandoffHead + handoff_cap, nullptr, s_stack, s_v0, s_e1, s_e2, &s_dir0, &s_slot,
-                      nullptr, nullptr, nullptr, nullptr);
+                      handoff + kHandoffHead + handoff_cap, forced_list, forced_cap, nullptr, s_stack, s_v0, s_e1,
+                      s_e2, &s_dir0, &s_slot, nullptr, nullptr, nullptr, nullptr);
 }
 
 // The second pass: the packets handed off, each resumed from its record,
@@ -1004,11 +1029,180 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t pk = handoff[kHandoffHead + i];
     const uint32_t* rec = handoff + kHandoffHead + handoff_cap + (size_t)i * kPStateWords;
+    if (rec[3] == kDone) continue;  // traced by the forced walk
     trace_packet_job<kPacketSplit>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count,
                                    faces_id, coords, depths, (int)(pk % (uint32_t)packets_x),
-                                   (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, nullptr, rec, s_stack, s_v0,
-                                   s_e1, s_e2, &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
+                                   (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, nullptr, nullptr, 0u, rec,
+                                   s_stack, s_v0, s_e1, s_e2, &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
     __syncthreads();  // the next packet rewrites the stack and s_dir0
+  }
+}
+
+// The packets holding a ray that passes every box test (the forcing ray), by
+// parts of kFRays rays on different CUs. While the forcing ray's depth is
+// above 0 it passes every box that box_ray passes (at t = +-0), so the packet
+// visits every such node, in the order its first ray gives, whatever its
+// other rays hold: each part walks that order from the packet's record on its
+// own, checking the forcing ray's box test at every node (a failure ends the
+// part: diverged), and each of its rays tests every face of every leaf in face
+// order against its own depth, kFParts threads per ray (consecutive faces of
+// each chunk each, merged in part order). The results go to temp;
+// k_packet_commit puts them in the outputs when no part diverged and the
+// forcing ray's final depth is above 0 (then it was above 0 all along), and
+// otherwise leaves the packet to k_trace_packet_split, from its record. On the
+// 1920x1080 heightfield (149 such packets), parts of 48, 24, 16 and 12 rays
+// gave 68.6, 47.4, 44.9 and 59.8 ms per frame (67.1 ms with all on one CU).
+#ifndef RK_REF_FORCED_RAYS
+#define RK_REF_FORCED_RAYS 16
+#endif
+constexpr int kFRays = RK_REF_FORCED_RAYS, kFThreads = kPT, kFParts = kFThreads / kFRays;
+constexpr int kFPackets = kPT / kFRays;  // parts of a packet
+constexpr uint32_t kForcedCap = 1024;    // forced packets per call
+constexpr uint32_t kForcedGrid = 2048;
+
+__global__ __launch_bounds__(kFThreads) void k_packet_forced(
+    const rk::Node* __restrict__ nodes, const uint4* __restrict__ faces, const float4* __restrict__ verts,
+    const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, int packets_x,
+    uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
+    const uint32_t* __restrict__ forced_list, uint32_t forced_cap, float4* __restrict__ temp,
+    uint32_t* __restrict__ fflags, const uint32_t* __restrict__ faces_id, const float2* __restrict__ coords,
+    const float* __restrict__ depths) {
+  __shared__ uint32_t s_stack[kPStack];
+  __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
+  __shared__ float s_pt[kFThreads], s_pu[kFThreads], s_pv[kFThreads];
+  __shared__ uint32_t s_pk[kFThreads];
+  const uint32_t nf = min(handoff[2], forced_cap);
+  const uint32_t jobs = nf * (uint32_t)kFPackets;
+  const uint32_t* state = handoff + kHandoffHead + handoff_cap;
+  const uint32_t max_visits = 2 * node_count + 2;
+  const int r = (int)threadIdx.x % kFRays, fp = (int)threadIdx.x / kFRays;
+  for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
+    const uint32_t f = job % nf, part = job / nf;  // part k of every packet before part k + 1 of any
+    const uint32_t slot = forced_list[f];
+    const uint32_t* rec = state + (size_t)slot * kPStateWords;
+    const uint32_t pk = handoff[kHandoffHead + slot];
+    const int bx = (int)(pk % (uint32_t)packets_x), by = (int)(pk / (uint32_t)packets_x);
+    const int q = (int)part * kFRays + r;  // the ray's place in the packet
+    const int x = bx * kPW + q % kPW, y = by * kPH + q / kPW;
+    const bool in_image = x < width && y < height;
+    const uint32_t l = in_image ? (uint32_t)(y * width + x) : 0u;
+    float4 orig = make_float4(0.f, 0.f, 0.f, 0.f), dir = orig;
+    float depth = 0.0f;
+    uint32_t fid = 0;
+    float2 coord = make_float2(0.f, 0.f);
+    if (in_image) {  // the hits so far, left in the outputs at the hand-off
+      orig = origins[l];
+      dir = dirs[l];
+      depth = depths[l];
+      fid = faces_id[l];
+      coord = coords[l];
+    }
+    const uint32_t R = rec[2];
+    const uint32_t lr = (uint32_t)((by * kPH + (int)R / kPW) * width + bx * kPW + (int)R % kPW);
+    const float4 orig_r = origins[lr], invdir_r = invdir_of(dirs[lr]);
+    const float4 dir0 = dirs[(uint32_t)(by * kPH) * (uint32_t)width + (uint32_t)(bx * kPW)];
+    int sp = (int)rec[0];
+    uint32_t visits = rec[1];
+    for (int k = (int)threadIdx.x; k < sp; k += kFThreads) s_stack[k] = rec[4 + k];
+    __syncthreads();
+    bool diverged = false;
+    while (sp > 0 && ++visits <= max_visits) {
+      const uint32_t nid = s_stack[--sp];
+      if (nid >= node_count) break;  // block-uniform; never taken for a tree built by this library
+      const NodeRegs nd = load_node(nodes, nid);
+      if (rk::node_is_leaf(nd.w0)) {
+        const uint32_t tri_id = nd.w1;
+        const uint32_t cnt = tri_id < face_count ? min(rk::leaf_count(nd.w0), face_count - tri_id) : 0u;
+        for (uint32_t c0 = 0; c0 < cnt; c0 += kPCache) {
+          const uint32_t cc = (cnt - c0) < (uint32_t)kPCache ? (cnt - c0) : (uint32_t)kPCache;
+          __syncthreads();
+          for (uint32_t k = threadIdx.x; k < cc; k += kFThreads) {
+            float4 v0, e1, e2;
+            tri_setup(faces, verts, tri_id + c0 + k, v0, e1, e2);
+            s_v0[k] = v0; s_e1[k] = e1; s_e2[k] = e2;
+          }
+          __syncthreads();
+          const uint32_t qn = (cc + kFParts - 1) / kFParts;
+          const uint32_t k0 = (uint32_t)fp * qn, k1 = min(cc, k0 + qn);
+          float pt = depth, pu = 0.0f, pv = 0.0f;
+          uint32_t pkf = kNone;
+          if (in_image) {
+            for (uint32_t k = k0; k < k1; ++k) {
+              float t, u, v;
+              if (tri_ray(orig, dir, s_v0[k], s_e1[k], s_e2[k], t, u, v)) {
+                if (t < pt) { pkf = k; pu = u; pv = v; pt = t; }
+              }
+            }
+          }
+          s_pt[threadIdx.x] = pt;
+          s_pu[threadIdx.x] = pu;
+          s_pv[threadIdx.x] = pv;
+          s_pk[threadIdx.x] = pkf;
+          __syncthreads();
+          if (in_image) {
+#pragma unroll
+            for (int p2 = 0; p2 < kFParts; ++p2) {
+              const int o = p2 * kFRays + r;
+              const uint32_t k = s_pk[o];
+              if (k != kNone && s_pt[o] < depth) {
+                depth = s_pt[o];
+                fid = tri_id + c0 + k;
+                coord = make_float2(s_pu[o], s_pv[o]);
+              }
+            }
+          }
+        }
+        __syncthreads();
+      } else {
+        float t;
+        if (!box_ray(nd.bmin, nd.bmax, orig_r, invdir_r, t)) {  // block-uniform
+          diverged = true;
+          break;
+        }
+        uint32_t nearc, farc;
+        children_order(nd.w0, nd.w1, dir0, nearc, farc);
+        if (sp + 2 > kPStack) break;  // block-uniform; cannot occur for trees of depth <= 126
+        __syncthreads();  // every thread has read the entry popped above
+        if (threadIdx.x == 0) { s_stack[sp] = farc; s_stack[sp + 1] = nearc; }
+        sp += 2;
+        __syncthreads();
+      }
+    }
+    if (in_image && fp == 0) temp[(size_t)f * kPT + (uint32_t)q] = make_float4(depth, coord.x, coord.y, __uint_as_float(fid));
+    if (threadIdx.x == 0) fflags[f * kFPackets + part] = diverged ? 1u : 0u;
+    __syncthreads();  // the next job rewrites the stack
+  }
+}
+
+// The forced walks' results, per packet: into the outputs (and the packet's
+// record marked kDone) when no part diverged and the forcing ray's final
+// depth is above 0.
+__global__ __launch_bounds__(kPT) void k_packet_commit(
+    int width, int height, int packets_x, uint32_t* __restrict__ handoff, uint32_t handoff_cap,
+    const uint32_t* __restrict__ forced_list, uint32_t forced_cap, const float4* __restrict__ temp,
+    const uint32_t* __restrict__ fflags, uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
+    float* __restrict__ depths) {
+  const uint32_t nf = min(handoff[2], forced_cap);
+  uint32_t* state = handoff + kHandoffHead + handoff_cap;
+  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
+    const uint32_t slot = forced_list[f];
+    uint32_t* rec = state + (size_t)slot * kPStateWords;
+    bool ok = temp[(size_t)f * kPT + rec[2]].x > 0.0f;
+    for (int p = 0; p < kFPackets; ++p) ok = ok && fflags[f * kFPackets + p] == 0u;
+    if (!ok) continue;  // block-uniform
+    const uint32_t pk = handoff[kHandoffHead + slot];
+    const int bx = (int)(pk % (uint32_t)packets_x), by = (int)(pk / (uint32_t)packets_x);
+    const int q = (int)threadIdx.x;
+    const int x = bx * kPW + q % kPW, y = by * kPH + q / kPW;
+    if (x < width && y < height) {
+      const uint32_t l = (uint32_t)(y * width + x);
+      const float4 v = temp[(size_t)f * kPT + (uint32_t)q];
+      depths[l] = v.x;
+      coords[l] = make_float2(v.y, v.z);
+      faces_id[l] = __float_as_uint(v.w);
+    }
+    __syncthreads();  // every thread has read the record before it is marked
+    if (threadIdx.x == 0) rec[3] = kDone;
   }
 }
 
@@ -1093,13 +1287,28 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     const uint32_t gx = (uint32_t)(width + kPW - 1) / kPW, gy = (uint32_t)(height + kPH - 1) / kPH;
     const uint32_t faces_limit = g_packet_faces.load(std::memory_order_relaxed);
     const uint32_t cap = std::min<uint32_t>(gx * gy, g_list_cap.load(std::memory_order_relaxed));
-    uint32_t* ho = handoff_alloc(st, kHandoffHead + (size_t)cap * (1u + kPStateWords), faces_limit);
+    const uint32_t fcap = std::min<uint32_t>(cap, kForcedCap);
+    // [head | list | records | forced list | forced flags (4 each) | forced results (192 float4 each)]
+    const size_t flist_off = kHandoffHead + (size_t)cap * (1u + kPStateWords);
+    const size_t temp_off = (flist_off + (1ull + kFPackets) * fcap + 3) & ~size_t(3);
+    uint32_t* ho = handoff_alloc(st, temp_off + 4ull * kPT * fcap, faces_limit);
+    uint32_t* flist = ho ? ho + flist_off : nullptr;
+    uint32_t* fflags = ho ? ho + flist_off + fcap : nullptr;
+    float4* temp = ho ? reinterpret_cast<float4*>(ho + temp_off) : nullptr;
     hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
                        height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths, ho,
-                       ho ? cap : 0u, faces_limit);
+                       ho ? cap : 0u, faces_limit, flist, fcap);
     if (ho) {
+      hipLaunchKernelGGL(k_packet_forced, dim3(std::min<uint32_t>(fcap * kFPackets, kForcedGrid)), dim3(kFThreads), 0,
+                         st, reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
+                         reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
+                         reinterpret_cast<const float4*>(directions), width, height, (int)gx, node_count, face_count,
+                         ho, cap, flist, fcap, temp, fflags, faces_id, reinterpret_cast<const float2*>(coords),
+                         depths);
+      hipLaunchKernelGGL(k_packet_commit, dim3(fcap), dim3(kPT), 0, st, width, height, (int)gx, ho, cap, flist, fcap,
+                         temp, fflags, faces_id, reinterpret_cast<float2*>(coords), depths);
       hipLaunchKernelGGL(k_trace_packet_split, dim3(std::min<uint32_t>(cap, kPacketGrid)), dim3(kPT * kPacketSplit),
                          0, st, reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                          reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),

@@ -264,7 +264,7 @@ def test_reference_trace_duplicate_faces(dev, rk, ora, depth, coherency, handoff
 
 @pytest.mark.parametrize("name,w,h,depth,coherency", [("hf128", 200, 150, 12, False), ("ico5", 128, 96, 12, False),
                                                       ("hf128", 200, 150, 31, False), ("hf160", 64, 48, 12, True),
-                                                      ("hf160", 64, 48, 31, True)])
+                                                      ("hf160", 64, 48, 31, True), ("ico5", 64, 48, 12, True)])
 def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth, coherency):
     """Rays with a direction component of exactly 0: the reference's normalised
     inverse direction is then (0, 0, NaN) up to order, every box test passes
@@ -276,10 +276,14 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
     assert F.shape[0] > 8192
     tree, ref = _compare_tree(dev, rk, ora, V, F, depth, True)
     O, D = _camera_rays(rk, name, w, h)
-    D = D.copy()
+    O, D = O.copy(), D.copy()
     n = w * h
     D[::5, 2] = 0.0
     D[1::11, 0] = 0.0
+    # some of them start at a vertex: a hit at t = 0 on a face there ends their
+    # walk's forcing of the packet's, and freezes their own result
+    sel = np.arange(0, n, 10)
+    O[sel, :3] = V[F[(sel * 7919) % F.shape[0], 0], :3]
     rng = np.random.default_rng(5)
     fid0 = rng.integers(0, 1000, n, dtype=np.uint32)
     uv0 = rng.random((n, 2), dtype=np.float32)
@@ -296,6 +300,7 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
         of, ouv, od, ovf = ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D, fid0, uv0, dep0)
     assert ovf == 0
     assert (od < dep0).sum() > n // 20
+    assert ((od == 0) & (dep0 > 0)).sum() > 0  # hits at t = 0
     np.testing.assert_array_equal(_h(out[0], np.uint32), of)
     np.testing.assert_array_equal(_h(out[1]), ouv)
     np.testing.assert_array_equal(_h(out[2]), od)
