@@ -225,7 +225,8 @@ rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32
  * traversal whose leaves pass ray_faces faces is traced again from the start by a wave of
  * its own; a packet whose leaves pass packet_faces faces is resumed where it stands with
  * five threads per ray. A ray with a direction component of 0 (every box test passes for
- * it, so it walks the whole tree), or a packet holding one, goes at its first leaf. The
+ * it, so it walks the whole tree), or a packet holding one, goes at its first leaf, to
+ * walks split by subtree or by parts of the packet on several CUs. The
  * results are the same. What is handed off goes to a per-call list of list_cap entries
  * (1 .. 65536, device memory from the stream's pool); past its end, rays and packets finish
  * in the first pass. UINT32_MAX turns a second pass off. Defaults 4096, 4096, 65536.
