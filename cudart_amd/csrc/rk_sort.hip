@@ -1,7 +1,7 @@
 // Stable LSD radix sort for gfx950, 8-bit digits, onesweep style:
 //   1. one histogram kernel counts the digits of every pass in a single read;
 //   2. one scatter kernel per pass. Each 512-thread workgroup sorts the
-//      8192-key tile of its block index: it ranks its keys stably with
+//      8192-key tile (4096 below 2^21 keys) of its block index: it ranks its keys stably with
 //      per-(wave, digit) lane masks in LDS, publishes its per-digit counts,
 //      resolves its global per-digit prefix by decoupled look-back over
 //      earlier tiles, reorders the tile by digit in LDS and writes it out in
@@ -29,11 +29,17 @@ namespace {
 #ifndef RK_SCATTER_ITEMS
 #define RK_SCATTER_ITEMS 16
 #endif
+#ifndef RK_SCATTER_ITEMS_SMALL
+#define RK_SCATTER_ITEMS_SMALL 8
+#endif
 constexpr int kWaves = RK_SCATTER_WAVES;    // waves per scatter workgroup
 constexpr int kThreads = 64 * kWaves;
 constexpr int kItems = RK_SCATTER_ITEMS;    // keys per lane
-constexpr int kWaveTile = 64 * kItems;
-constexpr int kTile = kThreads * kItems;    // keys per tile
+constexpr int kItemsSmall = RK_SCATTER_ITEMS_SMALL;  // keys per lane below kSmallTilesBelow keys
+// Below 2^21 keys the tiles are of 4096 keys: tiles of 8192 would leave most
+// of the 256 CUs idle (1M keys: 123 tiles). On the 1M-face heightfield build
+// 214 vs 224 us; on 10M keys the larger tiles are faster (1.73 vs 1.80 ms).
+constexpr size_t kSmallTilesBelow = size_t(1) << 21;
 static_assert(kThreads >= 256, "one thread per digit");
 constexpr uint32_t kAgg = 1u << 30;
 constexpr uint32_t kInc = 2u << 30;
@@ -114,25 +120,26 @@ __global__ __launch_bounds__(256) void k_rs_hist(const K* __restrict__ keys, uin
   }
 }
 
-// One pass over one tile of kTile keys. The tile's keys are ranked stably
+// One pass over one tile of kThreads * kIt keys. The tile's keys are ranked stably
 // per digit and moved to their place in the tile's digit order in LDS before
 // the look-back, so their registers are free while it waits; the values are
 // loaded during the look-back and follow through the same LDS buffer once the
-// keys are out, so a workgroup holds kTile keys of LDS, not keys and values.
-template <typename K, bool kVals>
+// keys are out, so a workgroup holds a tile of keys in LDS, not keys and values.
+template <typename K, bool kVals, int kIt>
 __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                          K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                          int shift, const uint32_t* __restrict__ hist,
                                                          uint32_t* __restrict__ status, uint32_t* __restrict__ err) {
+  constexpr int kTileT = kThreads * kIt, kWaveTileT = 64 * kIt;  // keys per tile and per wave
   __shared__ uint32_t s_cnt[kWaves][256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_tmp[8];
-  __shared__ K s_buf[kTile];
+  __shared__ K s_buf[kTileT];
 
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   // While ranking, s_buf holds one 64-bit lane mask per (wave, digit): the
   // lanes of the current 64 keys that carry that digit.
-  static_assert(sizeof(K) * kTile >= sizeof(uint64_t) * kWaves * 256, "digit masks fit in the tile buffer");
+  static_assert(sizeof(K) * kTileT >= sizeof(uint64_t) * kWaves * 256, "digit masks fit in the tile buffer");
   uint64_t* s_mask = reinterpret_cast<uint64_t*>(s_buf);
   for (int i = tid; i < kWaves * 256; i += kThreads) {
     (&s_cnt[0][0])[i] = 0;
@@ -140,13 +147,13 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
   }
   __syncthreads();
   const uint32_t tile = blockIdx.x;
-  const size_t base = (size_t)tile * kTile;
-  const size_t wbase = base + (size_t)wave * kWaveTile;
+  const size_t base = (size_t)tile * kTileT;
+  const size_t wbase = base + (size_t)wave * kWaveTileT;
 
-  K key[kItems];
-  uint32_t rank[kItems];
+  K key[kIt];
+  uint32_t rank[kIt];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
+  for (int j = 0; j < kIt; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
     key[j] = idx < n ? kin[idx] : (K)0;
   }
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
   const uint64_t lower_lanes = my_bit - 1;
   uint64_t* wmask = s_mask + wave * 256;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
+  for (int j = 0; j < kIt; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
     bool valid = idx < n;
     uint32_t d = (uint32_t)(key[j] >> shift) & 0xFFu;
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
+  for (int j = 0; j < kIt; ++j) {
     size_t idx = wbase + (size_t)j * 64 + lane;
     if (idx < n) {
       const uint32_t dd = (uint32_t)(key[j] >> shift) & 0xFFu;
@@ -209,10 +216,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
       s_buf[rank[j]] = key[j];
     }
   }
-  uint32_t val[kItems];
+  uint32_t val[kIt];
   if (kVals) {
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kIt; ++j) {
       size_t idx = wbase + (size_t)j * 64 + lane;
       val[j] = idx < n ? vin[idx] : 0u;
     }
@@ -262,10 +269,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
   }
   __syncthreads();
 
-  const uint32_t tile_n = (n - base) < (size_t)kTile ? (uint32_t)(n - base) : (uint32_t)kTile;
-  uint32_t opos[kItems];
+  const uint32_t tile_n = (n - base) < (size_t)kTileT ? (uint32_t)(n - base) : (uint32_t)kTileT;
+  uint32_t opos[kIt];
 #pragma unroll
-  for (int m = 0; m < kItems; ++m) {
+  for (int m = 0; m < kIt; ++m) {
     const uint32_t i = tid + (uint32_t)m * kThreads;
     opos[m] = 0xFFFFFFFFu;
     if (i < tile_n) {
@@ -282,13 +289,13 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const K* __restrict__ k
     uint32_t* s_val = reinterpret_cast<uint32_t*>(s_buf);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kIt; ++j) {
       size_t idx = wbase + (size_t)j * 64 + lane;
       if (idx < n) s_val[rank[j]] = val[j];
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < kItems; ++m) {
+    for (int m = 0; m < kIt; ++m) {
       const uint32_t i = tid + (uint32_t)m * kThreads;
       if (i < tile_n && opos[m] != 0xFFFFFFFFu) vout[opos[m]] = s_val[i];
     }
@@ -379,7 +386,11 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ v, uint32_t
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
 }
 
-inline uint32_t tiles_of(uint32_t n) { return (n + kTile - 1) / kTile; }
+inline int items_of(uint32_t n) { return (size_t)n < kSmallTilesBelow ? kItemsSmall : kItems; }
+inline uint32_t tiles_of(uint32_t n) {
+  const uint32_t tile = (uint32_t)(kThreads * items_of(n));
+  return (uint32_t)(((size_t)n + tile - 1) / tile);
+}
 // The error word of a sort called with this workspace: its first 4 bytes.
 inline uint32_t* ws_flags_word(void* ws, size_t ws_bytes) {
   return (ws && ws_bytes >= 4) ? static_cast<uint32_t*>(ws) : nullptr;
@@ -456,13 +467,23 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
   uint32_t* vd = valt;
   for (int p = 0; p < passes; ++p) {
     int shift = 8 * (p + first_pass);
+    const bool small = items_of(n) == kItemsSmall;
     if (vals) {
-      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(T), dim3(kThreads), 0, st, ks, vs, kd, vd, n, shift,
-                         hist + (size_t)p * 256, status + (size_t)p * T * 256, err);
+      if (small)
+        hipLaunchKernelGGL((k_rs_scatter<K, true, kItemsSmall>), dim3(T), dim3(kThreads), 0, st, ks, vs, kd, vd, n,
+                           shift, hist + (size_t)p * 256, status + (size_t)p * T * 256, err);
+      else
+        hipLaunchKernelGGL((k_rs_scatter<K, true, kItems>), dim3(T), dim3(kThreads), 0, st, ks, vs, kd, vd, n, shift,
+                           hist + (size_t)p * 256, status + (size_t)p * T * 256, err);
     } else {
-      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(T), dim3(kThreads), 0, st, ks, (const uint32_t*)nullptr, kd,
-                         (uint32_t*)nullptr, n, shift, hist + (size_t)p * 256,
-                         status + (size_t)p * T * 256, err);
+      if (small)
+        hipLaunchKernelGGL((k_rs_scatter<K, false, kItemsSmall>), dim3(T), dim3(kThreads), 0, st, ks,
+                           (const uint32_t*)nullptr, kd, (uint32_t*)nullptr, n, shift, hist + (size_t)p * 256,
+                           status + (size_t)p * T * 256, err);
+      else
+        hipLaunchKernelGGL((k_rs_scatter<K, false, kItems>), dim3(T), dim3(kThreads), 0, st, ks,
+                           (const uint32_t*)nullptr, kd, (uint32_t*)nullptr, n, shift, hist + (size_t)p * 256,
+                           status + (size_t)p * T * 256, err);
     }
     RK_CHECK_LAUNCH();
     std::swap(ks, kd);

@@ -107,7 +107,10 @@ __device__ __forceinline__ void children_order(uint32_t w0, uint32_t w1, float4 
 }
 
 constexpr int kStack = 96;  // STACK_SIZE, :45
-constexpr int kLdsStack = 32;  // entries per lane in LDS (8 KB per wave); the rest in scratch
+#ifndef RK_REF_LDS_STACK
+#define RK_REF_LDS_STACK 32
+#endif
+constexpr int kLdsStack = RK_REF_LDS_STACK;  // entries per lane in LDS (8 KB per wave); the rest in scratch
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
