@@ -78,10 +78,10 @@ TRACE_TRAFFIC_BYTES = (107_377 * 2 + 10_575) * 1024
 # k_fast_hierarchy (up to 2^21 faces); the soup's in k_fast_hierarchy_top.
 BUILD_KERNELS = {  # kernel: (us, HBM read KiB, HBM write KiB, launches per build)
     "heightfield": {
-        "k_fast_prep": (14.91, 32_282, 42_960, 1),
-        "k_rs_hist": (12.35, 3_991, 857, 1),
-        "k_rs_scatter": (18.84, 9_459, 9_218, 4),
-        "k_fast_hierarchy": (121.1, 52_260, 117_930, 1),
+        "k_fast_prep": (14.86, 32_290, 42_961, 1),
+        "k_rs_hist": (12.36, 3_992, 1_523, 1),
+        "k_rs_scatter": (16.51, 10_920, 10_406, 4),
+        "k_fast_hierarchy": (120.17, 52_280, 118_035, 1),
     },
     "soup": {
         "k_fast_prep": (208.4, 625_438, 468_776, 1),
