@@ -76,6 +76,10 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_t
 // with one LDS atomic instead of 64 to the same address.
 constexpr int kHistUnroll = 8;
 constexpr uint32_t kHistMaxBlocks = 1024;
+#ifndef RK_HIST_KEYS_PER_BLOCK
+#define RK_HIST_KEYS_PER_BLOCK (256 * kHistUnroll * 2)
+#endif
+constexpr size_t kHistKeysPerBlock = RK_HIST_KEYS_PER_BLOCK;
 
 // Also zeroes the look-back status words of the passes that follow (zero,
 // zero_words), which saves a fill launch per sort.
@@ -456,7 +460,7 @@ rk_status rk_radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, uint32_t n
     rk_status s = rk_fill_u32(reinterpret_cast<rk_stream>(st), ctl, 1 + (size_t)passes * 256, 0u);
     if (s != RK_SUCCESS) return s;
   }
-  const unsigned hist_blocks = grid_for(n, 256 * kHistUnroll * 2, kHistMaxBlocks);
+  const unsigned hist_blocks = grid_for(n, kHistKeysPerBlock, kHistMaxBlocks);
   const uint32_t chunk = (uint32_t)(((size_t)n + hist_blocks - 1) / hist_blocks);
   hipLaunchKernelGGL(k_rs_hist<K>, dim3(hist_blocks), dim3(256), 0, st, keys, n, passes, first_pass, chunk, hist,
                      status, (size_t)passes * T * 256);

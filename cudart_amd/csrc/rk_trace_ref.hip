@@ -30,6 +30,10 @@
 namespace {
 
 constexpr float kEps = 0.000001f;  // CUDA_raytraceMeshTree.cu:36
+constexpr float kUMargin = 0.001f;
+#ifndef RK_REF_U_PRETEST
+#define RK_REF_U_PRETEST 1
+#endif
 
 // intersectTriRay, CUDA_raytraceMeshTree.cu:57-89.
 __device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, float4 e1, float4 e2, float& t, float& u,
@@ -37,9 +41,19 @@ __device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, floa
   float4 pvec = rk::cross4(dir, e2);
   float det = rk::dot4(e1, pvec);
   if (det > -kEps && det < kEps) return false;
-  float inv_det = 1.0f / det;
   float4 tvec = rk::sub4(orig, v0);
-  u = rk::dot4(tvec, pvec) * inv_det;
+  const float un = rk::dot4(tvec, pvec);
+  // Most faces fail on u. Estimated with the hardware reciprocal (within
+  // 2^-21 of 1/det), a u below -kUMargin or above 1 + kUMargin means a u
+  // computed with the correctly rounded 1/det is outside [0, 1] too (with the
+  // same sign, also past the float range); NaN and every other case go on to
+  // the exact division. The result is the reference's in every case.
+#if RK_REF_U_PRETEST
+  const float ua = un * __builtin_amdgcn_rcpf(det);
+  if (ua < -kUMargin || ua > 1.0f + kUMargin) return false;
+#endif
+  float inv_det = 1.0f / det;
+  u = un * inv_det;
   if (u < 0.0f || u > 1.0f) return false;
   float4 qvec = rk::cross4(tvec, e1);
   v = rk::dot4(dir, qvec) * inv_det;
