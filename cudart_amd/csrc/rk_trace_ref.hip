@@ -100,6 +100,34 @@ __device__ __forceinline__ void tri_setup(const uint4* __restrict__ faces, const
   e2.w = 0.0f;
 }
 
+// A face's tri_setup terms, v0, e1 = v1 - v0 and e2 = v2 - v0 (w 0),
+// precomputed per call in face order (three float4 per face): one contiguous
+// load per face test instead of the face's indices, then its vertices.
+__global__ __launch_bounds__(256) void k_tri_records(const uint4* __restrict__ faces,
+                                                     const float4* __restrict__ verts, uint32_t face_count,
+                                                     float4* __restrict__ tris) {
+  const uint32_t f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= face_count) return;
+  float4 v0, e1, e2;
+  tri_setup(faces, verts, f, v0, e1, e2);
+  tris[3 * (size_t)f] = v0;
+  tris[3 * (size_t)f + 1] = e1;
+  tris[3 * (size_t)f + 2] = e2;
+}
+
+// tri_setup from the records when the call has them, else from the mesh.
+__device__ __forceinline__ void tri_get(const uint4* __restrict__ faces, const float4* __restrict__ verts,
+                                        const float4* __restrict__ tris, uint32_t f, float4& v0, float4& e1,
+                                        float4& e2) {
+  if (tris) {
+    v0 = tris[3 * (size_t)f];
+    e1 = tris[3 * (size_t)f + 1];
+    e2 = tris[3 * (size_t)f + 2];
+  } else {
+    tri_setup(faces, verts, f, v0, e1, e2);
+  }
+}
+
 struct NodeRegs { uint32_t w0, w1; float4 bmin, bmax; };
 
 __device__ __forceinline__ NodeRegs load_node(const rk::Node* __restrict__ nodes, uint32_t id) {
@@ -160,7 +188,8 @@ struct LeafBest { float t, u, v; uint32_t k; bool found; };
 // next round compares with that t.
 template <int kPer>
 __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, const float4* __restrict__ verts,
-                                              float4 o, float4 d, uint32_t first, uint32_t cnt, float dep) {
+                                              const float4* __restrict__ tris, float4 o, float4 d, uint32_t first,
+                                              uint32_t cnt, float dep) {
   const uint32_t lane = threadIdx.x & 63u;
   LeafBest r;
   r.t = dep;
@@ -173,17 +202,29 @@ __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, c
     const uint32_t m = min(cnt - c0, 64u * kPer);
     const uint32_t per = (m + 63u) >> 6;
     const uint32_t k0 = lane * per;
-    uint4 f[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j)
-      f[j] = ((uint32_t)j < per && k0 + j < m) ? faces[first + c0 + k0 + j] : make_uint4(0, 0, 0, 0);
     float4 p0[kPer], p1[kPer], p2[kPer];
+    if (tris) {  // v0, e1, e2
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      if ((uint32_t)j < per && k0 + j < m) {
-        p0[j] = verts[f[j].x];
-        p1[j] = verts[f[j].y];
-        p2[j] = verts[f[j].z];
+      for (int j = 0; j < kPer; ++j) {
+        if ((uint32_t)j < per && k0 + j < m) {
+          const size_t q = 3 * (size_t)(first + c0 + k0 + j);
+          p0[j] = tris[q];
+          p1[j] = tris[q + 1];
+          p2[j] = tris[q + 2];
+        }
+      }
+    } else {  // the vertices
+      uint4 f[kPer];
+#pragma unroll
+      for (int j = 0; j < kPer; ++j)
+        f[j] = ((uint32_t)j < per && k0 + j < m) ? faces[first + c0 + k0 + j] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        if ((uint32_t)j < per && k0 + j < m) {
+          p0[j] = verts[f[j].x];
+          p1[j] = verts[f[j].y];
+          p2[j] = verts[f[j].z];
+        }
       }
     }
     float bt = inf, bu = 0.0f, bv = 0.0f;
@@ -192,12 +233,14 @@ __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, c
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
       if ((uint32_t)j < per && k0 + j < m) {
-        // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
-        float4 v0 = p0[j];
-        float4 e1 = rk::sub4(p1[j], v0), e2 = rk::sub4(p2[j], v0);
-        v0.w = 0.0f;
-        e1.w = 0.0f;
-        e2.w = 0.0f;
+        float4 v0 = p0[j], e1 = p1[j], e2 = p2[j];
+        if (!tris) {  // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
+          e1 = rk::sub4(p1[j], v0);
+          e2 = rk::sub4(p2[j], v0);
+          v0.w = 0.0f;
+          e1.w = 0.0f;
+          e2.w = 0.0f;
+        }
         float t, u, v;
         if (tri_ray(o, d, v0, e1, e2, t, u, v) && t < r.t && t < bt) {
           bt = t;  // t < dep <= FLT_MAX or +inf: t is finite, below the sentinel
@@ -304,7 +347,8 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
                                                   uint32_t node_count, uint32_t face_count,
                                                   uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                   float* __restrict__ depths, uint32_t* __restrict__ handoff,
-                                                  uint32_t handoff_cap, uint32_t handoff_faces, UnitArgs ua) {
+                                                  uint32_t handoff_cap, uint32_t handoff_faces, UnitArgs ua,
+                                                  const float4* __restrict__ tris) {
   __shared__ uint32_t s_stack[kLdsStack * 64];
   const int lane = (int)threadIdx.x;
   const uint32_t l = blockIdx.x * 64 + (uint32_t)lane;
@@ -400,28 +444,42 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
       // each lane its own leaf, in face order
       if (at_leaf) {
         for (uint32_t k0 = 0; k0 < leaf_cnt; k0 += kLeafBatch) {
-          uint4 f[kLeafBatch];
-#pragma unroll
-          for (int b = 0; b < kLeafBatch; ++b)
-            f[b] = k0 + b < leaf_cnt ? faces[leaf_first + k0 + b] : make_uint4(0, 0, 0, 0);
           float4 p0[kLeafBatch], p1[kLeafBatch], p2[kLeafBatch];
+          if (tris) {  // v0, e1, e2
 #pragma unroll
-          for (int b = 0; b < kLeafBatch; ++b) {
-            if (k0 + b < leaf_cnt) {
-              p0[b] = verts[f[b].x];
-              p1[b] = verts[f[b].y];
-              p2[b] = verts[f[b].z];
+            for (int b = 0; b < kLeafBatch; ++b) {
+              if (k0 + b < leaf_cnt) {
+                const size_t q = 3 * (size_t)(leaf_first + k0 + b);
+                p0[b] = tris[q];
+                p1[b] = tris[q + 1];
+                p2[b] = tris[q + 2];
+              }
+            }
+          } else {  // the vertices
+            uint4 f[kLeafBatch];
+#pragma unroll
+            for (int b = 0; b < kLeafBatch; ++b)
+              f[b] = k0 + b < leaf_cnt ? faces[leaf_first + k0 + b] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < kLeafBatch; ++b) {
+              if (k0 + b < leaf_cnt) {
+                p0[b] = verts[f[b].x];
+                p1[b] = verts[f[b].y];
+                p2[b] = verts[f[b].z];
+              }
             }
           }
 #pragma unroll
           for (int b = 0; b < kLeafBatch; ++b) {
             if (k0 + b < leaf_cnt) {
-              // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
-              float4 v0 = p0[b];
-              float4 e1 = rk::sub4(p1[b], v0), e2 = rk::sub4(p2[b], v0);
-              v0.w = 0.0f;
-              e1.w = 0.0f;
-              e2.w = 0.0f;
+              float4 v0 = p0[b], e1 = p1[b], e2 = p2[b];
+              if (!tris) {  // tri_setup's arithmetic: v0, v1 - v0, v2 - v0, w cleared
+                e1 = rk::sub4(p1[b], v0);
+                e2 = rk::sub4(p2[b], v0);
+                v0.w = 0.0f;
+                e1.w = 0.0f;
+                e2.w = 0.0f;
+              }
               float t, u, v;
               if (tri_ray(orig, dir, v0, e1, e2, t, u, v)) {
                 if (t < depth) { fid = leaf_first + k0 + b; coord = make_float2(u, v); depth = t; }
@@ -436,7 +494,7 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
         const int L = __builtin_ctzll(pending);
         pending &= pending - 1;
         const uint32_t first = readlane_u32(leaf_first, L);
-        const LeafBest b = wave_leaf<1>(faces, verts, readlane_f4(orig, L), readlane_f4(dir, L), first,
+        const LeafBest b = wave_leaf<1>(faces, verts, tris, readlane_f4(orig, L), readlane_f4(dir, L), first,
                                      readlane_u32(leaf_cnt, L), readlane_f32(depth, L));
         if (lane == L && b.found) {
           depth = b.t;
@@ -531,7 +589,7 @@ __device__ __forceinline__ WalkResult wave_walk(uint32_t l, uint32_t root, uint3
                                                 const uint4* __restrict__ faces, const float4* __restrict__ verts,
                                                 const float4* __restrict__ origins,
                                                 const float4* __restrict__ dirs, uint32_t node_count,
-                                                uint32_t face_count) {
+                                                uint32_t face_count, const float4* __restrict__ tris) {
   const int lane = (int)(threadIdx.x & 63u);
   const uint32_t max_visits = 2 * node_count + 2;
   const float4 orig = origins[l], dir = dirs[l];
@@ -551,7 +609,7 @@ __device__ __forceinline__ WalkResult wave_walk(uint32_t l, uint32_t root, uint3
     if (rk::node_is_leaf(nd.w0)) {
       const uint32_t first = nd.w1;
       const uint32_t cnt = first < face_count ? min(rk::leaf_count(nd.w0), face_count - first) : 0u;
-      const LeafBest b = wave_leaf<kWavePer>(faces, verts, orig, dir, first, cnt, depth);
+      const LeafBest b = wave_leaf<kWavePer>(faces, verts, tris, orig, dir, first, cnt, depth);
       if (b.found) {
         depth = b.t;
         fid = first + b.k;
@@ -597,12 +655,13 @@ __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restric
                                                        const float4* __restrict__ dirs, uint32_t node_count,
                                                        uint32_t face_count, const uint32_t* __restrict__ handoff,
                                                        uint32_t handoff_cap, uint32_t* __restrict__ faces_id,
-                                                       float2* __restrict__ coords, float* __restrict__ depths) {
+                                                       float2* __restrict__ coords, float* __restrict__ depths,
+                                                       const float4* __restrict__ tris) {
   const uint32_t count = min(handoff[0], handoff_cap);
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t l = handoff[kHandoffHead + i];
     const WalkResult r = wave_walk(l, 0u, 0u, depths[l], faces_id[l], coords[l], nodes, faces, verts, origins,
-                                   dirs, node_count, face_count);
+                                   dirs, node_count, face_count, tris);
     if (threadIdx.x == 0) {
       faces_id[l] = r.fid;
       coords[l] = r.coord;
@@ -628,7 +687,8 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
                                                    const float4* __restrict__ origins,
                                                    const float4* __restrict__ dirs, const float* __restrict__ depths,
                                                    uint32_t node_count, uint32_t face_count,
-                                                   const uint32_t* __restrict__ handoff, UnitArgs ua) {
+                                                   const uint32_t* __restrict__ handoff, UnitArgs ua,
+                                                   const float4* __restrict__ tris) {
   __shared__ float4 s_v0[64], s_e1[64], s_e2[64];
   const int lane = (int)threadIdx.x;
   uint32_t bcount[8], groups = 0;
@@ -687,7 +747,7 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
           const uint32_t cc = min(64u, cnt - c0);
           if ((uint32_t)lane < cc) {
             float4 v0, e1, e2;
-            tri_setup(faces, verts, first + c0 + (uint32_t)lane, v0, e1, e2);
+            tri_get(faces, verts, tris, first + c0 + (uint32_t)lane, v0, e1, e2);
             s_v0[lane] = v0;
             s_e1[lane] = e1;
             s_e2[lane] = e2;
@@ -761,7 +821,8 @@ __global__ __launch_bounds__(64) void k_units_merge(const rk::Node* __restrict__
                                                     const float4* __restrict__ dirs, uint32_t node_count,
                                                     uint32_t face_count, const uint32_t* __restrict__ handoff,
                                                     UnitArgs ua, uint32_t* __restrict__ faces_id,
-                                                    float2* __restrict__ coords, float* __restrict__ depths) {
+                                                    float2* __restrict__ coords, float* __restrict__ depths,
+                                                    const float4* __restrict__ tris) {
   const uint32_t count = min(handoff[1], ua.cap);
   const uint64_t max_visits = 2 * node_count + 2;
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
@@ -783,7 +844,7 @@ __global__ __launch_bounds__(64) void k_units_merge(const rk::Node* __restrict__
     }
     if (pops > max_visits || ua.diverged[i] != 0u) {
       const WalkResult w = wave_walk(l, 0u, 0u, depths[l], faces_id[l], coords[l], nodes, faces, verts, origins,
-                                     dirs, node_count, face_count);
+                                     dirs, node_count, face_count, tris);
       depth = w.depth;
       fid = w.fid;
       coord = w.coord;
@@ -838,8 +899,9 @@ __device__ __forceinline__ bool trace_packet_job(
     uint32_t face_count, uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
     int bx, int by, uint32_t* __restrict__ handoff, uint32_t handoff_cap, uint32_t handoff_faces,
     uint32_t* __restrict__ state, uint32_t* __restrict__ forced_list, uint32_t forced_cap,
-    const uint32_t* __restrict__ resume, uint32_t* s_stack, float4* s_v0, float4* s_e1, float4* s_e2, float4* s_dir0,
-    uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv, uint32_t* s_pk) {
+    const uint32_t* __restrict__ resume, const float4* __restrict__ tris, uint32_t* s_stack, float4* s_v0,
+    float4* s_e1, float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv,
+    uint32_t* s_pk) {
   const int ray = (int)threadIdx.x % kPT, part = (int)threadIdx.x / kPT;
   const int tx = ray % kPW, ty = ray / kPW;
   const int x = bx * kPW + tx, y = by * kPH + ty;
@@ -932,7 +994,7 @@ __device__ __forceinline__ bool trace_packet_job(
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < cc; k += kPT * kS) {
           float4 v0, e1, e2;
-          tri_setup(faces, verts, tri_id + c0 + k, v0, e1, e2);
+          tri_get(faces, verts, tris, tri_id + c0 + k, v0, e1, e2);
           s_v0[k] = v0; s_e1[k] = e1; s_e2[k] = e2;
         }
         __syncthreads();
@@ -1013,15 +1075,16 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
                                                       uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                       float* __restrict__ depths, uint32_t* __restrict__ handoff,
                                                       uint32_t handoff_cap, uint32_t handoff_faces,
-                                                      uint32_t* __restrict__ forced_list, uint32_t forced_cap) {
+                                                      uint32_t* __restrict__ forced_list, uint32_t forced_cap,
+                                                      const float4* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float4 s_dir0;
   __shared__ uint32_t s_slot;
   trace_packet_job<1>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count, faces_id, coords,
                       depths, (int)blockIdx.x, (int)blockIdx.y, handoff, handoff_cap, handoff_faces,
-                      handoff + kHandoffHead + handoff_cap, forced_list, forced_cap, nullptr, s_stack, s_v0, s_e1,
-                      s_e2, &s_dir0, &s_slot, nullptr, nullptr, nullptr, nullptr);
+                      handoff + kHandoffHead + handoff_cap, forced_list, forced_cap, nullptr, tris, s_stack, s_v0,
+                      s_e1, s_e2, &s_dir0, &s_slot, nullptr, nullptr, nullptr, nullptr);
 }
 
 // The second pass: the packets handed off, each resumed from its record,
@@ -1035,7 +1098,8 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
     const rk::Node* __restrict__ nodes, const uint4* __restrict__ faces, const float4* __restrict__ verts,
     const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, int packets_x,
     uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
-    uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths) {
+    uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
+    const float4* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float4 s_dir0;
@@ -1050,7 +1114,7 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
     trace_packet_job<kPacketSplit>(nodes, faces, verts, origins, dirs, width, height, node_count, face_count,
                                    faces_id, coords, depths, (int)(pk % (uint32_t)packets_x),
                                    (int)(pk / (uint32_t)packets_x), nullptr, 0u, 0u, nullptr, nullptr, 0u, rec,
-                                   s_stack, s_v0, s_e1, s_e2, &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
+                                   tris, s_stack, s_v0, s_e1, s_e2, &s_dir0, &s_slot, s_pt, s_pu, s_pv, s_pk);
     __syncthreads();  // the next packet rewrites the stack and s_dir0
   }
 }
@@ -1083,7 +1147,7 @@ __global__ __launch_bounds__(kFThreads) void k_packet_forced(
     uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
     const uint32_t* __restrict__ forced_list, uint32_t forced_cap, float4* __restrict__ temp,
     uint32_t* __restrict__ fflags, const uint32_t* __restrict__ faces_id, const float2* __restrict__ coords,
-    const float* __restrict__ depths) {
+    const float* __restrict__ depths, const float4* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float s_pt[kFThreads], s_pu[kFThreads], s_pv[kFThreads];
@@ -1135,7 +1199,7 @@ __global__ __launch_bounds__(kFThreads) void k_packet_forced(
           __syncthreads();
           for (uint32_t k = threadIdx.x; k < cc; k += kFThreads) {
             float4 v0, e1, e2;
-            tri_setup(faces, verts, tri_id + c0 + k, v0, e1, e2);
+            tri_get(faces, verts, tris, tri_id + c0 + k, v0, e1, e2);
             s_v0[k] = v0; s_e1[k] = e1; s_e2[k] = e2;
           }
           __syncthreads();
@@ -1286,6 +1350,27 @@ static uint32_t* handoff_alloc(hipStream_t st, size_t words, uint32_t faces) {
   return ho;
 }
 
+#ifndef RK_REF_TRI_RECORDS
+#define RK_REF_TRI_RECORDS 1
+#endif
+
+// The call's triangle records (k_tri_records), from the same pool, or nullptr:
+// the kernels then read the faces and vertices.
+static float4* tri_records_alloc(hipStream_t st, const uint4* faces, const float4* verts, uint32_t face_count) {
+  if (!RK_REF_TRI_RECORDS) return nullptr;
+  float4* tris = nullptr;
+  hipMemPool_t pool = handoff_pool();
+  const size_t bytes = (size_t)face_count * 3 * sizeof(float4);
+  const hipError_t e = pool ? hipMallocFromPoolAsync(reinterpret_cast<void**>(&tris), bytes, pool, st)
+                            : hipMallocAsync(reinterpret_cast<void**>(&tris), bytes, st);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  hipLaunchKernelGGL(k_tri_records, dim3((face_count + 255) / 256), dim3(256), 0, st, faces, verts, face_count, tris);
+  return tris;
+}
+
 extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32_t node_count,
                                            const rk_uint4* faces, uint32_t face_count, const rk_float4* vertices,
                                            const rk_float4* origins,
@@ -1312,27 +1397,30 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     uint32_t* flist = ho ? ho + flist_off : nullptr;
     uint32_t* fflags = ho ? ho + flist_off + fcap : nullptr;
     float4* temp = ho ? reinterpret_cast<float4*>(ho + temp_off) : nullptr;
+    float4* tris = tri_records_alloc(st, reinterpret_cast<const uint4*>(faces),
+                                     reinterpret_cast<const float4*>(vertices), face_count);
     hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
                        reinterpret_cast<const float4*>(origins), reinterpret_cast<const float4*>(directions), width,
                        height, node_count, face_count, faces_id, reinterpret_cast<float2*>(coords), depths, ho,
-                       ho ? cap : 0u, faces_limit, flist, fcap);
+                       ho ? cap : 0u, faces_limit, flist, fcap, tris);
     if (ho) {
       hipLaunchKernelGGL(k_packet_forced, dim3(std::min<uint32_t>(fcap * kFPackets, kForcedGrid)), dim3(kFThreads), 0,
                          st, reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                          reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                          reinterpret_cast<const float4*>(directions), width, height, (int)gx, node_count, face_count,
                          ho, cap, flist, fcap, temp, fflags, faces_id, reinterpret_cast<const float2*>(coords),
-                         depths);
+                         depths, tris);
       hipLaunchKernelGGL(k_packet_commit, dim3(fcap), dim3(kPT), 0, st, width, height, (int)gx, ho, cap, flist, fcap,
                          temp, fflags, faces_id, reinterpret_cast<float2*>(coords), depths);
       hipLaunchKernelGGL(k_trace_packet_split, dim3(std::min<uint32_t>(cap, kPacketGrid)), dim3(kPT * kPacketSplit),
                          0, st, reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
                          reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
                          reinterpret_cast<const float4*>(directions), width, height, (int)gx, node_count, face_count,
-                         ho, cap, faces_id, reinterpret_cast<float2*>(coords), depths);
+                         ho, cap, faces_id, reinterpret_cast<float2*>(coords), depths, tris);
       if (hipFreeAsync(ho, st) != hipSuccess) return RK_ERROR_DEVICE;
     }
+    if (tris && hipFreeAsync(tris, st) != hipSuccess) return RK_ERROR_DEVICE;
   } else {
     // A ray whose leaves pass g_ray_faces faces, or a ray that passes every
     // box test (at its first leaf), is traced again by the second pass, one
@@ -1363,24 +1451,24 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     const float4* org = reinterpret_cast<const float4*>(origins);
     const float4* drs = reinterpret_cast<const float4*>(directions);
     float2* uv = reinterpret_cast<float2*>(coords);
+    float4* tris = tri_records_alloc(st, fc, vt, face_count);
     hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st, nd, fc, vt, org,
-                       drs, n, node_count, face_count, faces_id, uv, depths, ho, ho ? cap : 0u, faces_limit, ua);
+                       drs, n, node_count, face_count, faces_id, uv, depths, ho, ho ? cap : 0u, faces_limit, ua,
+                       tris);
     if (ho && ucap != 0u) {
       hipLaunchKernelGGL(k_units_split, dim3((ucap + 63) / 64), dim3(64), 0, st, nd, org, drs, depths, node_count,
                          ho, ua);
       hipLaunchKernelGGL(k_units_walk, dim3(std::min<uint32_t>(ucap * kUnits, kUnitGrid)), dim3(64), 0, st, nd, fc,
-                         vt, org, drs, depths, node_count, face_count, ho, ua);
+                         vt, org, drs, depths, node_count, face_count, ho, ua, tris);
       hipLaunchKernelGGL(k_units_merge, dim3(std::min<uint32_t>(ucap, kWaveGrid)), dim3(64), 0, st, nd, fc, vt, org,
-                         drs, node_count, face_count, ho, ua, faces_id, uv, depths);
+                         drs, node_count, face_count, ho, ua, faces_id, uv, depths, tris);
     }
     if (ho) {
-      hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kRayWaveGrid)), dim3(64), 0, st,
-                         reinterpret_cast<const rk::Node*>(nodes), reinterpret_cast<const uint4*>(faces),
-                         reinterpret_cast<const float4*>(vertices), reinterpret_cast<const float4*>(origins),
-                         reinterpret_cast<const float4*>(directions), node_count, face_count, ho, cap, faces_id,
-                         reinterpret_cast<float2*>(coords), depths);
+      hipLaunchKernelGGL(k_trace_ray_wave, dim3(std::min<uint32_t>(cap, kRayWaveGrid)), dim3(64), 0, st, nd, fc, vt,
+                         org, drs, node_count, face_count, ho, cap, faces_id, uv, depths, tris);
       if (hipFreeAsync(ho, st) != hipSuccess) return RK_ERROR_DEVICE;
     }
+    if (tris && hipFreeAsync(tris, st) != hipSuccess) return RK_ERROR_DEVICE;
   }
   RK_CHECK_LAUNCH();
   return RK_SUCCESS;
