@@ -55,6 +55,25 @@ def test_status_strings_and_errors(rk):
     assert L.rk_bubble_sort512(None, fake, fake, 513, 0) == 1
 
 
+def test_reference_handoff_limits(rk):
+    """rk_raytrace_set_handoff / get_handoff: defaults, bounds of the list size,
+    NULL outputs; host-only."""
+    import ctypes as C
+    L = rk.lib()
+    a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    get = lambda: (L.rk_raytrace_get_handoff(C.byref(a), C.byref(b), C.byref(c)), a.value, b.value, c.value)
+    assert get() == (0, 4096, 4096, 65536)
+    assert L.rk_raytrace_set_handoff(1, 2, 0) == 1
+    assert L.rk_raytrace_set_handoff(1, 2, 65537) == 1
+    assert get() == (0, 4096, 4096, 65536)  # a refused call changes nothing
+    assert L.rk_raytrace_get_handoff(None, C.byref(b), C.byref(c)) == 1
+    try:
+        assert L.rk_raytrace_set_handoff(0xFFFFFFFF, 7, 1) == 0
+        assert get() == (0, 0xFFFFFFFF, 7, 1)
+    finally:
+        assert L.rk_raytrace_set_handoff(4096, 4096, 65536) == 0
+
+
 def test_renderkit_exports_the_reference_sort_prototypes(rk):
     """The C++ free functions as the reference's callers declare them
     (samples/09_how_to_use_a_sorter.cpp:36-44, CUDA_radixSortGeneral.cu:27-30),

@@ -306,6 +306,34 @@ def test_reference_trace_rays_crossing_every_box(dev, rk, ora, name, w, h, depth
     np.testing.assert_array_equal(_h(out[2]), od)
 
 
+def test_reference_trace_two_streams(dev, rk, ora):
+    """Both traversals on two streams at once, three times over: each call has
+    its own hand-off buffers and triangle records (stream-ordered), so both
+    equal the oracle."""
+    import torch
+    V, F = _scene(rk, "hf128")
+    tree, ref = _compare_tree(dev, rk, ora, V, F, 12, True)
+    w, h = 200, 150
+    O, D = _camera_rays(rk, "hf128", w, h)
+    D = D.copy()
+    D[::5, 2] = 0.0  # rays that pass every box test: the split path
+    n = w * h
+    Vt, Ot, Dt = _t(V), _t(O), _t(D)
+    streams = (torch.cuda.Stream(), torch.cuda.Stream())
+    outs = [dev.new_outputs(n) for _ in range(6)]
+    torch.cuda.synchronize()
+    for k in range(6):
+        dev.trace_ref(tree, Vt, Ot, Dt, w, h, k % 2 == 1, out=outs[k], stream=streams[k % 2])
+    torch.cuda.synchronize()
+    expect = {False: ora.trace_ref_ray(ref.nodes, ref.faces, V, O, D), True: ora.trace_ref_packet(ref.nodes, ref.faces,
+                                                                                                  V, O, D, w, h)}
+    for k in range(6):
+        of, ouv, od, _ = expect[k % 2 == 1]
+        np.testing.assert_array_equal(_h(outs[k][0], np.uint32), of)
+        np.testing.assert_array_equal(_h(outs[k][1]), ouv)
+        np.testing.assert_array_equal(_h(outs[k][2]), od)
+
+
 def _chain_tree(V, levels, per_leaf):
     """A tree of `levels` inner nodes in a chain (split axis x, the mesh's box):
     each has a leaf of per_leaf faces and the next inner node (a leaf after the
