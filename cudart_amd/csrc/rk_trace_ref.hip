@@ -1294,8 +1294,20 @@ __global__ __launch_bounds__(kPT) void k_packet_commit(
 #endif
 constexpr uint32_t kPacketGrid = 1024;  // blocks of the packet traversal's second pass
 
-static std::atomic<uint32_t> g_ray_faces{RK_REF_HANDOFF_FACES};
-static std::atomic<uint32_t> g_packet_faces{RK_REF_PACKET_HANDOFF_FACES};
+static std::atomic<uint32_t> g_ray_faces{RK_HANDOFF_AUTO};
+static std::atomic<uint32_t> g_packet_faces{RK_HANDOFF_AUTO};
+// RK_HANDOFF_AUTO: below 2^19 rays the first pass holds fewer waves than the
+// GPU runs at once (about 5,000), and rays handed off early to a wave of their
+// own finish sooner. On the 256x256 icosphere (65,536 rays) the per-ray frame
+// took 0.85 ms past 256 faces against 1.69 ms past 4096, the packet frame
+// 2.05 ms past 512 against 2.29; on the 1920x1080 heightfield 4096 is best
+// (per-ray 8.1 ms against 11.1-11.5 ms past 256-2048; packets flat).
+constexpr uint32_t kAutoRaysBelow = 1u << 19;
+constexpr uint32_t kAutoSmallRayFaces = 256, kAutoSmallPacketFaces = 512;
+static uint32_t limit_of(uint32_t set, uint32_t n_rays, uint32_t small_limit, uint32_t large_limit) {
+  if (set != RK_HANDOFF_AUTO) return set;
+  return n_rays < kAutoRaysBelow ? small_limit : large_limit;
+}
 static std::atomic<uint32_t> g_list_cap{kHandoffCap};
 constexpr uint32_t kNoHandoff = 0xffffffffu;
 
@@ -1387,7 +1399,8 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     // its rays test every face; at its first leaf), is traced on by the second
     // pass, kPacketSplit threads per ray, from where it was left.
     const uint32_t gx = (uint32_t)(width + kPW - 1) / kPW, gy = (uint32_t)(height + kPH - 1) / kPH;
-    const uint32_t faces_limit = g_packet_faces.load(std::memory_order_relaxed);
+    const uint32_t faces_limit = limit_of(g_packet_faces.load(std::memory_order_relaxed), n, kAutoSmallPacketFaces,
+                                          RK_REF_PACKET_HANDOFF_FACES);
     const uint32_t cap = std::min<uint32_t>(gx * gy, g_list_cap.load(std::memory_order_relaxed));
     const uint32_t fcap = std::min<uint32_t>(cap, kForcedCap);
     // [head | list | records | forced list | forced flags (4 each) | forced results (192 float4 each)]
@@ -1425,7 +1438,8 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     // A ray whose leaves pass g_ray_faces faces, or a ray that passes every
     // box test (at its first leaf), is traced again by the second pass, one
     // wave per ray.
-    const uint32_t faces_limit = g_ray_faces.load(std::memory_order_relaxed);
+    const uint32_t faces_limit = limit_of(g_ray_faces.load(std::memory_order_relaxed), n, kAutoSmallRayFaces,
+                                          RK_REF_HANDOFF_FACES);
     const uint32_t cap = std::min<uint32_t>(n, g_list_cap.load(std::memory_order_relaxed));
     const uint32_t ucap = std::min<uint32_t>(cap, kCrossCap);
     // [head | list | split rays: list, nunits, top_pops, diverged, buckets (8) | unit_node, unit_sp |

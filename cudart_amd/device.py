@@ -205,7 +205,8 @@ def trace_ref(tree: RefTree, V: torch.Tensor, O: torch.Tensor, D: torch.Tensor, 
         return fid, uv, dep
 
 
-def set_ref_handoff(ray_faces: int = 4096, packet_faces: int = 4096, list_cap: int = 65536) -> None:
+def set_ref_handoff(ray_faces: int = N.RK_HANDOFF_AUTO, packet_faces: int = N.RK_HANDOFF_AUTO,
+                    list_cap: int = 65536) -> None:
     """Hand-off limits of trace_ref's second passes (process-wide; 0xFFFFFFFF turns one off)."""
     _check(N.lib().rk_raytrace_set_handoff(ray_faces, packet_faces, list_cap), "raytrace_set_handoff")
 

@@ -16,6 +16,7 @@ LIB_PATH = LIB_DIR / "librk.so"
 
 RK_SUCCESS = 0
 RK_MORTON_REF32, RK_MORTON_30, RK_MORTON_63 = 0, 1, 2
+RK_HANDOFF_AUTO = 0xFFFFFFFE  # rk_raytrace_set_handoff: the limit chosen per call
 RK_INFO_NODE_COUNT, RK_INFO_REAL_DEPTH, RK_INFO_FLAGS, RK_INFO_SPLITS, RK_INFO_LEAVES = 0, 1, 2, 3, 4
 RK_INFO_WORDS = 8
 RK_FLAG_WALK_PAST_END, RK_FLAG_WALK_PAST_DEPTH, RK_FLAG_MERGE2X_OOB, RK_FLAG_NODES_GT_N = 1, 2, 4, 8

@@ -229,8 +229,11 @@ rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* nodes, uint32
  * walks split by subtree or by parts of the packet on several CUs. The
  * results are the same. What is handed off goes to a per-call list of list_cap entries
  * (1 .. 65536, device memory from the stream's pool); past its end, rays and packets finish
- * in the first pass. UINT32_MAX turns a second pass off. Defaults 4096, 4096, 65536.
- * Process-wide, read by each call. */
+ * in the first pass. UINT32_MAX turns a second pass off. RK_HANDOFF_AUTO (the default for
+ * both limits) takes 256 faces per ray and 512 per packet when the call has fewer than 2^19
+ * rays (its first pass then holds too few waves to fill the GPU), else 4096 for both. The
+ * default list size is 65536. Process-wide, read by each call. */
+#define RK_HANDOFF_AUTO 0xFFFFFFFEu
 rk_status rk_raytrace_set_handoff(uint32_t ray_faces, uint32_t packet_faces, uint32_t list_cap);
 rk_status rk_raytrace_get_handoff(uint32_t* ray_faces, uint32_t* packet_faces, uint32_t* list_cap);
 

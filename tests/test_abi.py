@@ -62,16 +62,16 @@ def test_reference_handoff_limits(rk):
     L = rk.lib()
     a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
     get = lambda: (L.rk_raytrace_get_handoff(C.byref(a), C.byref(b), C.byref(c)), a.value, b.value, c.value)
-    assert get() == (0, 4096, 4096, 65536)
+    assert get() == (0, 0xFFFFFFFE, 0xFFFFFFFE, 65536)
     assert L.rk_raytrace_set_handoff(1, 2, 0) == 1
     assert L.rk_raytrace_set_handoff(1, 2, 65537) == 1
-    assert get() == (0, 4096, 4096, 65536)  # a refused call changes nothing
+    assert get() == (0, 0xFFFFFFFE, 0xFFFFFFFE, 65536)  # a refused call changes nothing
     assert L.rk_raytrace_get_handoff(None, C.byref(b), C.byref(c)) == 1
     try:
         assert L.rk_raytrace_set_handoff(0xFFFFFFFF, 7, 1) == 0
         assert get() == (0, 0xFFFFFFFF, 7, 1)
     finally:
-        assert L.rk_raytrace_set_handoff(4096, 4096, 65536) == 0
+        assert L.rk_raytrace_set_handoff(0xFFFFFFFE, 0xFFFFFFFE, 65536) == 0
 
 
 def test_renderkit_exports_the_reference_sort_prototypes(rk):

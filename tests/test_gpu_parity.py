@@ -190,7 +190,7 @@ def handoff(request, dev):
 
 
 def test_reference_handoff_limits(dev):
-    assert dev.get_ref_handoff() == (4096, 4096, 65536)
+    assert dev.get_ref_handoff() == (0xFFFFFFFE, 0xFFFFFFFE, 65536)
     dev.set_ref_handoff(1, 2, 3)
     assert dev.get_ref_handoff() == (1, 2, 3)
     with pytest.raises(Exception):
@@ -198,7 +198,7 @@ def test_reference_handoff_limits(dev):
     with pytest.raises(Exception):
         dev.set_ref_handoff(1, 2, 65537)
     dev.set_ref_handoff()
-    assert dev.get_ref_handoff() == (4096, 4096, 65536)
+    assert dev.get_ref_handoff() == (0xFFFFFFFE, 0xFFFFFFFE, 65536)
 
 
 @pytest.mark.parametrize("name,w,h", [("ico5", 256, 256), ("hf128", 200, 150), ("ico3", 33, 29),
