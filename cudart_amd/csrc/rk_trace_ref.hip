@@ -101,28 +101,38 @@ __device__ __forceinline__ void tri_setup(const uint4* __restrict__ faces, const
 }
 
 // A face's tri_setup terms, v0, e1 = v1 - v0 and e2 = v2 - v0 (w 0),
-// precomputed per call in face order (three float4 per face): one contiguous
-// load per face test instead of the face's indices, then its vertices.
+// precomputed per call in face order (nine floats, 36 B per face): one
+// contiguous load per face test instead of the face's indices, then its
+// vertices.
+constexpr size_t kRecFloats = 9;
 __global__ __launch_bounds__(256) void k_tri_records(const uint4* __restrict__ faces,
                                                      const float4* __restrict__ verts, uint32_t face_count,
-                                                     float4* __restrict__ tris) {
+                                                     float* __restrict__ tris) {
   const uint32_t f = blockIdx.x * 256 + threadIdx.x;
   if (f >= face_count) return;
   float4 v0, e1, e2;
   tri_setup(faces, verts, f, v0, e1, e2);
-  tris[3 * (size_t)f] = v0;
-  tris[3 * (size_t)f + 1] = e1;
-  tris[3 * (size_t)f + 2] = e2;
+  float3* r = reinterpret_cast<float3*>(tris + kRecFloats * (size_t)f);
+  r[0] = make_float3(v0.x, v0.y, v0.z);
+  r[1] = make_float3(e1.x, e1.y, e1.z);
+  r[2] = make_float3(e2.x, e2.y, e2.z);
+}
+
+// One of a record's three vectors, w 0 as tri_setup leaves it.
+__device__ __forceinline__ float4 rec_vec(const float* __restrict__ r) {
+  const float3 v = *reinterpret_cast<const float3*>(r);
+  return make_float4(v.x, v.y, v.z, 0.0f);
 }
 
 // tri_setup from the records when the call has them, else from the mesh.
 __device__ __forceinline__ void tri_get(const uint4* __restrict__ faces, const float4* __restrict__ verts,
-                                        const float4* __restrict__ tris, uint32_t f, float4& v0, float4& e1,
+                                        const float* __restrict__ tris, uint32_t f, float4& v0, float4& e1,
                                         float4& e2) {
   if (tris) {
-    v0 = tris[3 * (size_t)f];
-    e1 = tris[3 * (size_t)f + 1];
-    e2 = tris[3 * (size_t)f + 2];
+    const float* r = tris + kRecFloats * (size_t)f;
+    v0 = rec_vec(r);
+    e1 = rec_vec(r + 3);
+    e2 = rec_vec(r + 6);
   } else {
     tri_setup(faces, verts, f, v0, e1, e2);
   }
@@ -188,7 +198,7 @@ struct LeafBest { float t, u, v; uint32_t k; bool found; };
 // next round compares with that t.
 template <int kPer>
 __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, const float4* __restrict__ verts,
-                                              const float4* __restrict__ tris, float4 o, float4 d, uint32_t first,
+                                              const float* __restrict__ tris, float4 o, float4 d, uint32_t first,
                                               uint32_t cnt, float dep) {
   const uint32_t lane = threadIdx.x & 63u;
   LeafBest r;
@@ -207,10 +217,10 @@ __device__ __forceinline__ LeafBest wave_leaf(const uint4* __restrict__ faces, c
 #pragma unroll
       for (int j = 0; j < kPer; ++j) {
         if ((uint32_t)j < per && k0 + j < m) {
-          const size_t q = 3 * (size_t)(first + c0 + k0 + j);
-          p0[j] = tris[q];
-          p1[j] = tris[q + 1];
-          p2[j] = tris[q + 2];
+          const float* q = tris + kRecFloats * (size_t)(first + c0 + k0 + j);
+          p0[j] = rec_vec(q);
+          p1[j] = rec_vec(q + 3);
+          p2[j] = rec_vec(q + 6);
         }
       }
     } else {  // the vertices
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
                                                   uint32_t* __restrict__ faces_id, float2* __restrict__ coords,
                                                   float* __restrict__ depths, uint32_t* __restrict__ handoff,
                                                   uint32_t handoff_cap, uint32_t handoff_faces, UnitArgs ua,
-                                                  const float4* __restrict__ tris) {
+                                                  const float* __restrict__ tris) {
   __shared__ uint32_t s_stack[kLdsStack * 64];
   const int lane = (int)threadIdx.x;
   const uint32_t l = blockIdx.x * 64 + (uint32_t)lane;
@@ -449,10 +459,10 @@ __global__ __launch_bounds__(64) void k_trace_ray(const rk::Node* __restrict__ n
 #pragma unroll
             for (int b = 0; b < kLeafBatch; ++b) {
               if (k0 + b < leaf_cnt) {
-                const size_t q = 3 * (size_t)(leaf_first + k0 + b);
-                p0[b] = tris[q];
-                p1[b] = tris[q + 1];
-                p2[b] = tris[q + 2];
+                const float* q = tris + kRecFloats * (size_t)(leaf_first + k0 + b);
+                p0[b] = rec_vec(q);
+                p1[b] = rec_vec(q + 3);
+                p2[b] = rec_vec(q + 6);
               }
             }
           } else {  // the vertices
@@ -589,7 +599,7 @@ __device__ __forceinline__ WalkResult wave_walk(uint32_t l, uint32_t root, uint3
                                                 const uint4* __restrict__ faces, const float4* __restrict__ verts,
                                                 const float4* __restrict__ origins,
                                                 const float4* __restrict__ dirs, uint32_t node_count,
-                                                uint32_t face_count, const float4* __restrict__ tris) {
+                                                uint32_t face_count, const float* __restrict__ tris) {
   const int lane = (int)(threadIdx.x & 63u);
   const uint32_t max_visits = 2 * node_count + 2;
   const float4 orig = origins[l], dir = dirs[l];
@@ -656,7 +666,7 @@ __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restric
                                                        uint32_t face_count, const uint32_t* __restrict__ handoff,
                                                        uint32_t handoff_cap, uint32_t* __restrict__ faces_id,
                                                        float2* __restrict__ coords, float* __restrict__ depths,
-                                                       const float4* __restrict__ tris) {
+                                                       const float* __restrict__ tris) {
   const uint32_t count = min(handoff[0], handoff_cap);
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t l = handoff[kHandoffHead + i];
@@ -688,7 +698,7 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
                                                    const float4* __restrict__ dirs, const float* __restrict__ depths,
                                                    uint32_t node_count, uint32_t face_count,
                                                    const uint32_t* __restrict__ handoff, UnitArgs ua,
-                                                   const float4* __restrict__ tris) {
+                                                   const float* __restrict__ tris) {
   __shared__ float4 s_v0[64], s_e1[64], s_e2[64];
   const int lane = (int)threadIdx.x;
   uint32_t bcount[8], groups = 0;
@@ -822,7 +832,7 @@ __global__ __launch_bounds__(64) void k_units_merge(const rk::Node* __restrict__
                                                     uint32_t face_count, const uint32_t* __restrict__ handoff,
                                                     UnitArgs ua, uint32_t* __restrict__ faces_id,
                                                     float2* __restrict__ coords, float* __restrict__ depths,
-                                                    const float4* __restrict__ tris) {
+                                                    const float* __restrict__ tris) {
   const uint32_t count = min(handoff[1], ua.cap);
   const uint64_t max_visits = 2 * node_count + 2;
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
@@ -899,7 +909,7 @@ __device__ __forceinline__ bool trace_packet_job(
     uint32_t face_count, uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
     int bx, int by, uint32_t* __restrict__ handoff, uint32_t handoff_cap, uint32_t handoff_faces,
     uint32_t* __restrict__ state, uint32_t* __restrict__ forced_list, uint32_t forced_cap,
-    const uint32_t* __restrict__ resume, const float4* __restrict__ tris, uint32_t* s_stack, float4* s_v0,
+    const uint32_t* __restrict__ resume, const float* __restrict__ tris, uint32_t* s_stack, float4* s_v0,
     float4* s_e1, float4* s_e2, float4* s_dir0, uint32_t* s_slot, float* s_pt, float* s_pu, float* s_pv,
     uint32_t* s_pk) {
   const int ray = (int)threadIdx.x % kPT, part = (int)threadIdx.x / kPT;
@@ -1076,7 +1086,7 @@ __global__ __launch_bounds__(kPT) void k_trace_packet(const rk::Node* __restrict
                                                       float* __restrict__ depths, uint32_t* __restrict__ handoff,
                                                       uint32_t handoff_cap, uint32_t handoff_faces,
                                                       uint32_t* __restrict__ forced_list, uint32_t forced_cap,
-                                                      const float4* __restrict__ tris) {
+                                                      const float* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float4 s_dir0;
@@ -1099,7 +1109,7 @@ __global__ __launch_bounds__(kPT * kPacketSplit) void k_trace_packet_split(
     const float4* __restrict__ origins, const float4* __restrict__ dirs, int width, int height, int packets_x,
     uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
     uint32_t* __restrict__ faces_id, float2* __restrict__ coords, float* __restrict__ depths,
-    const float4* __restrict__ tris) {
+    const float* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float4 s_dir0;
@@ -1147,7 +1157,7 @@ __global__ __launch_bounds__(kFThreads) void k_packet_forced(
     uint32_t node_count, uint32_t face_count, const uint32_t* __restrict__ handoff, uint32_t handoff_cap,
     const uint32_t* __restrict__ forced_list, uint32_t forced_cap, float4* __restrict__ temp,
     uint32_t* __restrict__ fflags, const uint32_t* __restrict__ faces_id, const float2* __restrict__ coords,
-    const float* __restrict__ depths, const float4* __restrict__ tris) {
+    const float* __restrict__ depths, const float* __restrict__ tris) {
   __shared__ uint32_t s_stack[kPStack];
   __shared__ float4 s_v0[kPCache], s_e1[kPCache], s_e2[kPCache];
   __shared__ float s_pt[kFThreads], s_pu[kFThreads], s_pv[kFThreads];
@@ -1368,11 +1378,11 @@ static uint32_t* handoff_alloc(hipStream_t st, size_t words, uint32_t faces) {
 
 // The call's triangle records (k_tri_records), from the same pool, or nullptr:
 // the kernels then read the faces and vertices.
-static float4* tri_records_alloc(hipStream_t st, const uint4* faces, const float4* verts, uint32_t face_count) {
+static float* tri_records_alloc(hipStream_t st, const uint4* faces, const float4* verts, uint32_t face_count) {
   if (!RK_REF_TRI_RECORDS) return nullptr;
-  float4* tris = nullptr;
+  float* tris = nullptr;
   hipMemPool_t pool = handoff_pool();
-  const size_t bytes = (size_t)face_count * 3 * sizeof(float4);
+  const size_t bytes = (size_t)face_count * kRecFloats * sizeof(float);
   const hipError_t e = pool ? hipMallocFromPoolAsync(reinterpret_cast<void**>(&tris), bytes, pool, st)
                             : hipMallocAsync(reinterpret_cast<void**>(&tris), bytes, st);
   if (e != hipSuccess) {
@@ -1410,7 +1420,7 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     uint32_t* flist = ho ? ho + flist_off : nullptr;
     uint32_t* fflags = ho ? ho + flist_off + fcap : nullptr;
     float4* temp = ho ? reinterpret_cast<float4*>(ho + temp_off) : nullptr;
-    float4* tris = tri_records_alloc(st, reinterpret_cast<const uint4*>(faces),
+    float* tris = tri_records_alloc(st, reinterpret_cast<const uint4*>(faces),
                                      reinterpret_cast<const float4*>(vertices), face_count);
     hipLaunchKernelGGL(k_trace_packet, dim3(gx, gy), dim3(kPT), 0, st, reinterpret_cast<const rk::Node*>(nodes),
                        reinterpret_cast<const uint4*>(faces), reinterpret_cast<const float4*>(vertices),
@@ -1465,7 +1475,7 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     const float4* org = reinterpret_cast<const float4*>(origins);
     const float4* drs = reinterpret_cast<const float4*>(directions);
     float2* uv = reinterpret_cast<float2*>(coords);
-    float4* tris = tri_records_alloc(st, fc, vt, face_count);
+    float* tris = tri_records_alloc(st, fc, vt, face_count);
     hipLaunchKernelGGL(k_trace_ray<RK_REF_COOP_MAX>, dim3((n + 63) / 64), dim3(64), 0, st, nd, fc, vt, org,
                        drs, n, node_count, face_count, faces_id, uv, depths, ho, ho ? cap : 0u, faces_limit, ua,
                        tris);
