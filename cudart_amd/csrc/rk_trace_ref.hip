@@ -34,6 +34,9 @@ constexpr float kUMargin = 0.001f;
 #ifndef RK_REF_U_PRETEST
 #define RK_REF_U_PRETEST 1
 #endif
+#ifndef RK_REF_SHARED_TVEC
+#define RK_REF_SHARED_TVEC 1
+#endif
 
 // intersectTriRay, CUDA_raytraceMeshTree.cu:57-89.
 __device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, float4 e1, float4 e2, float& t, float& u,
@@ -60,6 +63,34 @@ __device__ __forceinline__ bool tri_ray(float4 orig, float4 dir, float4 v0, floa
   if (v < 0.0f || u + v > 1.0f) return false;
   t = rk::dot4(e2, qvec) * inv_det;
   return t >= 0.0f;
+}
+
+// tri_ray with tvec = orig - v0 given: for rays that share an origin (bit for
+// bit, w included) it is computed once per face, with tri_ray's operation.
+// The same results as tri_ray.
+__device__ __forceinline__ bool tri_ray_tv(float4 dir, float4 tvec, float4 e1, float4 e2, float& t, float& u,
+                                           float& v) {
+  float4 pvec = rk::cross4(dir, e2);
+  float det = rk::dot4(e1, pvec);
+  if (det > -kEps && det < kEps) return false;
+  const float un = rk::dot4(tvec, pvec);
+#if RK_REF_U_PRETEST
+  const float ua = un * __builtin_amdgcn_rcpf(det);
+  if (ua < -kUMargin || ua > 1.0f + kUMargin) return false;
+#endif
+  float inv_det = 1.0f / det;
+  u = un * inv_det;
+  if (u < 0.0f || u > 1.0f) return false;
+  float4 qvec = rk::cross4(tvec, e1);
+  v = rk::dot4(dir, qvec) * inv_det;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  t = rk::dot4(e2, qvec) * inv_det;
+  return t >= 0.0f;
+}
+
+__device__ __forceinline__ bool same_bits(float4 a, float4 b) {
+  return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) &&
+         __float_as_uint(a.z) == __float_as_uint(b.z) && __float_as_uint(a.w) == __float_as_uint(b.w);
 }
 
 // intersectBoxRay, CUDA_raytraceMeshTree.cu:91-105.
@@ -735,6 +766,9 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
     const float4 invdir = has_ray ? invdir_of(dir) : zero4;
     float depth = has_ray ? depths[l] : 0.0f;
     const float4 dir0 = readlane_f4(dir, 0);  // the octant's: the leader's direction
+    const float4 orig0 = readlane_f4(orig, 0);
+    // rays from one camera share their origin: tvec once per face
+    const bool shared = RK_REF_SHARED_TVEC && __ballot(has_ray && !same_bits(orig, orig0)) == 0ull;
     uint32_t fid = kNoFace;
     float2 coord = make_float2(0.0f, 0.0f);
     uint32_t st0 = lane == 0 ? root : 0u, st1 = 0u;
@@ -758,19 +792,25 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
           if ((uint32_t)lane < cc) {
             float4 v0, e1, e2;
             tri_get(faces, verts, tris, first + c0 + (uint32_t)lane, v0, e1, e2);
-            s_v0[lane] = v0;
+            s_v0[lane] = shared ? rk::sub4(orig0, v0) : v0;  // tvec, or v0
             s_e1[lane] = e1;
             s_e2[lane] = e2;
           }
           __syncthreads();
           if (walking) {
-#ifdef RK_REF_GROUP_UNROLL
-#pragma unroll RK_REF_GROUP_UNROLL
-#endif
-            for (uint32_t q = 0; q < cc; ++q) {
-              float t, u, v;
-              if (tri_ray(orig, dir, s_v0[q], s_e1[q], s_e2[q], t, u, v)) {
-                if (t < depth) { fid = first + c0 + q; coord = make_float2(u, v); depth = t; }
+            if (shared) {
+              for (uint32_t q = 0; q < cc; ++q) {
+                float t, u, v;
+                if (tri_ray_tv(dir, s_v0[q], s_e1[q], s_e2[q], t, u, v)) {
+                  if (t < depth) { fid = first + c0 + q; coord = make_float2(u, v); depth = t; }
+                }
+              }
+            } else {
+              for (uint32_t q = 0; q < cc; ++q) {
+                float t, u, v;
+                if (tri_ray(orig, dir, s_v0[q], s_e1[q], s_e2[q], t, u, v)) {
+                  if (t < depth) { fid = first + c0 + q; coord = make_float2(u, v); depth = t; }
+                }
               }
             }
           }
@@ -1210,7 +1250,9 @@ __global__ __launch_bounds__(kFThreads) void k_packet_forced(
           for (uint32_t k = threadIdx.x; k < cc; k += kFThreads) {
             float4 v0, e1, e2;
             tri_get(faces, verts, tris, tri_id + c0 + k, v0, e1, e2);
-            s_v0[k] = v0; s_e1[k] = e1; s_e2[k] = e2;
+            s_v0[k] = v0;
+            s_e1[k] = e1;
+            s_e2[k] = e2;
           }
           __syncthreads();
           const uint32_t qn = (cc + kFParts - 1) / kFParts;
