@@ -58,8 +58,20 @@ MUT = [
 only = sys.argv[1:]
 base = Path(tempfile.mkdtemp(prefix="oracle_mut_"))
 shutil.rmtree(base)
+def _ignored_top_dirs():
+    """Top-level directories that git ignores (working files, run outputs, caches)."""
+    r = subprocess.run(["git", "-C", str(REPO), "ls-files", "--others", "--ignored", "--exclude-standard",
+                        "--directory"], capture_output=True, text=True, check=True)
+    return {l.rstrip("/") for l in r.stdout.splitlines() if l.endswith("/") and "/" not in l.rstrip("/")}
+
+
+_TOP_SKIP = _ignored_top_dirs() | {".git"}
+
+
 def _skip(d, names):  # the oracle's own build is redone per mutation; the rest is copied as built
-    skip = {".git", "_work", "gpurun_out", "__pycache__", ".pytest_cache"}
+    skip = {"__pycache__", ".pytest_cache"}
+    if Path(d).resolve() == REPO:
+        skip |= _TOP_SKIP
     if Path(d).resolve() == (REPO / "oracle").resolve():
         skip.add("_build")
     return [n for n in names if n in skip]

@@ -1,7 +1,7 @@
 """Error of the hit distance t against FP64. The GPU computes t with the
 reference's FP32 Moller-Trumbore arithmetic (contraction off); the oracle
 recomputes t in double for the same face (oracle.hit_t_f64). The figures are
-printed and, when the directory exists, written to gpurun_out/accuracy.json."""
+printed and, when RK_ACCURACY_JSON names a file, written there as JSON."""
 import json
 import os
 
@@ -50,8 +50,9 @@ def _stats(name, ora, F, V, O, D, order, fid, dep):
          "max_ulp": float(ulp.max()), "mean_ulp": float(ulp.mean()), "p999_ulp": float(np.quantile(ulp, 0.999))}
     RESULTS[name] = s
     print(name, s)
-    if os.path.isdir("gpurun_out"):
-        with open("gpurun_out/accuracy.json", "w") as f:
+    out = os.environ.get("RK_ACCURACY_JSON")
+    if out:
+        with open(out, "w") as f:
             json.dump(RESULTS, f, indent=1)
     return s
 
