@@ -134,6 +134,16 @@ struct Part {
   uint32_t ref, end;
 };
 static_assert(sizeof(Part) == 32, "part is 32 B");
+// A part exchanged through global memory at a split that crosses chunks, with
+// the split key at its outer end (left end for a left child, right end for a
+// right child), which its parent's side test needs: carried with the part, so
+// the climb does not load keys after each exchange.
+struct GPart {
+  Part p;
+  uint64_t outer;
+  uint64_t pad;
+};
+static_assert(sizeof(GPart) == 48, "global part is 48 B");
 
 __device__ __forceinline__ void write_node(FNode* __restrict__ nodes, uint32_t idx, const Part& c0, const Part& c1,
                                            uint32_t n0, uint32_t n1) {
@@ -231,19 +241,23 @@ __device__ __forceinline__ Part finish_split(FNode* __restrict__ nodes, uint32_t
 // 16-B loads and stores that are coherent across XCDs, as relaxed agent-scope
 // atomics are (the same cache policy bit), but one instruction per 16 B.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void store_part_coherent(Part* p, const Part& v) {
+__device__ __forceinline__ void store_part_coherent(GPart* p, const Part& v, uint64_t outer) {
   const u32x4 a = {__float_as_uint(v.lox), __float_as_uint(v.loy), __float_as_uint(v.loz), __float_as_uint(v.hix)};
   const u32x4 b = {__float_as_uint(v.hiy), __float_as_uint(v.hiz), v.ref, v.end};
-  asm volatile("global_store_dwordx4 %0, %1, off sc1\n\tglobal_store_dwordx4 %0, %2, off offset:16 sc1"
-               :
-               : "v"(p), "v"(a), "v"(b)
-               : "memory");
-}
-__device__ __forceinline__ Part load_part_coherent(const Part* p) {
-  u32x4 a, b;
+  const u32x4 c = {(uint32_t)outer, (uint32_t)(outer >> 32), 0u, 0u};
   asm volatile(
-      "global_load_dwordx4 %0, %2, off sc1\n\tglobal_load_dwordx4 %1, %2, off offset:16 sc1\n\ts_waitcnt vmcnt(0)"
-      : "=&v"(a), "=&v"(b)
+      "global_store_dwordx4 %0, %1, off sc1\n\tglobal_store_dwordx4 %0, %2, off offset:16 sc1\n\t"
+      "global_store_dwordx4 %0, %3, off offset:32 sc1"
+      :
+      : "v"(p), "v"(a), "v"(b), "v"(c)
+      : "memory");
+}
+__device__ __forceinline__ Part load_part_coherent(const GPart* p, uint64_t& outer) {
+  u32x4 a, b, c;
+  asm volatile(
+      "global_load_dwordx4 %0, %3, off sc1\n\tglobal_load_dwordx4 %1, %3, off offset:16 sc1\n\t"
+      "global_load_dwordx4 %2, %3, off offset:32 sc1\n\ts_waitcnt vmcnt(0)"
+      : "=&v"(a), "=&v"(b), "=&v"(c)
       : "v"(p)
       : "memory");
   Part v;
@@ -255,6 +269,7 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
   v.hiz = __uint_as_float(b.y);
   v.ref = b.z;
   v.end = b.w;
+  outer = (uint64_t)c.x | ((uint64_t)c.y << 32);
   return v;
 }
 
@@ -264,32 +279,31 @@ __device__ __forceinline__ Part load_part_coherent(const Part* p) {
 // (published: already in its slot); the first to arrive stops, the second
 // merges both and climbs on. Every split above a crossing split crosses
 // chunks too, so the whole climb exchanges through global memory.
-template <typename K>
-__device__ __forceinline__ void climb_crossing(const K* __restrict__ keys, uint32_t n, uint32_t leaf_max,
-                                               uint32_t* __restrict__ gflags, Part* __restrict__ gparts,
-                                               FNode* __restrict__ nodes, uint32_t lo, uint32_t hi, Part me,
-                                               uint32_t l, uint32_t r, bool as_left, bool published) {
+// outer: the split key at the outer end of [l, r] (~0 past the array's end).
+__device__ __forceinline__ void climb_crossing(uint32_t n, uint32_t leaf_max, uint32_t* __restrict__ gflags,
+                                               GPart* __restrict__ gparts, FNode* __restrict__ nodes, uint32_t lo,
+                                               uint32_t hi, Part me, uint64_t outer, uint32_t l, uint32_t r,
+                                               bool as_left, bool published) {
   for (uint32_t guard = 0; guard < n; ++guard) {
     const uint32_t p = as_left ? r : l - 1;
     const uint32_t mine = as_left ? p : p + 1, other = as_left ? p + 1 : p;
     if (!published) {
       me.end = as_left ? l : r;
-      store_part_coherent(gparts + mine, me);
+      store_part_coherent(gparts + mine, me, outer);
       __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     published = false;
     const uint32_t old = __hip_atomic_fetch_add(gflags + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
-    const Part sib = load_part_coherent(gparts + other);
+    uint64_t sib_outer;
+    const Part sib = load_part_coherent(gparts + other, sib_outer);
     const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
     const bool root = nl == lo && nr == hi;  // the (mesh) root
     bool next_left = false;
-    if (!root) {
-      const bool has_l = nl > 0, has_r = nr + 1 < n;
-      const uint64_t kl = has_l ? split_key(keys, nl - 1) : ~0ull;
-      const uint64_t kr = has_r ? split_key(keys, nr) : ~0ull;
-      next_left = left_of_parent(has_l, has_r, kl, kr);
-    }
+    // the merged range's end keys: the left child's outer key and the right child's
+    const uint64_t kl = as_left ? outer : sib_outer, kr = as_left ? sib_outer : outer;
+    if (!root) next_left = left_of_parent(nl > 0, nr + 1 < n, kl, kr);
+    outer = next_left ? kl : kr;
     me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib, next_left)
                  : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me, next_left);
     if (root) return;
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
                                                                   const uint4* __restrict__ faces,
                                                                   const float4* __restrict__ verts, uint32_t n,
                                                                   uint32_t leaf_max, uint32_t* __restrict__ gflags,
-                                                                  Part* __restrict__ gparts,
+                                                                  GPart* __restrict__ gparts,
                                                                   FNode* __restrict__ nodes,
                                                                   float4* __restrict__ tris,
                                                                   const rk_batch_mesh* __restrict__ meshes,
@@ -414,15 +428,19 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       return;
     }
     // the side of [l, r] under its parent, from the split keys at its ends
-    auto side = [&](uint32_t l, uint32_t r) {
+    // (and the split key at the outer end of [l, r] in outer)
+    auto side = [&](uint32_t l, uint32_t r, uint64_t& outer) {
       const bool has_l = l > 0, has_r = r + 1 < n;
       uint64_t kl = ~0ull, kr = ~0ull;
       if (has_l) kl = (l >= c0 && l - c0 <= kChunk) ? s_key[l - c0] : split_key(keys, l - 1);
       if (has_r) kr = (r + 1 >= c0 && r + 1 - c0 <= kChunk) ? s_key[r + 1 - c0] : split_key(keys, r);
-      return left_of_parent(has_l, has_r, kl, kr);
+      const bool left = left_of_parent(has_l, has_r, kl, kr);
+      outer = left ? kl : kr;
+      return left;
     };
     uint32_t l = i, r = i;
-    bool as_left = side(l, r);
+    uint64_t outer = ~0ull;
+    bool as_left = side(l, r, outer);
     // Each step widens [l, r] by at least one position: n steps bound it.
     for (uint32_t guard = 0; guard < n; ++guard) {
       const uint32_t p = as_left ? r : l - 1;
@@ -430,11 +448,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       me.end = as_left ? l : r;
       if (!(p >= c0 && p - c0 < kChunk && s_local[p - c0])) {
         if (inline_top) {  // small trees: the chain above the chunks is short
-          climb_crossing<K>(keys, n, leaf_max, gflags, gparts, nodes, lo, hi, me, l, r, as_left, false);
+          climb_crossing(n, leaf_max, gflags, gparts, nodes, lo, hi, me, outer, l, r, as_left, false);
           return;
         }
         // crossing split: publish the part in its slot and hand the climb on
-        gparts[mine] = me;
+        gparts[mine].p = me;
+        gparts[mine].outer = outer;
         const uint32_t k = atomicAdd(&s_npend, 1u);
         pend[c0 + k] = mine | (as_left ? kPendLeft : 0u);
         return;
@@ -446,7 +465,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
       const Part sib = s_part[other - c0];
       const uint32_t nl = as_left ? l : sib.end, nr = as_left ? sib.end : r;
       const bool root = nl == lo && nr == hi;  // the (mesh) root
-      const bool next_left = root ? false : side(nl, nr);
+      const bool next_left = root ? false : side(nl, nr, outer);
       me = as_left ? finish_split(nodes, lo, hi, leaf_max, p, nl, nr, me, sib, next_left)
                    : finish_split(nodes, lo, hi, leaf_max, p, nl, nr, sib, me, next_left);
       if (root) return;
@@ -465,7 +484,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_fast_hierarchy(const K* __res
 // slot already (written by k_fast_hierarchy).
 template <typename K, bool kBatch>
 __global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__ keys, uint32_t n, uint32_t leaf_max,
-                                                           uint32_t* __restrict__ gflags, Part* __restrict__ gparts,
+                                                           uint32_t* __restrict__ gflags, GPart* __restrict__ gparts,
                                                            FNode* __restrict__ nodes,
                                                            const rk_batch_mesh* __restrict__ meshes,
                                                            const uint32_t* __restrict__ pend,
@@ -476,7 +495,8 @@ __global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__
     const uint32_t e = pend[c0 + k];
     const uint32_t mine = e & ~kPendLeft;
     const bool as_left = (e & kPendLeft) != 0;
-    const Part me = load_part_coherent(gparts + mine);
+    uint64_t outer;
+    const Part me = load_part_coherent(gparts + mine, outer);
     const uint32_t l = as_left ? me.end : mine, r = as_left ? mine : me.end;
     uint32_t lo = 0, hi = n - 1;
     if (kBatch) {
@@ -484,7 +504,7 @@ __global__ __launch_bounds__(64) void k_fast_hierarchy_top(const K* __restrict__
       lo = m.face_offset;
       hi = m.face_offset + m.face_count - 1;
     }
-    climb_crossing<K>(keys, n, leaf_max, gflags, gparts, nodes, lo, hi, me, l, r, as_left, true);
+    climb_crossing(n, leaf_max, gflags, gparts, nodes, lo, hi, me, outer, l, r, as_left, true);
   }
 }
 
@@ -990,17 +1010,17 @@ constexpr uint32_t kInlineTopFaces = RK_INLINE_TOP_FACES;
 // Workspace of the hierarchy passes over n positions.
 struct HierWs {
   uint32_t* flags;       // counters of splits that cross chunks (zeroed per build)
-  Part* parts;           // parts exchanged at splits that cross chunks
+  GPart* parts;          // parts exchanged at splits that cross chunks
   uint32_t* pend;        // per chunk, the climbs handed on to the top pass
   uint32_t* pend_count;  // per chunk, how many
 };
 inline size_t hier_ws_bytes(size_t n) {
-  return rk_align_up(n * 4) + rk_align_up(n * sizeof(Part)) + rk_align_up(n * 4) +
+  return rk_align_up(n * 4) + rk_align_up(n * sizeof(GPart)) + rk_align_up(n * 4) +
          rk_align_up(((n + kChunk - 1) / kChunk) * 4);
 }
 inline bool take_hier_ws(RkArena& ar, uint32_t n, HierWs& w) {
   w.flags = ar.take<uint32_t>(n);
-  w.parts = ar.take<Part>(n);
+  w.parts = ar.take<GPart>(n);
   w.pend = ar.take<uint32_t>(n);
   w.pend_count = ar.take<uint32_t>(blocks_for(n, kChunk));
   return w.flags && w.parts && w.pend && w.pend_count;
