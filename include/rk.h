@@ -311,7 +311,7 @@ rk_status rk_fast_build_keyed(rk_stream stream, const rk_uint4* faces, const rk_
  * workspace is a file-scope device array of RK_NOALLOC_SCRATCH_BYTES (128 MiB),
  * one per device, held until the process ends. face_count is limited to what
  * fits it, rk_fast_build_workspace_size(face_count, morton_mode) <=
- * RK_NOALLOC_SCRATCH_BYTES: 2,555,904 faces with RK_MORTON_30 and 2,199,712
+ * RK_NOALLOC_SCRATCH_BYTES: 1,944,805 faces with RK_MORTON_30 and 1,720,320
  * with RK_MORTON_63; larger counts return RK_ERROR_WORKSPACE. Everything the
  * build reads from the array is cleared by the build itself. All calls on one
  * device share that array, so calls that overlap on one device (concurrent

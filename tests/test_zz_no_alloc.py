@@ -15,7 +15,7 @@ from cudart_amd import native as N
 pytestmark = pytest.mark.gpu
 
 RK_ERROR_WORKSPACE = 4
-LIMIT = {N.RK_MORTON_30: 2_555_904, N.RK_MORTON_63: 2_199_712}
+LIMIT = {N.RK_MORTON_30: 1_944_805, N.RK_MORTON_63: 1_720_320}
 
 
 def _buffers(n):
