@@ -64,9 +64,9 @@ HEADLINE_DET_EPSILON = 1e-6  # the reference's absolute parallel-ray threshold (
 # HBM bytes per launch of the headline trace kernel (the default traversal
 # variant, tmax reset in the kernel), from two rocprofv3 --pmc runs of
 # `bench.py --profile-only` (profiles/fast_build_trace_heightfield708.md):
-# FETCH_SIZE 107,377 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 10,575 KiB
+# FETCH_SIZE 107,351 KiB, scaled x2 by a 256 MiB copy, plus WRITE_SIZE 10,576 KiB
 # (means of the two warm frames at the 1e-6 threshold).
-TRACE_TRAFFIC_BYTES = (107_377 * 2 + 10_575) * 1024
+TRACE_TRAFFIC_BYTES = (107_351 * 2 + 10_576) * 1024
 
 
 # The fast build's kernels, per launch, measured with rocprofv3 on the headline
@@ -78,17 +78,17 @@ TRACE_TRAFFIC_BYTES = (107_377 * 2 + 10_575) * 1024
 # k_fast_hierarchy (up to 2^21 faces); the soup's in k_fast_hierarchy_top.
 BUILD_KERNELS = {  # kernel: (us, HBM read KiB, HBM write KiB, launches per build)
     "heightfield": {
-        "k_fast_prep": (14.86, 32_290, 42_961, 1),
-        "k_rs_hist": (12.36, 3_992, 1_523, 1),
-        "k_rs_scatter": (16.51, 10_920, 10_406, 4),
-        "k_fast_hierarchy": (120.17, 52_280, 118_035, 1),
+        "k_fast_prep": (15.12, 32_289, 42_960, 1),
+        "k_rs_hist": (12.20, 3_992, 1_285, 1),
+        "k_rs_scatter": (16.46, 10_937, 10_366, 4),
+        "k_fast_hierarchy": (109.70, 51_675, 119_632, 1),
     },
     "soup": {
-        "k_fast_prep": (208.4, 625_438, 468_776, 1),
-        "k_rs_hist": (67.75, 78_284, 18_713, 1),
-        "k_rs_scatter": (61.57, 132_300, 135_800, 8),
-        "k_fast_hierarchy": (851.0, 2_928_140, 1_186_547, 1),
-        "k_fast_hierarchy_top": (142.8, 60_688, 32_715, 1),
+        "k_fast_prep": (211.5, 625_591, 468_775, 1),
+        "k_rs_hist": (66.08, 78_298, 18_713, 1),
+        "k_rs_scatter": (60.33, 132_323, 134_816, 8),
+        "k_fast_hierarchy": (854.0, 2_928_171, 1_192_198, 1),
+        "k_fast_hierarchy_top": (126.2, 51_644, 38_306, 1),
     },
 }
 HBM_PEAK_GBS = 8000.0
