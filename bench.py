@@ -753,6 +753,7 @@ def main():
         torch.cuda.synchronize()
         rtree.node_count = int(rtree.host_info()[native.RK_INFO_NODE_COUNT])
         dev.trace_ref(rtree, Vt, Ot, Dt, W, H, False)
+        dev.trace_ref(rtree, Vt, Ot, Dt, W, H, True)  # the packet traversal too
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB copy: scales FETCH/WRITE_SIZE
         b = torch.empty_like(a)
         b.copy_(a)

@@ -330,7 +330,17 @@ constexpr uint32_t kUnits = 1u << kUnitLevel;  // subtrees per ray at most: the 
 #define RK_REF_CROSS_CAP 4096
 #endif
 constexpr uint32_t kCrossCap = RK_REF_CROSS_CAP;          // rays split per call
-constexpr uint32_t kUnitGrid = 8192;          // one-wave workgroups walking the subtrees
+constexpr uint32_t kUnitGrid = 8192;          // workgroups walking the subtrees
+// A subtree walk takes two waves, each testing half of every batch of faces,
+// when the tree's leaves hold 64 faces or more on average (node_count / 2
+// leaves): the heightfield's depth-12 LBVH (155 faces per leaf) 7.7 -> 6.8 ms
+// per frame; with small leaves the second wave only repeats the walk
+// (depth-31 LBVH, 42 faces per leaf: 9.5 against 9.2 ms; SAH tree, 21 faces
+// per leaf: 5.6 against 6.7 ms), and four waves were slower on all three.
+#ifndef RK_REF_UNIT_WAVES
+#define RK_REF_UNIT_WAVES 2
+#endif
+constexpr uint32_t kUnitWaveFaces = 64;  // faces per leaf from which a walk takes RK_REF_UNIT_WAVES waves
 constexpr uint32_t kNoFace = 0xffffffffu;
 constexpr uint32_t kUnitStop = 1u;            // the walk stopped here (stack overflow, child out of range)
 
@@ -722,7 +732,8 @@ __global__ __launch_bounds__(64) void k_trace_ray_wave(const rk::Node* __restric
 // faces go to LDS 64 at a time, and each ray tests them in face order against
 // its own depth, as the reference does. A ray whose subtree list differs from
 // the leader's is marked diverged too. Each ray gets its UnitResult.
-__global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ nodes,
+template <int kUW>
+__global__ __launch_bounds__(64 * kUW) void k_units_walk(const rk::Node* __restrict__ nodes,
                                                    const uint4* __restrict__ faces,
                                                    const float4* __restrict__ verts,
                                                    const float4* __restrict__ origins,
@@ -730,8 +741,9 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
                                                    uint32_t node_count, uint32_t face_count,
                                                    const uint32_t* __restrict__ handoff, UnitArgs ua,
                                                    const float* __restrict__ tris) {
-  __shared__ float4 s_v0[64], s_e1[64], s_e2[64];
-  const int lane = (int)threadIdx.x;
+  __shared__ float4 s_v0[64 * kUW], s_e1[64 * kUW], s_e2[64 * kUW];
+  __shared__ float4 s_part[kUW > 1 ? 64 * kUW : 1];  // a wave's first smallest (t, u, v, face) per ray
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
   uint32_t bcount[8], groups = 0;
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
@@ -754,7 +766,7 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
     const uint32_t i = has_ray ? ua.bucket[o * ua.cap + g * 64u + (uint32_t)lane] : 0u;
     const uint32_t lead = readlane_u32(i, 0);
     const uint32_t nu_lead = ua.nunits[lead];
-    if (k == 0 && has_ray && ua.nunits[i] != nu_lead) ua.diverged[i] = 1u;
+    if (k == 0 && has_ray && wv == 0 && ua.nunits[i] != nu_lead) ua.diverged[i] = 1u;
     if (k >= nu_lead) continue;
     const uint32_t root = ua.unit_node[lead * kUnits + k], sp_base = ua.unit_sp[lead * kUnits + k];
     bool walking = has_ray && ua.nunits[i] == nu_lead && ua.unit_node[i * kUnits + k] == root &&
@@ -787,30 +799,51 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
       if (rk::node_is_leaf(nd.w0)) {
         const uint32_t first = nd.w1;
         const uint32_t cnt = first < face_count ? min(rk::leaf_count(nd.w0), face_count - first) : 0u;
-        for (uint32_t c0 = 0; c0 < cnt; c0 += 64u) {
-          const uint32_t cc = min(64u, cnt - c0);
-          if ((uint32_t)lane < cc) {
+        // Batches of 64 * kUW faces; wave w tests [w * 64, w * 64 + 64) of each,
+        // keeping its first smallest t below the depth at the batch's start,
+        // and the waves' results are merged in wave order: a sequential pass
+        // keeping t < depth ends on the same face (the first of smallest t).
+        for (uint32_t c0 = 0; c0 < cnt; c0 += 64u * kUW) {
+          const uint32_t cc = min(64u * kUW, cnt - c0);
+          if (threadIdx.x < cc) {
             float4 v0, e1, e2;
-            tri_get(faces, verts, tris, first + c0 + (uint32_t)lane, v0, e1, e2);
-            s_v0[lane] = shared ? rk::sub4(orig0, v0) : v0;  // tvec, or v0
-            s_e1[lane] = e1;
-            s_e2[lane] = e2;
+            tri_get(faces, verts, tris, first + c0 + threadIdx.x, v0, e1, e2);
+            s_v0[threadIdx.x] = shared ? rk::sub4(orig0, v0) : v0;  // tvec, or v0
+            s_e1[threadIdx.x] = e1;
+            s_e2[threadIdx.x] = e2;
           }
           __syncthreads();
+          const uint32_t q0 = (uint32_t)wv * 64u, q1 = min(cc, q0 + 64u);
+          float pt = depth, pu = 0.0f, pv = 0.0f;
+          uint32_t pq = 0xFFFFFFFFu;  // none
           if (walking) {
             if (shared) {
-              for (uint32_t q = 0; q < cc; ++q) {
+              for (uint32_t q = q0; q < q1; ++q) {
                 float t, u, v;
                 if (tri_ray_tv(dir, s_v0[q], s_e1[q], s_e2[q], t, u, v)) {
-                  if (t < depth) { fid = first + c0 + q; coord = make_float2(u, v); depth = t; }
+                  if (t < pt) { pq = q; pu = u; pv = v; pt = t; }
                 }
               }
             } else {
-              for (uint32_t q = 0; q < cc; ++q) {
+              for (uint32_t q = q0; q < q1; ++q) {
                 float t, u, v;
                 if (tri_ray(orig, dir, s_v0[q], s_e1[q], s_e2[q], t, u, v)) {
-                  if (t < depth) { fid = first + c0 + q; coord = make_float2(u, v); depth = t; }
+                  if (t < pt) { pq = q; pu = u; pv = v; pt = t; }
                 }
+              }
+            }
+          }
+          if (kUW == 1) {
+            if (pq != 0xFFFFFFFFu) { fid = first + c0 + pq; coord = make_float2(pu, pv); depth = pt; }
+          } else {
+            s_part[threadIdx.x] = make_float4(pt, pu, pv, __uint_as_float(pq));
+            __syncthreads();
+            if (walking) {
+#pragma unroll
+              for (int w = 0; w < kUW; ++w) {
+                const float4 r = s_part[w * 64 + lane];
+                const uint32_t rq = __float_as_uint(r.w);
+                if (rq != 0xFFFFFFFFu && r.x < depth) { fid = first + c0 + rq; coord = make_float2(r.y, r.z); depth = r.x; }
               }
             }
           }
@@ -844,7 +877,7 @@ __global__ __launch_bounds__(64) void k_units_walk(const rk::Node* __restrict__ 
         }
       }
     }
-    if (has_ray) {
+    if (has_ray && wv == 0) {
       if (diverged) ua.diverged[i] = 1u;
       UnitResult r;
       r.t = depth;
@@ -1524,8 +1557,13 @@ extern "C" rk_status rk_raytrace_mesh_tree(rk_stream stream, const rk_bvnode* no
     if (ho && ucap != 0u) {
       hipLaunchKernelGGL(k_units_split, dim3((ucap + 63) / 64), dim3(64), 0, st, nd, org, drs, depths, node_count,
                          ho, ua);
-      hipLaunchKernelGGL(k_units_walk, dim3(std::min<uint32_t>(ucap * kUnits, kUnitGrid)), dim3(64), 0, st, nd, fc,
-                         vt, org, drs, depths, node_count, face_count, ho, ua, tris);
+      const dim3 ugrid(std::min<uint32_t>(ucap * kUnits, kUnitGrid));
+      if ((uint64_t)face_count >= (uint64_t)kUnitWaveFaces * ((node_count + 1) / 2))
+        hipLaunchKernelGGL(k_units_walk<RK_REF_UNIT_WAVES>, ugrid, dim3(64 * RK_REF_UNIT_WAVES), 0, st, nd, fc, vt, org,
+                           drs, depths, node_count, face_count, ho, ua, tris);
+      else
+        hipLaunchKernelGGL(k_units_walk<1>, ugrid, dim3(64), 0, st, nd, fc, vt, org, drs, depths, node_count,
+                           face_count, ho, ua, tris);
       hipLaunchKernelGGL(k_units_merge, dim3(std::min<uint32_t>(ucap, kWaveGrid)), dim3(64), 0, st, nd, fc, vt, org,
                          drs, node_count, face_count, ho, ua, faces_id, uv, depths, tris);
     }
